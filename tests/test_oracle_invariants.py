@@ -1,0 +1,215 @@
+"""Operator-level and solution-level pins for the CPU oracle (CPU only).
+
+* Gauss's law      -- k = 0: the DLP row sums are -1/2 |T_i| on a closed
+                      surface (outward normal); the DLP potential of a unit
+                      density is -1 inside and 0 outside.
+* sphere spectrum  -- V 1 = sin(kR) e^{ikR} / k on a sphere of radius R
+                      (l = 0 eigenvalue ik R^2 j_0 h_0); W z = mu_1 z with
+                      mu_1 = -i k^3 R^2 j_1'(kR) h_1'(kR) (hypersingular, Maue
+                      form); both converge at O(h^2) on icospheres.
+* monopole         -- a source inside a closed surface radiates exactly
+                      G(x, s) outside: CBIE + Kirchhoff-Helmholtz reproduces
+                      it with the error falling ~h^2 (r-clamp off).
+* r-clamp          -- the reference clamps r >= 1e-3 (potential.h:16-23): at
+                      R = 0.1 it changes the solution by ~3e-3, at R = 1 by
+                      much less; the switch is visible.
+* FP32 mirror      -- agrees with FP64 to 1e-5 relative Frobenius.
+* BM combinations  -- bm_lhs = -K + (i/k) W, bm_rhs = -V - (i/k) K' (the
+                      relations the reference checks, check_with_bempp.py:145-163).
+* solvers          -- GMRES and the BiCGSTAB restatement against a direct
+                      solve; BiCGSTAB's silent non-convergence / early exit.
+"""
+import numpy as np
+import pytest
+from scipy.special import spherical_jn, spherical_yn
+
+import oracle
+from neuralat_amd import mesh
+
+
+def _areas(v, t):
+    return 0.5 * np.linalg.norm(np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]]), axis=1)
+
+
+def test_gauss_law_dlp_row_sums():
+    errs = []
+    for L in (1, 2):
+        v, t = mesh.icosphere(L, 1.0)
+        K = oracle.assemble(v, t, 0.0, "double", "DP0", clamp=False)
+        a = _areas(v, t)
+        errs.append(np.abs(K.sum(axis=1).real / a + 0.5).max())
+        assert np.abs(K.imag).max() == 0.0
+    # exact on a closed polyhedron for x inside a face: only quadrature error remains
+    assert max(errs) < 2e-5, errs
+
+
+def test_gauss_law_double_layer_potential():
+    v, t = mesh.icosphere(3, 1.0)
+    inside = mesh.random_exterior_points(8, 0.0, 0.4, seed=3)
+    outside = mesh.random_exterior_points(8, 2.0, 3.0, seed=4)
+    for basis, ones in (("DP0", np.ones(len(t))), ("P1", np.ones(len(v)))):
+        Di = oracle.potential(v, t, inside, 0.0, "double", basis)
+        Do = oracle.potential(v, t, outside, 0.0, "double", basis)
+        assert np.abs(Di @ ones + 1).max() < 1e-4
+        assert np.abs(Do @ ones).max() < 1e-4
+
+
+def test_sphere_single_layer_l0_eigenvalue():
+    R, k = 1.0, 1.5  # away from the zero of j0 at kR = pi
+    lam = np.sin(k * R) * np.exp(1j * k * R) / k  # ik R^2 j0(kR) h0(kR)
+    errs, vals = [], []
+    for L in (1, 2, 3):
+        v, t = mesh.icosphere(L, R)
+        S = oracle.assemble(v, t, k, "single", "DP0", clamp=False)
+        a = _areas(v, t)
+        got = S.sum() / a.sum()
+        vals.append(got)
+        errs.append(abs(got - lam) / abs(lam))
+    # O(h^2) geometric (polyhedron vs sphere) error; Richardson removes it
+    assert errs[1] / errs[2] > 3.0 and errs[0] / errs[1] > 3.0, errs
+    rich = (4 * vals[2] - vals[1]) / 3
+    assert abs(rich - lam) / abs(lam) < 3e-4, (errs, abs(rich - lam) / abs(lam))
+
+
+def _h1(x):
+    return spherical_jn(1, x) + 1j * spherical_yn(1, x)
+
+
+def test_sphere_hypersingular_l1_eigenvalue():
+    R, k = 1.0, 1.0  # away from the zero of j1' at kR = 2.08
+    jp = spherical_jn(1, k * R, derivative=True)
+    hp = jp + 1j * spherical_yn(1, k * R, derivative=True)
+    mu = -1j * k ** 3 * R ** 2 * jp * hp
+    errs, vals = [], []
+    for L in (1, 2, 3):
+        v, t = mesh.icosphere(L, R)
+        W = oracle.assemble(v, t, k, "hypersingular", "P1", clamp=False)
+        M = oracle.identity(v, t, "P1").real
+        z = v[:, 2]
+        got = (z @ W @ z) / (z @ M @ z)
+        vals.append(got)
+        errs.append(abs(got - mu) / abs(mu))
+    assert errs[1] / errs[2] > 3.0 and errs[0] / errs[1] > 3.0, errs
+    rich = (4 * vals[2] - vals[1]) / 3
+    assert abs(rich - mu) / abs(mu) < 2e-3, (errs, abs(rich - mu) / abs(mu))
+
+
+def _cbie_error(L, R, k, basis, clamp, precision="f64"):
+    v, t = mesh.icosphere(L, R)
+    val, dn = mesh.sources_field([[0.1 * R, -0.05 * R, 0.02 * R]], None, k)
+    S = oracle.assemble(v, t, k, "single", basis, clamp=clamp, precision=precision)
+    D = oracle.assemble(v, t, k, "double", basis, clamp=clamp, precision=precision)
+    M = oracle.identity(v, t, basis)
+    proj = mesh.project_dp0 if basis == "DP0" else mesh.project_p1
+    psi = proj(v, t, lambda x, n: dn(x, n))
+    phi = np.linalg.solve(0.5 * M - D, -S @ psi)
+    pts = mesh.random_exterior_points(64, 2 * R, 5 * R, 0)
+    p = (-oracle.potential(v, t, pts, k, "single", basis, clamp=clamp) @ psi
+         + oracle.potential(v, t, pts, k, "double", basis, clamp=clamp) @ phi)
+    pe = val(pts)
+    return np.linalg.norm(p - pe) / np.linalg.norm(pe), phi
+
+
+@pytest.mark.parametrize("basis", ["DP0", "P1"])
+def test_monopole_convergence_without_clamp(basis):
+    # off-centre source: the error falls as h^2 (4x per level)
+    e = [_cbie_error(L, 0.1, 18.3, basis, clamp=False)[0] for L in (1, 2, 3)]
+    assert e[2] < 4e-4, e
+    assert e[0] / e[1] > 3.5 and e[1] / e[2] > 3.5, e
+
+
+def test_clamp_effect_is_visible():
+    e_on, phi_on = _cbie_error(3, 0.1, 18.3, "DP0", clamp=True)
+    e_off, phi_off = _cbie_error(3, 0.1, 18.3, "DP0", clamp=False)
+    assert e_on > 5 * e_off, (e_on, e_off)
+    assert np.linalg.norm(phi_on - phi_off) / np.linalg.norm(phi_off) > 1e-3
+    # at R = 1 the L2 mesh has no quadrature pair closer than 1e-3 except in
+    # the innermost Sauter-Schwab points: the effect is far smaller
+    e1_on, p1 = _cbie_error(2, 1.0, 1.83, "DP0", clamp=True)
+    e1_off, p2 = _cbie_error(2, 1.0, 1.83, "DP0", clamp=False)
+    assert np.linalg.norm(p1 - p2) / np.linalg.norm(p2) < 1e-4
+
+
+@pytest.mark.parametrize("op", ["single", "double", "adjointdouble", "hypersingular", "bm_lhs", "bm_rhs"])
+def test_fp32_mirror_matches_fp64(op):
+    v, t = mesh.octasphere(2, 1.0)
+    for k in (1.0, 10.0):
+        a64 = oracle.assemble(v, t, k, op, "P1", precision="f64")
+        a32 = oracle.assemble(v, t, k, op, "P1", precision="f32")
+        assert np.linalg.norm(a32 - a64) / np.linalg.norm(a64) < 1e-5, (op, k)
+    if op in ("single", "double", "adjointdouble"):
+        a64 = oracle.assemble(v, t, 10.0, op, "DP0", precision="f64")
+        a32 = oracle.assemble(v, t, 10.0, op, "DP0", precision="f32")
+        assert np.linalg.norm(a32 - a64) / np.linalg.norm(a64) < 1e-5
+
+
+@pytest.mark.parametrize("quad", ["full", "approx", "approx1"])
+def test_burton_miller_combinations(quad):
+    v, t = mesh.octasphere(1, 1.0)
+    for k in (1.0, 10.0, -7.0):
+        beta = 1j / k
+        S = oracle.assemble(v, t, k, "single", "P1", quad=quad)
+        K = oracle.assemble(v, t, k, "double", "P1", quad=quad)
+        Kp = oracle.assemble(v, t, k, "adjointdouble", "P1", quad=quad)
+        W = oracle.assemble(v, t, k, "hypersingular", "P1", quad=quad)
+        L = oracle.assemble(v, t, k, "bm_lhs", "P1", quad=quad)
+        Rr = oracle.assemble(v, t, k, "bm_rhs", "P1", quad=quad)
+        assert np.linalg.norm(L - (-K + beta * W)) / np.linalg.norm(L) < 1e-12
+        assert np.linalg.norm(Rr - (-S - beta * Kp)) / np.linalg.norm(Rr) < 1e-12
+
+
+def test_hypersingular_annihilates_constants_at_k0():
+    v, t = mesh.icosphere(2, 1.0)
+    W = oracle.assemble(v, t, 0.0, "hypersingular", "P1")
+    assert np.abs(W.sum(axis=1)).max() < 1e-12 * np.abs(W).max() * len(v)
+
+
+def test_gmres_matches_direct_solve():
+    rng = np.random.default_rng(0)
+    n = 120
+    A = np.eye(n) * 2 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.5
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x_ref = np.linalg.solve(A, b)
+    for restart in (5, 30, 200):
+        x, its, conv, rel = oracle.gmres(lambda z: A @ z, b, tol=1e-10, restart=restart, maxiter=2000)
+        assert conv
+        assert np.linalg.norm(x - x_ref) / np.linalg.norm(x_ref) < 1e-8
+
+
+def test_bicgstab_restatement():
+    rng = np.random.default_rng(1)
+    n = 80
+    A = np.eye(n) * 3 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.4
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x, steps, conv = oracle.bicgstab(lambda z: A @ z, b)
+    assert conv and steps > 0
+    # tol = 1e-10 on the squared norm = 1e-5 relative residual
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-5
+    # silent non-convergence with a 1-step budget
+    x1, steps1, conv1 = oracle.bicgstab(lambda z: A @ z, b, nsteps=1)
+    assert steps1 == 1 and not conv1
+    # b = 0: returns x0 = 0 without iterating (BiCGSTAB.py:43,93-94)
+    x0, steps0, conv0 = oracle.bicgstab(lambda z: A @ z, np.zeros(n, complex))
+    assert steps0 == 0 and conv0 and np.all(x0 == 0)
+
+
+def test_triangle2vertex_average():
+    v, t = mesh.icosphere(1)
+    f = np.arange(len(t)) * (1 + 0.5j)
+    out = oracle.triangle2vertex(len(v), t, f)
+    for vi in (0, 5, len(v) - 1):
+        m = np.any(t == vi, axis=1)
+        assert abs(out[vi] - f[m].mean()) < 1e-12
+    assert np.allclose(oracle.triangle2vertex(len(v), t, np.full(len(t), 2 - 1j)), 2 - 1j)
+
+
+def test_neighbors_icosphere():
+    v, t = mesh.icosphere(2)
+    ptr, idx, sh, ncol = oracle.neighbors(t, len(v))
+    cnt = np.diff(ptr)
+    # 13 neighbours (self, 3 edge, 9 vertex) except at the 12 valence-5 vertices
+    assert set(np.unique(cnt)) <= {12, 13}
+    for tri in range(len(t)):
+        s = sh[ptr[tri]:ptr[tri + 1]]
+        assert (s == 3).sum() == 1 and (s == 2).sum() == 3
+    assert 4 <= ncol <= 16
