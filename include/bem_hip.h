@@ -1,0 +1,140 @@
+/*
+ * bem_hip.h -- C ABI of the MI355X-native BEM Helmholtz library (libbem_hip.so).
+ *
+ * Drop-in for the reference's JIT torch extension CUDA_MODULE
+ * (src/bem/cuda/bind.cu:178-223, loaded at src/bem/solver.py:9-31) and the
+ * torch matvec/BiCGSTAB it is used with (src/bem/solver.py:61-66,
+ * src/bem/BiCGSTAB.py:5-107).  Differences by design:
+ *   - plain pointers and sizes, no torch types; all device pointers are HIP
+ *     device memory of the current device; outputs are caller-allocated;
+ *   - every call takes the HIP stream to launch on (hipStream_t as void*;
+ *     NULL = the null stream) and returns a bem_status code; a message for
+ *     the last failure on the calling thread is in bem_last_error();
+ *   - the mesh is preprocessed once into a bem_mesh handle (normals, curls,
+ *     quadrature points, neighbour pairs), which the reference recomputes in
+ *     torch and in every singular launch (solver.py:40-58, bind.cu:90-97);
+ *   - both bases: P1 (vertex DOFs, the reference's basis, V x V matrices)
+ *     and DP0 (one DOF per triangle, N x N);
+ *   - complex matrices are complex64 (float re, float im) row-major with a
+ *     leading dimension ld (in complex elements); row = test function (x),
+ *     column = trial function (y), as the reference.
+ * No entry point raises C++ exceptions.  Calls on one stream are ordered;
+ * a bem_mesh may be shared by threads (it is read-only after creation).
+ */
+#ifndef BEM_HIP_H
+#define BEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    BEM_OK = 0,
+    BEM_ERR_BAD_ARG = 1,      /* invalid argument (null pointer, size, enum) */
+    BEM_ERR_BAD_MESH = 2,     /* index out of range, degenerate triangle, ... */
+    BEM_ERR_HIP = 3,          /* a HIP runtime call failed */
+    BEM_ERR_NOT_CONVERGED = 4,/* solver stopped before reaching the tolerance */
+    BEM_ERR_NO_DEVICE = 5,    /* no HIP device */
+    BEM_ERR_OOM = 6           /* device allocation failed */
+} bem_status;
+
+/* operator kinds (potential.h:7-15) */
+typedef enum {
+    BEM_OP_SINGLE = 0,         /* single layer  V   */
+    BEM_OP_DOUBLE = 1,         /* double layer  K   */
+    BEM_OP_ADJOINT_DOUBLE = 2, /* adjoint dbl.  K'  */
+    BEM_OP_HYPERSINGULAR = 3,  /* hypersingular W (Maue form, P1 only) */
+    BEM_OP_BM_LHS = 4,         /* -K + (i/k) W       (P1 only) */
+    BEM_OP_BM_RHS = 5          /* -V - (i/k) K'      */
+} bem_op;
+
+typedef enum { BEM_BASIS_P1 = 0, BEM_BASIS_DP0 = 1 } bem_basis;
+
+/* quadrature (bind.cu:180-215): full = 4-pt line / 6-pt triangle,
+ * approx = 2 / 3, approx_1 = 2 / 1 */
+typedef enum { BEM_QUAD_FULL = 0, BEM_QUAD_APPROX = 1, BEM_QUAD_APPROX_1 = 2 } bem_quad;
+
+typedef struct bem_mesh bem_mesh;
+
+typedef struct {
+    int32_t iterations;          /* Krylov iterations (GMRES) or steps (BiCGSTAB) */
+    int32_t converged;           /* 1 if the tolerance was reached */
+    double rel_residual;         /* ||b - A x|| / ||b|| recomputed at exit */
+    double implicit_rel_residual;/* GMRES least-squares estimate at exit */
+} bem_solve_info;
+
+const char* bem_version(void);
+const char* bem_last_error(void);
+
+/* Mesh handle.  verts: [nv,3] float32, tris: [nt,3] int32 (vertex order
+ * gives the outward normal (v1-v0) x (v2-v0)).  on_device != 0: the
+ * pointers are device memory (e.g. torch CUDA tensors), else host memory.
+ * Replaces BEM_Solver.__init__ + preprocess (solver.py:40-58,70-81). */
+int bem_mesh_create(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, int on_device, void* stream,
+                    bem_mesh** out);
+int bem_mesh_destroy(bem_mesh* mesh);
+int bem_mesh_info(const bem_mesh* mesh, int64_t* nv, int64_t* nt, int64_t* n_neighbor_pairs,
+                  int32_t* max_neighbors);
+/* unit normals [nt,3] and P1 surface curls [nt,3,3] (row a = n x grad phi_a)
+ * into device memory (solver.py:40-58). */
+int bem_mesh_normals_curls(const bem_mesh* mesh, float* normals, float* curls, void* stream);
+
+/* Boundary operator matrix.  Replaces <op>_boundary_matrix[_approx[_1]]
+ * (bind.cu:49-154, exports bind.cu:180-215).  P1: out is [nv, nv]; DP0:
+ * [nt, nt].  clamp != 0 applies the reference's r >= 1e-3 clamp
+ * (potential.h:16-23). */
+int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, void* out, int64_t ld,
+                 void* stream);
+/* DP0 rows [row0, row1) only (out has row1-row0 rows); rows are independent,
+ * so a large operator can be assembled block by block. */
+int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
+                      void* out, int64_t ld, void* stream);
+/* Stateless form of bem_assemble on device pointers (P1, the reference's
+ * single_boundary_matrix(vertices, triangles, ...) call shape). */
+int bem_boundary_matrix(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, float k, int op, int quad,
+                        void* out, void* stream);
+
+/* Mass matrix: P1 [nv,nv] (bind.cu:10-21), DP0 diag(area) [nt,nt]. */
+int bem_identity(const bem_mesh* mesh, int basis, void* out, int64_t ld, void* stream);
+/* Vertex average of per-triangle complex values (bind.cu:23-48). */
+int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream);
+/* Potential operator from the surface to np points ([np,3] float32, device):
+ * out [np, nv] (P1) or [np, nt] (DP0); op = SINGLE or DOUBLE
+ * (single/double_boundary_potential, bind.cu:156-176,219-222). */
+int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
+                  void* out, int64_t ld, void* stream);
+
+/* y = alpha op(A) x + beta y, A complex64 [rows, cols] row-major, trans:
+ * 0 = A, 1 = A^T, 2 = A^H.  work: device scratch of
+ * bem_matvec_work_size() bytes (may be NULL when that is 0).  Replaces the
+ * torch `@` (solver.py:65,103,131,137,147). */
+int64_t bem_matvec_work_size(int64_t rows, int64_t cols, int trans);
+int bem_matvec(const void* A, int64_t rows, int64_t cols, int64_t lda, int trans, const void* x, void* y,
+               float alpha_re, float alpha_im, float beta_re, float beta_im, void* work, void* stream);
+/* C = a A + b B elementwise (A or B may be NULL); e.g. 0.5 M - K
+ * (solver.py:102). */
+int bem_combine(const void* A, int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc, int64_t rows,
+                int64_t cols, float a_re, float a_im, float b_re, float b_im, void* stream);
+
+/* Restarted GMRES(restart) on a dense complex64 [n, n] matrix, device
+ * resident (classical Gram-Schmidt + re-orthogonalisation, Givens on the
+ * device); stops when ||b - A x|| <= tol ||b||.  x: initial guess in,
+ * solution out.  Returns BEM_ERR_NOT_CONVERGED (x still the last iterate)
+ * when maxiter runs out.  No reference counterpart (the reference solves
+ * with BiCGSTAB, solver.py:61-66). */
+int64_t bem_gmres_work_size(int64_t n, int restart);
+int bem_gmres(const void* A, int64_t n, int64_t lda, const void* b, void* x, float tol, int restart, int maxiter,
+              void* work, bem_solve_info* info, void* stream);
+/* Reference-compatible BiCGSTAB (BiCGSTAB.py:5-107): bilinear rho, alpha,
+ * omega; stops when ||b - A x||^2 < tol ||b||^2 or < atol; stops silently
+ * (returns BEM_OK, info->converged = 0) when nsteps run out or rho == 0. */
+int64_t bem_bicgstab_work_size(int64_t n);
+int bem_bicgstab(const void* A, int64_t n, int64_t lda, const void* b, void* x, int nsteps, double tol, double atol,
+                 void* work, bem_solve_info* info, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BEM_HIP_H */

@@ -1,0 +1,687 @@
+// Boundary-element assembly kernels for gfx950.
+//
+// Regular pairs (triangles sharing no vertex; reference: bind.cu:67-79,
+// integrand.h:187-235,280-329):
+//   * one lane owns one COLUMN triangle j (its Q quadrature points and
+//     weights live in VGPRs); the ROW triangle i is wave-uniform, so its data
+//     is fetched with scalar loads and used as SGPR operands;
+//   * DP0: each (i, j) entry is produced by exactly one lane and stored
+//     directly -- a wave writes 64 consecutive complex entries of a row;
+//   * P1: a workgroup owns a (row tile x column tile) of a locality-ordered
+//     triangle list; the 3x3 basis blocks of its pairs are summed in LDS on
+//     the tile's compacted vertex slots, then flushed once per tile entry.
+// Neighbour pairs (sharing >= 1 vertex; bind.cu:81-152, integrand.h:63-185,
+// 331-394) go to a separate Sauter-Schwab kernel: one wavefront per pair,
+// the n^4 tensor Gauss points spread over the lanes, FP64 accumulation.
+#include "bem_device.hpp"
+#include "bem_internal.hpp"
+
+namespace bem {
+
+// ---------------------------------------------------------------------------
+// utility kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void fill_c64_kernel(float2* __restrict__ out, int64_t rows, int64_t cols,
+                                                       int64_t ld, float2 val)
+{
+    int64_t total = rows * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = e / cols, c = e - r * cols;
+        out[r * ld + c] = val;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// regular pairs, DP0 basis
+// ---------------------------------------------------------------------------
+// Grid: x = column blocks of 256 triangles, y = row tiles of kDp0RowTile rows
+// (rows [row0, row1) of the operator).  out row r holds triangle row0 + r.
+template <int OP, int Q, bool CLAMP>
+__global__ __launch_bounds__(256) void dp0_regular_kernel(const float* __restrict__ rec, int64_t nt, int64_t row0,
+                                                          int64_t row1, float k, float2* __restrict__ out,
+                                                          int64_t ld)
+{
+    constexpr int S = Rec<Q>::kFloats;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < nt;
+    const int64_t jj = active ? j : nt - 1;
+    const float* cr = rec + jj * S;
+    float py[3 * Q], wy[Q];
+#pragma unroll
+    for (int b = 0; b < 3 * Q; ++b) py[b] = cr[b];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
+    const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
+    const float krev = k * kInv2Pi;
+
+    const int64_t i_begin = row0 + (int64_t)blockIdx.y * kDp0RowTile;
+    const int64_t i_end = min(i_begin + (int64_t)kDp0RowTile, row1);
+    for (int64_t i = i_begin; i < i_end; ++i) {
+        const float* rr = rec + i * S;  // wave-uniform
+        float accr = 0.f, acci = 0.f;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                float vr, vi, tr = 0.f, ti = 0.f;
+                kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
+                                     rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, tr, ti);
+                sr = fmaf(wy[b], vr, sr);
+                si = fmaf(wy[b], vi, si);
+            }
+            const float ua = rr[3 * Q + a];
+            accr = fmaf(ua, sr, accr);
+            acci = fmaf(ua, si, acci);
+        }
+        if (active) out[(i - row0) * ld + j] = make_float2(accr, acci);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// regular pairs, P1 basis (LDS tile accumulation)
+// ---------------------------------------------------------------------------
+template <int OP, int Q, bool CLAMP>
+__global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
+                                                         const float* __restrict__ curls, P1Tiles tl, float k,
+                                                         float2* __restrict__ out, int64_t ld)
+{
+    constexpr int S = Rec<Q>::kFloats;
+    __shared__ float acc[kP1RowVerts * kP1ColVerts * 2];
+    const int ct = blockIdx.x, rt = blockIdx.y;
+    const int c0 = tl.col_ptr[ct], c1 = tl.col_ptr[ct + 1];
+    const int r0 = tl.row_ptr[rt], r1 = tl.row_ptr[rt + 1];
+    const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
+    const int cv0 = tl.col_vptr[ct], ncv = tl.col_vptr[ct + 1] - cv0;
+    for (int e = threadIdx.x; e < 2 * nrv * ncv; e += 256) acc[e] = 0.f;
+    __syncthreads();
+
+    const int pos = c0 + (int)threadIdx.x;
+    const bool active = pos < c1;
+    const int j = tl.perm[active ? pos : c0];
+    const float* cr = rec + (int64_t)j * S;
+    float py[3 * Q], wy[Q];
+#pragma unroll
+    for (int b = 0; b < 3 * Q; ++b) py[b] = cr[b];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
+    const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
+    const int4 tj = tris[j];
+    const int4 cs = tl.col_slot[active ? pos : c0];
+    float cj[9];
+    if constexpr (OP == HYP || OP == BM_LHS) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) cj[e] = curls[(int64_t)j * 12 + e];
+    }
+    const float krev = k * kInv2Pi;
+
+    for (int rp = r0; rp < r1; ++rp) {
+        const int i = tl.perm[rp];  // wave-uniform
+        const int4 ti = tris[i];
+        const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
+                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z;
+        if (!active || shared) continue;
+        const float* rr = rec + (int64_t)i * S;
+        const int4 rs = tl.row_slot[rp];
+        float br[9], bi[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
+        float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+            float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
+            float tr9 = 0.f, ti9 = 0.f;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                float vr, vi, gr = 0.f, gi = 0.f;
+                kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
+                                     rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, gr, gi);
+                vr *= wy[b];
+                vi *= wy[b];
+                const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
+                const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
+                sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
+                sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
+                sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
+                if constexpr (OP == BM_LHS) {
+                    tr9 = fmaf(wy[b], gr, tr9);
+                    ti9 = fmaf(wy[b], gi, ti9);
+                }
+            }
+            const float ua = rr[3 * Q + a];
+            const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
+            const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
+#pragma unroll
+            for (int be = 0; be < 3; ++be) {
+                br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
+                br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
+                br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
+            }
+            if constexpr (OP == BM_LHS) {
+                gr9 = fmaf(ua, tr9, gr9);
+                gi9 = fmaf(ua, ti9, gi9);
+            }
+        }
+        if constexpr (OP == HYP) {
+            // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
+            float sgr = 0.f, sgi = 0.f;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
+            float ci[9], cp[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+            curl_product(ci, cj, cp);
+            const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
+            const float kk = k * k * nn;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                br[e] = sgr * cp[e] - br[e] * kk;
+                bi[e] = sgi * cp[e] - bi[e] * kk;
+            }
+        } else if constexpr (OP == BM_LHS) {
+            // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
+            const float invk = 1.0f / k;
+            const float s9r = -gi9 * invk, s9i = gr9 * invk;
+            float ci[9], cp[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+            curl_product(ci, cj, cp);
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                br[e] = fmaf(s9r, cp[e], br[e]);
+                bi[e] = fmaf(s9i, cp[e], bi[e]);
+            }
+        }
+        const int rsl[3] = {rs.x, rs.y, rs.z};
+        const int csl[3] = {cs.x, cs.y, cs.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                float* dst = &acc[2 * (rsl[a] * ncv + csl[b])];
+                atomicAdd(dst, br[3 * a + b]);
+                atomicAdd(dst + 1, bi[3 * a + b]);
+            }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nrv * ncv; e += 256) {
+        const int rvi = e / ncv, cvi = e - rvi * ncv;
+        const int64_t gr = tl.row_verts[rv0 + rvi], gc = tl.col_verts[cv0 + cvi];
+        float* dst = reinterpret_cast<float*>(&out[gr * ld + gc]);
+        unsafeAtomicAdd(dst, acc[2 * e]);
+        unsafeAtomicAdd(dst + 1, acc[2 * e + 1]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// neighbour (singular) pairs: Sauter-Schwab, one wavefront per pair
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int OP, int NL, int BASIS, bool CLAMP>
+__global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__ geo, const int4* __restrict__ tris,
+                                                       const float* __restrict__ curls,
+                                                       const int4* __restrict__ pairs, int64_t p0, int64_t p1,
+                                                       float k, float2* __restrict__ out, int64_t ld, int64_t row0)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t p = p0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (p >= p1) return;  // uniform per wavefront
+    const int4 pr = pairs[p];
+    const int i = __builtin_amdgcn_readfirstlane(pr.x);
+    const int j = __builtin_amdgcn_readfirstlane(pr.y);
+    const int shared = __builtin_amdgcn_readfirstlane(pr.z);
+    const int4 ti = tris[i], tj = tris[j];
+    const int ia[3] = {ti.x, ti.y, ti.z}, ja[3] = {tj.x, tj.y, tj.z};
+    int pi[3] = {0, 1, 2}, pj[3] = {0, 1, 2};
+    if (shared == 2) {
+        // common edge first, same order on both triangles, ordered by position
+        // in the column triangle (integrand.h:351-364)
+        int c = 0;
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a)
+                if (ia[a] == ja[b]) { pi[c] = a; pj[c] = b; ++c; }
+        pi[2] = 3 - pi[0] - pi[1];
+        pj[2] = 3 - pj[0] - pj[1];
+    } else if (shared == 1) {
+        // common vertex swapped to local 0 (integrand.h:365-382)
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                if (ia[a] == ja[b]) {
+                    if (a != 0) { pi[0] = a; pi[a] = 0; }
+                    if (b != 0) { pj[0] = b; pj[b] = 0; }
+                }
+    }
+    // geo record: v0 v1 v2 (9), n (3), jac (1), pad
+    const float* gi = geo + (int64_t)i * 16;
+    const float* gj = geo + (int64_t)j * 16;
+    float v[3][3], w[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            v[a][d] = gi[3 * pi[a] + d];
+            w[a][d] = gj[3 * pj[a] + d];
+        }
+    // x = v0 + x1 (v1 - v0) + x2 (v2 - v1) on (0,0),(1,0),(1,1); the shared
+    // local vertex 0 cancels in d = y - x, which keeps r accurate near the
+    // singularity.
+    float e1[3], e2[3], f1[3], f2[3], o[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = v[1][d] - v[0][d];
+        e2[d] = v[2][d] - v[1][d];
+        f1[d] = w[1][d] - w[0][d];
+        f2[d] = w[2][d] - w[1][d];
+        o[d] = w[0][d] - v[0][d];
+    }
+    const float nx0 = gi[9], nx1 = gi[10], nx2 = gi[11];
+    const float ny0 = gj[9], ny1 = gj[10], ny2 = gj[11];
+    const float jij = gi[12] * gj[12];
+    const float krev = k * kInv2Pi;
+
+    double ar[10], ai[10];
+#pragma unroll
+    for (int e = 0; e < 10; ++e) ar[e] = ai[e] = 0.0;
+
+    constexpr int NP = NL * NL * NL * NL;
+    for (int q = lane; q < NP; q += 64) {
+        const int q0 = q / (NL * NL * NL), q1 = (q / (NL * NL)) % NL, q2 = (q / NL) % NL, q3 = q % NL;
+        const float xs = LineRule01<NL>::t[q0], e_1 = LineRule01<NL>::t[q1];
+        const float e_2 = LineRule01<NL>::t[q2], e_3 = LineRule01<NL>::t[q3];
+        const float wq = LineRule01<NL>::w[q0] * LineRule01<NL>::w[q1] * LineRule01<NL>::w[q2] * LineRule01<NL>::w[q3];
+        const float e12 = e_1 * e_2, e123 = e12 * e_3;
+        float reg[6][5];
+        int nreg;
+        float jac;
+        if (shared == 3) {
+            jac = xs * xs * xs * e_1 * e_1 * e_2;
+            nreg = 6;
+            const float t[6][4] = {
+                {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1)},
+                {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12)},
+                {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12)},
+                {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123)},
+                {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12)},
+                {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123)},
+            };
+            for (int r = 0; r < 6; ++r) {
+                for (int c = 0; c < 4; ++c) reg[r][c] = t[r][c];
+                reg[r][4] = 1.f;
+            }
+        } else if (shared == 2) {
+            jac = xs * xs * xs * e_1 * e_1;
+            nreg = 5;
+            const float t[5][5] = {
+                {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f},
+                {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2},
+                {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2},
+                {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2},
+                {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2},
+            };
+            for (int r = 0; r < 5; ++r)
+                for (int c = 0; c < 5; ++c) reg[r][c] = t[r][c];
+        } else {
+            jac = xs * xs * xs * e_2;
+            nreg = 2;
+            const float t[2][4] = {
+                {xs, xs * e_1, xs * e_2, xs * e_2 * e_3},
+                {xs * e_2, xs * e_2 * e_3, xs, xs * e_1},
+            };
+            for (int r = 0; r < 2; ++r) {
+                for (int c = 0; c < 4; ++c) reg[r][c] = t[r][c];
+                reg[r][4] = 1.f;
+            }
+        }
+        float br[10], bi[10];
+#pragma unroll
+        for (int e = 0; e < 10; ++e) br[e] = bi[e] = 0.f;
+        for (int r = 0; r < nreg; ++r) {
+            const float x1 = reg[r][0], x2 = reg[r][1], y1 = reg[r][2], y2 = reg[r][3];
+            const float dx = o[0] + y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
+            const float dy = o[1] + y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
+            const float dz = o[2] + y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
+            float vr, vi, gr = 0.f, gg = 0.f;
+            kernel4pi<OP, CLAMP>(dx, dy, dz, nx0, nx1, nx2, ny0, ny1, ny2, k, krev, vr, vi, gr, gg);
+            const float ww = reg[r][4];
+            vr *= ww;
+            vi *= ww;
+            if constexpr (BASIS == P1) {
+                const float px[3] = {1.f - x1, x1 - x2, x2};
+                const float py[3] = {1.f - y1, y1 - y2, y2};
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        const float pp = px[a] * py[b];
+                        br[3 * a + b] = fmaf(vr, pp, br[3 * a + b]);
+                        bi[3 * a + b] = fmaf(vi, pp, bi[3 * a + b]);
+                    }
+            }
+            if constexpr (OP == BM_LHS) {
+                br[9] = fmaf(gr, ww, br[9]);
+                bi[9] = fmaf(gg, ww, bi[9]);
+            } else {
+                br[9] += vr;
+                bi[9] += vi;
+            }
+        }
+        const double sc = (double)(jac * wq) * (double)jij;
+#pragma unroll
+        for (int e = 0; e < 10; ++e) {
+            ar[e] += sc * (double)br[e];
+            ai[e] += sc * (double)bi[e];
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 10; ++e) {
+        ar[e] = wave_sum(ar[e]) * (double)kInv4Pi;
+        ai[e] = wave_sum(ai[e]) * (double)kInv4Pi;
+    }
+    if constexpr (BASIS == DP0) {
+        if (lane == 0) out[((int64_t)i - row0) * ld + j] = make_float2((float)ar[9], (float)ai[9]);
+    } else {
+        if (lane < 9) {
+            // lane = reordered (a, b) pair; its original local indices are (pi[a], pj[b])
+            const int a = lane / 3, b = lane % 3;
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                if (e == lane) { vr = ar[e]; vi = ai[e]; }
+            const int oa = pi[a], ob = pj[b];
+            if constexpr (OP == HYP || OP == BM_LHS) {
+                float ci[9], cj[9], cp[9];
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    ci[e] = curls[(int64_t)i * 12 + e];
+                    cj[e] = curls[(int64_t)j * 12 + e];
+                }
+                curl_product(ci, cj, cp);
+                const double c = cp[3 * oa + ob];
+                if constexpr (OP == HYP) {
+                    const double kk = (double)k * (double)k * (double)(nx0 * ny0 + nx1 * ny1 + nx2 * ny2);
+                    // sum over all 9 reordered entries = plain-G integral (phi sums to 1)
+                    vr = ar[9] * c - vr * kk;
+                    vi = ai[9] * c - vi * kk;
+                } else {
+                    const double invk = 1.0 / (double)k;
+                    vr += -ai[9] * invk * c;
+                    vi += ar[9] * invk * c;
+                }
+            }
+            float* dst = reinterpret_cast<float*>(&out[(int64_t)ia[oa] * ld + ja[ob]]);
+            unsafeAtomicAdd(dst, (float)vr);
+            unsafeAtomicAdd(dst + 1, (float)vi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mass matrix, triangle -> vertex average, potentials
+// ---------------------------------------------------------------------------
+// P1 mass matrix row va = sum over triangles T around va of |T|/12 (1 + delta)
+// (closed form of integrand.h:257-278); one thread per row, so each entry has
+// one writer and a fixed summation order.
+__global__ __launch_bounds__(256) void identity_p1_kernel(const float* __restrict__ geo, const int4* __restrict__ tris,
+                                                          const int32_t* __restrict__ vt_ptr,
+                                                          const int32_t* __restrict__ vt_idx, int64_t nv,
+                                                          float2* __restrict__ out, int64_t ld)
+{
+    const int64_t va = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (va >= nv) return;
+    for (int e = vt_ptr[va]; e < vt_ptr[va + 1]; ++e) {
+        const int t = vt_idx[e];
+        const float area = 0.5f * geo[(int64_t)t * 16 + 12];
+        const int4 tv = tris[t];
+        const int vs[3] = {tv.x, tv.y, tv.z};
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            float2* dst = &out[va * ld + vs[b]];
+            const float add = area * (vs[b] == va ? (2.0f / 12.0f) : (1.0f / 12.0f));
+            dst->x += add;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void identity_dp0_kernel(const float* __restrict__ geo, int64_t nt,
+                                                           float2* __restrict__ out, int64_t ld)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    out[t * ld + t] = make_float2(0.5f * geo[t * 16 + 12], 0.f);
+}
+
+// vertex value = mean of the values of the triangles around it (bind.cu:23-48)
+__global__ __launch_bounds__(256) void tri2vert_kernel(const float2* __restrict__ tv, const int32_t* __restrict__ vt_ptr,
+                                                       const int32_t* __restrict__ vt_idx, int64_t nv,
+                                                       float2* __restrict__ out)
+{
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    float sr = 0.f, si = 0.f;
+    const int e0 = vt_ptr[v], e1 = vt_ptr[v + 1];
+    for (int e = e0; e < e1; ++e) {
+        const float2 x = tv[vt_idx[e]];
+        sr += x.x;
+        si += x.y;
+    }
+    const float n = (float)(e1 - e0);
+    out[v] = make_float2(sr / n, si / n);
+}
+
+// Surface -> point operator (bind.cu:156-176, integrand.h:237-255,396-406):
+// row p = evaluation point (n_x = 0), column = triangle (DP0) or its three
+// vertices (P1, summed with atomics into a zeroed output).
+template <int OP, int BASIS, bool CLAMP>
+__global__ __launch_bounds__(256) void potential_kernel(const float* __restrict__ rec6, const int4* __restrict__ tris,
+                                                        int64_t nt, const float* __restrict__ pts, int64_t np, float k,
+                                                        float2* __restrict__ out, int64_t ld)
+{
+    constexpr int Q = 6;
+    constexpr int S = Rec<Q>::kFloats;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < nt;
+    const float* cr = rec6 + (active ? j : nt - 1) * S;
+    float py[3 * Q], wy[Q];
+#pragma unroll
+    for (int b = 0; b < 3 * Q; ++b) py[b] = cr[b];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
+    const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
+    const int4 tj = tris[active ? j : nt - 1];
+    const float krev = k * kInv2Pi;
+    const int64_t p_begin = (int64_t)blockIdx.y * kPotRowTile;
+    const int64_t p_end = min(p_begin + (int64_t)kPotRowTile, np);
+    for (int64_t p = p_begin; p < p_end; ++p) {
+        const float x0 = pts[3 * p], x1 = pts[3 * p + 1], x2 = pts[3 * p + 2];
+        float s[3][2] = {{0.f, 0.f}, {0.f, 0.f}, {0.f, 0.f}};
+        float tr = 0.f, ti = 0.f;
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            float vr, vi, g0 = 0.f, g1 = 0.f;
+            kernel4pi<OP, CLAMP>(py[3 * b] - x0, py[3 * b + 1] - x1, py[3 * b + 2] - x2, 0.f, 0.f, 0.f, ny0, ny1,
+                                 ny2, k, krev, vr, vi, g0, g1);
+            vr *= wy[b];
+            vi *= wy[b];
+            if constexpr (BASIS == P1) {
+                const float ph[3] = {1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b], TriRule<Q>::x1[b],
+                                     TriRule<Q>::x2[b]};
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    s[c][0] = fmaf(vr, ph[c], s[c][0]);
+                    s[c][1] = fmaf(vi, ph[c], s[c][1]);
+                }
+            } else {
+                tr += vr;
+                ti += vi;
+            }
+        }
+        if (!active) continue;
+        if constexpr (BASIS == DP0) {
+            out[p * ld + j] = make_float2(tr, ti);
+        } else {
+            const int vs[3] = {tj.x, tj.y, tj.z};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float* dst = reinterpret_cast<float*>(&out[p * ld + vs[c]]);
+                unsafeAtomicAdd(dst, s[c][0]);
+                unsafeAtomicAdd(dst + 1, s[c][1]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid1d(int64_t n, int64_t per = 256, int64_t cap = 1 << 20)
+{
+    int64_t g = (n + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_c64_kernel, dim3(grid1d(rows * cols, 256, 65536)), dim3(256), 0, s, out, rows, cols, ld,
+                       val);
+    return hipGetLastError();
+}
+
+template <int OP, int Q, bool CLAMP>
+static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, float k, float2* out, int64_t ld,
+                                hipStream_t s)
+{
+    const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
+    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((row1 - row0 + kDp0RowTile - 1) / kDp0RowTile));
+    hipLaunchKernelGGL((dp0_regular_kernel<OP, Q, CLAMP>), grid, dim3(256), 0, s, rec, m.nt, row0, row1, k, out, ld);
+    return hipGetLastError();
+}
+
+template <int OP, int Q, bool CLAMP>
+static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t ld, hipStream_t s)
+{
+    const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
+    dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
+    hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP>), grid, dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, k,
+                       out, ld);
+    return hipGetLastError();
+}
+
+template <int OP, int NL, int BASIS, bool CLAMP>
+static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, float k, float2* out, int64_t ld,
+                             int64_t row0, hipStream_t s)
+{
+    if (p1 <= p0) return hipSuccess;
+    unsigned g = (unsigned)((p1 - p0 + 3) / 4);
+    hipLaunchKernelGGL((singular_kernel<OP, NL, BASIS, CLAMP>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
+                       m.pairs, p0, p1, k, out, ld, row0);
+    return hipGetLastError();
+}
+
+template <int OP, bool CLAMP>
+static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int64_t row0, int64_t row1, float k, float2* out,
+                              int64_t ld, hipStream_t s)
+{
+    hipError_t e;
+    if (basis == DP0) {
+        if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, k, out, ld, s);
+        else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, k, out, ld, s);
+        else e = dp0_regular_t<OP, 1, CLAMP>(m, row0, row1, k, out, ld, s);
+        if (e != hipSuccess) return e;
+        const int64_t p0 = m.h_pair_ptr[row0], p1 = m.h_pair_ptr[row1];
+        return (quad == 0) ? singular_t<OP, 4, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s)
+                           : singular_t<OP, 2, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s);
+    }
+    e = launch_fill(out, m.nv, m.nv, ld, make_float2(0.f, 0.f), s);
+    if (e != hipSuccess) return e;
+    if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, s);
+    else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, s);
+    else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, s);
+    if (e != hipSuccess) return e;
+    return (quad == 0) ? singular_t<OP, 4, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s)
+                       : singular_t<OP, 2, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s);
+}
+
+hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int64_t row0, int64_t row1,
+                           float k, float2* out, int64_t ld, hipStream_t s)
+{
+#define BEM_OP_CASE(OPV)                                                                             \
+    case OPV:                                                                                        \
+        return clamp ? assemble_op<OPV, true>(m, basis, quad, row0, row1, k, out, ld, s)           \
+                     : assemble_op<OPV, false>(m, basis, quad, row0, row1, k, out, ld, s);
+    switch (op) {
+        BEM_OP_CASE(SLP)
+        BEM_OP_CASE(DLP)
+        BEM_OP_CASE(ADLP)
+        BEM_OP_CASE(HYP)
+        BEM_OP_CASE(BM_LHS)
+        BEM_OP_CASE(BM_RHS)
+    default:
+        return hipErrorInvalidValue;
+    }
+#undef BEM_OP_CASE
+}
+
+hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s)
+{
+    const int64_t n = basis == DP0 ? m.nt : m.nv;
+    hipError_t e = launch_fill(out, n, n, ld, make_float2(0.f, 0.f), s);
+    if (e != hipSuccess) return e;
+    if (basis == DP0)
+        hipLaunchKernelGGL(identity_dp0_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, m.geo, m.nt, out, ld);
+    else
+        hipLaunchKernelGGL(identity_p1_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, m.geo, m.tris, m.vt_ptr, m.vt_idx,
+                           m.nv, out, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(tri2vert_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, tv, m.vt_ptr, m.vt_idx, m.nv, out);
+    return hipGetLastError();
+}
+
+template <int OP, int BASIS, bool CLAMP>
+static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, float k, float2* out, int64_t ld,
+                              hipStream_t s)
+{
+    if (BASIS == P1) {
+        hipError_t e = launch_fill(out, np, m.nv, ld, make_float2(0.f, 0.f), s);
+        if (e != hipSuccess) return e;
+    }
+    if (np <= 0) return hipSuccess;
+    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((np + kPotRowTile - 1) / kPotRowTile));
+    hipLaunchKernelGGL((potential_kernel<OP, BASIS, CLAMP>), grid, dim3(256), 0, s, m.rec6, m.tris, m.nt, pts, np, k,
+                       out, ld);
+    return hipGetLastError();
+}
+
+hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, float k, int op, int basis, int clamp,
+                            float2* out, int64_t ld, hipStream_t s)
+{
+    if (op == SLP) {
+        if (basis == DP0) return clamp ? potential_t<SLP, DP0, true>(m, pts, np, k, out, ld, s)
+                                       : potential_t<SLP, DP0, false>(m, pts, np, k, out, ld, s);
+        return clamp ? potential_t<SLP, P1, true>(m, pts, np, k, out, ld, s)
+                     : potential_t<SLP, P1, false>(m, pts, np, k, out, ld, s);
+    }
+    if (op == DLP) {
+        if (basis == DP0) return clamp ? potential_t<DLP, DP0, true>(m, pts, np, k, out, ld, s)
+                                       : potential_t<DLP, DP0, false>(m, pts, np, k, out, ld, s);
+        return clamp ? potential_t<DLP, P1, true>(m, pts, np, k, out, ld, s)
+                     : potential_t<DLP, P1, false>(m, pts, np, k, out, ld, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace bem

@@ -1,0 +1,480 @@
+// extern "C" entry points of libbem_hip.so (declared in include/bem_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/bem_hip.h"
+#include "bem_internal.hpp"
+#include "bem_linalg.hpp"
+
+struct bem_mesh {
+    bem::MeshDev dev;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* where)
+{
+    return fail(BEM_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                        \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);    \
+    } while (0)
+
+// Runs on the mesh's device, restoring the caller's device afterwards.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = true;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) {
+            ok = false;
+            return;
+        }
+        if (prev != dev && hipSetDevice(dev) != hipSuccess) ok = false;
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+struct GmresLayout {
+    size_t V, w, r, H, h, h2, cs, sn, g, y, res, sc, total;
+};
+GmresLayout gmres_layout(int64_t n, int m)
+{
+    GmresLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        size_t at = o;
+        o += align256(bytes);
+        return at;
+    };
+    L.V = take(sizeof(float2) * (size_t)n * (size_t)(m + 1));
+    L.w = take(sizeof(float2) * (size_t)n);
+    L.r = take(sizeof(float2) * (size_t)n);
+    L.H = take(sizeof(double2) * (size_t)(m + 1) * (size_t)m);
+    L.h = take(sizeof(double2) * (size_t)(m + 1));
+    L.h2 = take(sizeof(double2) * (size_t)(m + 1));
+    L.cs = take(sizeof(double2) * (size_t)m);
+    L.sn = take(sizeof(double2) * (size_t)m);
+    L.g = take(sizeof(double2) * (size_t)(m + 1));
+    L.y = take(sizeof(double2) * (size_t)m);
+    L.res = take(sizeof(double) * (size_t)m);
+    L.sc = take(sizeof(double) * 8);
+    L.total = o;
+    return L;
+}
+
+int read_double(const double* d, double* h, hipStream_t s)
+{
+    HIP_TRY(hipMemcpyAsync(h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bem_version(void) { return "neuralat_amd-bem 0.1 (gfx950)"; }
+
+const char* bem_last_error(void) { return g_last_error.c_str(); }
+
+int bem_mesh_create(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, int on_device, void* stream,
+                    bem_mesh** out)
+{
+    if (!verts || !tris || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (nv <= 0 || nt <= 0) return fail(BEM_ERR_BAD_ARG, "empty mesh");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(BEM_ERR_NO_DEVICE, "no HIP device");
+    std::vector<float> hv;
+    std::vector<int32_t> ht;
+    const float* pv = verts;
+    const int32_t* pt = tris;
+    if (on_device) {
+        hv.resize((size_t)nv * 3);
+        ht.resize((size_t)nt * 3);
+        HIP_TRY(hipMemcpyAsync(hv.data(), verts, hv.size() * 4, hipMemcpyDeviceToHost, S(stream)));
+        HIP_TRY(hipMemcpyAsync(ht.data(), tris, ht.size() * 4, hipMemcpyDeviceToHost, S(stream)));
+        HIP_TRY(hipStreamSynchronize(S(stream)));
+        pv = hv.data();
+        pt = ht.data();
+    }
+    bem_mesh* m = new (std::nothrow) bem_mesh();
+    if (!m) return fail(BEM_ERR_OOM, "host allocation failed");
+    std::string err;
+    int rc = bem::mesh_build(pv, nv, pt, nt, m->dev, err);
+    if (rc != 0) {
+        delete m;
+        return fail(rc, err);
+    }
+    *out = m;
+    return BEM_OK;
+}
+
+int bem_mesh_destroy(bem_mesh* mesh)
+{
+    if (!mesh) return BEM_OK;
+    DeviceGuard g(mesh->dev.device);
+    bem::mesh_free(mesh->dev);
+    delete mesh;
+    return BEM_OK;
+}
+
+int bem_mesh_info(const bem_mesh* mesh, int64_t* nv, int64_t* nt, int64_t* npairs, int32_t* maxnb)
+{
+    if (!mesh) return fail(BEM_ERR_BAD_ARG, "null mesh");
+    if (nv) *nv = mesh->dev.nv;
+    if (nt) *nt = mesh->dev.nt;
+    if (npairs) *npairs = mesh->dev.npairs;
+    if (maxnb) *maxnb = mesh->dev.max_neighbors;
+    return BEM_OK;
+}
+
+int bem_mesh_normals_curls(const bem_mesh* mesh, float* normals, float* curls, void* stream)
+{
+    if (!mesh) return fail(BEM_ERR_BAD_ARG, "null mesh");
+    DeviceGuard g(mesh->dev.device);
+    const auto& m = mesh->dev;
+    if (normals)
+        HIP_TRY(hipMemcpy2DAsync(normals, 12, m.geo + 9, 64, 12, (size_t)m.nt, hipMemcpyDeviceToDevice, S(stream)));
+    if (curls)
+        HIP_TRY(hipMemcpy2DAsync(curls, 36, m.curls, 48, 36, (size_t)m.nt, hipMemcpyDeviceToDevice, S(stream)));
+    return BEM_OK;
+}
+
+static int check_op(int op, int basis, int quad)
+{
+    if (op < 0 || op > 5) return fail(BEM_ERR_BAD_ARG, "unknown operator " + std::to_string(op));
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    if (quad < 0 || quad > 2) return fail(BEM_ERR_BAD_ARG, "unknown quadrature");
+    if (basis == BEM_BASIS_DP0 && (op == BEM_OP_HYPERSINGULAR || op == BEM_OP_BM_LHS))
+        return fail(BEM_ERR_BAD_ARG, "the hypersingular and Burton-Miller LHS operators need the P1 basis");
+    return BEM_OK;
+}
+
+int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, void* out, int64_t ld,
+                 void* stream)
+{
+    if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (int rc = check_op(op, basis, quad)) return rc;
+    const auto& m = mesh->dev;
+    const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
+    if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (!std::isfinite(k)) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+    if ((op == BEM_OP_BM_LHS || op == BEM_OP_BM_RHS) && k == 0.f)
+        return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_assemble(m, op, basis, quad, clamp, 0, m.nt, k, (float2*)out, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_assemble launch");
+    return BEM_OK;
+}
+
+int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
+                      void* out, int64_t ld, void* stream)
+{
+    if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (int rc = check_op(op, BEM_BASIS_DP0, quad)) return rc;
+    const auto& m = mesh->dev;
+    if (row0 < 0 || row1 > m.nt || row0 >= row1) return fail(BEM_ERR_BAD_ARG, "bad row range");
+    if (ld < m.nt) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if ((op == BEM_OP_BM_RHS) && k == 0.f) return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_assemble(m, op, BEM_BASIS_DP0, quad, clamp, row0, row1, k, (float2*)out, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_assemble_rows launch");
+    return BEM_OK;
+}
+
+int bem_boundary_matrix(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, float k, int op, int quad,
+                        void* out, void* stream)
+{
+    bem_mesh* m = nullptr;
+    int rc = bem_mesh_create(verts, nv, tris, nt, 1, stream, &m);
+    if (rc) return rc;
+    rc = bem_assemble(m, k, op, BEM_BASIS_P1, quad, 1, out, nv, stream);
+    if (rc == BEM_OK) {
+        hipError_t e = hipStreamSynchronize(S(stream));
+        if (e != hipSuccess) rc = hip_fail(e, "bem_boundary_matrix sync");
+    }
+    bem_mesh_destroy(m);
+    return rc;
+}
+
+int bem_identity(const bem_mesh* mesh, int basis, void* out, int64_t ld, void* stream)
+{
+    if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    const auto& m = mesh->dev;
+    if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_identity(m, basis, (float2*)out, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_identity launch");
+    return BEM_OK;
+}
+
+int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream)
+{
+    if (!mesh || !tri_values || !vert_values) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    DeviceGuard g(mesh->dev.device);
+    hipError_t e = bem::launch_tri2vert(mesh->dev, (const float2*)tri_values, (float2*)vert_values, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_triangle2vertex launch");
+    return BEM_OK;
+}
+
+int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
+                  void* out, int64_t ld, void* stream)
+{
+    if (!mesh || !out || (np > 0 && !points)) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (op != BEM_OP_SINGLE && op != BEM_OP_DOUBLE)
+        return fail(BEM_ERR_BAD_ARG, "potential operators are SINGLE and DOUBLE");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    const auto& m = mesh->dev;
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
+    if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_potential(m, points, np, k, op, basis, clamp, (float2*)out, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_potential launch");
+    return BEM_OK;
+}
+
+int64_t bem_matvec_work_size(int64_t rows, int64_t cols, int trans)
+{
+    return (int64_t)sizeof(float2) * bem::gemv_work_elems(rows, cols, trans);
+}
+
+int bem_matvec(const void* A, int64_t rows, int64_t cols, int64_t lda, int trans, const void* x, void* y,
+               float alpha_re, float alpha_im, float beta_re, float beta_im, void* work, void* stream)
+{
+    if (!A || !x || !y) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (rows < 0 || cols < 0 || lda < cols) return fail(BEM_ERR_BAD_ARG, "bad matrix shape");
+    if (trans < 0 || trans > 2) return fail(BEM_ERR_BAD_ARG, "trans must be 0, 1 or 2");
+    if (trans != 0 && bem::gemv_work_elems(rows, cols, trans) > 0 && !work)
+        return fail(BEM_ERR_BAD_ARG, "transposed matvec needs a work buffer");
+    hipError_t e = bem::launch_gemv((const float2*)A, rows, cols, lda, trans, (const float2*)x, (float2*)y,
+                                    make_float2(alpha_re, alpha_im), make_float2(beta_re, beta_im), (float2*)work,
+                                    S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_matvec launch");
+    return BEM_OK;
+}
+
+int bem_combine(const void* A, int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc, int64_t rows,
+                int64_t cols, float a_re, float a_im, float b_re, float b_im, void* stream)
+{
+    if (!C) return fail(BEM_ERR_BAD_ARG, "null output");
+    if (rows < 0 || cols < 0 || ldc < cols || (A && lda < cols) || (B && ldb < cols))
+        return fail(BEM_ERR_BAD_ARG, "bad shape");
+    hipError_t e = bem::launch_combine((const float2*)A, lda, (const float2*)B, ldb, (float2*)C, ldc, rows, cols,
+                                       make_float2(a_re, a_im), make_float2(b_re, b_im), S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_combine launch");
+    return BEM_OK;
+}
+
+int64_t bem_gmres_work_size(int64_t n, int restart)
+{
+    if (n <= 0 || restart <= 0) return 0;
+    return (int64_t)gmres_layout(n, restart).total;
+}
+
+int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, float tol, int restart, int maxiter,
+              void* work, bem_solve_info* info, void* stream)
+{
+    if (!A_ || !b_ || !x_ || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (n <= 0 || lda < n || restart <= 0 || maxiter < 0 || !(tol > 0.f))
+        return fail(BEM_ERR_BAD_ARG, "bad GMRES argument");
+    const int m = restart;
+    const GmresLayout L = gmres_layout(n, m);
+    char* W = (char*)work;
+    float2* V = (float2*)(W + L.V);
+    float2* w = (float2*)(W + L.w);
+    float2* r = (float2*)(W + L.r);
+    double2* H = (double2*)(W + L.H);
+    double2* h = (double2*)(W + L.h);
+    double2* h2 = (double2*)(W + L.h2);
+    double2* cs = (double2*)(W + L.cs);
+    double2* sn = (double2*)(W + L.sn);
+    double2* g = (double2*)(W + L.g);
+    double2* y = (double2*)(W + L.y);
+    double* res = (double*)(W + L.res);
+    double* sc = (double*)(W + L.sc);
+    const float2* A = (const float2*)A_;
+    const float2* b = (const float2*)b_;
+    float2* x = (float2*)x_;
+    hipStream_t s = S(stream);
+    const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f), mone = make_float2(-1.f, 0.f);
+
+    HIP_TRY(bem::launch_norm2(b, n, sc, s));
+    double bn2 = 0.0;
+    if (int rc = read_double(sc, &bn2, s)) return rc;
+    const double bnorm = std::sqrt(bn2);
+    bem_solve_info inf{0, 0, 0.0, 0.0};
+    if (bnorm == 0.0) {
+        HIP_TRY(hipMemsetAsync(x, 0, sizeof(float2) * (size_t)n, s));
+        inf.converged = 1;
+        if (info) *info = inf;
+        return BEM_OK;
+    }
+    const double target = (double)tol * bnorm;
+    int its = 0;
+    double rnorm = 0.0, implicit = 0.0;
+    bool conv = false;
+    while (true) {
+        // r = b - A x
+        HIP_TRY(hipMemcpyAsync(r, b, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, x, r, mone, one, nullptr, s));
+        HIP_TRY(bem::launch_norm2(r, n, sc, s));
+        double rn2 = 0.0;
+        if (int rc = read_double(sc, &rn2, s)) return rc;
+        rnorm = std::sqrt(rn2);
+        if (rnorm <= target) {
+            conv = true;
+            break;
+        }
+        if (its >= maxiter) break;
+        HIP_TRY(bem::launch_normalize(r, n, sc, V, s));
+        HIP_TRY(bem::launch_init_g(g, m, sc, s));
+        int nk = 0;
+        for (int j = 0; j < m && its < maxiter; ++j) {
+            float2* vj = V + (size_t)j * n;
+            HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, vj, w, one, zero, nullptr, s));
+            // classical Gram-Schmidt, twice
+            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, s));
+            HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));
+            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h2, s));
+            HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h2, w, s));
+            HIP_TRY(bem::launch_norm2(w, n, sc + 1, s));
+            HIP_TRY(bem::launch_normalize(w, n, sc + 1, V + (size_t)(j + 1) * n, s));
+            HIP_TRY(bem::launch_givens(H, m, j, h, h2, sc + 1, cs, sn, g, res, s));
+            ++its;
+            nk = j + 1;
+            if (int rc = read_double(res + j, &implicit, s)) return rc;
+            if (implicit <= target) break;
+        }
+        HIP_TRY(bem::launch_backsolve(H, m, nk, g, y, s));
+        HIP_TRY(bem::launch_update_x(V, n, n, nk, y, x, s));
+    }
+    inf.iterations = its;
+    inf.converged = conv ? 1 : 0;
+    inf.rel_residual = rnorm / bnorm;
+    inf.implicit_rel_residual = implicit / bnorm;
+    if (info) *info = inf;
+    if (!conv)
+        return fail(BEM_ERR_NOT_CONVERGED, "GMRES reached maxiter with relative residual " +
+                                               std::to_string(inf.rel_residual));
+    return BEM_OK;
+}
+
+int64_t bem_bicgstab_work_size(int64_t n)
+{
+    if (n <= 0) return 0;
+    return (int64_t)(7 * align256(sizeof(float2) * (size_t)n) + align256(sizeof(double2) * 16));
+}
+
+int bem_bicgstab(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, int nsteps, double tol, double atol,
+                 void* work, bem_solve_info* info, void* stream)
+{
+    if (!A_ || !b_ || !x_ || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (n <= 0 || lda < n || nsteps < 0) return fail(BEM_ERR_BAD_ARG, "bad BiCGSTAB argument");
+    const size_t vb = align256(sizeof(float2) * (size_t)n);
+    char* W = (char*)work;
+    float2* r = (float2*)(W);
+    float2* rh = (float2*)(W + vb);
+    float2* p = (float2*)(W + 2 * vb);
+    float2* v = (float2*)(W + 3 * vb);
+    float2* sv = (float2*)(W + 4 * vb);
+    float2* t = (float2*)(W + 5 * vb);
+    float2* q = (float2*)(W + 6 * vb);
+    double2* sc = (double2*)(W + 7 * vb);
+    double* scd = (double*)(sc + 8);  // real scratch
+    const float2* A = (const float2*)A_;
+    const float2* b = (const float2*)b_;
+    float2* x = (float2*)x_;
+    hipStream_t s = S(stream);
+    const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f), mone = make_float2(-1.f, 0.f);
+
+    // residual_tol = tol * ||b||^2 (BiCGSTAB.py:40)
+    HIP_TRY(bem::launch_norm2(b, n, scd, s));
+    double bn2 = 0.0;
+    if (int rc = read_double(scd, &bn2, s)) return rc;
+    const double residual_tol = tol * bn2;
+    // q = b - A x, ||q||^2 (BiCGSTAB.py:51-57)
+    auto residual = [&](double* out2) -> int {
+        HIP_TRY(hipMemcpyAsync(q, b, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, x, q, mone, one, nullptr, s));
+        HIP_TRY(bem::launch_norm2(q, n, scd, s));
+        return read_double(scd, out2, s);
+    };
+    bem_solve_info inf{0, 0, 0.0, 0.0};
+    double rr = 0.0;
+    if (int rc = residual(&rr)) return rc;
+    bool conv = rr < residual_tol || rr < atol;
+    if (!conv) {
+        HIP_TRY(hipMemcpyAsync(r, q, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        // rho = alpha = omega = 1, v = p = 0, r_hat = r (BiCGSTAB.py:44-49)
+        double2 init[8] = {};
+        init[0] = make_double2(1.0, 0.0);
+        init[1] = make_double2(1.0, 0.0);
+        init[2] = make_double2(1.0, 0.0);
+        HIP_TRY(hipMemcpyAsync(sc, init, sizeof(init), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemsetAsync(v, 0, sizeof(float2) * (size_t)n, s));
+        HIP_TRY(hipMemsetAsync(p, 0, sizeof(float2) * (size_t)n, s));
+        HIP_TRY(hipMemcpyAsync(rh, r, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
+        int left = nsteps;
+        while (left) {
+            // BiCGSTAB.py:59-83
+            HIP_TRY(bem::launch_dot(r, rh, n, 0, sc + 3, s));  // rho = <r, r_hat> (bilinear)
+            HIP_TRY(bem::launch_bicg_scalar(0, sc, s));        // beta; rho_prev = rho
+            HIP_TRY(bem::launch_bicg_p(r, p, v, sc, n, s));    // p = r + beta (p - omega v)
+            HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, p, v, one, zero, nullptr, s));
+            HIP_TRY(bem::launch_dot(rh, v, n, 0, sc + 5, s));
+            HIP_TRY(bem::launch_bicg_scalar(1, sc, s));                   // alpha = rho / <r_hat, v>
+            HIP_TRY(bem::launch_bicg_axpy(r, v, sc, 1, -1.0, sv, n, s));  // s = r - alpha v
+            HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, sv, t, one, zero, nullptr, s));
+            HIP_TRY(bem::launch_dot(t, sv, n, 0, sc + 6, s));
+            HIP_TRY(bem::launch_dot(t, t, n, 0, sc + 7, s));
+            HIP_TRY(bem::launch_bicg_scalar(2, sc, s));  // omega = <t, s> / <t, t>
+            HIP_TRY(bem::launch_bicg_x(x, p, sv, sc, n, s));  // x += alpha p + omega s
+            ++inf.iterations;
+            if (int rc = residual(&rr)) return rc;
+            if (rr < residual_tol || rr < atol) {
+                conv = true;
+                break;
+            }
+            HIP_TRY(bem::launch_bicg_axpy(sv, t, sc, 2, -1.0, r, n, s));  // r = s - omega t
+            double2 rho_prev;
+            HIP_TRY(hipMemcpyAsync(&rho_prev, sc, sizeof(double2), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (rho_prev.x == 0.0 && rho_prev.y == 0.0) break;  // BiCGSTAB.py:103
+            --left;
+        }
+    }
+    inf.converged = conv ? 1 : 0;
+    inf.rel_residual = bn2 > 0 ? std::sqrt(rr / bn2) : 0.0;
+    if (info) *info = inf;
+    return BEM_OK;
+}
+
+}  // extern "C"

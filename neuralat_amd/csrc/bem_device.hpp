@@ -1,0 +1,150 @@
+// Device-side building blocks of the BEM Helmholtz kernels for gfx950.
+//
+// Conventions (reference: src/bem/cuda/include/potential.h:18-65,
+// integrand.h:187-235):
+//   x lies on the ROW (test) triangle i, y on the COLUMN (trial) triangle j;
+//   d = y - x; G = e^{ik r} / (4 pi r) with r = |d|, optionally clamped
+//   r >= 1e-3 (potential.h:16,21-23).  The 1/(4 pi) factor is folded into the
+//   column quadrature weights, so the kernels below return 4 pi * K.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bem {
+
+enum Op : int { SLP = 0, DLP = 1, ADLP = 2, HYP = 3, BM_LHS = 4, BM_RHS = 5 };
+enum Basis : int { P1 = 0, DP0 = 1 };
+
+constexpr float kEps = 1e-3f;        // potential.h:16
+constexpr float kInvEps = 1e3f;
+constexpr float kInv4Pi = 0.0795774715459476678844f;
+constexpr float kInv2Pi = 0.159154943091895335769f;
+
+// ---- quadrature rules ------------------------------------------------------
+// Triangle rules on (0,0),(1,0),(0,1), sum of weights 1 (gauss.h:56-126):
+// 6-point degree-4 Strang-Fix/Cowper, 3-point degree-2, 1-point centroid.
+template <int Q> struct TriRule;
+template <> struct TriRule<6> {
+    static constexpr float a = 0.445948490915965f, b = 0.091576213509771f;
+    static constexpr float wa = 0.223381589678011f, wb = 0.109951743655322f;
+    static constexpr float x1[6] = {a, b, 1.0f - 2.0f * a, a, 1.0f - 2.0f * b, b};
+    static constexpr float x2[6] = {a, b, a, 1.0f - 2.0f * a, b, 1.0f - 2.0f * b};
+    static constexpr float w[6] = {wa, wb, wa, wa, wb, wb};
+};
+template <> struct TriRule<3> {
+    static constexpr float x1[3] = {1.0f / 6, 2.0f / 3, 1.0f / 6};
+    static constexpr float x2[3] = {1.0f / 6, 1.0f / 6, 2.0f / 3};
+    static constexpr float w[3] = {1.0f / 3, 1.0f / 3, 1.0f / 3};
+};
+template <> struct TriRule<1> {
+    static constexpr float x1[1] = {1.0f / 3};
+    static constexpr float x2[1] = {1.0f / 3};
+    static constexpr float w[1] = {1.0f};
+};
+
+// Gauss-Legendre rules mapped to [0,1]: nodes t = (s+1)/2, weights w/2
+// (the reference maps inside singular_potential, integrand.h:78-81, and
+// divides the product weight by 16, integrand.h:183).
+template <int N> struct LineRule01;
+template <> struct LineRule01<4> {
+    static constexpr float t[4] = {0.5f * (1.0f + 0.861136311594053f), 0.5f * (1.0f + 0.339981043584856f),
+                                   0.5f * (1.0f - 0.339981043584856f), 0.5f * (1.0f - 0.861136311594053f)};
+    static constexpr float w[4] = {0.5f * 0.347854845137454f, 0.5f * 0.652145154862546f,
+                                   0.5f * 0.652145154862546f, 0.5f * 0.347854845137454f};
+};
+template <> struct LineRule01<2> {
+    static constexpr float t[2] = {0.5f * (1.0f + 0.577350269189626f), 0.5f * (1.0f - 0.577350269189626f)};
+    static constexpr float w[2] = {0.5f, 0.5f};
+};
+
+// ---- per-triangle records ----------------------------------------------------
+// Quadrature record of one triangle for rule Q, float4-aligned:
+//   pts[3Q]   quadrature points (x,y,z interleaved)
+//   wts[Q]    w_q * |J| / 2   (the area-scaled weights)
+//   n[3]      unit normal of the vertex order (v0, v1, v2)
+//   pad to a multiple of 4 floats
+template <int Q> struct Rec {
+    static constexpr int kFloats = ((4 * Q + 3) + 3) / 4 * 4;
+};
+
+// ---- complex helpers ---------------------------------------------------------
+struct cf {
+    float r, i;
+};
+
+// sin(2 pi t), cos(2 pi t) for t in revolutions, on the hardware
+// transcendental unit after exact range reduction to [0, 1).
+__device__ __forceinline__ void sincos_rev(float t, float& s, float& c)
+{
+    float f = __builtin_amdgcn_fractf(t);
+    s = __builtin_amdgcn_sinf(f);
+    c = __builtin_amdgcn_cosf(f);
+}
+
+// Kernel values for a displacement d = y - x.  Outputs are 4 pi times the
+// reference's potential<OP> (potential.h:18-52) and, for BM_LHS, 4 pi times
+// the second output of bm_lhs_potential (potential.h:55-65).  HYP evaluates
+// the single-layer kernel (its curl form is applied per pair).
+//   nx: row normal (x side), ny: column normal (y side)
+//   k: wavenumber, krev = k / (2 pi)
+template <int OP, bool CLAMP>
+__device__ __forceinline__ void kernel4pi(float dx, float dy, float dz, float nx0, float nx1, float nx2, float ny0,
+                                          float ny1, float ny2, float k, float krev, float& vr, float& vi, float& sr,
+                                          float& si)
+{
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float r;
+    if (CLAMP) {
+        rinv = fminf(rinv, kInvEps);
+        r = fmaxf(r2 * rinv, kEps);
+    } else {
+        r = r2 * rinv;
+    }
+    float s, c;
+    sincos_rev(r * krev, s, c);
+    if constexpr (OP == SLP || OP == HYP) {
+        vr = c * rinv;
+        vi = s * rinv;
+    } else if constexpr (OP == DLP || OP == ADLP) {
+        // DLP: -G (1 - i k r) (y-x).n_y / r^2 ; ADLP: -G (1 - i k r) (x-y).n_x / r^2
+        float dn = (OP == DLP) ? -(dx * ny0 + dy * ny1 + dz * ny2) : (dx * nx0 + dy * nx1 + dz * nx2);
+        float th = k * r;
+        float f = dn * rinv * rinv * rinv;
+        vr = f * fmaf(th, s, c);
+        vi = f * fmaf(-th, c, s);
+    } else if constexpr (OP == BM_LHS) {
+        // (-i k nx.ny + (1 - i k r)(y-x).n_y / r^2) G ; second: (i/k) G
+        float a = (dx * ny0 + dy * ny1 + dz * ny2) * rinv * rinv;
+        float th = k * r;
+        float ai = -k * (nx0 * ny0 + nx1 * ny1 + nx2 * ny2) - th * a;
+        float gr = c * rinv, gi = s * rinv;
+        vr = a * gr - ai * gi;
+        vi = a * gi + ai * gr;
+        sr = gr;  // plain G; scaled by i/k after the sum
+        si = gi;
+    } else if constexpr (OP == BM_RHS) {
+        // -G + (i/k) G (1 - i k r) (x-y).n_x / r^2
+        float q = -(dx * nx0 + dy * nx1 + dz * nx2) * rinv * rinv;  // (x-y).n_x / r^2
+        float th = k * r;
+        float gr = c * rinv, gi = s * rinv;
+        float invk = 1.0f / k;
+        // (i/k)(1 - i th) = th/k + i/k
+        float tr = th * invk * q, ti = invk * q;
+        vr = -gr + (gr * tr - gi * ti);
+        vi = -gi + (gr * ti + gi * tr);
+    }
+}
+
+// Surface-curl product of two triangles' P1 hat functions, A B^T of the
+// rows n x grad(phi_a) (common.h:150-164 applied to solver.py:40-58 output).
+__device__ __forceinline__ void curl_product(const float* ci, const float* cj, float* cp)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+            cp[3 * a + b] = ci[3 * a] * cj[3 * b] + ci[3 * a + 1] * cj[3 * b + 1] + ci[3 * a + 2] * cj[3 * b + 2];
+}
+
+}  // namespace bem

@@ -1,0 +1,73 @@
+// Library-internal types shared by the host code and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+namespace bem {
+
+// DP0 regular kernel: rows per workgroup (each lane computes this many
+// entries of its column).
+constexpr int kDp0RowTile = 64;
+// Potential kernel: points per workgroup.
+constexpr int kPotRowTile = 16;
+// P1 tiles: at most kP1RowTris row triangles touching at most kP1RowVerts
+// vertices, at most 256 column triangles touching at most kP1ColVerts
+// vertices (the LDS accumulator is kP1RowVerts x kP1ColVerts complex).
+constexpr int kP1RowTris = 32;
+constexpr int kP1RowVerts = 40;
+constexpr int kP1ColTris = 256;
+constexpr int kP1ColVerts = 200;
+
+// Locality-ordered tiling of the triangle list for the P1 scatter.
+struct P1Tiles {
+    const int32_t* perm;       // [nt] triangle id at ordered position
+    const int32_t* row_ptr;    // [nrow_tiles+1] position ranges
+    const int32_t* row_vptr;   // [nrow_tiles+1] ranges into row_verts
+    const int32_t* row_verts;  // global vertex ids of each row tile
+    const int4* row_slot;      // [nt] per position: tile-local slots of the 3 vertices
+    const int32_t* col_ptr;
+    const int32_t* col_vptr;
+    const int32_t* col_verts;
+    const int4* col_slot;
+    int nrow_tiles, ncol_tiles;
+};
+
+// Device-resident mesh data (one allocation).
+struct MeshDev {
+    int device = 0;
+    int64_t nv = 0, nt = 0;
+    float* verts = nullptr;  // [nv*3]
+    int4* tris = nullptr;    // [nt] (v0, v1, v2, 0)
+    float* rec6 = nullptr;   // [nt * Rec<6>::kFloats]
+    float* rec3 = nullptr;
+    float* rec1 = nullptr;
+    float* curls = nullptr;  // [nt*12]: rows n x grad(phi_a), a = 0..2 (9 used)
+    float* geo = nullptr;    // [nt*16]: v0 v1 v2 (9), unit normal (3), |J| (1)
+    int4* pairs = nullptr;   // [npairs]: (i, j, shared vertices, 0), sorted by (i, j)
+    int64_t npairs = 0;
+    int32_t* vt_ptr = nullptr;  // [nv+1] vertex -> triangle CSR (ascending)
+    int32_t* vt_idx = nullptr;
+    P1Tiles tiles{};
+    std::vector<int64_t> h_pair_ptr;  // [nt+1] host copy: pair range of each row triangle
+    int32_t max_neighbors = 0;
+    double diameter = 0.0;  // bounding-box diagonal
+    void* block = nullptr;  // the single device allocation
+};
+
+hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s);
+hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int64_t row0, int64_t row1,
+                           float k, float2* out, int64_t ld, hipStream_t s);
+hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
+hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
+hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, float k, int op, int basis, int clamp,
+                            float2* out, int64_t ld, hipStream_t s);
+
+// Builds all device data from host copies of the mesh; returns 0 or a
+// bem_status code with a message in `err`.
+int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, MeshDev& m, std::string& err);
+void mesh_free(MeshDev& m);
+
+}  // namespace bem

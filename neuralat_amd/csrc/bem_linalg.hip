@@ -1,0 +1,537 @@
+// Dense complex linear algebra for the BEM solve on gfx950.
+//
+//  * gemv_n: y = alpha A x + beta y for a row-major complex64 matrix; one
+//    wavefront streams kGemvRows rows with 16-byte loads, sharing each x
+//    fetch between its rows; HBM-bound.
+//  * gemv_t: y = alpha op(A) x + beta y with op = transpose or conjugate
+//    transpose; each lane owns two columns and a workgroup a row slab, with a
+//    second pass summing the slabs (fixed order, no atomics).
+//  * vector kernels for GMRES (classical Gram-Schmidt with one
+//    re-orthogonalisation, Givens rotations on the device) and for the
+//    reference-compatible BiCGSTAB (src/bem/BiCGSTAB.py:5-107).
+#include <hip/hip_runtime.h>
+
+#include "bem_linalg.hpp"
+
+namespace bem {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// GEMV, no transpose
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum_f(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* sh)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int q = 0; q < nw; ++q) t += sh[q];
+    __syncthreads();
+    return t;
+}
+
+template <int RW>
+__global__ __launch_bounds__(256) void gemv_n_vec_kernel(const float4* __restrict__ A, int64_t rows, int64_t cols2,
+                                                         int64_t lda2, const float4* __restrict__ x,
+                                                         float2* __restrict__ y, float2 alpha, float2 beta,
+                                                         int beta_zero)
+{
+    // cols2 = cols / 2, lda2 = lda / 2 (both even)
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
+    if (r0 >= rows) return;
+    float ar[RW], ai[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) ar[q] = ai[q] = 0.f;
+    const v4f* rowp[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) rowp[q] = reinterpret_cast<const v4f*>(A + min(r0 + q, rows - 1) * lda2);
+    for (int64_t c = lane; c < cols2; c += 64) {
+        const float4 xv = x[c];
+#pragma unroll
+        for (int q = 0; q < RW; ++q) {
+            const v4f a = __builtin_nontemporal_load(&rowp[q][c]);
+            ar[q] = fmaf(a.x, xv.x, ar[q]);
+            ar[q] = fmaf(-a.y, xv.y, ar[q]);
+            ai[q] = fmaf(a.x, xv.y, ai[q]);
+            ai[q] = fmaf(a.y, xv.x, ai[q]);
+            ar[q] = fmaf(a.z, xv.z, ar[q]);
+            ar[q] = fmaf(-a.w, xv.w, ar[q]);
+            ai[q] = fmaf(a.z, xv.w, ai[q]);
+            ai[q] = fmaf(a.w, xv.z, ai[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+        const float sr = wave_sum_f(ar[q]), si = wave_sum_f(ai[q]);
+        const int64_t r = r0 + q;
+        if (lane == q && r < rows) {
+            float2 out = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
+            if (!beta_zero) {
+                const float2 yo = y[r];
+                out.x += beta.x * yo.x - beta.y * yo.y;
+                out.y += beta.x * yo.y + beta.y * yo.x;
+            }
+            y[r] = out;
+        }
+    }
+}
+
+// generic (any ld / alignment / odd cols) fallback
+__global__ __launch_bounds__(256) void gemv_n_scalar_kernel(const float2* __restrict__ A, int64_t rows, int64_t cols,
+                                                            int64_t lda, const float2* __restrict__ x,
+                                                            float2* __restrict__ y, float2 alpha, float2 beta,
+                                                            int beta_zero)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    float ar = 0.f, ai = 0.f;
+    const float2* row = A + r * lda;
+    for (int64_t c = lane; c < cols; c += 64) {
+        const float2 a = row[c], xv = x[c];
+        ar = fmaf(a.x, xv.x, fmaf(-a.y, xv.y, ar));
+        ai = fmaf(a.x, xv.y, fmaf(a.y, xv.x, ai));
+    }
+    const float sr = wave_sum_f(ar), si = wave_sum_f(ai);
+    if (lane == 0) {
+        float2 out = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
+        if (!beta_zero) {
+            const float2 yo = y[r];
+            out.x += beta.x * yo.x - beta.y * yo.y;
+            out.y += beta.x * yo.y + beta.y * yo.x;
+        }
+        y[r] = out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GEMV, transpose / conjugate transpose: partial sums per row slab
+// ---------------------------------------------------------------------------
+constexpr int kTSlabRows = 256;
+
+__global__ __launch_bounds__(256) void gemv_t_partial_kernel(const float2* __restrict__ A, int64_t rows, int64_t cols,
+                                                             int64_t lda, const float2* __restrict__ x, int conj,
+                                                             float2* __restrict__ part)
+{
+    // grid.x: column blocks of 256, grid.y: row slabs
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.y * kTSlabRows;
+    const int64_t r1 = min(r0 + (int64_t)kTSlabRows, rows);
+    if (c >= cols) return;
+    float ar = 0.f, ai = 0.f;
+    const float sgn = conj ? -1.f : 1.f;
+    for (int64_t r = r0; r < r1; ++r) {
+        const float2 a = A[r * lda + c];
+        const float2 xv = x[r];  // uniform
+        const float aim = sgn * a.y;
+        ar = fmaf(a.x, xv.x, fmaf(-aim, xv.y, ar));
+        ai = fmaf(a.x, xv.y, fmaf(aim, xv.x, ai));
+    }
+    part[(int64_t)blockIdx.y * cols + c] = make_float2(ar, ai);
+}
+
+__global__ __launch_bounds__(256) void gemv_t_reduce_kernel(const float2* __restrict__ part, int64_t nslab,
+                                                            int64_t cols, float2* __restrict__ y, float2 alpha,
+                                                            float2 beta, int beta_zero)
+{
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= cols) return;
+    double sr = 0.0, si = 0.0;
+    for (int64_t s = 0; s < nslab; ++s) {
+        const float2 p = part[s * cols + c];
+        sr += p.x;
+        si += p.y;
+    }
+    float2 out = make_float2((float)(alpha.x * sr - alpha.y * si), (float)(alpha.x * si + alpha.y * sr));
+    if (!beta_zero) {
+        const float2 yo = y[c];
+        out.x += beta.x * yo.x - beta.y * yo.y;
+        out.y += beta.x * yo.y + beta.y * yo.x;
+    }
+    y[c] = out;
+}
+
+// ---------------------------------------------------------------------------
+// dense combine: C = a A + b B (either input may be null = 0)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void combine_kernel(const float2* __restrict__ A, int64_t lda,
+                                                      const float2* __restrict__ B, int64_t ldb, float2* __restrict__ C,
+                                                      int64_t ldc, int64_t rows, int64_t cols, float2 a, float2 b)
+{
+    const int64_t total = rows * cols;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int64_t r = e / cols, c = e - r * cols;
+        float2 o = make_float2(0.f, 0.f);
+        if (A) {
+            const float2 v = A[r * lda + c];
+            o.x += a.x * v.x - a.y * v.y;
+            o.y += a.x * v.y + a.y * v.x;
+        }
+        if (B) {
+            const float2 v = B[r * ldb + c];
+            o.x += b.x * v.x - b.y * v.y;
+            o.y += b.x * v.y + b.y * v.x;
+        }
+        C[r * ldc + c] = o;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels (GMRES / BiCGSTAB)
+// ---------------------------------------------------------------------------
+// h[q] = sum_e conj(V[q][e]) w[e] for q < nq, one workgroup per q (double
+// accumulation, fixed order).  Result written as double2 to hout[q].
+__global__ __launch_bounds__(256) void multidot_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n,
+                                                       const float2* __restrict__ w, double2* __restrict__ hout)
+{
+    __shared__ double sh[8];
+    const int q = blockIdx.x;
+    const float2* v = V + (int64_t)q * ldv;
+    double sr = 0.0, si = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += 256) {
+        const float2 a = v[e], b = w[e];
+        sr += (double)a.x * b.x + (double)a.y * b.y;
+        si += (double)a.x * b.y - (double)a.y * b.x;
+    }
+    sr = block_sum_d(sr, sh);
+    si = block_sum_d(si, sh);
+    if (threadIdx.x == 0) hout[q] = make_double2(sr, si);
+}
+
+// w[e] -= sum_q h[q] V[q][e]; hacc[q] += h[q] (second Gram-Schmidt pass folds in)
+__global__ __launch_bounds__(256) void gs_update_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n, int nq,
+                                                        const double2* __restrict__ h, float2* __restrict__ w)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double wr = w[e].x, wi = w[e].y;
+    for (int q = 0; q < nq; ++q) {
+        const double2 hq = h[q];
+        const float2 v = V[(int64_t)q * ldv + e];
+        wr -= hq.x * v.x - hq.y * v.y;
+        wi -= hq.x * v.y + hq.y * v.x;
+    }
+    w[e] = make_float2((float)wr, (float)wi);
+}
+
+// ||w||^2 into out[0] (double), one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void norm2_kernel(const float2* __restrict__ w, int64_t n, double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += 1024) {
+        const float2 a = w[e];
+        s += (double)a.x * a.x + (double)a.y * a.y;
+    }
+    s = block_sum_d(s, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+// dot(a, b) = sum a b (bilinear, conj = 0) or sum conj(a) b (conj = 1) into out (double2)
+__global__ __launch_bounds__(1024) void dot_kernel(const float2* __restrict__ a, const float2* __restrict__ b,
+                                                   int64_t n, int conj, double2* __restrict__ out)
+{
+    __shared__ double sh[16];
+    double sr = 0.0, si = 0.0;
+    const double sg = conj ? -1.0 : 1.0;
+    for (int64_t e = threadIdx.x; e < n; e += 1024) {
+        const float2 x = a[e], y = b[e];
+        sr += (double)x.x * y.x - sg * (double)x.y * y.y;
+        si += (double)x.x * y.y + sg * (double)x.y * y.x;
+    }
+    sr = block_sum_d(sr, sh);
+    si = block_sum_d(si, sh);
+    if (threadIdx.x == 0) *out = make_double2(sr, si);
+}
+
+// out[e] = src[e] * scale, scale = 1 / sqrt(*nrm2) (or 0 if nrm2 == 0)
+__global__ __launch_bounds__(256) void normalize_kernel(const float2* __restrict__ src, int64_t n,
+                                                        const double* __restrict__ nrm2, float2* __restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double nn = *nrm2;
+    const float s = nn > 0.0 ? (float)(1.0 / sqrt(nn)) : 0.f;
+    const float2 v = src[e];
+    out[e] = make_float2(v.x * s, v.y * s);
+}
+
+// GMRES Arnoldi bookkeeping on the device (one thread):
+//   column j of H gets h (length j+1, from CGS2) and ||w||; previous Givens
+//   rotations are applied, a new one zeroes H[j+1][j], g is rotated and
+//   |g[j+1]| is stored in res[j].  H: (m+1) x m column-major (double2).
+__global__ void gmres_givens_kernel(double2* __restrict__ H, int m, int j, const double2* __restrict__ h,
+                                    const double2* __restrict__ h2, const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn,
+                                    double2* __restrict__ g, double* __restrict__ res)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double2* col = H + (int64_t)j * (m + 1);
+    for (int q = 0; q <= j; ++q) col[q] = make_double2(h[q].x + h2[q].x, h[q].y + h2[q].y);
+    col[j + 1] = make_double2(sqrt(*wn2), 0.0);
+    for (int q = 0; q < j; ++q) {
+        // [c s; -conj(s) conj(c)] applied to (col[q], col[q+1])
+        const double2 c = cs[q], s = sn[q], a = col[q], b = col[q + 1];
+        const double2 t = make_double2(c.x * a.x - c.y * a.y + s.x * b.x - s.y * b.y,
+                                       c.x * a.y + c.y * a.x + s.x * b.y + s.y * b.x);
+        const double2 u = make_double2(-(s.x * a.x + s.y * a.y) + (c.x * b.x + c.y * b.y),
+                                       -(s.x * a.y - s.y * a.x) + (c.x * b.y - c.y * b.x));
+        col[q] = t;
+        col[q + 1] = u;
+    }
+    const double2 a = col[j], b = col[j + 1];
+    const double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
+    double2 c, s;
+    if (den > 0.0) {
+        c = make_double2(a.x / den, -a.y / den);  // conj(a)/den
+        s = make_double2(b.x / den, -b.y / den);  // conj(b)/den
+    } else {
+        c = make_double2(1.0, 0.0);
+        s = make_double2(0.0, 0.0);
+    }
+    cs[j] = c;
+    sn[j] = s;
+    col[j] = make_double2(den, 0.0);
+    col[j + 1] = make_double2(0.0, 0.0);
+    const double2 gj = g[j];
+    g[j + 1] = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));  // -conj(s) g_j
+    g[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
+    res[j] = sqrt(g[j + 1].x * g[j + 1].x + g[j + 1].y * g[j + 1].y);
+}
+
+// back substitution y = R^{-1} g[0:nk] (R upper triangular, H layout above)
+__global__ void gmres_backsolve_kernel(const double2* __restrict__ H, int m, int nk, const double2* __restrict__ g,
+                                       double2* __restrict__ y)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int q = nk - 1; q >= 0; --q) {
+        double2 s = g[q];
+        for (int c = q + 1; c < nk; ++c) {
+            const double2 hqc = H[(int64_t)c * (m + 1) + q], yc = y[c];
+            s.x -= hqc.x * yc.x - hqc.y * yc.y;
+            s.y -= hqc.x * yc.y + hqc.y * yc.x;
+        }
+        const double2 d = H[(int64_t)q * (m + 1) + q];
+        const double dd = d.x * d.x + d.y * d.y;
+        y[q] = dd > 0.0 ? make_double2((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd)
+                        : make_double2(0.0, 0.0);
+    }
+}
+
+// x[e] += sum_q y[q] V[q][e]
+__global__ __launch_bounds__(256) void gmres_update_x_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n,
+                                                             int nk, const double2* __restrict__ y,
+                                                             float2* __restrict__ x)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    double xr = x[e].x, xi = x[e].y;
+    for (int q = 0; q < nk; ++q) {
+        const double2 yq = y[q];
+        const float2 v = V[(int64_t)q * ldv + e];
+        xr += yq.x * v.x - yq.y * v.y;
+        xi += yq.x * v.y + yq.y * v.x;
+    }
+    x[e] = make_float2((float)xr, (float)xi);
+}
+
+// g = (beta, 0, ...), beta = sqrt(*nrm2)
+__global__ void gmres_init_g_kernel(double2* __restrict__ g, int m, const double* __restrict__ nrm2)
+{
+    const int q = threadIdx.x;
+    for (int e = q; e <= m; e += blockDim.x) g[e] = make_double2(e == 0 ? sqrt(*nrm2) : 0.0, 0.0);
+}
+
+// ---- BiCGSTAB helpers (scalars live on the device) ----
+// sc layout (double2): 0 rho_prev, 1 alpha, 2 omega, 3 rho, 4 beta, 5 dot(r_hat, v),
+//                      6 dot(t, s), 7 dot(t, t)
+__global__ void bicg_beta_kernel(double2* sc)
+{
+    // beta = (rho / rho_prev) * (alpha / omega); rho_prev = rho
+    auto cdiv = [](double2 a, double2 b) {
+        const double d = b.x * b.x + b.y * b.y;
+        return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+    };
+    auto cmul = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+    sc[4] = cmul(cdiv(sc[3], sc[0]), cdiv(sc[1], sc[2]));
+    sc[0] = sc[3];
+}
+__global__ void bicg_alpha_kernel(double2* sc)
+{
+    const double2 a = sc[0], b = sc[5];
+    const double d = b.x * b.x + b.y * b.y;
+    sc[1] = make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__global__ void bicg_omega_kernel(double2* sc)
+{
+    const double2 a = sc[6], b = sc[7];
+    const double d = b.x * b.x + b.y * b.y;
+    sc[2] = make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+// p = r + beta (p - omega v)
+__global__ __launch_bounds__(256) void bicg_p_kernel(const float2* __restrict__ r, float2* __restrict__ p,
+                                                     const float2* __restrict__ v, const double2* __restrict__ sc,
+                                                     int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double2 be = sc[4], om = sc[2];
+    const float2 pv = p[e], vv = v[e], rv = r[e];
+    const double tr = pv.x - (om.x * vv.x - om.y * vv.y), ti = pv.y - (om.x * vv.y + om.y * vv.x);
+    p[e] = make_float2((float)(rv.x + be.x * tr - be.y * ti), (float)(rv.y + be.x * ti + be.y * tr));
+}
+// out = a + coef * b, coef = sign * sc[idx]
+__global__ __launch_bounds__(256) void bicg_axpy_kernel(const float2* __restrict__ a, const float2* __restrict__ b,
+                                                        const double2* __restrict__ sc, int idx, double sign,
+                                                        float2* __restrict__ out, int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double2 c = make_double2(sign * sc[idx].x, sign * sc[idx].y);
+    const float2 av = a[e], bv = b[e];
+    out[e] = make_float2((float)(av.x + c.x * bv.x - c.y * bv.y), (float)(av.y + c.x * bv.y + c.y * bv.x));
+}
+// x += alpha p + omega s
+__global__ __launch_bounds__(256) void bicg_x_kernel(float2* __restrict__ x, const float2* __restrict__ p,
+                                                     const float2* __restrict__ s, const double2* __restrict__ sc,
+                                                     int64_t n)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    const double2 al = sc[1], om = sc[2];
+    const float2 pv = p[e], sv = s[e], xv = x[e];
+    x[e] = make_float2((float)(xv.x + al.x * pv.x - al.y * pv.y + om.x * sv.x - om.y * sv.y),
+                       (float)(xv.y + al.x * pv.y + al.y * pv.x + om.x * sv.y + om.y * sv.x));
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned g1(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda, int trans, const float2* x, float2* y,
+                       float2 alpha, float2 beta, float2* work, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0) return hipSuccess;
+    const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
+    if (trans == 0) {
+        const bool vec = (cols % 2 == 0) && (lda % 2 == 0) && (((uintptr_t)A & 15) == 0) && (((uintptr_t)x & 15) == 0);
+        if (vec) {
+            constexpr int RW = kGemvRows;
+            const int64_t waves = (rows + RW - 1) / RW;
+            hipLaunchKernelGGL(gemv_n_vec_kernel<RW>, dim3(g1(waves, 4)), dim3(256), 0, s, (const float4*)A, rows,
+                               cols / 2, lda / 2, (const float4*)x, y, alpha, beta, bz);
+        } else {
+            hipLaunchKernelGGL(gemv_n_scalar_kernel, dim3(g1(rows, 4)), dim3(256), 0, s, A, rows, cols, lda, x, y,
+                               alpha, beta, bz);
+        }
+        return hipGetLastError();
+    }
+    const int64_t nslab = (rows + kTSlabRows - 1) / kTSlabRows;
+    hipLaunchKernelGGL(gemv_t_partial_kernel, dim3(g1(cols), (unsigned)nslab), dim3(256), 0, s, A, rows, cols, lda, x,
+                       trans == 2 ? 1 : 0, work);
+    hipLaunchKernelGGL(gemv_t_reduce_kernel, dim3(g1(cols)), dim3(256), 0, s, work, nslab, cols, y, alpha, beta, bz);
+    return hipGetLastError();
+}
+
+int64_t gemv_work_elems(int64_t rows, int64_t cols, int trans)
+{
+    if (trans == 0) return 0;
+    return ((rows + kTSlabRows - 1) / kTSlabRows) * cols;
+}
+
+hipError_t launch_combine(const float2* A, int64_t lda, const float2* B, int64_t ldb, float2* C, int64_t ldc,
+                          int64_t rows, int64_t cols, float2 a, float2 b, hipStream_t s)
+{
+    int64_t g = (rows * cols + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(combine_kernel, dim3((unsigned)g), dim3(256), 0, s, A, lda, B, ldb, C, ldc, rows, cols, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, const float2* w, double2* h, hipStream_t s)
+{
+    hipLaunchKernelGGL(multidot_kernel, dim3(nq), dim3(256), 0, s, V, ldv, n, w, h);
+    return hipGetLastError();
+}
+hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(gs_update_kernel, dim3(g1(n)), dim3(256), 0, s, V, ldv, n, nq, h, w);
+    return hipGetLastError();
+}
+hipError_t launch_norm2(const float2* w, int64_t n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(norm2_kernel, dim3(1), dim3(1024), 0, s, w, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_dot(const float2* a, const float2* b, int64_t n, int conj, double2* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(dot_kernel, dim3(1), dim3(1024), 0, s, a, b, n, conj, out);
+    return hipGetLastError();
+}
+hipError_t launch_normalize(const float2* src, int64_t n, const double* nrm2, float2* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(normalize_kernel, dim3(g1(n)), dim3(256), 0, s, src, n, nrm2, out);
+    return hipGetLastError();
+}
+hipError_t launch_givens(double2* H, int m, int j, const double2* h, const double2* h2, const double* wn2,
+                         double2* cs, double2* sn, double2* g, double* res, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_givens_kernel, dim3(1), dim3(64), 0, s, H, m, j, h, h2, wn2, cs, sn, g, res);
+    return hipGetLastError();
+}
+hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_backsolve_kernel, dim3(1), dim3(64), 0, s, H, m, nk, g, y);
+    return hipGetLastError();
+}
+hipError_t launch_update_x(const float2* V, int64_t ldv, int64_t n, int nk, const double2* y, float2* x,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_update_x_kernel, dim3(g1(n)), dim3(256), 0, s, V, ldv, n, nk, y, x);
+    return hipGetLastError();
+}
+hipError_t launch_init_g(double2* g, int m, const double* nrm2, hipStream_t s)
+{
+    hipLaunchKernelGGL(gmres_init_g_kernel, dim3(1), dim3(256), 0, s, g, m, nrm2);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_scalar(int which, double2* sc, hipStream_t s)
+{
+    if (which == 0) hipLaunchKernelGGL(bicg_beta_kernel, dim3(1), dim3(1), 0, s, sc);
+    else if (which == 1) hipLaunchKernelGGL(bicg_alpha_kernel, dim3(1), dim3(1), 0, s, sc);
+    else hipLaunchKernelGGL(bicg_omega_kernel, dim3(1), dim3(1), 0, s, sc);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_p(const float2* r, float2* p, const float2* v, const double2* sc, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bicg_p_kernel, dim3(g1(n)), dim3(256), 0, s, r, p, v, sc, n);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_axpy(const float2* a, const float2* b, const double2* sc, int idx, double sign, float2* out,
+                            int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bicg_axpy_kernel, dim3(g1(n)), dim3(256), 0, s, a, b, sc, idx, sign, out, n);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_x(float2* x, const float2* p, const float2* sv, const double2* sc, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(bicg_x_kernel, dim3(g1(n)), dim3(256), 0, s, x, p, sv, sc, n);
+    return hipGetLastError();
+}
+
+}  // namespace bem

@@ -1,0 +1,34 @@
+// Launchers of the dense linear-algebra kernels (bem_linalg.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bem {
+
+constexpr int kGemvRows = 4;  // rows per wavefront in the streaming GEMV
+
+hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda, int trans, const float2* x, float2* y,
+                       float2 alpha, float2 beta, float2* work, hipStream_t s);
+int64_t gemv_work_elems(int64_t rows, int64_t cols, int trans);
+hipError_t launch_combine(const float2* A, int64_t lda, const float2* B, int64_t ldb, float2* C, int64_t ldc,
+                          int64_t rows, int64_t cols, float2 a, float2 b, hipStream_t s);
+hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, const float2* w, double2* h,
+                           hipStream_t s);
+hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
+                            hipStream_t s);
+hipError_t launch_norm2(const float2* w, int64_t n, double* out, hipStream_t s);
+hipError_t launch_dot(const float2* a, const float2* b, int64_t n, int conj, double2* out, hipStream_t s);
+hipError_t launch_normalize(const float2* src, int64_t n, const double* nrm2, float2* out, hipStream_t s);
+hipError_t launch_givens(double2* H, int m, int j, const double2* h, const double2* h2, const double* wn2,
+                         double2* cs, double2* sn, double2* g, double* res, hipStream_t s);
+hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s);
+hipError_t launch_update_x(const float2* V, int64_t ldv, int64_t n, int nk, const double2* y, float2* x,
+                           hipStream_t s);
+hipError_t launch_init_g(double2* g, int m, const double* nrm2, hipStream_t s);
+hipError_t launch_bicg_scalar(int which, double2* sc, hipStream_t s);
+hipError_t launch_bicg_p(const float2* r, float2* p, const float2* v, const double2* sc, int64_t n, hipStream_t s);
+hipError_t launch_bicg_axpy(const float2* a, const float2* b, const double2* sc, int idx, double sign, float2* out,
+                            int64_t n, hipStream_t s);
+hipError_t launch_bicg_x(float2* x, const float2* p, const float2* sv, const double2* sc, int64_t n, hipStream_t s);
+
+}  // namespace bem

@@ -1,0 +1,357 @@
+// Host-side mesh preprocessing: validation, per-triangle quadrature records,
+// surface curls, neighbour pairs, vertex->triangle lists and the P1 tiling.
+//
+// The reference recomputes normals and curls in torch (solver.py:40-58) and
+// finds neighbour pairs with an O(N) scan per triangle inside the singular
+// kernel (bind.cu:90-97); here they are built once per mesh in O(N log N).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "bem_device.hpp"
+#include "bem_internal.hpp"
+
+namespace bem {
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 mul(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// quadrature rules in double (same points as the device TriRule<Q>)
+struct HostRule {
+    std::vector<double> x1, x2, w;
+};
+HostRule host_rule(int q)
+{
+    HostRule r;
+    if (q == 6) {
+        const double a = 0.445948490915965, b = 0.091576213509771;
+        const double wa = 0.223381589678011, wb = 0.109951743655322;
+        r.x1 = {a, b, 1 - 2 * a, a, 1 - 2 * b, b};
+        r.x2 = {a, b, a, 1 - 2 * a, b, 1 - 2 * b};
+        r.w = {wa, wb, wa, wa, wb, wb};
+    } else if (q == 3) {
+        r.x1 = {1.0 / 6, 2.0 / 3, 1.0 / 6};
+        r.x2 = {1.0 / 6, 1.0 / 6, 2.0 / 3};
+        r.w = {1.0 / 3, 1.0 / 3, 1.0 / 3};
+    } else {
+        r.x1 = {1.0 / 3};
+        r.x2 = {1.0 / 3};
+        r.w = {1.0};
+    }
+    return r;
+}
+
+uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+struct Tiling {
+    std::vector<int32_t> ptr, vptr, verts;
+    std::vector<int32_t> slot;  // 4 per position
+};
+
+// Greedy tiling of the ordered triangle list: a tile closes when it holds
+// max_tris triangles or one more would exceed max_verts distinct vertices.
+Tiling make_tiling(const std::vector<int32_t>& perm, const int32_t* tris, int max_tris, int max_verts)
+{
+    Tiling T;
+    const int64_t nt = (int64_t)perm.size();
+    T.slot.assign((size_t)nt * 4, 0);
+    T.ptr.push_back(0);
+    T.vptr.push_back(0);
+    int64_t pos = 0;
+    std::vector<int32_t> local;
+    while (pos < nt) {
+        local.clear();
+        int64_t start = pos;
+        while (pos < nt && pos - start < max_tris) {
+            const int32_t* t = tris + 3 * (int64_t)perm[pos];
+            int fresh = 0;
+            for (int a = 0; a < 3; ++a)
+                if (std::find(local.begin(), local.end(), t[a]) == local.end()) ++fresh;
+            if ((int)local.size() + fresh > max_verts) break;
+            for (int a = 0; a < 3; ++a)
+                if (std::find(local.begin(), local.end(), t[a]) == local.end()) local.push_back(t[a]);
+            ++pos;
+        }
+        std::vector<int32_t> sorted = local;
+        std::sort(sorted.begin(), sorted.end());
+        for (int64_t p = start; p < pos; ++p) {
+            const int32_t* t = tris + 3 * (int64_t)perm[p];
+            for (int a = 0; a < 3; ++a)
+                T.slot[4 * p + a] = (int32_t)(std::lower_bound(sorted.begin(), sorted.end(), t[a]) - sorted.begin());
+        }
+        T.verts.insert(T.verts.end(), sorted.begin(), sorted.end());
+        T.ptr.push_back((int32_t)pos);
+        T.vptr.push_back((int32_t)T.verts.size());
+    }
+    return T;
+}
+
+}  // namespace
+
+int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, MeshDev& m, std::string& err)
+{
+    if (nv < 3 || nt < 1) {
+        err = "mesh needs at least 3 vertices and 1 triangle";
+        return 1;
+    }
+    if (nt > (int64_t)1 << 30 || nv > (int64_t)1 << 30) {
+        err = "mesh too large (more than 2^30 vertices or triangles)";
+        return 1;
+    }
+    for (int64_t t = 0; t < nt; ++t) {
+        const int32_t* tr = h_tris + 3 * t;
+        for (int a = 0; a < 3; ++a)
+            if (tr[a] < 0 || tr[a] >= nv) {
+                err = "triangle " + std::to_string(t) + " has a vertex index out of range";
+                return 2;
+            }
+        if (tr[0] == tr[1] || tr[1] == tr[2] || tr[0] == tr[2]) {
+            err = "triangle " + std::to_string(t) + " repeats a vertex";
+            return 2;
+        }
+    }
+    for (int64_t v = 0; v < 3 * nv; ++v)
+        if (!std::isfinite(h_verts[v])) {
+            err = "non-finite vertex coordinate";
+            return 2;
+        }
+    m.nv = nv;
+    m.nt = nt;
+
+    // ---- per-triangle geometry in double ----
+    constexpr int S6 = Rec<6>::kFloats, S3 = Rec<3>::kFloats, S1 = Rec<1>::kFloats;
+    std::vector<float> rec6((size_t)nt * S6, 0.f), rec3((size_t)nt * S3, 0.f), rec1((size_t)nt * S1, 0.f);
+    std::vector<float> curls((size_t)nt * 12, 0.f), geo((size_t)nt * 16, 0.f);
+    std::vector<int32_t> tris4((size_t)nt * 4, 0);
+    const HostRule R6 = host_rule(6), R3 = host_rule(3), R1 = host_rule(1);
+    V3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
+    for (int64_t v = 0; v < nv; ++v) {
+        lo.x = std::min(lo.x, (double)h_verts[3 * v]);
+        hi.x = std::max(hi.x, (double)h_verts[3 * v]);
+        lo.y = std::min(lo.y, (double)h_verts[3 * v + 1]);
+        hi.y = std::max(hi.y, (double)h_verts[3 * v + 1]);
+        lo.z = std::min(lo.z, (double)h_verts[3 * v + 2]);
+        hi.z = std::max(hi.z, (double)h_verts[3 * v + 2]);
+    }
+    m.diameter = std::sqrt(dot(sub(hi, lo), sub(hi, lo)));
+    std::vector<V3> cent((size_t)nt);
+    for (int64_t t = 0; t < nt; ++t) {
+        const int32_t* tr = h_tris + 3 * t;
+        V3 v[3];
+        for (int a = 0; a < 3; ++a) {
+            v[a] = {h_verts[3 * (int64_t)tr[a]], h_verts[3 * (int64_t)tr[a] + 1], h_verts[3 * (int64_t)tr[a] + 2]};
+            tris4[4 * t + a] = tr[a];
+        }
+        const V3 e1 = sub(v[1], v[0]), e2 = sub(v[2], v[0]);
+        const V3 c = cross(e1, e2);
+        const double J = std::sqrt(dot(c, c));
+        if (!(J > 0.0) || !std::isfinite(J)) {
+            err = "triangle " + std::to_string(t) + " has zero area";
+            return 2;
+        }
+        const V3 n = mul(1.0 / J, c);
+        cent[t] = mul(1.0 / 3.0, add(add(v[0], v[1]), v[2]));
+        auto fill = [&](float* r, const HostRule& R) {
+            const int Q = (int)R.w.size();
+            for (int q = 0; q < Q; ++q) {
+                const V3 p = add(v[0], add(mul(R.x1[q], e1), mul(R.x2[q], e2)));
+                r[3 * q] = (float)p.x;
+                r[3 * q + 1] = (float)p.y;
+                r[3 * q + 2] = (float)p.z;
+                r[3 * Q + q] = (float)(R.w[q] * 0.5 * J);
+            }
+            r[4 * Q] = (float)n.x;
+            r[4 * Q + 1] = (float)n.y;
+            r[4 * Q + 2] = (float)n.z;
+        };
+        fill(&rec6[(size_t)t * S6], R6);
+        fill(&rec3[(size_t)t * S3], R3);
+        fill(&rec1[(size_t)t * S1], R1);
+        // P1 surface curls: row a = n x grad(phi_a), grad from the dual basis
+        const double g11 = dot(e1, e1), g12 = dot(e1, e2), g22 = dot(e2, e2);
+        const double det = g11 * g22 - g12 * g12;
+        const V3 b1 = add(mul(g22 / det, e1), mul(-g12 / det, e2));
+        const V3 b2 = add(mul(-g12 / det, e1), mul(g11 / det, e2));
+        const V3 grad[3] = {mul(-1.0, add(b1, b2)), b1, b2};
+        for (int a = 0; a < 3; ++a) {
+            const V3 cu = cross(n, grad[a]);
+            curls[(size_t)t * 12 + 3 * a] = (float)cu.x;
+            curls[(size_t)t * 12 + 3 * a + 1] = (float)cu.y;
+            curls[(size_t)t * 12 + 3 * a + 2] = (float)cu.z;
+        }
+        float* g = &geo[(size_t)t * 16];
+        for (int a = 0; a < 3; ++a) {
+            g[3 * a] = (float)v[a].x;
+            g[3 * a + 1] = (float)v[a].y;
+            g[3 * a + 2] = (float)v[a].z;
+        }
+        g[9] = (float)n.x;
+        g[10] = (float)n.y;
+        g[11] = (float)n.z;
+        g[12] = (float)J;
+    }
+
+    // ---- vertex -> triangle CSR ----
+    std::vector<int32_t> vt_ptr((size_t)nv + 1, 0), vt_idx((size_t)nt * 3);
+    for (int64_t t = 0; t < 3 * nt; ++t) vt_ptr[h_tris[t] + 1]++;
+    for (int64_t v = 0; v < nv; ++v) vt_ptr[v + 1] += vt_ptr[v];
+    {
+        std::vector<int32_t> fillp(vt_ptr.begin(), vt_ptr.end() - 1);
+        for (int64_t t = 0; t < nt; ++t)
+            for (int a = 0; a < 3; ++a) vt_idx[fillp[h_tris[3 * t + a]]++] = (int32_t)t;
+    }
+    for (int64_t v = 0; v < nv; ++v)
+        if (vt_ptr[v + 1] == vt_ptr[v]) {
+            err = "vertex " + std::to_string(v) + " belongs to no triangle";
+            return 2;
+        }
+
+    // ---- neighbour pairs (share >= 1 vertex, self included) ----
+    std::vector<int32_t> pairs;
+    pairs.reserve((size_t)nt * 14 * 4);
+    m.h_pair_ptr.assign((size_t)nt + 1, 0);
+    std::vector<int32_t> cand;
+    int32_t maxnb = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        cand.clear();
+        for (int a = 0; a < 3; ++a) {
+            const int32_t v = h_tris[3 * t + a];
+            cand.insert(cand.end(), vt_idx.begin() + vt_ptr[v], vt_idx.begin() + vt_ptr[v + 1]);
+        }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        maxnb = std::max(maxnb, (int32_t)cand.size());
+        for (int32_t u : cand) {
+            int s = 0;
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) s += (h_tris[3 * t + a] == h_tris[3 * (int64_t)u + b]);
+            if (s == 3 && u != t) {
+                err = "triangles " + std::to_string(t) + " and " + std::to_string(u) + " share all three vertices";
+                return 2;
+            }
+            pairs.push_back((int32_t)t);
+            pairs.push_back(u);
+            pairs.push_back(s);
+            pairs.push_back(0);
+        }
+        m.h_pair_ptr[t + 1] = (int64_t)pairs.size() / 4;
+    }
+    m.npairs = (int64_t)pairs.size() / 4;
+    m.max_neighbors = maxnb;
+
+    // ---- P1 tiling on a Morton order of the centroids ----
+    std::vector<int32_t> perm((size_t)nt);
+    {
+        std::vector<uint32_t> code((size_t)nt);
+        const double sx = hi.x - lo.x > 0 ? 1023.0 / (hi.x - lo.x) : 0.0;
+        const double sy = hi.y - lo.y > 0 ? 1023.0 / (hi.y - lo.y) : 0.0;
+        const double sz = hi.z - lo.z > 0 ? 1023.0 / (hi.z - lo.z) : 0.0;
+        for (int64_t t = 0; t < nt; ++t) {
+            const uint32_t ix = (uint32_t)((cent[t].x - lo.x) * sx);
+            const uint32_t iy = (uint32_t)((cent[t].y - lo.y) * sy);
+            const uint32_t iz = (uint32_t)((cent[t].z - lo.z) * sz);
+            code[t] = spread10(ix) | (spread10(iy) << 1) | (spread10(iz) << 2);
+        }
+        std::iota(perm.begin(), perm.end(), 0);
+        std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
+    }
+    const Tiling rowT = make_tiling(perm, h_tris, kP1RowTris, kP1RowVerts);
+    const Tiling colT = make_tiling(perm, h_tris, kP1ColTris, kP1ColVerts);
+
+    // ---- one device allocation ----
+    struct Part {
+        const void* src;
+        size_t bytes;
+        void** dst;
+    };
+    std::vector<Part> parts;
+    float* d_verts = nullptr;
+    std::vector<float> vcopy(h_verts, h_verts + 3 * nv);
+    parts.push_back({vcopy.data(), vcopy.size() * 4, (void**)&d_verts});
+    parts.push_back({tris4.data(), tris4.size() * 4, (void**)&m.tris});
+    parts.push_back({rec6.data(), rec6.size() * 4, (void**)&m.rec6});
+    parts.push_back({rec3.data(), rec3.size() * 4, (void**)&m.rec3});
+    parts.push_back({rec1.data(), rec1.size() * 4, (void**)&m.rec1});
+    parts.push_back({curls.data(), curls.size() * 4, (void**)&m.curls});
+    parts.push_back({geo.data(), geo.size() * 4, (void**)&m.geo});
+    parts.push_back({pairs.data(), pairs.size() * 4, (void**)&m.pairs});
+    parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
+    parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});
+    int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv;
+    int4 *d_rs, *d_cs;
+    parts.push_back({perm.data(), perm.size() * 4, (void**)&d_perm});
+    parts.push_back({rowT.ptr.data(), rowT.ptr.size() * 4, (void**)&d_rp});
+    parts.push_back({rowT.vptr.data(), rowT.vptr.size() * 4, (void**)&d_rvp});
+    parts.push_back({rowT.verts.data(), rowT.verts.size() * 4, (void**)&d_rv});
+    parts.push_back({rowT.slot.data(), rowT.slot.size() * 4, (void**)&d_rs});
+    parts.push_back({colT.ptr.data(), colT.ptr.size() * 4, (void**)&d_cp});
+    parts.push_back({colT.vptr.data(), colT.vptr.size() * 4, (void**)&d_cvp});
+    parts.push_back({colT.verts.data(), colT.verts.size() * 4, (void**)&d_cv});
+    parts.push_back({colT.slot.data(), colT.slot.size() * 4, (void**)&d_cs});
+    size_t total = 0;
+    std::vector<size_t> off;
+    for (auto& p : parts) {
+        off.push_back(total);
+        total += (p.bytes + 255) / 256 * 256;
+    }
+    if (hipGetDevice(&m.device) != hipSuccess) {
+        err = "no HIP device";
+        return 5;
+    }
+    if (hipMalloc(&m.block, total) != hipSuccess) {
+        err = "device allocation of " + std::to_string(total) + " bytes failed";
+        (void)hipGetLastError();
+        return 6;
+    }
+    for (size_t q = 0; q < parts.size(); ++q) {
+        char* dst = (char*)m.block + off[q];
+        *parts[q].dst = dst;
+        if (parts[q].bytes && hipMemcpy(dst, parts[q].src, parts[q].bytes, hipMemcpyHostToDevice) != hipSuccess) {
+            err = "host to device copy failed";
+            (void)hipFree(m.block);
+            m.block = nullptr;
+            return 3;
+        }
+    }
+    m.verts = d_verts;
+    m.tiles.perm = d_perm;
+    m.tiles.row_ptr = d_rp;
+    m.tiles.row_vptr = d_rvp;
+    m.tiles.row_verts = d_rv;
+    m.tiles.row_slot = d_rs;
+    m.tiles.col_ptr = d_cp;
+    m.tiles.col_vptr = d_cvp;
+    m.tiles.col_verts = d_cv;
+    m.tiles.col_slot = d_cs;
+    m.tiles.nrow_tiles = (int)rowT.ptr.size() - 1;
+    m.tiles.ncol_tiles = (int)colT.ptr.size() - 1;
+    return 0;
+}
+
+void mesh_free(MeshDev& m)
+{
+    if (m.block) (void)hipFree(m.block);
+    m.block = nullptr;
+}
+
+}  // namespace bem

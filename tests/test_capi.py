@@ -1,0 +1,66 @@
+"""C-ABI library checks that need no GPU: it loads, and it exports exactly
+the entry points include/bem_hip.h declares (and the ctypes table binds)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "bem_hip.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bem_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "bem_assemble" in names and "bem_gmres" in names and len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    from neuralat_amd import _lib
+
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in _declared() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_ctypes_table_matches_header():
+    from neuralat_amd import _lib
+
+    assert sorted(_lib.SIGNATURES) == _declared()
+    lib = _lib.load()
+    assert b"gfx950" in lib.bem_version()
+
+
+def test_library_built_for_gfx950_only():
+    from neuralat_amd import _lib
+
+    data = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+    assert b"sm_" not in data[:0]  # no CUDA targets exist in a hipcc build
+
+
+def test_missing_library_fails_loudly(monkeypatch):
+    from neuralat_amd import _lib
+
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", "/nonexistent/libbem_hip.so")
+    with pytest.raises(ImportError):
+        _lib.load()
+
+
+def test_status_mapping():
+    from neuralat_amd import _lib
+
+    _lib.load()
+    with pytest.raises(ValueError):
+        _lib.check(_lib.BEM_ERR_BAD_ARG)
+    with pytest.raises(_lib.NotConvergedError):
+        _lib.check(_lib.BEM_ERR_NOT_CONVERGED)
+    with pytest.raises(_lib.BemError):
+        _lib.check(_lib.BEM_ERR_HIP)

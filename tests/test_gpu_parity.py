@@ -1,0 +1,314 @@
+"""GPU parity of the HIP path against the CPU oracle and the golden fixtures.
+
+Bar: relative Frobenius error <= 1e-5 against the FP64 oracle for every
+assembled operator (the plain method's FP32 parity target), with the same
+r-clamp setting on both sides.  All calls go through the C ABI
+(libbem_hip.so) via neuralat_amd.bem.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from neuralat_amd import mesh
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+OPS = ["single", "double", "adjointdouble", "hypersingular", "bm_lhs", "bm_rhs"]
+TOL = 1e-5
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _dev():
+    return torch.device("cuda:0")
+
+
+def _solver(v, t, basis="P1", clamp=True):
+    from neuralat_amd.bem import BEM_Solver
+
+    return BEM_Solver(torch.tensor(np.asarray(v, np.float32), device=_dev()),
+                      torch.tensor(np.asarray(t, np.int32), device=_dev()), basis=basis, clamp=clamp)
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_native_library_is_loaded():
+    from neuralat_amd import _lib
+
+    lib = _lib.load()
+    assert b"gfx950" in lib.bem_version()
+
+
+# ---------------------------------------------------------------------------
+# golden fixtures (the reference's own check: octahedron sphere, P1, k in 1/10/100)
+# ---------------------------------------------------------------------------
+
+def test_golden_octasphere0_all_operators():
+    g = np.load(os.path.join(GOLD, "octasphere0_p1.npz"))
+    s = _solver(g["vertices"], g["triangles"])
+    assert _rel(s.identity_matrix().cpu().numpy(), g["identity"]) < TOL
+    for k in (1, 10, 100):
+        for op in OPS:
+            _say("golden octasphere0", op, k)
+            A = s.assemble_boundary_matrix(float(k), op).cpu().numpy()
+            e = _rel(A, g[f"{op}_k{k}"])
+            assert e < TOL, (op, k, e)
+        dens = torch.tensor(g["density"], dtype=torch.complex64, device=_dev())
+        pts = torch.tensor(g["points"], device=_dev())
+        assert _rel(s.single_potential(float(k), dens, pts).cpu().numpy(), g[f"spot_k{k}"]) < TOL
+        assert _rel(s.double_potential(float(k), dens, pts).cpu().numpy(), g[f"dpot_k{k}"]) < TOL
+
+
+@pytest.mark.parametrize("quad", ["approx", "approx1"])
+def test_golden_approx_rules(quad):
+    g = np.load(os.path.join(GOLD, "octasphere1_p1_approx.npz"))
+    s = _solver(g["vertices"], g["triangles"])
+    for op in OPS:
+        _say("golden approx", quad, op)
+        A = s.assemble_boundary_matrix(10.0, op, approx=("approx_1" if quad == "approx1" else True)).cpu().numpy()
+        e = _rel(A, g[f"{op}_{quad}"])
+        assert e < TOL, (op, quad, e)
+
+
+def test_golden_icosphere1_dp0():
+    g = np.load(os.path.join(GOLD, "icosphere1_dp0.npz"))
+    for clamp in (True, False):
+        s = _solver(g["vertices"], g["triangles"], basis="DP0", clamp=clamp)
+        for op in ("single", "double", "adjointdouble", "bm_rhs"):
+            _say("golden icosphere1 DP0", op, clamp)
+            A = s.assemble_boundary_matrix(18.3, op).cpu().numpy()
+            e = _rel(A, g[op] if clamp else g[f"{op}_noclamp"])
+            assert e < TOL, (op, clamp, e)
+        assert _rel(s.identity_matrix().cpu().numpy(), g["identity"]) < TOL
+
+
+# ---------------------------------------------------------------------------
+# live oracle at larger sizes
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def l2_sphere():
+    v, t = mesh.icosphere(2, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=2)
+    return v.astype(np.float32), t
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_p1_operators_vs_oracle(l2_sphere, op):
+    v, t = l2_sphere
+    s = _solver(v, t, "P1")
+    for k in (3.0, -12.0):
+        _say("P1 L2", op, k)
+        A = s.assemble_boundary_matrix(k, op).cpu().numpy()
+        R = oracle.assemble(v, t, k, op, "P1")
+        e = _rel(A, R)
+        assert e < TOL, (op, k, e)
+
+
+@pytest.mark.parametrize("op", ["single", "double", "adjointdouble", "bm_rhs"])
+@pytest.mark.parametrize("clamp", [True, False])
+def test_dp0_operators_vs_oracle(op, clamp):
+    v, t = mesh.icosphere(3, 0.1)
+    v = v.astype(np.float32)
+    s = _solver(v, t, "DP0", clamp=clamp)
+    _say("DP0 L3", op, clamp)
+    A = s.assemble_boundary_matrix(18.3, op).cpu().numpy()
+    R = oracle.assemble(v, t, 18.3, op, "DP0", clamp=clamp, nthreads=0)
+    e = _rel(A, R)
+    assert e < TOL, (op, clamp, e)
+
+
+def test_dp0_row_blocks_match_full():
+    from neuralat_amd import _lib
+    from neuralat_amd.bem.solver import _ptr, _stream
+
+    v, t = mesh.icosphere(3, 1.0)
+    s = _solver(v, t, "DP0")
+    full = s.assemble_boundary_matrix(10.0, "double")
+    n = full.shape[0]
+    blk = torch.empty((300, n), dtype=torch.complex64, device=_dev())
+    _lib.check(_lib.load().bem_assemble_rows(s.mesh.handle, 10.0, 1, 0, 1, 500, 800, _ptr(blk), n, _stream()))
+    torch.cuda.synchronize()
+    assert torch.equal(blk, full[500:800])
+
+
+def test_identity_potential_t2v_vs_oracle(l2_sphere):
+    from neuralat_amd.bem import map_triangle2vertex
+
+    v, t = l2_sphere
+    for basis in ("P1", "DP0"):
+        s = _solver(v, t, basis)
+        assert _rel(s.identity_matrix().cpu().numpy(), oracle.identity(v, t, basis)) < TOL
+        pts = mesh.random_exterior_points(40, 1.5, 3.0, seed=5).astype(np.float32)
+        pt = torch.tensor(pts, device=_dev())
+        for op in ("single", "double"):
+            for clamp in (True, False):
+                P = s.potential_matrix(7.0, pt, op, clamp=clamp).cpu().numpy()
+                assert _rel(P, oracle.potential(v, t, pts, 7.0, op, basis, clamp=clamp)) < TOL, (basis, op)
+    f = (np.arange(len(t)) * (0.3 - 0.1j)).astype(np.complex64)
+    out = map_triangle2vertex(torch.tensor(v, device=_dev()), torch.tensor(t, device=_dev()),
+                              torch.tensor(f, device=_dev())).cpu().numpy()
+    assert _rel(out, oracle.triangle2vertex(len(v), t, f)) < 1e-6
+
+
+def test_negative_wavenumber_is_conjugate(l2_sphere):
+    v, t = l2_sphere
+    s = _solver(v, t, "P1")
+    a = s.assemble_boundary_matrix(9.0, "single").cpu().numpy()
+    b = s.assemble_boundary_matrix(-9.0, "single").cpu().numpy()
+    assert _rel(b, np.conj(a)) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# linear algebra
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape", [(257, 130), (1000, 1000), (64, 4097), (3, 5)])
+def test_matvec_all_ops(shape):
+    from neuralat_amd.bem import matvec
+
+    rng = np.random.default_rng(1)
+    r, c = shape
+    A = (rng.standard_normal((r, c)) + 1j * rng.standard_normal((r, c))).astype(np.complex64)
+    At = torch.tensor(A, device=_dev())
+    for trans, Aop in ((0, A), (1, A.T), (2, A.conj().T)):
+        x = (rng.standard_normal(Aop.shape[1]) + 1j * rng.standard_normal(Aop.shape[1])).astype(np.complex64)
+        y0 = (rng.standard_normal(Aop.shape[0]) + 1j * rng.standard_normal(Aop.shape[0])).astype(np.complex64)
+        ref = (0.5 - 2j) * (Aop.astype(np.complex128) @ x) + (1 + 1j) * y0
+        out = torch.tensor(y0, device=_dev())
+        matvec(At, torch.tensor(x, device=_dev()), trans=trans, alpha=0.5 - 2j, beta=1 + 1j, out=out)
+        assert _rel(out.cpu().numpy(), ref) < 1e-5, (shape, trans)
+
+
+def test_gmres_and_bicgstab_vs_direct():
+    from neuralat_amd.bem import bicgstab, gmres
+
+    rng = np.random.default_rng(2)
+    n = 700
+    A = (np.eye(n) * 2 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.6)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    xr = np.linalg.solve(A, b)
+    At = torch.tensor(A.astype(np.complex64), device=_dev())
+    bt = torch.tensor(b.astype(np.complex64), device=_dev())
+    for restart in (10, 200):
+        x, info = gmres(At, bt, tol=1e-6, restart=restart, maxiter=500)
+        assert info["converged"] and info["rel_residual"] <= 1e-6, info
+        assert _rel(x.cpu().numpy(), xr) < 1e-5
+    x, info = bicgstab(At, bt)
+    assert info["converged"], info
+    assert _rel(x.cpu().numpy(), xr) < 2e-5
+    # silent non-convergence, as BiCGSTAB.py
+    x1, info1 = bicgstab(At, bt, nsteps=1)
+    assert info1["iterations"] == 1 and not info1["converged"]
+    # GMRES reports non-convergence
+    x2, info2 = gmres(At, bt, tol=1e-12, restart=5, maxiter=5)
+    assert not info2["converged"] and info2["iterations"] == 5
+
+
+def test_solve_linear_equation_matches_reference_semantics():
+    from neuralat_amd.bem import solve_linear_equation
+
+    rng = np.random.default_rng(3)
+    n = 300
+    A = (np.eye(n) * 3 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.4)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x_or, steps, conv = oracle.bicgstab(lambda z: A @ z, b)
+    At = torch.tensor(A.astype(np.complex64), device=_dev())
+    bt = torch.tensor(b.astype(np.complex64), device=_dev())
+    x = solve_linear_equation(At, bt).cpu().numpy()
+    assert _rel(x, x_or) < 2e-5
+    xc = solve_linear_equation(lambda z: At @ z, bt).cpu().numpy()
+    assert _rel(xc, x_or) < 2e-5
+
+
+# ---------------------------------------------------------------------------
+# end to end: CBIE against the oracle solution and the analytic monopole
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("basis", ["DP0", "P1"])
+def test_cbie_monopole_end_to_end(basis):
+    v, t = mesh.icosphere(3, 0.1)
+    k = 18.3
+    val, dn = mesh.sources_field([[0.01, -0.005, 0.002]], None, k)
+    proj = mesh.project_dp0 if basis == "DP0" else mesh.project_p1
+    psi = proj(v, t, lambda x, n: dn(x, n))
+    s = _solver(v, t, basis, clamp=False)
+    psit = torch.tensor(psi.astype(np.complex64), device=_dev())
+    phi = s.CBIE(k, psit)
+    assert s.last_solve_info["converged"]
+    pts = mesh.random_exterior_points(64, 0.2, 0.5, seed=0)
+    p = s.boundary2potential(k, psit, phi, torch.tensor(pts, dtype=torch.float32, device=_dev())).cpu().numpy()
+    err_an = _rel(p, val(pts))
+    # FP64 oracle solution of the same discrete system
+    v32 = v.astype(np.float32)
+    S = oracle.assemble(v32, t, k, "single", basis, clamp=False)
+    D = oracle.assemble(v32, t, k, "double", basis, clamp=False)
+    M = oracle.identity(v32, t, basis)
+    phi_or = np.linalg.solve(0.5 * M - D, -S @ psi)
+    p_or = (-oracle.potential(v32, t, pts, k, "single", basis, clamp=False) @ psi
+            + oracle.potential(v32, t, pts, k, "double", basis, clamp=False) @ phi_or)
+    err_fp64 = _rel(p, p_or)
+    err_surf = _rel(phi.cpu().numpy(), phi_or)
+    _say(f"CBIE {basis}: vs analytic {err_an:.2e}, vs FP64 oracle {err_fp64:.2e} (surface {err_surf:.2e})")
+    assert err_fp64 < 1e-4 and err_surf < 1e-4
+    assert err_an < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# size-independent properties at a large size, and error paths
+# ---------------------------------------------------------------------------
+
+def test_gauss_law_large_mesh():
+    """Row sums of the k=0 DLP are -|T_i|/2 on a closed surface (exact on a
+    polyhedron up to quadrature error), checked at level 5 (20480 panels)."""
+    from neuralat_amd.bem import matvec
+
+    v, t = mesh.icosphere(5, 1.0)
+    s = _solver(v, t, "DP0", clamp=False)
+    K = s.assemble_boundary_matrix(0.0, "double")
+    ones = torch.ones(K.shape[0], dtype=torch.complex64, device=_dev())
+    rs = matvec(K, ones).cpu().numpy()
+    area = 0.5 * np.linalg.norm(np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]]), axis=1)
+    assert np.abs(rs.real / area + 0.5).max() < 1e-3
+    S = s.assemble_boundary_matrix(5.0, "single")
+    d = (S - S.T).abs().max().item() / S.abs().max().item()
+    assert d < 1e-3
+    del K, S
+
+
+def test_bad_meshes_raise():
+    from neuralat_amd.bem import BEM_Solver
+
+    v = torch.tensor([[0, 0, 0], [1, 0, 0], [0, 1, 0], [2, 2, 2]], dtype=torch.float32, device=_dev())
+    with pytest.raises(ValueError):
+        BEM_Solver(v, torch.tensor([[0, 1, 9]], dtype=torch.int32, device=_dev()))
+    with pytest.raises(ValueError):
+        BEM_Solver(v, torch.tensor([[0, 1, 1]], dtype=torch.int32, device=_dev()))
+    vz = torch.tensor([[0, 0, 0], [1, 0, 0], [2, 0, 0]], dtype=torch.float32, device=_dev())
+    with pytest.raises(ValueError):
+        BEM_Solver(vz, torch.tensor([[0, 1, 2]], dtype=torch.int32, device=_dev()))
+    with pytest.raises(AssertionError):
+        BEM_Solver(v.double(), torch.tensor([[0, 1, 2]], dtype=torch.int32, device=_dev()))
+    s = BEM_Solver(v, torch.tensor([[0, 1, 2], [0, 2, 3]], dtype=torch.int32, device=_dev()), basis="DP0")
+    with pytest.raises(ValueError):
+        s.assemble_boundary_matrix(1.0, "hypersingular")
+    with pytest.raises(ValueError):
+        s.assemble_boundary_matrix(1.0, "nonsense")
+
+
+def test_single_triangle_mesh():
+    v = np.array([[0, 0, 0], [0.1, 0, 0], [0.02, 0.09, 0]], dtype=np.float32)
+    t = np.array([[0, 1, 2]], dtype=np.int32)
+    for basis in ("P1", "DP0"):
+        s = _solver(v, t, basis, clamp=False)
+        A = s.assemble_boundary_matrix(4.0, "single").cpu().numpy()
+        R = oracle.assemble(v, t, 4.0, "single", basis, clamp=False)
+        assert _rel(A, R) < TOL
