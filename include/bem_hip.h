@@ -87,6 +87,12 @@ int bem_mesh_normals_curls(const bem_mesh* mesh, float* normals, float* curls, v
  * (potential.h:16-23). */
 int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, void* out, int64_t ld,
                  void* stream);
+/* bem_assemble restricted to parts of the pair set: parts = 1 regular pairs
+ * only (P1: also zero-fills out first), 2 neighbour (singular) pairs only
+ * (added / written on top of out), 3 both (= bem_assemble).  Lets a caller
+ * time or overlap the two kernels separately. */
+int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
+                    int64_t ld, void* stream);
 /* DP0 rows [row0, row1) only (out has row1-row0 rows); rows are independent,
  * so a large operator can be assembled block by block. */
 int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
@@ -98,6 +104,10 @@ int bem_boundary_matrix(const float* verts, int64_t nv, const int32_t* tris, int
 
 /* Mass matrix: P1 [nv,nv] (bind.cu:10-21), DP0 diag(area) [nt,nt]. */
 int bem_identity(const bem_mesh* mesh, int basis, void* out, int64_t ld, void* stream);
+/* A += alpha * M in place (M the mass matrix of the basis), e.g. the 0.5 M
+ * of the CBIE left-hand side (solver.py:102) without a dense identity. */
+int bem_add_identity(const bem_mesh* mesh, int basis, float alpha_re, float alpha_im, void* A, int64_t ld,
+                     void* stream);
 /* Vertex average of per-triangle complex values (bind.cu:23-48). */
 int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream);
 /* Potential operator from the surface to np points ([np,3] float32, device):

@@ -126,6 +126,24 @@ def combine(A, a, B, b):
     return C
 
 
+def scale_(A: torch.Tensor, a):
+    """A *= a in place."""
+    a = complex(a)
+    rows, cols = A.shape
+    _lib.check(_lib.load().bem_combine(_ptr(A), A.stride(0), None, cols, _ptr(A), A.stride(0), rows, cols, a.real,
+                                       a.imag, 0.0, 0.0, _stream()), "bem_combine")
+    return A
+
+
+def add_identity_(solver, A: torch.Tensor, alpha, basis=None):
+    """A += alpha M in place (M the mass matrix of the solver's basis)."""
+    basis = basis or solver.basis
+    alpha = complex(alpha)
+    _lib.check(_lib.load().bem_add_identity(solver.mesh.handle, _lib.BASES[basis], alpha.real, alpha.imag, _ptr(A),
+                                            A.stride(0), _stream()), "bem_add_identity")
+    return A
+
+
 def gmres(A: torch.Tensor, b: torch.Tensor, x0=None, tol=1e-6, restart=100, maxiter=1000, raise_on_fail=False):
     """Device GMRES(restart) on a dense complex64 matrix.  Returns (x, info)."""
     check_tensor(A, torch.complex64)
@@ -232,12 +250,12 @@ class BEM_Solver:
 
     def CBIE(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):
         """Conventional BIE (solver.py:97-104): (M/2 - K) phi = -V psi."""
-        identity = self.identity_matrix()
         slp = self.assemble_boundary_matrix(k, "single")
-        dlp = self.assemble_boundary_matrix(k, "double")
-        LHS = combine(identity, 0.5, dlp, -1.0)
-        del identity, dlp
         RHS = matvec(slp, neumann.to(torch.complex64).contiguous(), alpha=-1.0)
+        del slp
+        LHS = self.assemble_boundary_matrix(k, "double")
+        scale_(LHS, -1.0)
+        add_identity_(self, LHS, 0.5)
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 
     def HBIE(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):

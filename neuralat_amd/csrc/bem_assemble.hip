@@ -271,16 +271,14 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
             w[a][d] = gj[3 * pj[a] + d];
         }
     // x = v0 + x1 (v1 - v0) + x2 (v2 - v1) on (0,0),(1,0),(1,1); the shared
-    // local vertex 0 cancels in d = y - x, which keeps r accurate near the
-    // singularity.
-    float e1[3], e2[3], f1[3], f2[3], o[3];
+    // local vertex 0 (v0 = w0 in all three cases) cancels in d = y - x.
+    float e1[3], e2[3], f1[3], f2[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         e1[d] = v[1][d] - v[0][d];
         e2[d] = v[2][d] - v[1][d];
         f1[d] = w[1][d] - w[0][d];
         f2[d] = w[2][d] - w[1][d];
-        o[d] = w[0][d] - v[0][d];
     }
     const float nx0 = gi[9], nx1 = gi[10], nx2 = gi[11];
     const float ny0 = gj[9], ny1 = gj[10], ny2 = gj[11];
@@ -298,56 +296,72 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
         const float e_2 = LineRule01<NL>::t[q2], e_3 = LineRule01<NL>::t[q3];
         const float wq = LineRule01<NL>::w[q0] * LineRule01<NL>::w[q1] * LineRule01<NL>::w[q2] * LineRule01<NL>::w[q3];
         const float e12 = e_1 * e_2, e123 = e12 * e_3;
-        float reg[6][5];
+        // Region table rows: (x1, x2, y1, y2, extra weight, d1, d2) with
+        // (d1, d2) = (y1 - x1, y2 - x2) formed from the eta products
+        // directly (coincident, edge: d1 only), which keeps y - x accurate
+        // where the two points nearly coincide.
+        float reg[6][7];
         int nreg;
         float jac;
         if (shared == 3) {
             jac = xs * xs * xs * e_1 * e_1 * e_2;
             nreg = 6;
-            const float t[6][4] = {
-                {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1)},
-                {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12)},
-                {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12)},
-                {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123)},
-                {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12)},
-                {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123)},
+            const float e12c = e12 * (1.f - e_3);  // eta1 eta2 - eta1 eta2 eta3
+            const float t[6][7] = {
+                {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1), 1.f, -xs * e123, -xs * e12},
+                {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12), 1.f, xs * e123, xs * e12},
+                {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12), 1.f, -xs * e12, -xs * e123},
+                {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123), 1.f, xs * e12, xs * e123},
+                {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12), 1.f, xs * e123, -xs * e12c},
+                {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123), 1.f, -xs * e123, xs * e12c},
             };
-            for (int r = 0; r < 6; ++r) {
-                for (int c = 0; c < 4; ++c) reg[r][c] = t[r][c];
-                reg[r][4] = 1.f;
-            }
+            for (int r = 0; r < 6; ++r)
+                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
         } else if (shared == 2) {
             jac = xs * xs * xs * e_1 * e_1;
             nreg = 5;
-            const float t[5][5] = {
-                {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f},
-                {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2},
-                {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2},
-                {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2},
-                {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2},
+            const float t[5][7] = {
+                {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f, -xs * e12, 0.f},
+                {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2, -xs * e123, 0.f},
+                {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2, xs * e12, 0.f},
+                {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2, xs * e123, 0.f},
+                {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2, xs * e123, 0.f},
             };
             for (int r = 0; r < 5; ++r)
-                for (int c = 0; c < 5; ++c) reg[r][c] = t[r][c];
+                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
         } else {
             jac = xs * xs * xs * e_2;
             nreg = 2;
-            const float t[2][4] = {
-                {xs, xs * e_1, xs * e_2, xs * e_2 * e_3},
-                {xs * e_2, xs * e_2 * e_3, xs, xs * e_1},
+            const float t[2][7] = {
+                {xs, xs * e_1, xs * e_2, xs * e_2 * e_3, 1.f, 0.f, 0.f},
+                {xs * e_2, xs * e_2 * e_3, xs, xs * e_1, 1.f, 0.f, 0.f},
             };
-            for (int r = 0; r < 2; ++r) {
-                for (int c = 0; c < 4; ++c) reg[r][c] = t[r][c];
-                reg[r][4] = 1.f;
-            }
+            for (int r = 0; r < 2; ++r)
+                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
         }
         float br[10], bi[10];
 #pragma unroll
         for (int e = 0; e < 10; ++e) br[e] = bi[e] = 0.f;
         for (int r = 0; r < nreg; ++r) {
             const float x1 = reg[r][0], x2 = reg[r][1], y1 = reg[r][2], y2 = reg[r][3];
-            const float dx = o[0] + y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
-            const float dy = o[1] + y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
-            const float dz = o[2] + y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
+            const float d1 = reg[r][5], d2 = reg[r][6];
+            float dx, dy, dz;
+            if (shared == 3) {
+                // same triangle: y - x = d1 e1 + d2 e2
+                dx = d1 * e1[0] + d2 * e2[0];
+                dy = d1 * e1[1] + d2 * e2[1];
+                dz = d1 * e1[2] + d2 * e2[2];
+            } else if (shared == 2) {
+                // common edge v0 v1 = w0 w1 (so f1 = e1): y - x = d1 e1 + y2 f2 - x2 e2
+                dx = d1 * e1[0] + y2 * f2[0] - x2 * e2[0];
+                dy = d1 * e1[1] + y2 * f2[1] - x2 * e2[1];
+                dz = d1 * e1[2] + y2 * f2[2] - x2 * e2[2];
+            } else {
+                // common vertex v0 = w0
+                dx = y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
+                dy = y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
+                dz = y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
+            }
             float vr, vi, gr = 0.f, gg = 0.f;
             kernel4pi<OP, CLAMP>(dx, dy, dz, nx0, nx1, nx2, ny0, ny1, ny2, k, krev, vr, vi, gr, gg);
             const float ww = reg[r][4];
@@ -588,37 +602,44 @@ static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, float k, 
     return hipGetLastError();
 }
 
+// parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour pairs
 template <int OP, bool CLAMP>
-static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int64_t row0, int64_t row1, float k, float2* out,
-                              int64_t ld, hipStream_t s)
+static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, int64_t row0, int64_t row1, float k,
+                              float2* out, int64_t ld, hipStream_t s)
 {
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (basis == DP0) {
-        if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, k, out, ld, s);
-        else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, k, out, ld, s);
-        else e = dp0_regular_t<OP, 1, CLAMP>(m, row0, row1, k, out, ld, s);
-        if (e != hipSuccess) return e;
+        if (parts & 1) {
+            if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, k, out, ld, s);
+            else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, k, out, ld, s);
+            else e = dp0_regular_t<OP, 1, CLAMP>(m, row0, row1, k, out, ld, s);
+            if (e != hipSuccess) return e;
+        }
+        if (!(parts & 2)) return hipSuccess;
         const int64_t p0 = m.h_pair_ptr[row0], p1 = m.h_pair_ptr[row1];
         return (quad == 0) ? singular_t<OP, 4, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s)
                            : singular_t<OP, 2, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s);
     }
-    e = launch_fill(out, m.nv, m.nv, ld, make_float2(0.f, 0.f), s);
-    if (e != hipSuccess) return e;
-    if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, s);
-    else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, s);
-    else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, s);
-    if (e != hipSuccess) return e;
+    if (parts & 1) {
+        e = launch_fill(out, m.nv, m.nv, ld, make_float2(0.f, 0.f), s);
+        if (e != hipSuccess) return e;
+        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, s);
+        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, s);
+        else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, s);
+        if (e != hipSuccess) return e;
+    }
+    if (!(parts & 2)) return hipSuccess;
     return (quad == 0) ? singular_t<OP, 4, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s)
                        : singular_t<OP, 2, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s);
 }
 
-hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int64_t row0, int64_t row1,
-                           float k, float2* out, int64_t ld, hipStream_t s)
+hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
+                           int64_t row1, float k, float2* out, int64_t ld, hipStream_t s)
 {
 #define BEM_OP_CASE(OPV)                                                                             \
     case OPV:                                                                                        \
-        return clamp ? assemble_op<OPV, true>(m, basis, quad, row0, row1, k, out, ld, s)           \
-                     : assemble_op<OPV, false>(m, basis, quad, row0, row1, k, out, ld, s);
+        return clamp ? assemble_op<OPV, true>(m, basis, quad, parts, row0, row1, k, out, ld, s)    \
+                     : assemble_op<OPV, false>(m, basis, quad, parts, row0, row1, k, out, ld, s);
     switch (op) {
         BEM_OP_CASE(SLP)
         BEM_OP_CASE(DLP)
@@ -642,6 +663,51 @@ hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld,
     else
         hipLaunchKernelGGL(identity_p1_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, m.geo, m.tris, m.vt_ptr, m.vt_idx,
                            m.nv, out, ld);
+    return hipGetLastError();
+}
+
+// out += alpha * M (mass matrix) in place: DP0 diagonal, P1 one thread per row
+__global__ __launch_bounds__(256) void add_identity_dp0_kernel(const float* __restrict__ geo, int64_t nt, float2 alpha,
+                                                               float2* __restrict__ out, int64_t ld)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const float a = 0.5f * geo[t * 16 + 12];
+    float2* d = &out[t * ld + t];
+    d->x += alpha.x * a;
+    d->y += alpha.y * a;
+}
+
+__global__ __launch_bounds__(256) void add_identity_p1_kernel(const float* __restrict__ geo,
+                                                              const int4* __restrict__ tris,
+                                                              const int32_t* __restrict__ vt_ptr,
+                                                              const int32_t* __restrict__ vt_idx, int64_t nv,
+                                                              float2 alpha, float2* __restrict__ out, int64_t ld)
+{
+    const int64_t va = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (va >= nv) return;
+    for (int e = vt_ptr[va]; e < vt_ptr[va + 1]; ++e) {
+        const int t = vt_idx[e];
+        const float area = 0.5f * geo[(int64_t)t * 16 + 12];
+        const int4 tv = tris[t];
+        const int vs[3] = {tv.x, tv.y, tv.z};
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float m = area * (vs[b] == va ? (2.0f / 12.0f) : (1.0f / 12.0f));
+            float2* d = &out[va * ld + vs[b]];
+            d->x += alpha.x * m;
+            d->y += alpha.y * m;
+        }
+    }
+}
+
+hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s)
+{
+    if (basis == DP0)
+        hipLaunchKernelGGL(add_identity_dp0_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, m.geo, m.nt, alpha, out, ld);
+    else
+        hipLaunchKernelGGL(add_identity_p1_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, m.geo, m.tris, m.vt_ptr,
+                           m.vt_idx, m.nv, alpha, out, ld);
     return hipGetLastError();
 }
 

@@ -178,6 +178,13 @@ static int check_op(int op, int basis, int quad)
 int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, void* out, int64_t ld,
                  void* stream)
 {
+    return bem_assemble_ex(mesh, k, op, basis, quad, clamp, 3, out, ld, stream);
+}
+
+int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
+                    int64_t ld, void* stream)
+{
+    if (parts < 1 || parts > 3) return fail(BEM_ERR_BAD_ARG, "parts must be 1, 2 or 3");
     if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
     if (int rc = check_op(op, basis, quad)) return rc;
     const auto& m = mesh->dev;
@@ -188,7 +195,7 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
         return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
     DeviceGuard g(m.device);
     if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
-    hipError_t e = bem::launch_assemble(m, op, basis, quad, clamp, 0, m.nt, k, (float2*)out, ld, S(stream));
+    hipError_t e = bem::launch_assemble(m, op, basis, quad, clamp, parts, 0, m.nt, k, (float2*)out, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_assemble launch");
     return BEM_OK;
 }
@@ -203,7 +210,8 @@ int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp
     if (ld < m.nt) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
     if ((op == BEM_OP_BM_RHS) && k == 0.f) return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
     DeviceGuard g(m.device);
-    hipError_t e = bem::launch_assemble(m, op, BEM_BASIS_DP0, quad, clamp, row0, row1, k, (float2*)out, ld, S(stream));
+    hipError_t e =
+        bem::launch_assemble(m, op, BEM_BASIS_DP0, quad, clamp, 3, row0, row1, k, (float2*)out, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_assemble_rows launch");
     return BEM_OK;
 }
@@ -232,6 +240,19 @@ int bem_identity(const bem_mesh* mesh, int basis, void* out, int64_t ld, void* s
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_identity(m, basis, (float2*)out, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_identity launch");
+    return BEM_OK;
+}
+
+int bem_add_identity(const bem_mesh* mesh, int basis, float alpha_re, float alpha_im, void* A, int64_t ld,
+                     void* stream)
+{
+    if (!mesh || !A) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    const auto& m = mesh->dev;
+    if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_add_identity(m, basis, make_float2(alpha_re, alpha_im), (float2*)A, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_add_identity launch");
     return BEM_OK;
 }
 

@@ -58,8 +58,9 @@ struct MeshDev {
 };
 
 hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s);
-hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int64_t row0, int64_t row1,
-                           float k, float2* out, int64_t ld, hipStream_t s);
+hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
+                           int64_t row1, float k, float2* out, int64_t ld, hipStream_t s);
+hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
 hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, float k, int op, int basis, int clamp,

@@ -63,8 +63,15 @@ def test_golden_octasphere0_all_operators():
             assert e < TOL, (op, k, e)
         dens = torch.tensor(g["density"], dtype=torch.complex64, device=_dev())
         pts = torch.tensor(g["points"], device=_dev())
-        assert _rel(s.single_potential(float(k), dens, pts).cpu().numpy(), g[f"spot_k{k}"]) < TOL
-        assert _rel(s.double_potential(float(k), dens, pts).cpu().numpy(), g[f"dpot_k{k}"]) < TOL
+        for op, key, fn in (("single", "spot", s.single_potential), ("double", "dpot", s.double_potential)):
+            # at k = 100 the phase k r ~ 200 is resolved to ~1e-5 in FP32, so the
+            # bar is the larger of 1e-5 and 1.5x the error of the FP32 mirror of
+            # the reference arithmetic on the same inputs
+            mirror = oracle.potential(g["vertices"], g["triangles"], g["points"], float(k), op, "P1",
+                                      precision="f32") @ g["density"]
+            bar = max(TOL, 1.5 * _rel(mirror, g[f"{key}_k{k}"]))
+            e = _rel(fn(float(k), dens, pts).cpu().numpy(), g[f"{key}_k{k}"])
+            assert e < bar, (op, k, e, bar)
 
 
 @pytest.mark.parametrize("quad", ["approx", "approx1"])
