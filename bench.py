@@ -13,9 +13,10 @@ One step = the three solves (assembly of K and V, GMRES, exterior potential).
 ``value`` = operator entries assembled (2 N^2 per wavenumber) per second of
 the whole timed region; ``assembly_entries_per_s`` counts assembly time only.
 
-The ``roofline`` object is for the dominant kernel, the regular-pair DP0
-double-layer assembly kernel: algorithmic FLOP per launch = 29 FLOP x 36
-Gauss-point pairs x (N^2 - neighbour pairs), divided by its mean duration
+The ``roofline`` object is for the dominant kernel, the fused regular-pair
+DP0 kernel that evaluates the double layer (stored as -K) and the single
+layer (contracted with psi): algorithmic FLOP per launch = (29 + 16) FLOP x
+36 Gauss-point pairs x (N^2 - neighbour pairs), divided by its mean duration
 from HIP events recorded on the stream it runs on.
 
 ``cpu_baseline``: the CPU oracle's FP32 mirror of the reference arithmetic
@@ -51,7 +52,6 @@ def parse():
     p.add_argument("--tol", type=float, default=1e-6)
     p.add_argument("--restart", type=int, default=200)
     p.add_argument("--maxiter", type=int, default=3000)
-    p.add_argument("--block-rows", type=int, default=8192)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--check-level", type=int, default=3, help="mesh level of the FP64 oracle check (0 = skip)")
@@ -78,7 +78,7 @@ def make_workload(level, seed=0):
 class Solver:
     """The plain-method CBIE pipeline on the C ABI (device resident)."""
 
-    def __init__(self, v, t, block_rows, clamp=True):
+    def __init__(self, v, t, clamp=True):
         import torch
 
         from neuralat_amd.bem.solver import Mesh
@@ -90,16 +90,15 @@ class Solver:
         self.mesh = Mesh(self.vt, self.tt)
         self.n = self.mesh.nt
         self.clamp = int(clamp)
-        self.block_rows = min(block_rows, self.n)
+        from neuralat_amd import _lib
+
         self.lhs = torch.empty((self.n, self.n), dtype=torch.complex64, device=self.dev)
-        self.blk = torch.empty((self.block_rows, self.n), dtype=torch.complex64, device=self.dev)
         self.rhs = torch.empty(self.n, dtype=torch.complex64, device=self.dev)
+        self.work = torch.empty(_lib.load().bem_cbie_work_size(self.mesh.handle), dtype=torch.uint8, device=self.dev)
         self.reg_ms = []
 
     def assemble(self, k, psi):
-        """LHS = M/2 - K in place, rhs = -V psi; returns assembly ms (events)."""
-        import ctypes
-
+        """lhs = M/2 - K, rhs = -V psi in one fused pass (V never stored)."""
         from neuralat_amd import _lib
         from neuralat_amd.bem.solver import _ptr, _stream
 
@@ -107,26 +106,19 @@ class Solver:
         lib = _lib.load()
         st = _stream()
         n = self.n
-        e0, e1, e2, e3 = (torch.cuda.Event(enable_timing=True) for _ in range(4))
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         e0.record()
-        _lib.check(lib.bem_assemble_ex(self.mesh.handle, float(k), 1, 1, 0, self.clamp, 1, _ptr(self.lhs), n, st))
+        _lib.check(lib.bem_cbie_dp0_ex(self.mesh.handle, float(k), 0, self.clamp, 1, _ptr(psi), _ptr(self.lhs), n,
+                                       _ptr(self.rhs), _ptr(self.work), st))
         e1.record()
-        _lib.check(lib.bem_assemble_ex(self.mesh.handle, float(k), 1, 1, 0, self.clamp, 2, _ptr(self.lhs), n, st))
-        _lib.check(lib.bem_combine(_ptr(self.lhs), n, None, n, _ptr(self.lhs), n, n, n, -1.0, 0.0, 0.0, 0.0, st))
-        _lib.check(lib.bem_add_identity(self.mesh.handle, 1, 0.5, 0.0, _ptr(self.lhs), n, st))
+        _lib.check(lib.bem_cbie_dp0_ex(self.mesh.handle, float(k), 0, self.clamp, 2, _ptr(psi), _ptr(self.lhs), n,
+                                       _ptr(self.rhs), _ptr(self.work), st))
         e2.record()
-        for r0 in range(0, n, self.block_rows):
-            r1 = min(n, r0 + self.block_rows)
-            _lib.check(lib.bem_assemble_rows(self.mesh.handle, float(k), 0, 0, self.clamp, r0, r1, _ptr(self.blk), n,
-                                             st))
-            out = ctypes.c_void_p(self.rhs.data_ptr() + 8 * r0)
-            _lib.check(lib.bem_matvec(_ptr(self.blk), r1 - r0, n, n, 0, _ptr(psi), out, -1.0, 0.0, 0.0, 0.0, None, st))
-        e3.record()
-        self._events = (e0, e1, e2, e3)
+        self._events = (e0, e1, e2)
 
     def timings(self):
-        e0, e1, e2, e3 = self._events
-        return e0.elapsed_time(e1), e0.elapsed_time(e2), e0.elapsed_time(e3)
+        e0, e1, e2 = self._events
+        return e0.elapsed_time(e1), e0.elapsed_time(e2)
 
     def solve(self, k, psi, pts_t, tol, restart, maxiter):
         from neuralat_amd.bem.solver import gmres
@@ -161,7 +153,7 @@ def fp64_check(level, ks):
     out = {}
     import torch
 
-    s = Solver(v32, t, 4096)
+    s = Solver(v32, t)
     for k in ks:
         val, dn = mesh.sources_field(src, None, k)
         psi = mesh.project_dp0(v32, t, lambda x, n: dn(x, n))
@@ -237,7 +229,7 @@ def main():
         fp64 = fp64_check(args.check_level, ks)
         log(f"bench: fp64 check {fp64}")
     log(f"bench: allocating {n}x{n} complex64 ({8 * n * n / 1e9:.1f} GB)")
-    s = Solver(v.astype(np.float32), t, args.block_rows)
+    s = Solver(v.astype(np.float32), t)
     dev = s.dev
     psis, fields = {}, {}
     for k in ks:
@@ -251,10 +243,10 @@ def main():
         res = []
         for k in ks:
             phi, p, info = s.solve(k, psis[k], pts_t, args.tol, args.restart, args.maxiter)
-            reg_ms, lhs_ms, asm_ms = s.timings()
+            reg_ms, asm_ms = s.timings()
             res.append((k, p, info, reg_ms, asm_ms))
             if record:
-                log(f"bench: k={k} regular DLP kernel {reg_ms:.1f} ms, assembly {asm_ms:.1f} ms, "
+                log(f"bench: k={k} fused regular kernel {reg_ms:.1f} ms, assembly {asm_ms:.1f} ms, "
                     f"GMRES its {info['iterations']} res {info['rel_residual']:.2e}")
         return res
 
@@ -280,7 +272,7 @@ def main():
     nsolves = args.steps * len(ks)
     entries = 2 * n * n * nsolves
     reg_mean_s = float(np.mean(reg)) / 1e3
-    flop = DLP_FLOP_PER_EVAL * 36 * (n * n - npairs)
+    flop = (DLP_FLOP_PER_EVAL + SLP_FLOP_PER_EVAL) * 36 * (n * n - npairs)
     achieved = flop / reg_mean_s
     cpu = None
     if not args.no_cpu:
@@ -307,10 +299,11 @@ def main():
         "gmres_iterations": its,
         "rel_err_vs_analytic": errs,
         "rel_err_vs_fp64_small_mesh": {"level": args.check_level, **fp64},
-        "roofline": {"bound": "valu", "kernel": "dp0_regular_kernel<DLP,6>", "achieved": achieved / 1e12,
+        "roofline": {"bound": "valu", "kernel": "dp0_cbie_kernel<6> (fused single+double layer)", "achieved": achieved / 1e12,
                      "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK, "traffic": None,
                      "flop_per_launch": flop, "mean_launch_ms": reg_mean_s * 1e3,
-                     "transcendentals_per_s": 4 * 36 * (n * n - npairs) / reg_mean_s},
+                     "transcendentals_per_s": 4 * 36 * (n * n - npairs) / reg_mean_s,
+                     "flop_count": "45 FLOP (single 16 + double 29) x 36 Gauss-point pairs x (N^2 - neighbour pairs)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

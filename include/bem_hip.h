@@ -102,6 +102,19 @@ int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp
 int bem_boundary_matrix(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, float k, int op, int quad,
                         void* out, void* stream);
 
+/* The DP0 CBIE system of solver.py:97-104 in one pass: lhs = M/2 - K
+ * ([nt, nt]) and rhs = -V psi ([nt]) for psi [nt] (complex64, device).  V
+ * is evaluated alongside K (shared distances and e^{ikr}) and contracted with
+ * psi on the fly, never stored.  work: bem_cbie_work_size() bytes of device
+ * scratch.  Deterministic (fixed summation order). */
+int64_t bem_cbie_work_size(const bem_mesh* mesh);
+int bem_cbie_dp0(const bem_mesh* mesh, float k, int quad, int clamp, const void* psi, void* lhs, int64_t ld,
+                 void* rhs, void* work, void* stream);
+/* bem_cbie_dp0 in two halves: parts = 1 the fused regular-pair kernel,
+ * 2 the neighbour pairs and the final reduction (after 1), 3 both. */
+int bem_cbie_dp0_ex(const bem_mesh* mesh, float k, int quad, int clamp, int parts, const void* psi, void* lhs,
+                    int64_t ld, void* rhs, void* work, void* stream);
+
 /* Mass matrix: P1 [nv,nv] (bind.cu:10-21), DP0 diag(area) [nt,nt]. */
 int bem_identity(const bem_mesh* mesh, int basis, void* out, int64_t ld, void* stream);
 /* A += alpha * M in place (M the mass matrix of the basis), e.g. the 0.5 M

@@ -64,6 +64,9 @@ SIGNATURES = {
     "bem_mesh_normals_curls": (_I32, [_P, _P, _P, _P]),
     "bem_assemble": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_assemble_ex": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_cbie_work_size": (_I64, [_P]),
+    "bem_cbie_dp0": (_I32, [_P, _F, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
+    "bem_cbie_dp0_ex": (_I32, [_P, _F, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
     "bem_add_identity": (_I32, [_P, _I32, _F, _F, _P, _I64, _P]),
     "bem_assemble_rows": (_I32, [_P, _F, _I32, _I32, _I32, _I64, _I64, _P, _I64, _P]),
     "bem_boundary_matrix": (_I32, [_P, _I64, _P, _I64, _F, _I32, _I32, _P, _P]),

@@ -249,7 +249,11 @@ class BEM_Solver:
         return x
 
     def CBIE(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):
-        """Conventional BIE (solver.py:97-104): (M/2 - K) phi = -V psi."""
+        """Conventional BIE (solver.py:97-104): (M/2 - K) phi = -V psi.  For
+        DP0 the system is built in one fused pass (V psi without storing V)."""
+        if self.basis == "DP0":
+            LHS, RHS = self.cbie_system(k, neumann)
+            return self._solve(LHS, RHS, solver, tol, restart, maxiter)
         slp = self.assemble_boundary_matrix(k, "single")
         RHS = matvec(slp, neumann.to(torch.complex64).contiguous(), alpha=-1.0)
         del slp
@@ -257,6 +261,21 @@ class BEM_Solver:
         scale_(LHS, -1.0)
         add_identity_(self, LHS, 0.5)
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
+
+    def cbie_system(self, k, neumann, approx=False, out=None):
+        """(M/2 - K, -V psi) for the DP0 basis in one fused pass."""
+        if self.basis != "DP0":
+            raise ValueError("the fused CBIE system is built for the DP0 basis")
+        n = self.mesh.nt
+        psi = neumann.to(torch.complex64).contiguous()
+        lhs = torch.empty((n, n), dtype=torch.complex64, device=self.device) if out is None else out
+        rhs = torch.empty(n, dtype=torch.complex64, device=self.device)
+        lib = _lib.load()
+        work = torch.empty(lib.bem_cbie_work_size(self.mesh.handle), dtype=torch.uint8, device=self.device)
+        quad = "approx_1" if approx == "approx_1" else ("approx" if approx else "full")
+        _lib.check(lib.bem_cbie_dp0(self.mesh.handle, float(k), _lib.QUADS[quad], int(self.clamp), _ptr(psi),
+                                    _ptr(lhs), lhs.stride(0), _ptr(rhs), _ptr(work), _stream()), "bem_cbie_dp0")
+        return lhs, rhs
 
     def HBIE(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):
         """Burton-Miller combined BIE (solver.py:106-116), beta = i/k:

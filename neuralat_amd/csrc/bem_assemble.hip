@@ -58,25 +58,194 @@ __global__ __launch_bounds__(256) void dp0_regular_kernel(const float* __restric
     const int64_t i_end = min(i_begin + (int64_t)kDp0RowTile, row1);
     for (int64_t i = i_begin; i < i_end; ++i) {
         const float* rr = rec + i * S;  // wave-uniform
+        const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
         float accr = 0.f, acci = 0.f;
+        if constexpr (OP == SLP || OP == DLP || OP == ADLP) {
+            // Specialised evaluation.  The normal factor is constant along one
+            // triangle: (y - x).n_y = (y_0 - x_a).n_y depends on the row point only
+            // (DLP), (y - x).n_x = (y_b - x_0).n_x on the column point only (ADLP);
+            // both are formed from a point difference first to stay accurate for
+            // nearly coplanar pairs.
+            float qb[Q];
+            if constexpr (OP == ADLP) {
 #pragma unroll
-        for (int a = 0; a < Q; ++a) {
-            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
-            float sr = 0.f, si = 0.f;
-#pragma unroll
-            for (int b = 0; b < Q; ++b) {
-                float vr, vi, tr = 0.f, ti = 0.f;
-                kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
-                                     rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, tr, ti);
-                sr = fmaf(wy[b], vr, sr);
-                si = fmaf(wy[b], vi, si);
+                for (int b = 0; b < Q; ++b)
+                    qb[b] = wy[b] * ((py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                                     (py[3 * b + 2] - rr[2]) * nx2);
             }
-            const float ua = rr[3 * Q + a];
-            accr = fmaf(ua, sr, accr);
-            acci = fmaf(ua, si, acci);
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (CLAMP) r2 = fmaxf(r2, kEps2);  // max(r, eps) = sqrt(max(r^2, eps^2))
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * rinv;
+                    float sn, cs;
+                    sincos_rev(r * krev, sn, cs);
+                    if constexpr (OP == SLP) {
+                        const float g = rinv * wy[b];
+                        sr = fmaf(g, cs, sr);
+                        si = fmaf(g, sn, si);
+                    } else {
+                        const float th = r * k;
+                        const float re = fmaf(th, sn, cs), im = fmaf(-th, cs, sn);
+                        const float p = rinv * rinv * (rinv * (OP == DLP ? wy[b] : qb[b]));
+                        sr = fmaf(p, re, sr);
+                        si = fmaf(p, im, si);
+                    }
+                }
+                float ua = rr[3 * Q + a];
+                if constexpr (OP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
+                    ua *= (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
+                else if constexpr (OP == ADLP)  // -G (1 - ikr)(x - y).n_x / r^2 = +G (...) (y - x).n_x / r^2
+                    ;
+                accr = fmaf(ua, sr, accr);
+                acci = fmaf(ua, si, acci);
+            }
+        } else {
+#pragma unroll
+            for (int a = 0; a < Q; ++a) {
+                const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    float vr, vi, tr = 0.f, ti = 0.f;
+                    kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, nx0, nx1, nx2,
+                                         ny0, ny1, ny2, k, krev, vr, vi, tr, ti);
+                    sr = fmaf(wy[b], vr, sr);
+                    si = fmaf(wy[b], vi, si);
+                }
+                const float ua = rr[3 * Q + a];
+                accr = fmaf(ua, sr, accr);
+                acci = fmaf(ua, si, acci);
+            }
         }
         if (active) out[(i - row0) * ld + j] = make_float2(accr, acci);
     }
+}
+
+// ---------------------------------------------------------------------------
+// fused DP0 CBIE system: lhs = -K (regular pairs) and partial sums of V psi
+// ---------------------------------------------------------------------------
+// The single- and double-layer kernels share the point difference, r and
+// e^{ikr}; the single-layer entries are contracted with psi on the fly and
+// never stored.  Pairs sharing a vertex are left out of the V psi sums (their
+// entries come from the singular kernel); their lhs entries are overwritten
+// by it.  partial[cb * nt + i] = sum over the 256 columns of block cb.
+__device__ __forceinline__ float wave_sum_f32(float v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int Q, bool CLAMP>
+__global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
+                                                       int64_t nt, float k, const float2* __restrict__ psi,
+                                                       float2* __restrict__ lhs, int64_t ld,
+                                                       float2* __restrict__ partial)
+{
+    constexpr int S = Rec<Q>::kFloats;
+    __shared__ float2 red[4][kDp0RowTile];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < nt;
+    const int64_t jj = active ? j : nt - 1;
+    const float* cr = rec + jj * S;
+    float py[3 * Q], wy[Q];
+#pragma unroll
+    for (int b = 0; b < 3 * Q; ++b) py[b] = cr[b];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
+    const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
+    const int4 tj = tris[jj];
+    const float2 pj = active ? psi[jj] : make_float2(0.f, 0.f);
+    const float krev = k * kInv2Pi;
+
+    const int64_t i_begin = (int64_t)blockIdx.y * kDp0RowTile;
+    const int64_t i_end = min(i_begin + (int64_t)kDp0RowTile, nt);
+    for (int64_t i = i_begin; i < i_end; ++i) {
+        const float* rr = rec + i * S;  // wave-uniform
+        float kr = 0.f, ki = 0.f, vr = 0.f, vi = 0.f;
+#pragma unroll
+        for (int a = 0; a < Q; ++a) {
+            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+            float dr = 0.f, di = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int b = 0; b < Q; ++b) {
+                const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+                float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (CLAMP) r2 = fmaxf(r2, kEps2);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                float sn, cs;
+                sincos_rev(r * krev, sn, cs);
+                const float g = rinv * wy[b];
+                sr = fmaf(g, cs, sr);
+                si = fmaf(g, sn, si);
+                const float th = r * k;
+                const float p = g * rinv * rinv;
+                dr = fmaf(p, fmaf(th, sn, cs), dr);
+                di = fmaf(p, fmaf(-th, cs, sn), di);
+            }
+            const float ua = rr[3 * Q + a];
+            const float un = ua * ((xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2);
+            kr = fmaf(un, dr, kr);
+            ki = fmaf(un, di, ki);
+            vr = fmaf(ua, sr, vr);
+            vi = fmaf(ua, si, vi);
+        }
+        if (active) lhs[i * ld + j] = make_float2(-kr, -ki);
+        const int4 ti = tris[i];
+        const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
+                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z;
+        float tr = shared ? 0.f : vr * pj.x - vi * pj.y;
+        float tim = shared ? 0.f : vr * pj.y + vi * pj.x;
+        tr = wave_sum_f32(tr);
+        tim = wave_sum_f32(tim);
+        if (lane == 0) red[wave][i - i_begin] = make_float2(tr, tim);
+    }
+    __syncthreads();
+    const int64_t nrow = i_end - i_begin;
+    if (threadIdx.x < nrow) {
+        float2 a = red[0][threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            a.x += red[w][threadIdx.x].x;
+            a.y += red[w][threadIdx.x].y;
+        }
+        partial[(int64_t)blockIdx.x * nt + i_begin + threadIdx.x] = a;
+    }
+}
+
+// rhs_i = -(sum of the column-block partials + sum of the singular V psi
+// terms of row i); lhs_ii += |T_i| / 2 (the M/2 of the CBIE).  One thread per
+// row, fixed summation order.
+__global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __restrict__ partial, int64_t nblk,
+                                                            const float2* __restrict__ vpsi,
+                                                            const int64_t* __restrict__ pair_ptr,
+                                                            const float* __restrict__ geo, int64_t nt,
+                                                            float2* __restrict__ lhs, int64_t ld,
+                                                            float2* __restrict__ rhs)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt) return;
+    double sr = 0.0, si = 0.0;
+    for (int64_t b = 0; b < nblk; ++b) {
+        const float2 p = partial[b * nt + i];
+        sr += p.x;
+        si += p.y;
+    }
+    for (int64_t e = pair_ptr[i]; e < pair_ptr[i + 1]; ++e) {
+        sr += vpsi[e].x;
+        si += vpsi[e].y;
+    }
+    rhs[i] = make_float2((float)-sr, (float)-si);
+    lhs[i * ld + i].x += 0.25f * geo[i * 16 + 12];  // M_ii / 2 = |T_i| / 2 = |J_i| / 4
 }
 
 // ---------------------------------------------------------------------------
@@ -225,11 +394,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-template <int OP, int NL, int BASIS, bool CLAMP>
+// MODE 0: write (DP0: out = alpha * entry) or add (P1) the entries;
+// MODE 1 (DP0): vpsi[p] = entry * psi[j], the pair's term of V psi.
+template <int OP, int NL, int BASIS, bool CLAMP, int MODE = 0>
 __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__ geo, const int4* __restrict__ tris,
                                                        const float* __restrict__ curls,
                                                        const int4* __restrict__ pairs, int64_t p0, int64_t p1,
-                                                       float k, float2* __restrict__ out, int64_t ld, int64_t row0)
+                                                       float k, float2* __restrict__ out, int64_t ld, int64_t row0,
+                                                       float alpha, const float2* __restrict__ psi,
+                                                       float2* __restrict__ vpsi)
 {
     const int lane = threadIdx.x & 63;
     const int64_t p = p0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -399,8 +572,14 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
         ar[e] = wave_sum(ar[e]) * (double)kInv4Pi;
         ai[e] = wave_sum(ai[e]) * (double)kInv4Pi;
     }
-    if constexpr (BASIS == DP0) {
-        if (lane == 0) out[((int64_t)i - row0) * ld + j] = make_float2((float)ar[9], (float)ai[9]);
+    if constexpr (MODE == 1) {
+        if (lane == 0) {
+            const float2 pj = psi[j];
+            vpsi[p] = make_float2((float)(ar[9] * pj.x - ai[9] * pj.y), (float)(ar[9] * pj.y + ai[9] * pj.x));
+        }
+    } else if constexpr (BASIS == DP0) {
+        if (lane == 0)
+            out[((int64_t)i - row0) * ld + j] = make_float2((float)(alpha * ar[9]), (float)(alpha * ai[9]));
     } else {
         if (lane < 9) {
             // lane = reordered (a, b) pair; its original local indices are (pi[a], pj[b])
@@ -598,8 +777,49 @@ static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, float k, 
     if (p1 <= p0) return hipSuccess;
     unsigned g = (unsigned)((p1 - p0 + 3) / 4);
     hipLaunchKernelGGL((singular_kernel<OP, NL, BASIS, CLAMP>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
-                       m.pairs, p0, p1, k, out, ld, row0);
+                       m.pairs, p0, p1, k, out, ld, row0, 1.0f, nullptr, nullptr);
     return hipGetLastError();
+}
+
+// ---- fused DP0 CBIE system -------------------------------------------------
+int64_t cbie_work_bytes(const MeshDev& m)
+{
+    const int64_t nblk = (m.nt + 255) / 256;
+    return (int64_t)sizeof(float2) * (nblk * m.nt + m.npairs);
+}
+
+template <int Q, int NL, bool CLAMP>
+static hipError_t cbie_t(const MeshDev& m, float k, const float2* psi, float2* lhs, int64_t ld, float2* rhs,
+                         void* work, int parts, hipStream_t s)
+{
+    const int64_t nblk = (m.nt + 255) / 256;
+    float2* partial = (float2*)work;
+    float2* vpsi = partial + nblk * m.nt;
+    const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
+    dim3 grid((unsigned)nblk, (unsigned)((m.nt + kDp0RowTile - 1) / kDp0RowTile));
+    if (parts & 1)
+        hipLaunchKernelGGL((dp0_cbie_kernel<Q, CLAMP>), grid, dim3(256), 0, s, rec, m.tris, m.nt, k, psi, lhs, ld,
+                           partial);
+    if (!(parts & 2)) return hipGetLastError();
+    const unsigned g = (unsigned)((m.npairs + 3) / 4);
+    hipLaunchKernelGGL((singular_kernel<DLP, NL, DP0, CLAMP, 0>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
+                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, -1.0f, nullptr, nullptr);
+    hipLaunchKernelGGL((singular_kernel<SLP, NL, DP0, CLAMP, 1>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
+                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, 1.0f, psi, vpsi);
+    hipLaunchKernelGGL(cbie_finalize_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, partial, nblk, vpsi, m.pair_ptr,
+                       m.geo, m.nt, lhs, ld, rhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
+                           int64_t ld, float2* rhs, void* work, hipStream_t s)
+{
+    if (quad == 0) return clamp ? cbie_t<6, 4, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
+                                : cbie_t<6, 4, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
+    if (quad == 1) return clamp ? cbie_t<3, 2, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
+                                : cbie_t<3, 2, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
+    return clamp ? cbie_t<1, 2, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
+                 : cbie_t<1, 2, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
 }
 
 // parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour pairs

@@ -55,6 +55,7 @@ struct DeviceGuard {
 };
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+constexpr int64_t kDp0RowTileHost = bem::kDp0RowTile;
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
@@ -253,6 +254,35 @@ int bem_add_identity(const bem_mesh* mesh, int basis, float alpha_re, float alph
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_add_identity(m, basis, make_float2(alpha_re, alpha_im), (float2*)A, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_add_identity launch");
+    return BEM_OK;
+}
+
+int64_t bem_cbie_work_size(const bem_mesh* mesh)
+{
+    if (!mesh) return 0;
+    return bem::cbie_work_bytes(mesh->dev);
+}
+
+int bem_cbie_dp0(const bem_mesh* mesh, float k, int quad, int clamp, const void* psi, void* lhs, int64_t ld,
+                 void* rhs, void* work, void* stream)
+{
+    return bem_cbie_dp0_ex(mesh, k, quad, clamp, 3, psi, lhs, ld, rhs, work, stream);
+}
+
+int bem_cbie_dp0_ex(const bem_mesh* mesh, float k, int quad, int clamp, int parts, const void* psi, void* lhs,
+                    int64_t ld, void* rhs, void* work, void* stream)
+{
+    if (!mesh || !psi || !lhs || !rhs || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (parts < 1 || parts > 3) return fail(BEM_ERR_BAD_ARG, "parts must be 1, 2 or 3");
+    if (quad < 0 || quad > 2) return fail(BEM_ERR_BAD_ARG, "unknown quadrature");
+    const auto& m = mesh->dev;
+    if (ld < m.nt) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (!std::isfinite(k)) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+    if ((m.nt + kDp0RowTileHost - 1) / kDp0RowTileHost > 65535) return fail(BEM_ERR_BAD_ARG, "mesh too large");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_cbie_dp0(m, k, quad, clamp, parts, (const float2*)psi, (float2*)lhs, ld, (float2*)rhs,
+                                        work, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_cbie_dp0 launch");
     return BEM_OK;
 }
 

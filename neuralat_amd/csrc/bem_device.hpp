@@ -17,6 +17,7 @@ enum Basis : int { P1 = 0, DP0 = 1 };
 
 constexpr float kEps = 1e-3f;        // potential.h:16
 constexpr float kInvEps = 1e3f;
+constexpr float kEps2 = 1e-6f;
 constexpr float kInv4Pi = 0.0795774715459476678844f;
 constexpr float kInv2Pi = 0.159154943091895335769f;
 
@@ -93,14 +94,9 @@ __device__ __forceinline__ void kernel4pi(float dx, float dy, float dz, float nx
                                           float& si)
 {
     float r2 = dx * dx + dy * dy + dz * dz;
-    float rinv = __builtin_amdgcn_rsqf(r2);
-    float r;
-    if (CLAMP) {
-        rinv = fminf(rinv, kInvEps);
-        r = fmaxf(r2 * rinv, kEps);
-    } else {
-        r = r2 * rinv;
-    }
+    if (CLAMP) r2 = fmaxf(r2, kEps2);  // max(r, eps) = sqrt(max(r^2, eps^2))
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
     float s, c;
     sincos_rev(r * krev, s, c);
     if constexpr (OP == SLP || OP == HYP) {

@@ -47,6 +47,7 @@ struct MeshDev {
     float* curls = nullptr;  // [nt*12]: rows n x grad(phi_a), a = 0..2 (9 used)
     float* geo = nullptr;    // [nt*16]: v0 v1 v2 (9), unit normal (3), |J| (1)
     int4* pairs = nullptr;   // [npairs]: (i, j, shared vertices, 0), sorted by (i, j)
+    int64_t* pair_ptr = nullptr;  // [nt+1] pair range of each row triangle
     int64_t npairs = 0;
     int32_t* vt_ptr = nullptr;  // [nv+1] vertex -> triangle CSR (ascending)
     int32_t* vt_idx = nullptr;
@@ -60,6 +61,9 @@ struct MeshDev {
 hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s);
 hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
                            int64_t row1, float k, float2* out, int64_t ld, hipStream_t s);
+int64_t cbie_work_bytes(const MeshDev& m);
+hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
+                           int64_t ld, float2* rhs, void* work, hipStream_t s);
 hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);

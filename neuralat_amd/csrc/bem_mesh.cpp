@@ -295,6 +295,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({curls.data(), curls.size() * 4, (void**)&m.curls});
     parts.push_back({geo.data(), geo.size() * 4, (void**)&m.geo});
     parts.push_back({pairs.data(), pairs.size() * 4, (void**)&m.pairs});
+    parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
     parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});
     int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv;

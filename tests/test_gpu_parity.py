@@ -133,6 +133,35 @@ def test_dp0_operators_vs_oracle(op, clamp):
     assert e < TOL, (op, clamp, e)
 
 
+@pytest.mark.parametrize("approx", [False, True])
+@pytest.mark.parametrize("clamp", [True, False])
+def test_fused_cbie_system_matches_separate_operators(approx, clamp):
+    from neuralat_amd.bem import matvec
+
+    v, t = mesh.icosphere(3, 0.1)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=7).astype(np.float32)
+    k = 18.3
+    s = _solver(v, t, "DP0", clamp=clamp)
+    rng = np.random.default_rng(4)
+    psi = torch.tensor((rng.standard_normal(len(t)) + 1j * rng.standard_normal(len(t))).astype(np.complex64),
+                       device=_dev())
+    lhs, rhs = s.cbie_system(k, psi, approx=approx)
+    S = s.assemble_boundary_matrix(k, "single", approx=approx)
+    D = s.assemble_boundary_matrix(k, "double", approx=approx)
+    M = s.identity_matrix()
+    ref_lhs = (0.5 * M - D).cpu().numpy()
+    ref_rhs = matvec(S, psi, alpha=-1.0).cpu().numpy()
+    assert _rel(lhs.cpu().numpy(), ref_lhs) < 1e-6
+    assert _rel(rhs.cpu().numpy(), ref_rhs) < 1e-6
+    # and against the FP64 oracle
+    quad = "approx" if approx else "full"
+    S64 = oracle.assemble(v, t, k, "single", "DP0", quad=quad, clamp=clamp)
+    D64 = oracle.assemble(v, t, k, "double", "DP0", quad=quad, clamp=clamp)
+    M64 = oracle.identity(v, t, "DP0")
+    assert _rel(lhs.cpu().numpy(), 0.5 * M64 - D64) < TOL
+    assert _rel(rhs.cpu().numpy(), -S64 @ psi.cpu().numpy().astype(np.complex128)) < TOL
+
+
 def test_dp0_row_blocks_match_full():
     from neuralat_amd import _lib
     from neuralat_amd.bem.solver import _ptr, _stream
