@@ -157,6 +157,44 @@ int64_t bem_bicgstab_work_size(int64_t n);
 int bem_bicgstab(const void* A, int64_t n, int64_t lda, const void* b, void* x, int nsteps, double tol, double atol,
                  void* work, bem_solve_info* info, void* stream);
 
+/* ---- mixed precision (no reference counterpart; the reference is FP32 only) ----
+ * A complex64 matrix is stored as two 16-bit planes Ar, Ai (dtype 1 = bf16,
+ * 2 = fp16) holding round(scale * A), with a leading dimension ldlp that is a
+ * multiple of 32 (bem_lowp_ld); columns beyond cols are zero.  The 16-RHS
+ * products run on the matrix cores with FP32 accumulation. */
+int64_t bem_lowp_ld(int64_t cols);
+/* scale <= 0 picks it: 1 for bf16, the power of two putting max|A| at 2^14
+ * for fp16; the value used is returned in *scale_used. */
+int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int dtype, float scale, void* Ar,
+                     void* Ai, int64_t ldlp, float* scale_used, void* stream);
+/* Y = alpha A X + beta Y for nrhs right-hand sides (X: nrhs systems of cols
+ * entries at stride ldx; Y likewise with ldy), complex64 A on FP32 matrix
+ * cores. */
+int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const void* X, int64_t ldx, int nrhs, void* Y,
+               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* stream);
+/* Same on the 16-bit planes (result multiplied by inv_scale = 1/scale). */
+int64_t bem_matmul_lowp_work_size(int64_t ldlp);
+int bem_matmul_lowp(const void* Ar, const void* Ai, int dtype, int64_t rows, int64_t cols, int64_t ldlp,
+                    float inv_scale, const void* X, int64_t ldx, int nrhs, void* Y, int64_t ldy, float alpha_re,
+                    float alpha_im, float beta_re, float beta_im, void* work, void* stream);
+/* GMRES on nrhs systems sharing one matrix (complex64 A when dtype = 0, else
+ * the 16-bit planes), in lockstep: one multi-RHS product per Arnoldi step,
+ * per-system Gram-Schmidt, Givens and convergence on the device.  info:
+ * nrhs entries. */
+int64_t bem_gmres_multi_work_size(int64_t n, int nrhs, int restart, int64_t ldlp);
+int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp,
+                    float inv_scale, int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol,
+                    int restart, int maxiter, void* work, bem_solve_info* info, void* stream);
+/* Mixed-precision iterative refinement: x = 0; repeat { solve the 16-bit
+ * system for the correction to inner_tol with bem_gmres_multi; x += d;
+ * r = b - A x with the complex64 A } until ||r|| <= tol ||b|| for every
+ * system.  info[q].iterations = total inner iterations. */
+int64_t bem_solve_ir_work_size(int64_t n, int nrhs, int restart, int64_t ldlp);
+int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp, float inv_scale,
+                 int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol, float inner_tol,
+                 int restart, int max_inner, int max_outer, void* work, bem_solve_info* info,
+                 int32_t* outer_iterations, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -80,6 +80,18 @@ SIGNATURES = {
     "bem_gmres": (_I32, [_P, _I64, _I64, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
     "bem_bicgstab_work_size": (_I64, [_I64]),
     "bem_bicgstab": (_I32, [_P, _I64, _I64, _P, _P, _I32, _D, _D, _P, ctypes.POINTER(SolveInfo), _P]),
+    "bem_lowp_ld": (_I64, [_I64]),
+    "bem_convert_lowp": (_I32, [_P, _I64, _I64, _I64, _I32, _F, _P, _P, _I64, ctypes.POINTER(_F), _P]),
+    "bem_matmul": (_I32, [_P, _I64, _I64, _I64, _P, _I64, _I32, _P, _I64, _F, _F, _F, _F, _P]),
+    "bem_matmul_lowp_work_size": (_I64, [_I64]),
+    "bem_matmul_lowp": (_I32, [_P, _P, _I32, _I64, _I64, _I64, _F, _P, _I64, _I32, _P, _I64, _F, _F, _F, _F, _P,
+                               _P]),
+    "bem_gmres_multi_work_size": (_I64, [_I64, _I32, _I32, _I64]),
+    "bem_gmres_multi": (_I32, [_P, _I64, _P, _P, _I32, _I64, _F, _I64, _I32, _P, _I64, _P, _I64, _F, _I32, _I32, _P,
+                               ctypes.POINTER(SolveInfo), _P]),
+    "bem_solve_ir_work_size": (_I64, [_I64, _I32, _I32, _I64]),
+    "bem_solve_ir": (_I32, [_P, _I64, _P, _P, _I32, _I64, _F, _I64, _I32, _P, _I64, _P, _I64, _F, _F, _I32, _I32,
+                            _I32, _P, ctypes.POINTER(SolveInfo), ctypes.POINTER(ctypes.c_int32), _P]),
 }
 
 _lib = None

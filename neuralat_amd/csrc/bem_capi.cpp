@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -9,6 +10,8 @@
 #include "../../include/bem_hip.h"
 #include "bem_internal.hpp"
 #include "bem_linalg.hpp"
+#include "bem_lowp.hpp"
+#include "bem_msolve.hpp"
 
 struct bem_mesh {
     bem::MeshDev dev;
@@ -525,6 +528,175 @@ int bem_bicgstab(const void* A_, int64_t n, int64_t lda, const void* b_, void* x
     inf.converged = conv ? 1 : 0;
     inf.rel_residual = bn2 > 0 ? std::sqrt(rr / bn2) : 0.0;
     if (info) *info = inf;
+    return BEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// mixed precision: 16-bit storage, matrix-core products, refinement
+// ---------------------------------------------------------------------------
+int64_t bem_lowp_ld(int64_t cols) { return cols <= 0 ? 0 : (cols + 31) / 32 * 32; }
+
+int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int dtype, float scale, void* Ar,
+                     void* Ai, int64_t ldlp, float* scale_used, void* stream)
+{
+    if (!A || !Ar || !Ai) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (dtype != bem::kDtBF16 && dtype != bem::kDtFP16) return fail(BEM_ERR_BAD_ARG, "dtype must be BF16 or FP16");
+    if (rows <= 0 || cols <= 0 || lda < cols || ldlp < bem_lowp_ld(cols) || ldlp % 32)
+        return fail(BEM_ERR_BAD_ARG, "bad shape (ldlp must be a multiple of 32 >= cols)");
+    hipStream_t st = S(stream);
+    if (!(scale > 0.f)) {
+        if (dtype == bem::kDtBF16) {
+            scale = 1.f;
+        } else {
+            // largest power of two keeping max |A| * scale <= 2^14 (fp16 max 65504)
+            const int nb = 1024;
+            float* part = nullptr;
+            HIP_TRY(hipMallocAsync((void**)&part, sizeof(float) * nb, st));
+            HIP_TRY(bem::launch_absmax((const float2*)A, rows, cols, lda, part, nb, st));
+            std::vector<float> hp(nb);
+            HIP_TRY(hipMemcpyAsync(hp.data(), part, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipFreeAsync(part, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            float mx = 0.f;
+            for (float v : hp) mx = std::max(mx, v);
+            if (!std::isfinite(mx)) return fail(BEM_ERR_BAD_ARG, "matrix has non-finite entries");
+            scale = mx > 0.f ? std::ldexp(1.f, (int)std::floor(std::log2(16384.0 / mx))) : 1.f;
+        }
+    }
+    if (scale_used) *scale_used = scale;
+    HIP_TRY(bem::launch_to_lowp(dtype, (const float2*)A, rows, cols, lda, scale, (uint16_t*)Ar, (uint16_t*)Ai, ldlp,
+                                st));
+    return BEM_OK;
+}
+
+int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const void* X, int64_t ldx, int nrhs, void* Y,
+               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* stream)
+{
+    if (!A || !X || !Y) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (rows <= 0 || cols <= 0 || lda < cols || nrhs <= 0 || ldx < cols || ldy < rows)
+        return fail(BEM_ERR_BAD_ARG, "bad shape");
+    for (int s0 = 0; s0 < nrhs; s0 += 16) {
+        const int cnt = std::min(16, nrhs - s0);
+        HIP_TRY(bem::launch_gemm_c64((const float2*)A, rows, cols, lda, (const float2*)X + (int64_t)s0 * ldx, ldx, cnt,
+                                     (float2*)Y + (int64_t)s0 * ldy, ldy, make_float2(alpha_re, alpha_im),
+                                     make_float2(beta_re, beta_im), S(stream)));
+    }
+    return BEM_OK;
+}
+
+int64_t bem_matmul_lowp_work_size(int64_t ldlp) { return (int64_t)bem::lowp_scratch_bytes(ldlp); }
+
+int bem_matmul_lowp(const void* Ar, const void* Ai, int dtype, int64_t rows, int64_t cols, int64_t ldlp,
+                    float inv_scale, const void* X, int64_t ldx, int nrhs, void* Y, int64_t ldy, float alpha_re,
+                    float alpha_im, float beta_re, float beta_im, void* work, void* stream)
+{
+    if (!Ar || !Ai || !X || !Y || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (dtype != bem::kDtBF16 && dtype != bem::kDtFP16) return fail(BEM_ERR_BAD_ARG, "dtype must be BF16 or FP16");
+    if (rows <= 0 || cols <= 0 || ldlp < bem_lowp_ld(cols) || ldlp % 32 || nrhs <= 0 || ldx < cols || ldy < rows)
+        return fail(BEM_ERR_BAD_ARG, "bad shape");
+    uint16_t* Xr = (uint16_t*)work;
+    uint16_t* Xi = Xr + 32 * ldlp;
+    hipStream_t st = S(stream);
+    if (nrhs == 1) {
+        HIP_TRY(bem::launch_gemv_lowp(dtype, (const uint16_t*)Ar, (const uint16_t*)Ai, rows, cols, ldlp, inv_scale,
+                                      (const float2*)X, (float2*)Y, make_float2(alpha_re, alpha_im),
+                                      make_float2(beta_re, beta_im), st));
+        return BEM_OK;
+    }
+    for (int s0 = 0; s0 < nrhs; s0 += 16) {
+        const int cnt = std::min(16, nrhs - s0);
+        HIP_TRY(bem::launch_vec_to_lowp(dtype, (const float2*)X + (int64_t)s0 * ldx, cols, ldx, cnt, Xr, Xi, ldlp, st));
+        HIP_TRY(bem::launch_gemm_lowp(dtype, (const uint16_t*)Ar, (const uint16_t*)Ai, rows, ldlp, inv_scale, Xr, Xi,
+                                      ldlp, cnt, (float2*)Y + (int64_t)s0 * ldy, ldy, make_float2(alpha_re, alpha_im),
+                                      make_float2(beta_re, beta_im), st));
+    }
+    return BEM_OK;
+}
+
+static int make_op(bem::MultiOp& op, const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype,
+                   int64_t ldlp, float inv_scale, int64_t n, void* scratch)
+{
+    op.n = n;
+    op.dtype = dtype;
+    if (dtype == 0) {
+        if (!A || lda < n) return fail(BEM_ERR_BAD_ARG, "complex64 matrix missing or lda < n");
+        op.A = (const float2*)A;
+        op.lda = lda;
+        return BEM_OK;
+    }
+    if (dtype != bem::kDtBF16 && dtype != bem::kDtFP16) return fail(BEM_ERR_BAD_ARG, "unknown dtype");
+    if (!Ar || !Ai || ldlp < bem_lowp_ld(n) || ldlp % 32) return fail(BEM_ERR_BAD_ARG, "16-bit planes missing/bad ldlp");
+    op.Ar = (const uint16_t*)Ar;
+    op.Ai = (const uint16_t*)Ai;
+    op.ldlp = ldlp;
+    op.inv_scale = inv_scale;
+    op.Xr = (uint16_t*)scratch;  // Xr (16 ldlp) + its low plane (16 ldlp), then Xi likewise
+    op.Xi = op.Xr + 32 * ldlp;
+    return BEM_OK;
+}
+
+int64_t bem_gmres_multi_work_size(int64_t n, int nrhs, int restart, int64_t ldlp)
+{
+    if (n <= 0 || nrhs <= 0 || restart <= 0) return 0;
+    return (int64_t)(bem::gmres_multi_bytes(n, nrhs, restart) + bem::lowp_scratch_bytes(std::max<int64_t>(ldlp, 32)));
+}
+
+int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp,
+                    float inv_scale, int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol,
+                    int restart, int maxiter, void* work, bem_solve_info* info, void* stream)
+{
+    if (!B || !X || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (n <= 0 || nrhs <= 0 || restart <= 0 || maxiter < 0 || ldb < n || ldx < n || !(tol > 0.f))
+        return fail(BEM_ERR_BAD_ARG, "bad argument");
+    bem::MultiOp op;
+    char* scratch = (char*)work + bem::gmres_multi_bytes(n, nrhs, restart);
+    if (int rc = make_op(op, A, lda, Ar, Ai, dtype, ldlp, inv_scale, n, scratch)) return rc;
+    bem::MultiInfo mi;
+    HIP_TRY(bem::gmres_multi(op, n, nrhs, (const float2*)B, ldb, (float2*)X, ldx, tol, restart, maxiter, work, &mi,
+                             S(stream)));
+    if (info)
+        for (int q = 0; q < nrhs; ++q) {
+            info[q].iterations = mi.iterations;
+            info[q].converged = mi.rel_residual[q] <= tol ? 1 : 0;
+            info[q].rel_residual = mi.rel_residual[q];
+            info[q].implicit_rel_residual = 0.0;
+        }
+    if (!mi.converged) return fail(BEM_ERR_NOT_CONVERGED, "multi-RHS GMRES reached maxiter");
+    return BEM_OK;
+}
+
+int64_t bem_solve_ir_work_size(int64_t n, int nrhs, int restart, int64_t ldlp)
+{
+    if (n <= 0 || nrhs <= 0 || restart <= 0) return 0;
+    return (int64_t)(bem::solve_ir_bytes(n, nrhs, restart) + bem::lowp_scratch_bytes(std::max<int64_t>(ldlp, 32)));
+}
+
+int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp, float inv_scale,
+                 int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol, float inner_tol,
+                 int restart, int max_inner, int max_outer, void* work, bem_solve_info* info,
+                 int32_t* outer_iterations, void* stream)
+{
+    if (!B || !X || !work || !A) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (n <= 0 || nrhs <= 0 || restart <= 0 || max_inner < 1 || max_outer < 1 || ldb < n || ldx < n ||
+        !(tol > 0.f) || !(inner_tol > 0.f))
+        return fail(BEM_ERR_BAD_ARG, "bad argument");
+    if (dtype != bem::kDtBF16 && dtype != bem::kDtFP16) return fail(BEM_ERR_BAD_ARG, "inner dtype must be BF16/FP16");
+    bem::MultiOp hi, lo;
+    char* scratch = (char*)work + bem::solve_ir_bytes(n, nrhs, restart);
+    if (int rc = make_op(hi, A, lda, nullptr, nullptr, 0, 0, 1.f, n, nullptr)) return rc;
+    if (int rc = make_op(lo, nullptr, 0, Ar, Ai, dtype, ldlp, inv_scale, n, scratch)) return rc;
+    bem::IrInfo ir;
+    HIP_TRY(bem::solve_ir(hi, lo, n, nrhs, (const float2*)B, ldb, (float2*)X, ldx, tol, inner_tol, restart, max_inner,
+                          max_outer, work, &ir, S(stream)));
+    if (outer_iterations) *outer_iterations = ir.outer;
+    if (info)
+        for (int q = 0; q < nrhs; ++q) {
+            info[q].iterations = ir.inner;
+            info[q].converged = ir.rel_residual[q] <= tol ? 1 : 0;
+            info[q].rel_residual = ir.rel_residual[q];
+            info[q].implicit_rel_residual = 0.0;
+        }
+    if (!ir.converged) return fail(BEM_ERR_NOT_CONVERGED, "iterative refinement reached max_outer");
     return BEM_OK;
 }
 

@@ -417,6 +417,204 @@ __global__ __launch_bounds__(256) void bicg_x_kernel(float2* __restrict__ x, con
 }
 
 // ---------------------------------------------------------------------------
+// batched Krylov kernels: ns systems in lockstep; vectors stored system-major
+// (system s of basis vector q at V + (q * ns + s) * ldn)
+// ---------------------------------------------------------------------------
+// h[s * hs + q] = <V_q,s, w_s> for q < nq; grid (nq, ns)
+__global__ __launch_bounds__(256) void bmultidot_kernel(const float2* __restrict__ V, int64_t ldn, int ns, int64_t n,
+                                                        const float2* __restrict__ w, double2* __restrict__ h, int hs,
+                                                        const int* __restrict__ done)
+{
+    __shared__ double sh[8];
+    const int q = blockIdx.x, sy = blockIdx.y;
+    if (done[sy]) return;  // uniform per workgroup
+    const float2* v = V + ((int64_t)q * ns + sy) * ldn;
+    const float2* ww = w + (int64_t)sy * ldn;
+    double sr = 0.0, si = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += 256) {
+        const float2 a = v[e], b = ww[e];
+        sr += (double)a.x * b.x + (double)a.y * b.y;
+        si += (double)a.x * b.y - (double)a.y * b.x;
+    }
+    sr = block_sum_d(sr, sh);
+    si = block_sum_d(si, sh);
+    if (threadIdx.x == 0) h[(int64_t)sy * hs + q] = make_double2(sr, si);
+}
+
+// w_s -= sum_q h[s][q] V_q,s ; grid (ceil(n/256), ns)
+__global__ __launch_bounds__(256) void bgs_update_kernel(const float2* __restrict__ V, int64_t ldn, int ns, int64_t n,
+                                                         int nq, const double2* __restrict__ h, int hs,
+                                                         float2* __restrict__ w, const int* __restrict__ done)
+{
+    const int sy = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (done[sy] || e >= n) return;
+    float2* ww = w + (int64_t)sy * ldn;
+    double wr = ww[e].x, wi = ww[e].y;
+    for (int q = 0; q < nq; ++q) {
+        const double2 hq = h[(int64_t)sy * hs + q];
+        const float2 v = V[((int64_t)q * ns + sy) * ldn + e];
+        wr -= hq.x * v.x - hq.y * v.y;
+        wi -= hq.x * v.y + hq.y * v.x;
+    }
+    ww[e] = make_float2((float)wr, (float)wi);
+}
+
+// out[s] = ||w_s||^2 ; grid ns
+__global__ __launch_bounds__(1024) void bnorm2_kernel(const float2* __restrict__ w, int64_t ldn, int64_t n,
+                                                      double* __restrict__ out)
+{
+    __shared__ double sh[16];
+    const int sy = blockIdx.x;
+    const float2* ww = w + (int64_t)sy * ldn;
+    double s = 0.0;
+    for (int64_t e = threadIdx.x; e < n; e += 1024) {
+        const float2 a = ww[e];
+        s += (double)a.x * a.x + (double)a.y * a.y;
+    }
+    s = block_sum_d(s, sh);
+    if (threadIdx.x == 0) out[sy] = s;
+}
+
+// dst_s = src_s / sqrt(nrm2[s]) ; grid (ceil(n/256), ns)
+__global__ __launch_bounds__(256) void bnormalize_kernel(const float2* __restrict__ src, int64_t lds, int64_t n,
+                                                         const double* __restrict__ nrm2, float2* __restrict__ dst,
+                                                         int64_t ldd, const int* __restrict__ done)
+{
+    const int sy = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n || (done && done[sy])) return;
+    const double nn = nrm2[sy];
+    const float sc = nn > 0.0 ? (float)(1.0 / sqrt(nn)) : 0.f;
+    const float2 v = src[(int64_t)sy * lds + e];
+    dst[(int64_t)sy * ldd + e] = make_float2(v.x * sc, v.y * sc);
+}
+
+// Per-system restart set-up: g = (beta, 0...), done = (beta <= target), nk = 0.
+__global__ void binit_kernel(double2* __restrict__ g, int m, const double* __restrict__ nrm2,
+                             const double* __restrict__ target, int* __restrict__ done, int* __restrict__ nk, int ns)
+{
+    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sy >= ns) return;
+    const double beta = sqrt(nrm2[sy]);
+    for (int e = 0; e <= m; ++e) g[(int64_t)sy * (m + 1) + e] = make_double2(e == 0 ? beta : 0.0, 0.0);
+    done[sy] = beta <= target[sy] ? 1 : 0;
+    nk[sy] = 0;
+}
+
+// Per-system Arnoldi bookkeeping (one thread per system): as gmres_givens_kernel,
+// plus the convergence test |g_{j+1}| <= target -> done, nk = j + 1.
+__global__ void bgivens_kernel(double2* __restrict__ H, int m, int j, const double2* __restrict__ h,
+                               const double2* __restrict__ h2, int hs, const double* __restrict__ wn2,
+                               double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+                               double* __restrict__ res, const double* __restrict__ target, int* __restrict__ done,
+                               int* __restrict__ nk, int ns)
+{
+    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sy >= ns || done[sy]) return;
+    double2* col = H + (int64_t)sy * (m + 1) * m + (int64_t)j * (m + 1);
+    const double2* hh = h + (int64_t)sy * hs;
+    const double2* hh2 = h2 + (int64_t)sy * hs;
+    double2* c_ = cs + (int64_t)sy * m;
+    double2* s_ = sn + (int64_t)sy * m;
+    double2* gg = g + (int64_t)sy * (m + 1);
+    for (int q = 0; q <= j; ++q) col[q] = make_double2(hh[q].x + hh2[q].x, hh[q].y + hh2[q].y);
+    col[j + 1] = make_double2(sqrt(wn2[sy]), 0.0);
+    for (int q = 0; q < j; ++q) {
+        const double2 c = c_[q], s = s_[q], a = col[q], b = col[q + 1];
+        col[q] = make_double2(c.x * a.x - c.y * a.y + s.x * b.x - s.y * b.y, c.x * a.y + c.y * a.x + s.x * b.y + s.y * b.x);
+        col[q + 1] = make_double2(-(s.x * a.x + s.y * a.y) + (c.x * b.x + c.y * b.y),
+                                  -(s.x * a.y - s.y * a.x) + (c.x * b.y - c.y * b.x));
+    }
+    const double2 a = col[j], b = col[j + 1];
+    const double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
+    double2 c = make_double2(1.0, 0.0), s = make_double2(0.0, 0.0);
+    if (den > 0.0) {
+        c = make_double2(a.x / den, -a.y / den);
+        s = make_double2(b.x / den, -b.y / den);
+    }
+    c_[j] = c;
+    s_[j] = s;
+    col[j] = make_double2(den, 0.0);
+    col[j + 1] = make_double2(0.0, 0.0);
+    const double2 gj = gg[j];
+    gg[j + 1] = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+    gg[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
+    const double rj = sqrt(gg[j + 1].x * gg[j + 1].x + gg[j + 1].y * gg[j + 1].y);
+    res[sy] = rj;
+    nk[sy] = j + 1;
+    if (rj <= target[sy]) done[sy] = 1;
+}
+
+// y_s = R_s^{-1} g_s[0:nk_s] ; one thread per system
+__global__ void bbacksolve_kernel(const double2* __restrict__ H, int m, const int* __restrict__ nk,
+                                  const double2* __restrict__ g, double2* __restrict__ y, int ns)
+{
+    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sy >= ns) return;
+    const double2* Hs = H + (int64_t)sy * (m + 1) * m;
+    const double2* gs = g + (int64_t)sy * (m + 1);
+    double2* ys = y + (int64_t)sy * m;
+    const int n_k = nk[sy];
+    for (int q = n_k - 1; q >= 0; --q) {
+        double2 acc = gs[q];
+        for (int c = q + 1; c < n_k; ++c) {
+            const double2 hqc = Hs[(int64_t)c * (m + 1) + q], yc = ys[c];
+            acc.x -= hqc.x * yc.x - hqc.y * yc.y;
+            acc.y -= hqc.x * yc.y + hqc.y * yc.x;
+        }
+        const double2 d = Hs[(int64_t)q * (m + 1) + q];
+        const double dd = d.x * d.x + d.y * d.y;
+        ys[q] = dd > 0.0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
+                         : make_double2(0.0, 0.0);
+    }
+}
+
+// x_s += sum_{q < nk_s} y_s[q] V_q,s ; grid (ceil(n/256), ns)
+__global__ __launch_bounds__(256) void bupdate_x_kernel(const float2* __restrict__ V, int64_t ldn, int ns, int64_t n,
+                                                        const int* __restrict__ nk, const double2* __restrict__ y,
+                                                        int m, float2* __restrict__ x, int64_t ldx)
+{
+    const int sy = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float2* xs = x + (int64_t)sy * ldx;
+    double xr = xs[e].x, xi = xs[e].y;
+    const int n_k = nk[sy];
+    for (int q = 0; q < n_k; ++q) {
+        const double2 yq = y[(int64_t)sy * m + q];
+        const float2 v = V[((int64_t)q * ns + sy) * ldn + e];
+        xr += yq.x * v.x - yq.y * v.y;
+        xi += yq.x * v.y + yq.y * v.x;
+    }
+    xs[e] = make_float2((float)xr, (float)xi);
+}
+
+// dst_s[e] = src_s[e] (strided copies of ns vectors)
+__global__ __launch_bounds__(256) void bcopy_kernel(const float2* __restrict__ src, int64_t lds, float2* __restrict__ dst,
+                                                    int64_t ldd, int64_t n)
+{
+    const int sy = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    dst[(int64_t)sy * ldd + e] = src[(int64_t)sy * lds + e];
+}
+
+// x_s += d_s
+__global__ __launch_bounds__(256) void badd_kernel(float2* __restrict__ x, int64_t ldx, const float2* __restrict__ d,
+                                                   int64_t ldd, int64_t n)
+{
+    const int sy = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n) return;
+    float2 a = x[(int64_t)sy * ldx + e];
+    const float2 b = d[(int64_t)sy * ldd + e];
+    a.x += b.x;
+    a.y += b.y;
+    x[(int64_t)sy * ldx + e] = a;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 static inline unsigned g1(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
@@ -531,6 +729,69 @@ hipError_t launch_bicg_axpy(const float2* a, const float2* b, const double2* sc,
 hipError_t launch_bicg_x(float2* x, const float2* p, const float2* sv, const double2* sc, int64_t n, hipStream_t s)
 {
     hipLaunchKernelGGL(bicg_x_kernel, dim3(g1(n)), dim3(256), 0, s, x, p, sv, sc, n);
+    return hipGetLastError();
+}
+
+// ---- batched launchers ----
+static inline dim3 gns(int64_t n, int ns) { return dim3((unsigned)((n + 255) / 256), (unsigned)ns); }
+
+hipError_t launch_bmultidot(const float2* V, int64_t ldn, int ns, int64_t n, int nq, const float2* w, double2* h,
+                            int hs, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL(bmultidot_kernel, dim3(nq, ns), dim3(256), 0, s, V, ldn, ns, n, w, h, hs, done);
+    return hipGetLastError();
+}
+hipError_t launch_bgs_update(const float2* V, int64_t ldn, int ns, int64_t n, int nq, const double2* h, int hs,
+                             float2* w, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL(bgs_update_kernel, gns(n, ns), dim3(256), 0, s, V, ldn, ns, n, nq, h, hs, w, done);
+    return hipGetLastError();
+}
+hipError_t launch_bnorm2(const float2* w, int64_t ldn, int64_t n, int ns, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(bnorm2_kernel, dim3(ns), dim3(1024), 0, s, w, ldn, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_bnormalize(const float2* src, int64_t lds, int64_t n, int ns, const double* nrm2, float2* dst,
+                             int64_t ldd, const int* done, hipStream_t s)
+{
+    hipLaunchKernelGGL(bnormalize_kernel, gns(n, ns), dim3(256), 0, s, src, lds, n, nrm2, dst, ldd, done);
+    return hipGetLastError();
+}
+hipError_t launch_binit(double2* g, int m, const double* nrm2, const double* target, int* done, int* nk, int ns,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(binit_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, g, m, nrm2, target, done, nk, ns);
+    return hipGetLastError();
+}
+hipError_t launch_bgivens(double2* H, int m, int j, const double2* h, const double2* h2, int hs, const double* wn2,
+                          double2* cs, double2* sn, double2* g, double* res, const double* target, int* done, int* nk,
+                          int ns, hipStream_t s)
+{
+    hipLaunchKernelGGL(bgivens_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, H, m, j, h, h2, hs, wn2, cs, sn, g, res,
+                       target, done, nk, ns);
+    return hipGetLastError();
+}
+hipError_t launch_bbacksolve(const double2* H, int m, const int* nk, const double2* g, double2* y, int ns,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(bbacksolve_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, H, m, nk, g, y, ns);
+    return hipGetLastError();
+}
+hipError_t launch_bupdate_x(const float2* V, int64_t ldn, int ns, int64_t n, const int* nk, const double2* y, int m,
+                            float2* x, int64_t ldx, hipStream_t s)
+{
+    hipLaunchKernelGGL(bupdate_x_kernel, gns(n, ns), dim3(256), 0, s, V, ldn, ns, n, nk, y, m, x, ldx);
+    return hipGetLastError();
+}
+hipError_t launch_bcopy(const float2* src, int64_t lds, float2* dst, int64_t ldd, int64_t n, int ns, hipStream_t s)
+{
+    hipLaunchKernelGGL(bcopy_kernel, gns(n, ns), dim3(256), 0, s, src, lds, dst, ldd, n);
+    return hipGetLastError();
+}
+hipError_t launch_badd(float2* x, int64_t ldx, const float2* d, int64_t ldd, int64_t n, int ns, hipStream_t s)
+{
+    hipLaunchKernelGGL(badd_kernel, gns(n, ns), dim3(256), 0, s, x, ldx, d, ldd, n);
     return hipGetLastError();
 }
 

@@ -31,4 +31,24 @@ hipError_t launch_bicg_axpy(const float2* a, const float2* b, const double2* sc,
                             int64_t n, hipStream_t s);
 hipError_t launch_bicg_x(float2* x, const float2* p, const float2* sv, const double2* sc, int64_t n, hipStream_t s);
 
+// batched Krylov kernels (ns systems, system-major vectors)
+hipError_t launch_bmultidot(const float2* V, int64_t ldn, int ns, int64_t n, int nq, const float2* w, double2* h,
+                            int hs, const int* done, hipStream_t s);
+hipError_t launch_bgs_update(const float2* V, int64_t ldn, int ns, int64_t n, int nq, const double2* h, int hs,
+                             float2* w, const int* done, hipStream_t s);
+hipError_t launch_bnorm2(const float2* w, int64_t ldn, int64_t n, int ns, double* out, hipStream_t s);
+hipError_t launch_bnormalize(const float2* src, int64_t lds, int64_t n, int ns, const double* nrm2, float2* dst,
+                             int64_t ldd, const int* done, hipStream_t s);
+hipError_t launch_binit(double2* g, int m, const double* nrm2, const double* target, int* done, int* nk, int ns,
+                        hipStream_t s);
+hipError_t launch_bgivens(double2* H, int m, int j, const double2* h, const double2* h2, int hs, const double* wn2,
+                          double2* cs, double2* sn, double2* g, double* res, const double* target, int* done, int* nk,
+                          int ns, hipStream_t s);
+hipError_t launch_bbacksolve(const double2* H, int m, const int* nk, const double2* g, double2* y, int ns,
+                             hipStream_t s);
+hipError_t launch_bupdate_x(const float2* V, int64_t ldn, int ns, int64_t n, const int* nk, const double2* y, int m,
+                            float2* x, int64_t ldx, hipStream_t s);
+hipError_t launch_bcopy(const float2* src, int64_t lds, float2* dst, int64_t ldd, int64_t n, int ns, hipStream_t s);
+hipError_t launch_badd(float2* x, int64_t ldx, const float2* d, int64_t ldd, int64_t n, int ns, hipStream_t s);
+
 }  // namespace bem

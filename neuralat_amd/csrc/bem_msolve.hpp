@@ -1,0 +1,49 @@
+// Multi-right-hand-side GMRES and mixed-precision iterative refinement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace bem {
+
+// A dense operator applied to blocks of right-hand sides: complex64
+// (dtype 0) or 16-bit planes (dtype kDtBF16 / kDtFP16, holding scale * A).
+struct MultiOp {
+    int dtype = 0;
+    int64_t n = 0;
+    const float2* A = nullptr;
+    int64_t lda = 0;
+    const uint16_t* Ar = nullptr;
+    const uint16_t* Ai = nullptr;
+    int64_t ldlp = 0;
+    float inv_scale = 1.f;
+    uint16_t* Xr = nullptr;  // scratch for the 16-bit right-hand sides (16 x ldlp each)
+    uint16_t* Xi = nullptr;
+    hipError_t apply(const float2* X, int64_t ldx, int ns, float2* Y, int64_t ldy, float2 alpha, float2 beta,
+                     hipStream_t s) const;
+};
+
+struct MultiInfo {
+    int iterations = 0;
+    int converged = 0;
+    std::vector<double> rel_residual;
+};
+
+struct IrInfo {
+    int outer = 0;
+    int inner = 0;
+    int converged = 0;
+    std::vector<double> rel_residual;
+};
+
+size_t lowp_scratch_bytes(int64_t ldlp);
+size_t gmres_multi_bytes(int64_t n, int ns, int m);
+hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, int64_t ldb, float2* X, int64_t ldx,
+                       double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s);
+size_t solve_ir_bytes(int64_t n, int ns, int m);
+hipError_t solve_ir(const MultiOp& hi, const MultiOp& lo, int64_t n, int ns, const float2* B, int64_t ldb, float2* X,
+                    int64_t ldx, double tol, double inner_tol, int m, int max_inner, int max_outer, void* work,
+                    IrInfo* info, hipStream_t s);
+
+}  // namespace bem
