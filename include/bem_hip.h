@@ -195,6 +195,43 @@ int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int
                  int restart, int max_inner, int max_outer, void* work, bem_solve_info* info,
                  int32_t* outer_iterations, void* stream);
 
+/* ---- batched items (no reference counterpart; the reference solves one
+ * (mesh, k) item at a time, src/scene.py:270-291) ---- */
+typedef struct {
+    const bem_mesh* mesh;
+    float k;
+    const void* psi; /* [nt] complex64 Neumann data */
+    void* lhs;       /* [nt, ld] complex64 out: M/2 - K */
+    int64_t ld;
+    void* rhs;       /* [nt] complex64 out: -V psi */
+} bem_batch_item;
+/* The DP0 CBIE systems of n_items items (any sizes, any wavenumbers) with the
+ * regular pairs of all items in ONE persistent launch over a device-side
+ * queue of (item, row tile, 256-column block) tasks, largest items first.
+ * row_tile: rows per task (16/32/64/128) or 0 for the value bem_batch_autotune
+ * stored for this size class (64 if none). */
+int64_t bem_batch_cbie_work_size(const bem_batch_item* items, int n_items);
+int bem_batch_cbie(const bem_batch_item* items, int n_items, int clamp, int row_tile, void* work, void* stream);
+/* Times the four row tiles on the largest item, keeps the fastest for its
+ * size class (process-wide, mutex-guarded), returns it and the times. */
+int bem_batch_autotune(const bem_batch_item* items, int n_items, int clamp, void* work, int* best_row_tile,
+                       float* ms4, void* stream);
+typedef struct {
+    const void* A; /* [n, lda] complex64 */
+    int64_t lda;
+    int64_t n;
+    const void* b; /* [n] */
+    void* x;       /* [n] initial guess in, solution out */
+} bem_batch_system;
+/* GMRES(restart) on all systems in lockstep (different matrices and sizes),
+ * one restart cycle per hipGraph replay when use_graph != 0; stops when every
+ * system has ||b - A x|| <= tol ||b|| or after max_cycles cycles.  info[q].
+ * iterations = cycles x restart (an upper bound per system). */
+int64_t bem_batch_gmres_work_size(const bem_batch_system* sys, int n_sys, int restart);
+int bem_batch_gmres(const bem_batch_system* sys, int n_sys, float tol, int restart, int max_cycles, int use_graph,
+                    void* work, bem_solve_info* info, void* stream);
+const char* bem_batch_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

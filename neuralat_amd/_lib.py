@@ -92,6 +92,12 @@ SIGNATURES = {
     "bem_solve_ir_work_size": (_I64, [_I64, _I32, _I32, _I64]),
     "bem_solve_ir": (_I32, [_P, _I64, _P, _P, _I32, _I64, _F, _I64, _I32, _P, _I64, _P, _I64, _F, _F, _I32, _I32,
                             _I32, _P, ctypes.POINTER(SolveInfo), ctypes.POINTER(ctypes.c_int32), _P]),
+    "bem_batch_cbie_work_size": (_I64, [_P, _I32]),
+    "bem_batch_cbie": (_I32, [_P, _I32, _I32, _I32, _P, _P]),
+    "bem_batch_autotune": (_I32, [_P, _I32, _I32, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_F), _P]),
+    "bem_batch_gmres_work_size": (_I64, [_P, _I32, _I32]),
+    "bem_batch_gmres": (_I32, [_P, _I32, _F, _I32, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
+    "bem_batch_last_error": (ctypes.c_char_p, []),
 }
 
 _lib = None

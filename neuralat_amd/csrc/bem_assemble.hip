@@ -143,16 +143,17 @@ __device__ __forceinline__ float wave_sum_f32(float v)
     return v;
 }
 
-template <int Q, bool CLAMP>
-__global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
-                                                       int64_t nt, float k, const float2* __restrict__ psi,
-                                                       float2* __restrict__ lhs, int64_t ld,
-                                                       float2* __restrict__ partial)
+// One task: rows [by * RT, by * RT + RT) x columns [bx * 256, bx * 256 + 256).
+// red: LDS scratch of 4 x RT complex.  Called by all 256 threads.
+template <int Q, bool CLAMP, int RT>
+__device__ __forceinline__ void cbie_tile(const float* __restrict__ rec, const int4* __restrict__ tris, int64_t nt,
+                                          float k, const float2* __restrict__ psi, float2* __restrict__ lhs,
+                                          int64_t ld, float2* __restrict__ partial, int64_t bx, int64_t by,
+                                          float2 (*red)[RT])
 {
     constexpr int S = Rec<Q>::kFloats;
-    __shared__ float2 red[4][kDp0RowTile];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j = bx * 256 + threadIdx.x;
     const bool active = j < nt;
     const int64_t jj = active ? j : nt - 1;
     const float* cr = rec + jj * S;
@@ -166,8 +167,8 @@ __global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__
     const float2 pj = active ? psi[jj] : make_float2(0.f, 0.f);
     const float krev = k * kInv2Pi;
 
-    const int64_t i_begin = (int64_t)blockIdx.y * kDp0RowTile;
-    const int64_t i_end = min(i_begin + (int64_t)kDp0RowTile, nt);
+    const int64_t i_begin = by * RT;
+    const int64_t i_end = min(i_begin + (int64_t)RT, nt);
     for (int64_t i = i_begin; i < i_end; ++i) {
         const float* rr = rec + i * S;  // wave-uniform
         float kr = 0.f, ki = 0.f, vr = 0.f, vi = 0.f;
@@ -218,7 +219,90 @@ __global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__
             a.x += red[w][threadIdx.x].x;
             a.y += red[w][threadIdx.x].y;
         }
-        partial[(int64_t)blockIdx.x * nt + i_begin + threadIdx.x] = a;
+        partial[bx * nt + i_begin + threadIdx.x] = a;
+    }
+}
+
+template <int Q, bool CLAMP>
+__global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
+                                                       int64_t nt, float k, const float2* __restrict__ psi,
+                                                       float2* __restrict__ lhs, int64_t ld,
+                                                       float2* __restrict__ partial)
+{
+    __shared__ float2 red[4][kDp0RowTile];
+    cbie_tile<Q, CLAMP, kDp0RowTile>(rec, tris, nt, k, psi, lhs, ld, partial, blockIdx.x, blockIdx.y, red);
+}
+
+// ---------------------------------------------------------------------------
+// batched (mesh, k) items: one persistent launch over a queue of tiles
+// ---------------------------------------------------------------------------
+// Workgroups pull task indices from a device counter until the queue is
+// empty; task t belongs to the item whose task0 is the largest <= t and is
+// the tile (t - task0) % nbx (columns), (t - task0) / nbx (rows) of its fused
+// CBIE system.  Items are ordered largest first, so the long tiles start early.
+template <int Q, bool CLAMP, int RT>
+__global__ __launch_bounds__(256) void batch_cbie_persistent(const BatchItemDev* __restrict__ items, int nitems,
+                                                             int64_t total_tasks,
+                                                             unsigned long long* __restrict__ counter)
+{
+    __shared__ float2 red[4][RT];
+    __shared__ long long task_sh;
+    while (true) {
+        __syncthreads();
+        if (threadIdx.x == 0) task_sh = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long t = task_sh;
+        if (t >= total_tasks) break;  // the same value for the whole workgroup
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].task0 <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchItemDev it = items[lo];
+        const int64_t local = t - it.task0;
+        const int64_t bx = local % it.nbx, by = local / it.nbx;
+        cbie_tile<Q, CLAMP, RT>(it.rec, it.tris, it.nt, it.k, it.psi, it.lhs, it.ld, it.partial, bx, by, red);
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
+template <int RT>
+static hipError_t batch_persistent_t(const BatchItemDev* items, int nitems, int64_t total, unsigned long long* counter,
+                                     int clamp, hipStream_t s)
+{
+    int dev = 0, ncu = 0, per = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    if (clamp)
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_cbie_persistent<6, true, RT>, 256, 0);
+    else
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, batch_cbie_persistent<6, false, RT>, 256, 0);
+    if (ncu <= 0) ncu = 256;
+    if (per <= 0) per = 2;
+    int64_t grid = (int64_t)ncu * per;
+    if (grid > total) grid = total;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, counter);
+    if (clamp)
+        hipLaunchKernelGGL((batch_cbie_persistent<6, true, RT>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems,
+                           total, counter);
+    else
+        hipLaunchKernelGGL((batch_cbie_persistent<6, false, RT>), dim3((unsigned)grid), dim3(256), 0, s, items, nitems,
+                           total, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_cbie_persistent(const BatchItemDev* items, int nitems, int64_t total, int rt,
+                                        unsigned long long* counter, int clamp, hipStream_t s)
+{
+    switch (rt) {
+    case 16: return batch_persistent_t<16>(items, nitems, total, counter, clamp, s);
+    case 32: return batch_persistent_t<32>(items, nitems, total, counter, clamp, s);
+    case 64: return batch_persistent_t<64>(items, nitems, total, counter, clamp, s);
+    case 128: return batch_persistent_t<128>(items, nitems, total, counter, clamp, s);
+    default: return hipErrorInvalidValue;
     }
 }
 

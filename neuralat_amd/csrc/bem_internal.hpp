@@ -61,6 +61,22 @@ struct MeshDev {
 hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s);
 hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
                            int64_t row1, float k, float2* out, int64_t ld, hipStream_t s);
+// One (mesh, k) item of a batched fused-CBIE assembly (device-resident table).
+struct BatchItemDev {
+    const float* rec;   // 6-point records of the mesh
+    const int4* tris;
+    int64_t nt;
+    float k;
+    int pad;
+    const float2* psi;  // [nt]
+    float2* lhs;        // [nt, ld]
+    int64_t ld;
+    float2* partial;    // [ceil(nt/256), nt]
+    int64_t task0;      // first task of the item in the queue
+    int64_t nbx;        // 256-column blocks
+};
+hipError_t launch_batch_cbie_persistent(const BatchItemDev* items, int nitems, int64_t total, int rt,
+                                        unsigned long long* counter, int clamp, hipStream_t s);
 int64_t cbie_work_bytes(const MeshDev& m);
 hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
                            int64_t ld, float2* rhs, void* work, hipStream_t s);
