@@ -150,6 +150,10 @@ int bem_combine(const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
 int64_t bem_gmres_work_size(int64_t n, int restart);
 int bem_gmres(const void* A, int64_t n, int64_t lda, const void* b, void* x, float tol, int restart, int maxiter,
               void* work, bem_solve_info* info, void* stream);
+/* bem_gmres on op(A): trans 0 = A, 1 = A^T, 2 = A^H (e.g. the adjoint
+ * solve of a gradient, reusing the assembled matrix). */
+int bem_gmres_ex(const void* A, int64_t n, int64_t lda, int trans, const void* b, void* x, float tol, int restart,
+                 int maxiter, void* work, bem_solve_info* info, void* stream);
 /* Reference-compatible BiCGSTAB (BiCGSTAB.py:5-107): bilinear rho, alpha,
  * omega; stops when ||b - A x||^2 < tol ||b||^2 or < atol; stops silently
  * (returns BEM_OK, info->converged = 0) when nsteps run out or rho == 0. */
@@ -194,6 +198,22 @@ int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int
                  int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol, float inner_tol,
                  int restart, int max_inner, int max_outer, void* work, bem_solve_info* info,
                  int32_t* outer_iterations, void* stream);
+
+/* ---- differentiation (no reference counterpart; the reference has no
+ * gradients).  For the DP0 CBIE pressure p = D_pot phi - S_pot psi with
+ * (M/2 - K) phi = -V psi and a real loss L with pbar = dL/dp* (half of torch's
+ * complex gradient) and lambda solving (M/2 - K)^H lambda = D_pot^H pbar, these
+ * ADD into FP64 accumulators:
+ *   grad_vertices [nv,3], grad_k [1]: the parts of dL/dv, dL/dk through K, V,
+ *     M (bem_adjoint_cbie_dp0) and through S_pot, D_pot (bem_adjoint_potential_dp0);
+ *   vhl [nt] (complex, as 2 doubles): V^H lambda (needed for dL/dpsi).
+ * Derivative blocks are recomputed per Gauss-point pair, never stored. */
+int64_t bem_adjoint_work_size(const bem_mesh* mesh);
+int bem_adjoint_cbie_dp0(const bem_mesh* mesh, float k, int clamp, const void* lam, const void* psi, const void* phi,
+                         double* grad_vertices, double* grad_k, double* vhl, void* work, void* stream);
+int bem_adjoint_potential_dp0(const bem_mesh* mesh, const float* points, int64_t np, float k, int clamp,
+                              const void* pbar, const void* psi, const void* phi, double* grad_vertices,
+                              double* grad_k, void* stream);
 
 /* ---- batched items (no reference counterpart; the reference solves one
  * (mesh, k) item at a time, src/scene.py:270-291) ---- */

@@ -1,0 +1,128 @@
+"""Differentiable exterior pressure of the DP0 CBIE solve.
+
+``exterior_pressure(vertices, triangles, k, psi, points)`` returns
+p = D_pot phi - S_pot psi with (M/2 - K) phi = -V psi (the reference's
+CBIE + boundary2potential, src/bem/solver.py:97-104,140-147) as a
+``torch.autograd.Function``: gradients flow to the vertex positions, to k and
+to the Neumann data psi.  The backward pass solves the adjoint system
+(M/2 - K)^H lambda = D_pot^H pbar with GMRES on the stored matrix and runs
+the library's adjoint kernels, which recompute every derivative block on the
+fly.  The reference has no gradients; this is checked against central finite
+differences and the FP64 oracle in tests/test_gpu_grad.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+
+from .. import _lib
+from .solver import Mesh, _ptr, _stream, gmres, matvec
+
+_RULE6 = ((0.445948490915965, 0.445948490915965, 0.223381589678011),
+          (0.091576213509771, 0.091576213509771, 0.109951743655322),
+          (1 - 2 * 0.445948490915965, 0.445948490915965, 0.223381589678011),
+          (0.445948490915965, 1 - 2 * 0.445948490915965, 0.223381589678011),
+          (1 - 2 * 0.091576213509771, 0.091576213509771, 0.109951743655322),
+          (0.091576213509771, 1 - 2 * 0.091576213509771, 0.109951743655322))
+
+
+def monopole_neumann_dp0(vertices, triangles, sources, k, amplitudes=None):
+    """Mean over each triangle (6-point rule) of d/dn of sum_s a_s e^{ik|x-s|}/(4 pi |x-s|),
+    with torch ops so that it is differentiable in vertices, k and sources."""
+    tri = triangles.long()
+    v0, v1, v2 = vertices[tri[:, 0]], vertices[tri[:, 1]], vertices[tri[:, 2]]
+    c = torch.cross(v1 - v0, v2 - v0, dim=1)
+    n = c / torch.linalg.vector_norm(c, dim=1, keepdim=True)
+    k = torch.as_tensor(k, dtype=vertices.dtype, device=vertices.device)
+    out = torch.zeros(tri.shape[0], dtype=torch.complex64, device=vertices.device)
+    srcs = torch.as_tensor(sources, dtype=vertices.dtype, device=vertices.device).reshape(-1, 3)
+    for q, s in enumerate(srcs):
+        a = 1.0 if amplitudes is None else amplitudes[q]
+        for x1, x2, w in _RULE6:
+            x = v0 + x1 * (v1 - v0) + x2 * (v2 - v0)
+            d = x - s
+            r = torch.linalg.vector_norm(d, dim=1)
+            g = torch.exp(1j * k * r) / (4 * math.pi * r)
+            out = out + a * w * g * (1j * k - 1.0 / r) * (d * n).sum(dim=1) / r
+    return out
+
+
+class _ExteriorPressure(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, vertices, k, psi, triangles, points, clamp, tol, restart, maxiter):
+        verts = vertices.detach().contiguous().float()
+        mesh = Mesh(verts, triangles)
+        kf = float(k)
+        nt = mesh.nt
+        dev = verts.device
+        psi = psi.detach().to(torch.complex64).contiguous()
+        lib = _lib.load()
+        lhs = torch.empty((nt, nt), dtype=torch.complex64, device=dev)
+        rhs = torch.empty(nt, dtype=torch.complex64, device=dev)
+        work = torch.empty(lib.bem_cbie_work_size(mesh.handle), dtype=torch.uint8, device=dev)
+        _lib.check(lib.bem_cbie_dp0(mesh.handle, kf, 0, int(clamp), _ptr(psi), _ptr(lhs), nt, _ptr(rhs),
+                                    _ptr(work), _stream()), "bem_cbie_dp0")
+        del work
+        phi, info = gmres(lhs, rhs, tol=tol, restart=restart, maxiter=maxiter)
+        pts = points.detach().to(torch.float32).contiguous()
+        S = _potential(mesh, pts, kf, 0, clamp)
+        D = _potential(mesh, pts, kf, 1, clamp)
+        p = matvec(D, phi)
+        matvec(S, psi, alpha=-1.0, beta=1.0, out=p)
+        ctx.mesh, ctx.lhs, ctx.S, ctx.D = mesh, lhs, S, D
+        ctx.kf, ctx.clamp, ctx.tol, ctx.restart, ctx.maxiter = kf, clamp, tol, restart, maxiter
+        ctx.save_for_backward(phi, psi, pts)
+        ctx.info = info
+        return p
+
+    @staticmethod
+    def backward(ctx, gp):
+        phi, psi, pts = ctx.saved_tensors
+        mesh, lhs, S, D = ctx.mesh, ctx.lhs, ctx.S, ctx.D
+        dev = phi.device
+        nt = mesh.nt
+        pbar = (0.5 * gp).to(torch.complex64).contiguous()
+        lib = _lib.load()
+        # adjoint solve (M/2 - K)^H lambda = D^H pbar, on the stored matrix
+        c = matvec(D, pbar, trans=2)
+        lam = torch.zeros(nt, dtype=torch.complex64, device=dev)
+        restart = int(min(ctx.restart, nt))
+        wk = torch.empty(lib.bem_gmres_work_size(nt, restart), dtype=torch.uint8, device=dev)
+        info = _lib.SolveInfo()
+        rc = lib.bem_gmres_ex(_ptr(lhs), nt, nt, 2, _ptr(c), _ptr(lam), float(ctx.tol), restart, int(ctx.maxiter),
+                              _ptr(wk), ctypes.byref(info), _stream())
+        if rc not in (_lib.BEM_OK, _lib.BEM_ERR_NOT_CONVERGED):
+            _lib.check(rc, "bem_gmres_ex")
+        del wk
+        gvert = torch.zeros((mesh.nv, 3), dtype=torch.float64, device=dev)
+        gk = torch.zeros(1, dtype=torch.float64, device=dev)
+        vhl = torch.zeros((nt, 2), dtype=torch.float64, device=dev)
+        _lib.check(lib.bem_adjoint_potential_dp0(mesh.handle, _ptr(pts), pts.shape[0], ctx.kf, int(ctx.clamp),
+                                                 _ptr(pbar), _ptr(psi), _ptr(phi), _ptr(gvert), _ptr(gk),
+                                                 _stream()), "bem_adjoint_potential_dp0")
+        work = torch.empty(max(lib.bem_adjoint_work_size(mesh.handle), 1), dtype=torch.uint8, device=dev)
+        _lib.check(lib.bem_adjoint_cbie_dp0(mesh.handle, ctx.kf, int(ctx.clamp), _ptr(lam), _ptr(psi), _ptr(phi),
+                                            _ptr(gvert), _ptr(gk), _ptr(vhl), _ptr(work), _stream()),
+                   "bem_adjoint_cbie_dp0")
+        # dL/dpsi (torch convention, 2 dL/dpsi*): -2 (S^H pbar + V^H lambda)
+        shp = matvec(S, pbar, trans=2)
+        gpsi = -2.0 * (shp + torch.complex(vhl[:, 0], vhl[:, 1]).to(torch.complex64))
+        ctx.adjoint_info = info.as_dict()
+        gk_t = gk.reshape(())
+        return (gvert.to(torch.float32), gk_t.to(torch.float32), gpsi, None, None, None, None, None, None)
+
+
+def _potential(mesh, pts, k, op, clamp):
+    out = torch.empty((pts.shape[0], mesh.nt), dtype=torch.complex64, device=pts.device)
+    _lib.check(_lib.load().bem_potential(mesh.handle, _ptr(pts), pts.shape[0], float(k), op, 1, int(clamp),
+                                         _ptr(out), mesh.nt, _stream()), "bem_potential")
+    return out
+
+
+def exterior_pressure(vertices, triangles, k, psi, points, clamp=True, tol=1e-6, restart=200, maxiter=2000):
+    """Differentiable p(points) of the DP0 CBIE solve; see the module doc."""
+    if not torch.is_tensor(k):
+        k = torch.tensor(float(k), dtype=torch.float32, device=vertices.device)
+    return _ExteriorPressure.apply(vertices, k, psi, triangles, points, clamp, tol, restart, maxiter)

@@ -63,7 +63,7 @@ constexpr int64_t kDp0RowTileHost = bem::kDp0RowTile;
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct GmresLayout {
-    size_t V, w, r, H, h, h2, cs, sn, g, y, res, sc, total;
+    size_t V, w, r, H, h, h2, cs, sn, g, y, res, sc, tw, total;
 };
 GmresLayout gmres_layout(int64_t n, int m)
 {
@@ -86,6 +86,7 @@ GmresLayout gmres_layout(int64_t n, int m)
     L.y = take(sizeof(double2) * (size_t)m);
     L.res = take(sizeof(double) * (size_t)m);
     L.sc = take(sizeof(double) * 8);
+    L.tw = take(sizeof(float2) * (size_t)bem::gemv_work_elems(n, n, 1));
     L.total = o;
     return L;
 }
@@ -289,6 +290,40 @@ int bem_cbie_dp0_ex(const bem_mesh* mesh, float k, int quad, int clamp, int part
     return BEM_OK;
 }
 
+int64_t bem_adjoint_work_size(const bem_mesh* mesh)
+{
+    if (!mesh) return 0;
+    return bem::adjoint_work_bytes(mesh->dev);
+}
+
+int bem_adjoint_cbie_dp0(const bem_mesh* mesh, float k, int clamp, const void* lam, const void* psi, const void* phi,
+                         double* grad_vertices, double* grad_k, double* vhl, void* work, void* stream)
+{
+    if (!mesh || !lam || !psi || !phi || !grad_vertices || !grad_k || !vhl || !work)
+        return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (!std::isfinite(k)) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+    DeviceGuard g(mesh->dev.device);
+    hipError_t e = bem::launch_adjoint_pairs(mesh->dev, k, clamp, (const float2*)lam, (const float2*)psi,
+                                             (const float2*)phi, grad_vertices, grad_k, vhl, work, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_adjoint_cbie_dp0 launch");
+    return BEM_OK;
+}
+
+int bem_adjoint_potential_dp0(const bem_mesh* mesh, const float* points, int64_t np, float k, int clamp,
+                              const void* pbar, const void* psi, const void* phi, double* grad_vertices,
+                              double* grad_k, void* stream)
+{
+    if (!mesh || !points || !pbar || !psi || !phi || !grad_vertices || !grad_k)
+        return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    DeviceGuard g(mesh->dev.device);
+    hipError_t e = bem::launch_adjoint_potential(mesh->dev, points, np, k, clamp, (const float2*)pbar,
+                                                 (const float2*)psi, (const float2*)phi, grad_vertices, grad_k,
+                                                 S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_adjoint_potential_dp0 launch");
+    return BEM_OK;
+}
+
 int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream)
 {
     if (!mesh || !tri_values || !vert_values) return fail(BEM_ERR_BAD_ARG, "null pointer");
@@ -356,7 +391,14 @@ int64_t bem_gmres_work_size(int64_t n, int restart)
 int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, float tol, int restart, int maxiter,
               void* work, bem_solve_info* info, void* stream)
 {
+    return bem_gmres_ex(A_, n, lda, 0, b_, x_, tol, restart, maxiter, work, info, stream);
+}
+
+int bem_gmres_ex(const void* A_, int64_t n, int64_t lda, int trans, const void* b_, void* x_, float tol, int restart,
+                 int maxiter, void* work, bem_solve_info* info, void* stream)
+{
     if (!A_ || !b_ || !x_ || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (trans < 0 || trans > 2) return fail(BEM_ERR_BAD_ARG, "trans must be 0, 1 or 2");
     if (n <= 0 || lda < n || restart <= 0 || maxiter < 0 || !(tol > 0.f))
         return fail(BEM_ERR_BAD_ARG, "bad GMRES argument");
     const int m = restart;
@@ -374,6 +416,7 @@ int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, 
     double2* y = (double2*)(W + L.y);
     double* res = (double*)(W + L.res);
     double* sc = (double*)(W + L.sc);
+    float2* tw = (float2*)(W + L.tw);
     const float2* A = (const float2*)A_;
     const float2* b = (const float2*)b_;
     float2* x = (float2*)x_;
@@ -398,7 +441,7 @@ int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, 
     while (true) {
         // r = b - A x
         HIP_TRY(hipMemcpyAsync(r, b, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
-        HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, x, r, mone, one, nullptr, s));
+        HIP_TRY(bem::launch_gemv(A, n, n, lda, trans, x, r, mone, one, tw, s));
         HIP_TRY(bem::launch_norm2(r, n, sc, s));
         double rn2 = 0.0;
         if (int rc = read_double(sc, &rn2, s)) return rc;
@@ -413,7 +456,7 @@ int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, 
         int nk = 0;
         for (int j = 0; j < m && its < maxiter; ++j) {
             float2* vj = V + (size_t)j * n;
-            HIP_TRY(bem::launch_gemv(A, n, n, lda, 0, vj, w, one, zero, nullptr, s));
+            HIP_TRY(bem::launch_gemv(A, n, n, lda, trans, vj, w, one, zero, tw, s));
             // classical Gram-Schmidt, twice
             HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, s));
             HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));

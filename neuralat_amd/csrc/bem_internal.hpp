@@ -80,6 +80,12 @@ hipError_t launch_batch_cbie_persistent(const BatchItemDev* items, int nitems, i
 int64_t cbie_work_bytes(const MeshDev& m);
 hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
                            int64_t ld, float2* rhs, void* work, hipStream_t s);
+int64_t adjoint_work_bytes(const MeshDev& m);
+hipError_t launch_adjoint_pairs(const MeshDev& m, float k, int clamp, const float2* lam, const float2* psi,
+                                const float2* phi, double* gvert, double* gk, double* vhl, void* work, hipStream_t s);
+hipError_t launch_adjoint_potential(const MeshDev& m, const float* pts, int64_t np, float k, int clamp,
+                                    const float2* pbar, const float2* psi, const float2* phi, double* gvert, double* gk,
+                                    hipStream_t s);
 hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
