@@ -93,8 +93,10 @@ __global__ __launch_bounds__(256) void absmax_kernel(const float2* __restrict__ 
 }
 
 // X (complex64, systems of length n at stride ldx_in) -> 16-bit planes
-// (ldx, zero-padded).  bf16 keeps a second, low-order plane per part
-// (x = hi + lo, both bf16), so the right-hand sides carry ~16 mantissa bits:
+// (ldx, zero-padded) with a second, low-order plane per part (x = hi + lo,
+// both in the 16-bit format), so the right-hand sides carry about twice the
+// format's mantissa (fp16 low parts may be subnormal: their absolute
+// resolution 2^-24 is far below a normalised Krylov vector's entries):
 // planes Xr, Xi, Xr_lo = Xr + 16 ldx, Xi_lo = Xi + 16 ldx.
 template <int DT>
 __global__ __launch_bounds__(256) void vec_to_lowp_kernel(const float2* __restrict__ X, int64_t n, int64_t ldx_in,
@@ -109,10 +111,8 @@ __global__ __launch_bounds__(256) void vec_to_lowp_kernel(const float2* __restri
         const uint16_t hr = to_lp<DT>(v.x), hi = to_lp<DT>(v.y);
         Xr[e] = hr;
         Xi[e] = hi;
-        if constexpr (DT == kDtBF16) {
-            Xr[e + 16 * ldx] = to_lp<DT>(v.x - from_lp<DT>(hr));
-            Xi[e + 16 * ldx] = to_lp<DT>(v.y - from_lp<DT>(hi));
-        }
+        Xr[e + 16 * ldx] = to_lp<DT>(v.x - from_lp<DT>(hr));
+        Xi[e + 16 * ldx] = to_lp<DT>(v.y - from_lp<DT>(hi));
     }
 }
 
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
     const uint16_t* pxi = Xi + (int64_t)s * ldx + 8 * q;
     const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
     const bool live = li < nrhs;
-    constexpr bool kSplit = (DT == kDtBF16);  // hi + lo right-hand-side planes
+    constexpr bool kSplit = true;  // hi + lo right-hand-side planes
     f32x4 a_rr = {0, 0, 0, 0}, a_ii = a_rr, a_ri = a_rr, a_ir = a_rr;
     f32x4 b_rr = a_rr, b_ii = a_rr, b_ri = a_rr, b_ir = a_rr;
     const int64_t kend = ld;  // ld is a multiple of 32; padded columns are zero
