@@ -38,8 +38,18 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BEM entries assembled/s + solves/s at <=1e-3 rel err vs FP64 (plain vs batched)"
 FP32_PEAK = 157.3e12  # MI355X FP32 vector/matrix peak, FLOP/s
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# HBM bytes per matrix entry per launch (FETCH_SIZE + WRITE_SIZE, scaled by
+# the counters' reading for a copy of known size), from the hardware-counter
+# passes over ``bench.py --traffic-probe`` at level 6 kept under profiles/.
+GEMV_TRAFFIC_PER_ENTRY = None
+CBIE_TRAFFIC_PER_ENTRY = None
 DLP_FLOP_PER_EVAL = 29  # algorithmic count per Gauss-point pair (SLP: 16)
 SLP_FLOP_PER_EVAL = 16
+
+
+def _per_entry(x, n):
+    return None if x is None else x * n * n
 
 
 def parse():
@@ -55,6 +65,11 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--check-level", type=int, default=3, help="mesh level of the FP64 oracle check (0 = skip)")
+    p.add_argument("--no-forms", action="store_true", help="skip the precision / batching / differentiation forms")
+    p.add_argument("--batch-meshes", type=int, default=8)
+    p.add_argument("--batch-ks", type=int, default=8)
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="run one fused assembly launch, one GEMV and a 4 GiB copy, for hardware-counter passes")
     return p.parse_args()
 
 
@@ -124,7 +139,7 @@ class Solver:
         from neuralat_amd.bem.solver import gmres
 
         self.assemble(k, psi)
-        phi, info = gmres(self.lhs, self.rhs, tol=tol, restart=restart, maxiter=maxiter)
+        phi, info = gmres(self.lhs, self.rhs, tol=tol, restart=restart, maxiter=maxiter, time_products=True)
         S = self.potential(k, pts_t, "single")
         D = self.potential(k, pts_t, "double")
         from neuralat_amd.bem.solver import matvec
@@ -175,6 +190,299 @@ def fp64_check(level, ks):
     return out
 
 
+def matvec_roofline(s, reps=5):
+    """HBM roofline of the dense complex GEMV (the GMRES matvec) on the
+    assembled operator: 8 N^2 + 16 N bytes per launch over its HIP-event time."""
+    import torch
+
+    from neuralat_amd.bem.solver import matvec
+
+    n = s.n
+    g = torch.Generator(device=s.dev).manual_seed(3)
+    x = torch.randn(n, dtype=torch.complex64, device=s.dev, generator=g)
+    y = torch.empty_like(x)
+    matvec(s.lhs, x, out=y)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record()
+    for r in range(reps):
+        matvec(s.lhs, x, out=y)
+        ev[r + 1].record()
+    torch.cuda.synchronize()
+    ms = [ev[r].elapsed_time(ev[r + 1]) for r in range(reps)]
+    nbytes = 8 * n * n + 16 * n
+    sec = float(np.mean(ms)) / 1e3
+    return {"bound": "hbm", "kernel": "gemv_n_vec_kernel<4> (complex64 GEMV)", "achieved": nbytes / sec / 1e9,
+            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": nbytes / sec / HBM_PEAK, "traffic": None,
+            "bytes_per_launch": nbytes, "mean_launch_ms": sec * 1e3}
+
+
+def _monopole_psis(v, t, k, srcs):
+    from neuralat_amd import mesh
+
+    out = []
+    for src in srcs:
+        _, dn = mesh.sources_field([src], None, k)
+        out.append(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)))
+    return np.array(out, dtype=np.complex64)
+
+
+def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
+    """16 right-hand sides on the assembled level-6 system at k: plain FP32
+    (one GMRES solve per right-hand side, timed on one of them, with the fused
+    assembly and potentials of the plain path) against 16-bit storage +
+    matrix-core multi-RHS GMRES + FP32 refinement (K and V assembled once,
+    -V Psi on the FP32 matrix cores).  Errors: each solution against the
+    analytic field at 64 exterior points, the FP32 solution of the same system,
+    and the FP64 oracle on a level-``check_level`` mesh."""
+    import torch
+
+    from neuralat_amd import _lib, mesh
+    from neuralat_amd.bem.precision import LowPrecisionMatrix, matmul, solve_ir
+    from neuralat_amd.bem.solver import _ptr, _stream, gmres, matvec
+
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(seed)
+    d = rng.standard_normal((nrhs, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    srcs = d * rng.uniform(0.0, 0.5, size=(nrhs, 1))
+    psi = torch.tensor(_monopole_psis(v, t, k, srcs), device=s.dev)
+    pts_t = torch.tensor(pts.astype(np.float32), device=s.dev)
+    fields = np.array([mesh.sources_field([q], None, k)[0](pts) for q in srcs])
+    n = s.n
+    lib = _lib.load()
+    res = {"k": k, "nrhs": nrhs}
+
+    # plain FP32 path, one right-hand side (the same for each of the 16)
+    t0 = time.perf_counter()
+    phi0, p0, info0 = s.solve(k, psi[0], pts_t, 1e-6, 200, 3000)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    res["plain_fp32"] = {"seconds_per_solve": t_plain, "solves_per_s": 1.0 / t_plain,
+                         "gmres_iterations": info0["iterations"],
+                         "rel_err_vs_analytic": float(np.linalg.norm(p0.cpu().numpy() - fields[0])
+                                                      / np.linalg.norm(fields[0]))}
+    log(f"bench: precision plain FP32 {t_plain:.2f} s/solve, {info0['iterations']} its")
+    # 16-bit: K (fused pass, lhs) and V assembled once; B = -V Psi on matrix cores
+    for dt in ("bf16", "fp16"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.assemble(k, psi[0])
+        Vm = torch.empty((n, n), dtype=torch.complex64, device=s.dev)
+        _lib.check(lib.bem_assemble(s.mesh.handle, float(k), _lib.OPS["single"], _lib.BASES["DP0"], 0, s.clamp,
+                                    _ptr(Vm), n, _stream()), "bem_assemble")
+        B = matmul(Vm, psi, alpha=-1.0)
+        del Vm
+        L = LowPrecisionMatrix(s.lhs, dt)
+        X, info = solve_ir(s.lhs, L, B, tol=1e-6, inner_tol=1e-2 if dt == "bf16" else 1e-3, restart=100,
+                           max_inner=2000, max_outer=20)
+        Sp = s.potential(k, pts_t, "single")
+        Dp = s.potential(k, pts_t, "double")
+        P = matmul(Dp, X)
+        matmul(Sp, psi, alpha=-1.0, beta=1.0, out=P)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        del L
+        Pc = P.cpu().numpy()
+        errs = [float(np.linalg.norm(Pc[q] - fields[q]) / np.linalg.norm(fields[q])) for q in range(nrhs)]
+        x0 = X[0]
+        rel_fp32 = float((torch.linalg.vector_norm(x0 - phi0) / torch.linalg.vector_norm(phi0)).item())
+        res[dt] = {"seconds": el, "solves_per_s": nrhs / el, "speedup_vs_plain": (nrhs / el) * t_plain,
+                   "outer_iterations": info["outer"], "inner_iterations": info["inner"],
+                   "max_rel_residual": float(max(info["rel_residual"])), "converged": bool(info["converged"]),
+                   "surface_rel_diff_vs_fp32_solution": rel_fp32,
+                   "max_rel_err_vs_analytic": float(max(errs))}
+        log(f"bench: precision {dt}: {el:.2f} s for {nrhs} solves ({nrhs / el:.2f}/s, "
+            f"{(nrhs / el) * t_plain:.1f}x plain), outer {info['outer']}, converged {info['converged']}")
+    if check_level:
+        res["fp64_check"] = precision_fp64_check(check_level, k, srcs[:2])
+    return res
+
+
+def precision_fp64_check(level, k, srcs):
+    """The 16-bit + refinement pipeline on a small mesh of the same family
+    against the FP64 oracle solution of the same discrete system."""
+    import torch
+
+    import oracle
+    from neuralat_amd.bem.precision import LowPrecisionMatrix, solve_ir
+
+    v, t, _, pts = make_workload(level)
+    v32 = v.astype(np.float32)
+    s = Solver(v32, t)
+    psis = _monopole_psis(v32, t, k, srcs)
+    S = oracle.assemble(v32, t, k, "single", "DP0")
+    D = oracle.assemble(v32, t, k, "double", "DP0")
+    M = oracle.identity(v32, t, "DP0")
+    A64 = 0.5 * M - D
+    out = {"level": level}
+    for dt in ("bf16", "fp16"):
+        rhs = []
+        for q in range(len(srcs)):
+            s.assemble(k, torch.tensor(psis[q], device=s.dev))
+            rhs.append(s.rhs.clone())
+        L = LowPrecisionMatrix(s.lhs, dt)
+        X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=1e-6, inner_tol=1e-2 if dt == "bf16" else 1e-3)
+        Xc = X.cpu().numpy()
+        errs = []
+        for q in range(len(srcs)):
+            x64 = np.linalg.solve(A64, -S @ psis[q])
+            errs.append(float(np.linalg.norm(Xc[q] - x64) / np.linalg.norm(x64)))
+        out[dt] = {"max_surface_rel_err": max(errs), "converged": bool(info["converged"])}
+    del s
+    return out
+
+
+def grad_form(ndir=4, seed=0):
+    """Level-4 differentiation workload: loss = sum |p|^2 at 32 listeners from
+    one interior monopole at k = 25; adjoint gradient w.r.t. all vertices and
+    k, backward/forward time, and central differences on a few directions."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.grad import exterior_pressure, monopole_neumann_dp0
+
+    v, t = mesh.icosphere(4, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=seed).astype(np.float32)
+    rng = np.random.default_rng(seed + 10)
+    pts = mesh.random_exterior_points(32, 1.5, 3.0, seed=seed + 3).astype(np.float32)
+    src = rng.uniform(-0.2, 0.2, size=3)
+    dev = torch.device("cuda:0")
+    tt = torch.tensor(t, device=dev)
+    pts_t = torch.tensor(pts, device=dev)
+    k0 = 25.0
+
+    def loss(vt, k, tol=1e-7):
+        psi = monopole_neumann_dp0(vt, tt, src, k)
+        p = exterior_pressure(vt, tt, k, psi, pts_t, tol=tol)
+        return (p.real ** 2 + p.imag ** 2).sum()
+
+    tf, tb = [], []
+    for _ in range(3):
+        vt = torch.tensor(v, device=dev, requires_grad=True)
+        kt = torch.tensor(k0, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = loss(vt, kt, tol=1e-6)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        tf.append(t1 - t0)
+        tb.append(time.perf_counter() - t1)
+    vt = torch.tensor(v, device=dev, requires_grad=True)
+    kt = torch.tensor(k0, device=dev, requires_grad=True)
+    L = loss(vt, kt)
+    L.backward()
+    gv = vt.grad.double().cpu().numpy()
+    dirs = rng.standard_normal((ndir,) + v.shape)
+    dirs /= np.linalg.norm(dirs.reshape(ndir, -1), axis=1)[:, None, None]
+    ad, fd = [], []
+    with torch.no_grad():
+        for d in dirs:
+            dp = torch.tensor((v + 1e-3 * d).astype(np.float32), device=dev)
+            dm = torch.tensor((v - 1e-3 * d).astype(np.float32), device=dev)
+            fd.append(float(loss(dp, k0)) - float(loss(dm, k0)))
+            ad.append(float(np.sum(gv * (dp - dm).double().cpu().numpy())))
+        hk = 1e-5 * k0
+        fdk = (float(loss(vt.detach(), k0 + hk)) - float(loss(vt.detach(), k0 - hk))) / (2 * hk)
+    ad, fd = np.array(ad), np.array(fd)
+    return {"workload": "icosphere L4 (5120 triangles) + SH noise, k=25, 32 listeners, loss sum |p|^2",
+            "forward_ms": 1e3 * min(tf), "backward_ms": 1e3 * min(tb), "backward_over_forward": min(tb) / min(tf),
+            "fd_vertex_rel_diff": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)), "fd_directions": ndir,
+            "dL_dk_adjoint": float(kt.grad), "dL_dk_fd": fdk}
+
+
+def make_batch(nmesh, nk, seed=7):
+    """Seeded uneven meshes (icosphere levels 2-5 in turn, anisotropic scale
+    in [0.5, 2]^3, SH noise) and nk wavenumbers each, log-uniform in [5, 80]."""
+    from neuralat_amd import mesh
+
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(nmesh):
+        level = 2 + q % 4
+        v, t = mesh.icosphere(level, 1.0)
+        v = mesh.anisotropic_scale(mesh.sh_perturb(v, 8, 0.05, seed=seed + q), rng.uniform(0.5, 2.0, 3))
+        ks = np.exp(rng.uniform(np.log(5.0), np.log(80.0), size=nk))
+        out.append((v.astype(np.float32), t, [float(x) for x in ks]))
+    return out
+
+
+def batch_form(nmesh=8, nk=8, restart=50, max_cycles=40, seed=7):
+    """Plain per-item loop (fused assembly + GMRES per (mesh, k)) against one
+    persistent launch for all items + lockstep graph-captured GMRES.  Neumann
+    data: one interior monopole per mesh."""
+    import torch
+
+    import oracle
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.batch import allocate_systems, batch_autotune, batch_cbie, batch_gmres
+    from neuralat_amd.bem.solver import BEM_Solver, gmres
+
+    dev = torch.device("cuda:0")
+    items = make_batch(nmesh, nk, seed)
+    solvers, meshes, ks, psis, host = [], [], [], [], []
+    for v, t, kl in items:
+        sv = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="DP0")
+        solvers.append(sv)
+        for k in kl:
+            _, dn = mesh.sources_field([[0.05, -0.02, 0.01]], None, k)
+            meshes.append(sv.mesh)
+            ks.append(k)
+            psis.append(torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64),
+                                     device=dev))
+            host.append((v, t, k))
+    outs = allocate_systems(meshes, dev)
+    ntri = [m.nt for m in meshes]
+    entries = float(sum(2 * n * n for n in ntri))
+    # plain
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    xs_plain, conv_plain = [], 0
+    q = 0
+    for sv, (v, t, kl) in zip(solvers, items):
+        for k in kl:
+            lhs, rhs = sv.cbie_system(k, psis[q], out=outs[q][0])
+            x, info = gmres(lhs, rhs, tol=1e-6, restart=restart, maxiter=restart * max_cycles)
+            xs_plain.append(x)
+            conv_plain += int(info["converged"])
+            q += 1
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"bench: batch plain loop {t_plain:.2f} s for {len(ks)} solves")
+    best, tune_ms = batch_autotune(meshes, ks, psis, outs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch_cbie(meshes, ks, psis, outs, row_tile=best)
+    xs, infos = batch_gmres([o[0] for o in outs], [o[1] for o in outs], tol=1e-6, restart=restart,
+                            max_cycles=max_cycles, use_graph=True)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    log(f"bench: batch persistent + graph GMRES {t_batch:.2f} s")
+    conv = [bool(i["converged"]) for i in infos]
+    diffs = [float((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item())
+             for a, b, c in zip(xs, xs_plain, conv) if c]
+    # FP64 oracle of the level-2 items (320 triangles)
+    errs = []
+    for q, (v, t, k) in enumerate(host):
+        if t.shape[0] != 320 or not conv[q]:
+            continue
+        S = oracle.assemble(v, t, k, "single", "DP0")
+        D = oracle.assemble(v, t, k, "double", "DP0")
+        M = oracle.identity(v, t, "DP0")
+        x64 = np.linalg.solve(0.5 * M - D, -S @ psis[q].cpu().numpy().astype(np.complex128))
+        errs.append(float(np.linalg.norm(xs[q].cpu().numpy() - x64) / np.linalg.norm(x64)))
+    return {"workload": f"{nmesh} meshes (icosphere L2-L5, anisotropic scale [0.5,2], SH noise) x {nk} k "
+                        f"log-uniform [5,80] = {len(ks)} DP0 CBIE solves, GMRES({restart}) tol 1e-6",
+            "triangles": sorted(set(ntri)), "plain_seconds": t_plain, "batched_seconds": t_batch,
+            "plain_solves_per_s": len(ks) / t_plain, "batched_solves_per_s": len(ks) / t_batch,
+            "speedup": t_plain / t_batch, "entries_per_s_batched": entries / t_batch,
+            "converged_batched": int(sum(conv)), "converged_plain": conv_plain,
+            "autotune_row_tile": best, "autotune_ms": tune_ms,
+            "max_rel_diff_batched_vs_plain": max(diffs) if diffs else None,
+            "max_surface_rel_err_vs_fp64_level2": max(errs) if errs else None}
+
+
 def cpu_baseline(v, t, k, seconds):
     """FP32 mirror of the reference arithmetic on rows [0, R) of the DP0 SLP
     and DLP operators, all cores of this process's affinity."""
@@ -208,6 +516,31 @@ def cpu_baseline(v, t, k, seconds):
                       f"OpenMP; CPU: {model}"}
 
 
+def traffic_probe(level, k):
+    """One launch each of the fused assembly kernel and the GEMV on the
+    level-``level`` workload, then a device copy of 4 GiB (read 4 GiB, write
+    4 GiB) that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.solver import matvec
+
+    v, t, src, _ = make_workload(level)
+    s = Solver(v.astype(np.float32), t)
+    _, dn = mesh.sources_field(src, None, k)
+    psi = torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64), device=s.dev)
+    s.assemble(k, psi)
+    torch.cuda.synchronize()
+    y = matvec(s.lhs, psi)
+    torch.cuda.synchronize()
+    a = torch.ones(1 << 30, dtype=torch.float32, device=s.dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"triangles": s.n, "cbie_entries": s.n * s.n, "gemv_bytes": 8 * s.n * s.n + 16 * s.n,
+                      "copy_read_bytes": 4 << 30, "copy_write_bytes": 4 << 30, "y0": str(complex(y[0].item()))}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -220,6 +553,9 @@ def main():
 
     _lib.load()
     ks = [float(x) for x in args.ks.split(",")]
+    if args.traffic_probe:
+        traffic_probe(args.level, ks[0])
+        return
     log(f"bench: building level-{args.level} workload")
     v, t, src, pts = make_workload(args.level)
     n = t.shape[0]
@@ -264,6 +600,10 @@ def main():
 
     reg = [r[3] for res in all_res for r in res]
     asm = [r[4] for res in all_res for r in res]
+    gemv_ms = sum(r[2]["gemv_ms"] for res in all_res for r in res)
+    gemv_n = sum(r[2]["gemv_count"] for res in all_res for r in res)
+    gemv_bytes = 8 * n * n + 16 * n
+    gemv_avg_s = gemv_ms / 1e3 / max(gemv_n, 1)
     errs, its = {}, {}
     for (k, p, info, _, _) in all_res[-1]:
         pe = fields[k]
@@ -274,6 +614,18 @@ def main():
     reg_mean_s = float(np.mean(reg)) / 1e3
     flop = (DLP_FLOP_PER_EVAL + SLP_FLOP_PER_EVAL) * 36 * (n * n - npairs)
     achieved = flop / reg_mean_s
+    mv = matvec_roofline(s)
+    log(f"bench: matvec {mv['achieved']:.0f} GB/s ({mv['frac']:.3f} of peak)")
+    forms = {}
+    if not args.no_forms:
+        forms["precision"] = precision_form(s, v, t, ks[-1], pts)
+    del s
+    torch.cuda.empty_cache()
+    if not args.no_forms:
+        forms["batching"] = batch_form(args.batch_meshes, args.batch_ks)
+        torch.cuda.empty_cache()
+        forms["differentiation"] = grad_form()
+        log(f"bench: forms {json.dumps(forms)}")
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline")
@@ -299,11 +651,22 @@ def main():
         "gmres_iterations": its,
         "rel_err_vs_analytic": errs,
         "rel_err_vs_fp64_small_mesh": {"level": args.check_level, **fp64},
-        "roofline": {"bound": "valu", "kernel": "dp0_cbie_kernel<6> (fused single+double layer)", "achieved": achieved / 1e12,
-                     "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s", "frac": achieved / FP32_PEAK, "traffic": None,
-                     "flop_per_launch": flop, "mean_launch_ms": reg_mean_s * 1e3,
-                     "transcendentals_per_s": 4 * 36 * (n * n - npairs) / reg_mean_s,
-                     "flop_count": "45 FLOP (single 16 + double 29) x 36 Gauss-point pairs x (N^2 - neighbour pairs)"},
+        "roofline": {"bound": "hbm", "kernel": "gemv_n_vec_kernel<4> (complex64 GEMV in GMRES)",
+                     "achieved": gemv_bytes / gemv_avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": gemv_bytes / gemv_avg_s / HBM_PEAK, "traffic": _per_entry(GEMV_TRAFFIC_PER_ENTRY, n),
+                     "bytes_per_launch": gemv_bytes, "mean_launch_ms": gemv_avg_s * 1e3, "launches": gemv_n,
+                     "share_of_timed_region": gemv_ms / 1e3 / wall,
+                     "byte_count": "8 B per matrix entry (complex64) + 16 B per row (x read, y written)"},
+        "roofline_assembly": {"bound": "valu", "kernel": "dp0_cbie_kernel<6> (fused single+double layer)",
+                              "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12, "unit": "TFLOP/s",
+                              "frac": achieved / FP32_PEAK, "traffic": _per_entry(CBIE_TRAFFIC_PER_ENTRY, n),
+                              "flop_per_launch": flop, "mean_launch_ms": reg_mean_s * 1e3,
+                              "transcendentals_per_s": 4 * 36 * (n * n - npairs) / reg_mean_s,
+                              "share_of_timed_region": float(np.sum(reg)) / 1e3 / wall,
+                              "flop_count": "45 FLOP (single 16 + double 29) x 36 Gauss-point pairs x "
+                                            "(N^2 - neighbour pairs)"},
+        "roofline_matvec_standalone": mv,
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

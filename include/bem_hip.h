@@ -154,6 +154,12 @@ int bem_gmres(const void* A, int64_t n, int64_t lda, const void* b, void* x, flo
  * solve of a gradient, reusing the assembled matrix). */
 int bem_gmres_ex(const void* A, int64_t n, int64_t lda, int trans, const void* b, void* x, float tol, int restart,
                  int maxiter, void* work, bem_solve_info* info, void* stream);
+/* bem_gmres_ex that also returns the summed HIP-event time (ms) and the
+ * number of its matrix-vector products (either pointer may be NULL), for
+ * measuring the product kernel inside a solve.  No reference counterpart. */
+int bem_gmres_timed(const void* A, int64_t n, int64_t lda, int trans, const void* b, void* x, float tol, int restart,
+                    int maxiter, void* work, bem_solve_info* info, double* gemv_ms, int64_t* gemv_count,
+                    void* stream);
 /* Reference-compatible BiCGSTAB (BiCGSTAB.py:5-107): bilinear rho, alpha,
  * omega; stops when ||b - A x||^2 < tol ||b||^2 or < atol; stops silently
  * (returns BEM_OK, info->converged = 0) when nsteps run out or rho == 0. */
@@ -227,15 +233,17 @@ typedef struct {
 } bem_batch_item;
 /* The DP0 CBIE systems of n_items items (any sizes, any wavenumbers) with the
  * regular pairs of all items in ONE persistent launch over a device-side
- * queue of (item, row tile, 256-column block) tasks, largest items first.
- * row_tile: rows per task (16/32/64/128) or 0 for the value bem_batch_autotune
- * stored for this size class (64 if none). */
+ * queue of (item, 64-column block, row tile) wavefront tasks, largest items
+ * first.  row_tile: rows per task (16/32/64) or 0 for the value
+ * bem_batch_autotune stored for this size class (64 if none).  The result
+ * does not depend on row_tile (bit for bit). */
 int64_t bem_batch_cbie_work_size(const bem_batch_item* items, int n_items);
 int bem_batch_cbie(const bem_batch_item* items, int n_items, int clamp, int row_tile, void* work, void* stream);
-/* Times the four row tiles on the largest item, keeps the fastest for its
- * size class (process-wide, mutex-guarded), returns it and the times. */
+/* Times the three row tiles on the largest item, keeps the fastest for its
+ * size class (process-wide, mutex-guarded), returns it and the times
+ * (ms3[0..2] for 16, 32, 64 rows). */
 int bem_batch_autotune(const bem_batch_item* items, int n_items, int clamp, void* work, int* best_row_tile,
-                       float* ms4, void* stream);
+                       float* ms3, void* stream);
 typedef struct {
     const void* A; /* [n, lda] complex64 */
     int64_t lda;

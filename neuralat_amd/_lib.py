@@ -99,6 +99,8 @@ SIGNATURES = {
     "bem_batch_gmres": (_I32, [_P, _I32, _F, _I32, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
     "bem_batch_last_error": (ctypes.c_char_p, []),
     "bem_gmres_ex": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
+    "bem_gmres_timed": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64), _P]),
     "bem_adjoint_work_size": (_I64, [_P]),
     "bem_adjoint_cbie_dp0": (_I32, [_P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bem_adjoint_potential_dp0": (_I32, [_P, _P, _I64, _F, _I32, _P, _P, _P, _P, _P, _P]),

@@ -70,10 +70,10 @@ def batch_autotune(meshes, ks, psis, outs=None, clamp=True):
     lib = _lib.load()
     work = torch.empty(lib.bem_batch_cbie_work_size(arr, len(arr)), dtype=torch.uint8, device=device)
     best = ctypes.c_int()
-    ms = (ctypes.c_float * 4)()
+    ms = (ctypes.c_float * 3)()
     _check(lib.bem_batch_autotune(arr, len(arr), int(clamp), _ptr(work), ctypes.byref(best), ms, _stream()),
            "bem_batch_autotune")
-    return int(best.value), dict(zip((16, 32, 64, 128), list(ms)))
+    return int(best.value), dict(zip((16, 32, 64), list(ms)))
 
 
 def batch_gmres(As, bs, xs=None, tol=1e-6, restart=50, max_cycles=40, use_graph=True, raise_on_fail=False):

@@ -144,8 +144,11 @@ def add_identity_(solver, A: torch.Tensor, alpha, basis=None):
     return A
 
 
-def gmres(A: torch.Tensor, b: torch.Tensor, x0=None, tol=1e-6, restart=100, maxiter=1000, raise_on_fail=False):
-    """Device GMRES(restart) on a dense complex64 matrix.  Returns (x, info)."""
+def gmres(A: torch.Tensor, b: torch.Tensor, x0=None, tol=1e-6, restart=100, maxiter=1000, raise_on_fail=False,
+          time_products=False):
+    """Device GMRES(restart) on a dense complex64 matrix.  Returns (x, info);
+    with ``time_products`` info also holds the summed HIP-event time of the
+    matrix-vector products ("gemv_ms") and their number ("gemv_count")."""
     check_tensor(A, torch.complex64)
     check_tensor(b, torch.complex64)
     n = A.shape[0]
@@ -154,12 +157,20 @@ def gmres(A: torch.Tensor, b: torch.Tensor, x0=None, tol=1e-6, restart=100, maxi
     restart = int(min(restart, max(n, 1)))
     work = torch.empty(lib.bem_gmres_work_size(n, restart), dtype=torch.uint8, device=A.device)
     info = _lib.SolveInfo()
-    rc = lib.bem_gmres(_ptr(A), n, A.stride(0), _ptr(b), _ptr(x), float(tol), restart, int(maxiter), _ptr(work),
-                       ctypes.byref(info), _stream())
-    if rc == _lib.BEM_ERR_NOT_CONVERGED and not raise_on_fail:
-        return x, info.as_dict()
-    _lib.check(rc, "bem_gmres")
-    return x, info.as_dict()
+    ms, cnt = ctypes.c_double(), ctypes.c_int64()
+    if time_products:
+        rc = lib.bem_gmres_timed(_ptr(A), n, A.stride(0), 0, _ptr(b), _ptr(x), float(tol), restart, int(maxiter),
+                                 _ptr(work), ctypes.byref(info), ctypes.byref(ms), ctypes.byref(cnt), _stream())
+    else:
+        rc = lib.bem_gmres(_ptr(A), n, A.stride(0), _ptr(b), _ptr(x), float(tol), restart, int(maxiter), _ptr(work),
+                           ctypes.byref(info), _stream())
+    if not (rc == _lib.BEM_ERR_NOT_CONVERGED and not raise_on_fail):
+        _lib.check(rc, "bem_gmres")
+    out = info.as_dict()
+    if time_products:
+        out["gemv_ms"] = float(ms.value)
+        out["gemv_count"] = int(cnt.value)
+    return x, out
 
 
 def bicgstab(A: torch.Tensor, b: torch.Tensor, x=None, nsteps=500, tol=1e-10, atol=1e-16):

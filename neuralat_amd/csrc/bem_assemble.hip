@@ -133,9 +133,30 @@ __global__ __launch_bounds__(256) void dp0_regular_kernel(const float* __restric
 // ---------------------------------------------------------------------------
 // The single- and double-layer kernels share the point difference, r and
 // e^{ikr}; the single-layer entries are contracted with psi on the fly and
-// never stored.  Pairs sharing a vertex are left out of the V psi sums (their
-// entries come from the singular kernel); their lhs entries are overwritten
-// by it.  partial[cb * nt + i] = sum over the 256 columns of block cb.
+// never stored.  Both operators are evaluated once per UNORDERED pair {i, j}:
+// the 6 x 6 Gauss-point pairs of (i, j) and (j, i) are the same points with the
+// same weight products (w_a |J_i| / 2) (w_b |J_j| / 2), so
+//   V_ji = V_ij,
+//   K_ij = sum_a (x_a - y_0).n_j  sum_b F_ab,
+//   K_ji = sum_b (y_b - x_0).n_i  sum_a F_ab,
+// with F_ab = w_a w_b e^{ikr} (1 - ikr) / r^3 (x_0, y_0: any point of the
+// plane of T_i, T_j).  One evaluation of F_ab feeds both entries, which halves
+// the transcendental work of the pair sweep.
+//
+// A task is a wavefront: columns J*64 + lane (lane = column triangle, its
+// points in VGPRs), rows [I*64 + sub*RT, +RT) with I <= J (row data
+// wave-uniform, in SGPRs).  In diagonal blocks (I == J) only j > i is taken.
+// K_ij is stored directly (64 consecutive entries of row i); K_ji is staged
+// in LDS for 16 rows and written as 16-entry row segments (4 full 128-byte
+// lines per store).  Pairs sharing a vertex are left out of the V psi sums
+// (the singular kernel provides them); their lhs entries are overwritten by
+// it.  Partial sums (deterministic, independent of RT):
+//   prow[J * nt + i]               = sum over columns j in block J of V_ij psi_j,
+//   pcol[((I * 64 / RT) + sub) * nt + j] = sum over the task's rows of V_ij psi_i,
+// the latter formed per 16-row chunk and combined as ((c0 + c1) + (c2 + c3)).
+constexpr int kSymBlock = 64;
+constexpr int kStageCols = 16;
+
 __device__ __forceinline__ float wave_sum_f32(float v)
 {
 #pragma unroll
@@ -143,19 +164,37 @@ __device__ __forceinline__ float wave_sum_f32(float v)
     return v;
 }
 
-// One task: rows [by * RT, by * RT + RT) x columns [bx * 256, bx * 256 + 256).
-// red: LDS scratch of 4 x RT complex.  Called by all 256 threads.
-template <int Q, bool CLAMP, int RT>
-__device__ __forceinline__ void cbie_tile(const float* __restrict__ rec, const int4* __restrict__ tris, int64_t nt,
-                                          float k, const float2* __restrict__ psi, float2* __restrict__ lhs,
-                                          int64_t ld, float2* __restrict__ partial, int64_t bx, int64_t by,
-                                          float2 (*red)[RT])
+// block pair index p -> (I, J), I <= J, p = J (J + 1) / 2 + I
+__device__ __forceinline__ void tri_block_pair(int64_t p, int64_t& I, int64_t& J)
 {
+    int64_t jb = (int64_t)((sqrt(8.0 * (double)p + 1.0) - 1.0) * 0.5);
+    while ((jb + 1) * (jb + 2) / 2 <= p) ++jb;
+    while (jb * (jb + 1) / 2 > p) --jb;
+    J = jb;
+    I = p - jb * (jb + 1) / 2;
+}
+
+// a wave-uniform 64-bit value held in SGPRs (v must be uniform, or come from lane 0)
+__device__ __forceinline__ int64_t uniform_i64(int64_t v)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <int Q, bool CLAMP, int RT>
+__device__ __forceinline__ void cbie_sym_task(const float* __restrict__ rec, const int4* __restrict__ tris, int64_t nt,
+                                              float k, const float2* __restrict__ psi, float2* __restrict__ lhs,
+                                              int64_t ld, float2* __restrict__ prow, float2* __restrict__ pcol,
+                                              int64_t I, int64_t sub, int64_t J,
+                                              float2 (*__restrict__ stage)[kStageCols + 1])
+{
+    static_assert(RT % kStageCols == 0 && kSymBlock % RT == 0, "row tile");
     constexpr int S = Rec<Q>::kFloats;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t j = bx * 256 + threadIdx.x;
-    const bool active = j < nt;
-    const int64_t jj = active ? j : nt - 1;
+    const int lane = threadIdx.x & 63;
+    const int64_t j = J * kSymBlock + lane;
+    const bool jin = j < nt;
+    const int64_t jj = jin ? j : nt - 1;
     const float* cr = rec + jj * S;
     float py[3 * Q], wy[Q];
 #pragma unroll
@@ -164,95 +203,171 @@ __device__ __forceinline__ void cbie_tile(const float* __restrict__ rec, const i
     for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
     const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
     const int4 tj = tris[jj];
-    const float2 pj = active ? psi[jj] : make_float2(0.f, 0.f);
+    const float2 pj = jin ? psi[jj] : make_float2(0.f, 0.f);
     const float krev = k * kInv2Pi;
+    const bool diag = I == J;
 
-    const int64_t i_begin = by * RT;
+    const int64_t i_begin = I * kSymBlock + sub * RT;
     const int64_t i_end = min(i_begin + (int64_t)RT, nt);
-    for (int64_t i = i_begin; i < i_end; ++i) {
-        const float* rr = rec + i * S;  // wave-uniform
-        float kr = 0.f, ki = 0.f, vr = 0.f, vi = 0.f;
+    // column sums of 16-row chunks: pa = c0 (+ c1), pb = c2 (+ c3)
+    float par = 0.f, pai = 0.f, pbr = 0.f, pbi = 0.f;
+
+#pragma unroll 1
+    for (int64_t i0 = i_begin; i0 < i_end; i0 += kStageCols) {
+        const int chunk = (int)((i0 - i_begin) / kStageCols);
+        const int64_t i1 = min(i0 + (int64_t)kStageCols, i_end);
+        float cr_acc = 0.f, ci_acc = 0.f;
+#pragma unroll 1
+        for (int64_t i = i0; i < i1; ++i) {
+            const float* rr = rec + i * S;  // wave-uniform
+            const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+            float er[Q], ei[Q];
 #pragma unroll
-        for (int a = 0; a < Q; ++a) {
-            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
-            float dr = 0.f, di = 0.f, sr = 0.f, si = 0.f;
+            for (int b = 0; b < Q; ++b) er[b] = ei[b] = 0.f;
+            float kr = 0.f, ki = 0.f, vr = 0.f, vi = 0.f;
+#pragma unroll 1  // 6 independent evaluations per iteration; unrolling a as well costs registers
+            for (int a = 0; a < Q; ++a) {
+                const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+                const float ua = rr[3 * Q + a];
+                float dr = 0.f, di = 0.f, sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int b = 0; b < Q; ++b) {
+                    const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (CLAMP) r2 = fmaxf(r2, kEps2);
+                    const float rinv = __builtin_amdgcn_rsqf(r2);
+                    const float r = r2 * rinv;
+                    float sn, cs;
+                    sincos_rev(r * krev, sn, cs);
+                    const float g = rinv * wy[b];
+                    sr = fmaf(g, cs, sr);
+                    si = fmaf(g, sn, si);
+                    const float th = r * k;
+                    const float p = g * rinv * rinv;
+                    const float t1 = fmaf(th, sn, cs), t2 = fmaf(-th, cs, sn);
+                    dr = fmaf(p, t1, dr);
+                    di = fmaf(p, t2, di);
+                    const float pu = p * ua;
+                    er[b] = fmaf(pu, t1, er[b]);
+                    ei[b] = fmaf(pu, t2, ei[b]);
+                }
+                const float un = ua * ((xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2);
+                kr = fmaf(un, dr, kr);
+                ki = fmaf(un, di, ki);
+                vr = fmaf(ua, sr, vr);
+                vi = fmaf(ua, si, vi);
+            }
+            // K_ji = sum_b (y_b - x_0).n_i E_b
+            float tr = 0.f, ti = 0.f;
 #pragma unroll
             for (int b = 0; b < Q; ++b) {
-                const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
-                float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                if (CLAMP) r2 = fmaxf(r2, kEps2);
-                const float rinv = __builtin_amdgcn_rsqf(r2);
-                const float r = r2 * rinv;
-                float sn, cs;
-                sincos_rev(r * krev, sn, cs);
-                const float g = rinv * wy[b];
-                sr = fmaf(g, cs, sr);
-                si = fmaf(g, sn, si);
-                const float th = r * k;
-                const float p = g * rinv * rinv;
-                dr = fmaf(p, fmaf(th, sn, cs), dr);
-                di = fmaf(p, fmaf(-th, cs, sn), di);
+                const float mb = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                                 (py[3 * b + 2] - rr[2]) * nx2;
+                tr = fmaf(mb, er[b], tr);
+                ti = fmaf(mb, ei[b], ti);
             }
-            const float ua = rr[3 * Q + a];
-            const float un = ua * ((xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2);
-            kr = fmaf(un, dr, kr);
-            ki = fmaf(un, di, ki);
-            vr = fmaf(ua, sr, vr);
-            vi = fmaf(ua, si, vi);
+            const bool valid = jin && (!diag || j > i);
+            if (valid) lhs[i * ld + j] = make_float2(-kr, -ki);
+            stage[lane][i - i0] = make_float2(-tr, -ti);
+            const int4 ti4 = tris[i];
+            const bool shared = ti4.x == tj.x || ti4.x == tj.y || ti4.x == tj.z || ti4.y == tj.x ||
+                                ti4.y == tj.y || ti4.y == tj.z || ti4.z == tj.x || ti4.z == tj.y || ti4.z == tj.z;
+            const bool use = valid && !shared;
+            const float2 pi_ = psi[i];
+            float rsr = use ? vr * pj.x - vi * pj.y : 0.f;
+            float rsi = use ? vr * pj.y + vi * pj.x : 0.f;
+            rsr = wave_sum_f32(rsr);
+            rsi = wave_sum_f32(rsi);
+            if (lane == 0) prow[J * nt + i] = make_float2(rsr, rsi);
+            if (use) {
+                cr_acc = fmaf(vr, pi_.x, fmaf(-vi, pi_.y, cr_acc));
+                ci_acc = fmaf(vr, pi_.y, fmaf(vi, pi_.x, ci_acc));
+            }
         }
-        if (active) lhs[i * ld + j] = make_float2(-kr, -ki);
-        const int4 ti = tris[i];
-        const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
-                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z;
-        float tr = shared ? 0.f : vr * pj.x - vi * pj.y;
-        float tim = shared ? 0.f : vr * pj.y + vi * pj.x;
-        tr = wave_sum_f32(tr);
-        tim = wave_sum_f32(tim);
-        if (lane == 0) red[wave][i - i_begin] = make_float2(tr, tim);
+        if (chunk == 0) {
+            par = cr_acc;
+            pai = ci_acc;
+        } else if (chunk == 1) {
+            par = par + cr_acc;
+            pai = pai + ci_acc;
+        } else if (chunk == 2) {
+            pbr = cr_acc;
+            pbi = ci_acc;
+        } else {
+            pbr = pbr + cr_acc;
+            pbi = pbi + ci_acc;
+        }
+        // transposed entries: stage[jl][c] is (row J*64 + jl, column i0 + c)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int c = lane & (kStageCols - 1);
+        const int64_t ic = i0 + c;
+#pragma unroll 4
+        for (int s4 = 0; s4 < kSymBlock / 4; ++s4) {
+            const int jl = s4 * 4 + (lane >> 4);
+            const int64_t jr = J * kSymBlock + jl;
+            if (ic < i1 && jr < nt && (!diag || jr > ic)) lhs[jr * ld + ic] = stage[jl][c];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    __syncthreads();
-    const int64_t nrow = i_end - i_begin;
-    if (threadIdx.x < nrow) {
-        float2 a = red[0][threadIdx.x];
-#pragma unroll
-        for (int w = 1; w < 4; ++w) {
-            a.x += red[w][threadIdx.x].x;
-            a.y += red[w][threadIdx.x].y;
-        }
-        partial[bx * nt + i_begin + threadIdx.x] = a;
+    if (jin) {
+        // chunk sums combined pairwise, so that tasks of 16, 32 or 64 rows give
+        // the same bits after the finalize combines their slots the same way
+        if constexpr (RT == 64)
+            pcol[I * nt + j] = make_float2(par + pbr, pai + pbi);
+        else
+            pcol[(I * (kSymBlock / RT) + sub) * nt + j] = make_float2(par, pai);
     }
 }
 
-template <int Q, bool CLAMP>
+// Tasks of one mesh: task t -> block pair t / (64 / RT), row sub-tile t % (64 / RT).
+__host__ __device__ inline int64_t cbie_sym_tasks(int64_t nt, int rt)
+{
+    const int64_t nb = (nt + kSymBlock - 1) / kSymBlock;
+    return nb * (nb + 1) / 2 * (kSymBlock / rt);
+}
+
+template <int Q, bool CLAMP, int RT>
 __global__ __launch_bounds__(256) void dp0_cbie_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                        int64_t nt, float k, const float2* __restrict__ psi,
                                                        float2* __restrict__ lhs, int64_t ld,
-                                                       float2* __restrict__ partial)
+                                                       float2* __restrict__ prow, float2* __restrict__ pcol,
+                                                       int64_t ntasks)
 {
-    __shared__ float2 red[4][kDp0RowTile];
-    cbie_tile<Q, CLAMP, kDp0RowTile>(rec, tris, nt, k, psi, lhs, ld, partial, blockIdx.x, blockIdx.y, red);
+    __shared__ float2 stage[4][kSymBlock][kStageCols + 1];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t t = (int64_t)blockIdx.x * 4 + wave;
+    if (t >= ntasks) return;  // wave-uniform; the kernel has no workgroup barrier
+    int64_t I, J;
+    tri_block_pair(t / (kSymBlock / RT), I, J);
+    cbie_sym_task<Q, CLAMP, RT>(rec, tris, nt, k, psi, lhs, ld, prow, pcol, I, t % (kSymBlock / RT), J,
+                                stage[wave]);
 }
 
 // ---------------------------------------------------------------------------
-// batched (mesh, k) items: one persistent launch over a queue of tiles
+// batched (mesh, k) items: one persistent launch over a queue of tasks
 // ---------------------------------------------------------------------------
-// Workgroups pull task indices from a device counter until the queue is
-// empty; task t belongs to the item whose task0 is the largest <= t and is
-// the tile (t - task0) % nbx (columns), (t - task0) / nbx (rows) of its fused
-// CBIE system.  Items are ordered largest first, so the long tiles start early.
+// Each wavefront pulls task indices from a device counter until the queue is
+// empty; task t belongs to the item whose task0 is the largest <= t.  Items
+// are ordered largest first, so the long-running items start early.  Waves
+// are independent (no workgroup barrier), so a wave that finds the queue
+// empty simply exits.
 template <int Q, bool CLAMP, int RT>
 __global__ __launch_bounds__(256) void batch_cbie_persistent(const BatchItemDev* __restrict__ items, int nitems,
                                                              int64_t total_tasks,
                                                              unsigned long long* __restrict__ counter)
 {
-    __shared__ float2 red[4][RT];
-    __shared__ long long task_sh;
+    __shared__ float2 stage[4][kSymBlock][kStageCols + 1];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63;
     while (true) {
-        __syncthreads();
-        if (threadIdx.x == 0) task_sh = (long long)atomicAdd(counter, 1ull);
-        __syncthreads();
-        const long long t = task_sh;
-        if (t >= total_tasks) break;  // the same value for the whole workgroup
+        unsigned long long got = 0ull;
+        if (lane == 0) got = atomicAdd(counter, 1ull);
+        const int64_t t = uniform_i64((int64_t)got);  // lane 0 is the first active lane
+        if (t >= total_tasks) break;
         int lo = 0, hi = nitems - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -261,8 +376,10 @@ __global__ __launch_bounds__(256) void batch_cbie_persistent(const BatchItemDev*
         }
         const BatchItemDev it = items[lo];
         const int64_t local = t - it.task0;
-        const int64_t bx = local % it.nbx, by = local / it.nbx;
-        cbie_tile<Q, CLAMP, RT>(it.rec, it.tris, it.nt, it.k, it.psi, it.lhs, it.ld, it.partial, bx, by, red);
+        int64_t I, J;
+        tri_block_pair(local / (kSymBlock / RT), I, J);
+        cbie_sym_task<Q, CLAMP, RT>(it.rec, it.tris, it.nt, it.k, it.psi, it.lhs, it.ld, it.prow, it.pcol, I,
+                                    local % (kSymBlock / RT), J, stage[wave]);
     }
 }
 
@@ -282,7 +399,7 @@ static hipError_t batch_persistent_t(const BatchItemDev* items, int nitems, int6
     if (ncu <= 0) ncu = 256;
     if (per <= 0) per = 2;
     int64_t grid = (int64_t)ncu * per;
-    if (grid > total) grid = total;
+    if (grid * 4 > total) grid = (total + 3) / 4;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, s, counter);
     if (clamp)
@@ -301,15 +418,18 @@ hipError_t launch_batch_cbie_persistent(const BatchItemDev* items, int nitems, i
     case 16: return batch_persistent_t<16>(items, nitems, total, counter, clamp, s);
     case 32: return batch_persistent_t<32>(items, nitems, total, counter, clamp, s);
     case 64: return batch_persistent_t<64>(items, nitems, total, counter, clamp, s);
-    case 128: return batch_persistent_t<128>(items, nitems, total, counter, clamp, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-// rhs_i = -(sum of the column-block partials + sum of the singular V psi
-// terms of row i); lhs_ii += |T_i| / 2 (the M/2 of the CBIE).  One thread per
-// row, fixed summation order.
-__global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __restrict__ partial, int64_t nblk,
+int64_t batch_cbie_tasks(int64_t nt, int rt) { return cbie_sym_tasks(nt, rt); }
+
+// rhs_i = -(row partials + column partials + singular V psi terms of row i);
+// lhs_ii += |T_i| / 2 (the M/2 of the CBIE).  One thread per row, fixed
+// summation order; column slots of one 64-row block are combined pairwise
+// exactly as a 64-row task combines its 16-row chunks.
+__global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __restrict__ prow,
+                                                            const float2* __restrict__ pcol, int nsub,
                                                             const float2* __restrict__ vpsi,
                                                             const int64_t* __restrict__ pair_ptr,
                                                             const float* __restrict__ geo, int64_t nt,
@@ -318,11 +438,27 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= nt) return;
+    const int64_t nb = (nt + kSymBlock - 1) / kSymBlock;
+    const int64_t bi = i / kSymBlock;
     double sr = 0.0, si = 0.0;
-    for (int64_t b = 0; b < nblk; ++b) {
-        const float2 p = partial[b * nt + i];
+    for (int64_t J = bi; J < nb; ++J) {
+        const float2 p = prow[J * nt + i];
         sr += p.x;
         si += p.y;
+    }
+    for (int64_t I = 0; I <= bi; ++I) {
+        const float2* c = pcol + I * nsub * nt + i;
+        float2 v;
+        if (nsub == 1) {
+            v = c[0];
+        } else if (nsub == 2) {
+            v = make_float2(c[0].x + c[nt].x, c[0].y + c[nt].y);
+        } else {
+            v = make_float2((c[0].x + c[nt].x) + (c[2 * nt].x + c[3 * nt].x),
+                            (c[0].y + c[nt].y) + (c[2 * nt].y + c[3 * nt].y));
+        }
+        sr += v.x;
+        si += v.y;
     }
     for (int64_t e = pair_ptr[i]; e < pair_ptr[i + 1]; ++e) {
         sr += vpsi[e].x;
@@ -866,44 +1002,55 @@ static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, float k, 
 }
 
 // ---- fused DP0 CBIE system -------------------------------------------------
-int64_t cbie_work_bytes(const MeshDev& m)
+// work: prow [nb, nt] | pcol [nb * 64 / rt, nt] | vpsi [npairs] (complex64)
+int64_t cbie_work_bytes(const MeshDev& m, int rt)
 {
-    const int64_t nblk = (m.nt + 255) / 256;
-    return (int64_t)sizeof(float2) * (nblk * m.nt + m.npairs);
+    const int64_t nb = (m.nt + kSymBlock - 1) / kSymBlock;
+    return (int64_t)sizeof(float2) * (nb * m.nt + nb * (kSymBlock / rt) * m.nt + m.npairs);
+}
+
+CbieWork cbie_work_split(const MeshDev& m, int rt, void* work)
+{
+    const int64_t nb = (m.nt + kSymBlock - 1) / kSymBlock;
+    CbieWork w;
+    w.prow = (float2*)work;
+    w.pcol = w.prow + nb * m.nt;
+    w.vpsi = w.pcol + nb * (kSymBlock / rt) * m.nt;
+    w.nsub = kSymBlock / rt;
+    return w;
 }
 
 template <int Q, int NL, bool CLAMP>
 static hipError_t cbie_t(const MeshDev& m, float k, const float2* psi, float2* lhs, int64_t ld, float2* rhs,
-                         void* work, int parts, hipStream_t s)
+                         const CbieWork& w, int parts, hipStream_t s)
 {
-    const int64_t nblk = (m.nt + 255) / 256;
-    float2* partial = (float2*)work;
-    float2* vpsi = partial + nblk * m.nt;
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
-    dim3 grid((unsigned)nblk, (unsigned)((m.nt + kDp0RowTile - 1) / kDp0RowTile));
-    if (parts & 1)
-        hipLaunchKernelGGL((dp0_cbie_kernel<Q, CLAMP>), grid, dim3(256), 0, s, rec, m.tris, m.nt, k, psi, lhs, ld,
-                           partial);
+    if (parts & 1) {
+        if (w.nsub != 1) return hipErrorInvalidValue;
+        const int64_t ntasks = cbie_sym_tasks(m.nt, kSymBlock);
+        hipLaunchKernelGGL((dp0_cbie_kernel<Q, CLAMP, kSymBlock>), dim3((unsigned)((ntasks + 3) / 4)), dim3(256), 0,
+                           s, rec, m.tris, m.nt, k, psi, lhs, ld, w.prow, w.pcol, ntasks);
+    }
     if (!(parts & 2)) return hipGetLastError();
     const unsigned g = (unsigned)((m.npairs + 3) / 4);
     hipLaunchKernelGGL((singular_kernel<DLP, NL, DP0, CLAMP, 0>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
                        m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, -1.0f, nullptr, nullptr);
     hipLaunchKernelGGL((singular_kernel<SLP, NL, DP0, CLAMP, 1>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
-                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, 1.0f, psi, vpsi);
-    hipLaunchKernelGGL(cbie_finalize_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, partial, nblk, vpsi, m.pair_ptr,
-                       m.geo, m.nt, lhs, ld, rhs);
+                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, 1.0f, psi, w.vpsi);
+    hipLaunchKernelGGL(cbie_finalize_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, w.prow, w.pcol, w.nsub, w.vpsi,
+                       m.pair_ptr, m.geo, m.nt, lhs, ld, rhs);
     return hipGetLastError();
 }
 
 hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
-                           int64_t ld, float2* rhs, void* work, hipStream_t s)
+                           int64_t ld, float2* rhs, const CbieWork& w, hipStream_t s)
 {
-    if (quad == 0) return clamp ? cbie_t<6, 4, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
-                                : cbie_t<6, 4, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
-    if (quad == 1) return clamp ? cbie_t<3, 2, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
-                                : cbie_t<3, 2, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
-    return clamp ? cbie_t<1, 2, true>(m, k, psi, lhs, ld, rhs, work, parts, s)
-                 : cbie_t<1, 2, false>(m, k, psi, lhs, ld, rhs, work, parts, s);
+    if (quad == 0) return clamp ? cbie_t<6, 4, true>(m, k, psi, lhs, ld, rhs, w, parts, s)
+                                : cbie_t<6, 4, false>(m, k, psi, lhs, ld, rhs, w, parts, s);
+    if (quad == 1) return clamp ? cbie_t<3, 2, true>(m, k, psi, lhs, ld, rhs, w, parts, s)
+                                : cbie_t<3, 2, false>(m, k, psi, lhs, ld, rhs, w, parts, s);
+    return clamp ? cbie_t<1, 2, true>(m, k, psi, lhs, ld, rhs, w, parts, s)
+                 : cbie_t<1, 2, false>(m, k, psi, lhs, ld, rhs, w, parts, s);
 }
 
 // parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour pairs

@@ -274,7 +274,7 @@ int64_t bem_batch_cbie_work_size(const bem_batch_item* items, int n_items)
     size_t b = al(sizeof(BatchItemDev) * n_items) + 256;
     for (int q = 0; q < n_items; ++q) {
         if (!items[q].mesh) return 0;
-        b += al((size_t)cbie_work_bytes(items[q].mesh->dev));
+        b += al((size_t)cbie_work_bytes(items[q].mesh->dev, 16));  // the largest row-tile layout
     }
     return (int64_t)b;
 }
@@ -293,7 +293,7 @@ static int batch_run(const bem_batch_item* items, int n_items, int clamp, int rt
     std::vector<char*> wk(n_items);
     for (int q = 0; q < n_items; ++q) {
         wk[q] = cur;
-        cur += al((size_t)cbie_work_bytes(items[q].mesh->dev));
+        cur += al((size_t)cbie_work_bytes(items[q].mesh->dev, 16));
     }
     std::vector<BatchItemDev> h(n_items);
     int64_t task = 0;
@@ -308,10 +308,11 @@ static int batch_run(const bem_batch_item* items, int n_items, int clamp, int rt
         d.psi = (const float2*)items[q].psi;
         d.lhs = (float2*)items[q].lhs;
         d.ld = items[q].ld;
-        d.partial = (float2*)wk[q];
-        d.nbx = (m.nt + 255) / 256;
+        const CbieWork w = cbie_work_split(m, rt, wk[q]);
+        d.prow = w.prow;
+        d.pcol = w.pcol;
         d.task0 = task;
-        task += d.nbx * ((m.nt + rt - 1) / rt);
+        task += batch_cbie_tasks(m.nt, rt);
         h[e] = d;
     }
     XTRY(hipMemcpyAsync(table, h.data(), sizeof(BatchItemDev) * n_items, hipMemcpyHostToDevice, s));
@@ -320,7 +321,8 @@ static int batch_run(const bem_batch_item* items, int n_items, int clamp, int rt
     if (singular)
         for (int q = 0; q < n_items; ++q)
             XTRY(launch_cbie_dp0(items[q].mesh->dev, items[q].k, 0, clamp, 2, (const float2*)items[q].psi,
-                                 (float2*)items[q].lhs, items[q].ld, (float2*)items[q].rhs, wk[q], s));
+                                 (float2*)items[q].lhs, items[q].ld, (float2*)items[q].rhs,
+                                 cbie_work_split(items[q].mesh->dev, rt, wk[q]), s));
     return BEM_OK;
 }
 
@@ -333,13 +335,13 @@ int bem_batch_autotune(const bem_batch_item* items, int n_items, int clamp, void
     int big = 0;
     for (int q = 1; q < n_items; ++q)
         if (items[q].mesh->dev.nt > items[big].mesh->dev.nt) big = q;
-    const int cands[4] = {16, 32, 64, 128};
+    const int cands[3] = {16, 32, 64};
     hipEvent_t e0, e1;
     XTRY(hipEventCreate(&e0));
     XTRY(hipEventCreate(&e1));
     float bestms = 1e30f;
     int bestv = 64;
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < 3; ++c) {
         for (int rep = 0; rep < 2; ++rep) {  // first run warms up
             XTRY(hipEventRecord(e0, s));
             if (int rc = batch_run(&items[big], 1, clamp, cands[c], work, s, false)) return rc;
@@ -388,8 +390,8 @@ int bem_batch_cbie(const bem_batch_item* items, int n_items, int clamp, int row_
         auto f = g_tune.find({dev, bucket});
         row_tile = f != g_tune.end() ? f->second : 64;
     }
-    if (row_tile != 16 && row_tile != 32 && row_tile != 64 && row_tile != 128) {
-        g_batch_err = "row_tile must be 0 (tuned), 16, 32, 64 or 128";
+    if (row_tile != 16 && row_tile != 32 && row_tile != 64) {
+        g_batch_err = "row_tile must be 0 (tuned), 16, 32 or 64";
         return BEM_ERR_BAD_ARG;
     }
     return batch_run(items, n_items, clamp, row_tile, work, reinterpret_cast<hipStream_t>(stream), true);

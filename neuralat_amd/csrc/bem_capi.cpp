@@ -264,7 +264,7 @@ int bem_add_identity(const bem_mesh* mesh, int basis, float alpha_re, float alph
 int64_t bem_cbie_work_size(const bem_mesh* mesh)
 {
     if (!mesh) return 0;
-    return bem::cbie_work_bytes(mesh->dev);
+    return bem::cbie_work_bytes(mesh->dev, 64);
 }
 
 int bem_cbie_dp0(const bem_mesh* mesh, float k, int quad, int clamp, const void* psi, void* lhs, int64_t ld,
@@ -285,7 +285,7 @@ int bem_cbie_dp0_ex(const bem_mesh* mesh, float k, int quad, int clamp, int part
     if ((m.nt + kDp0RowTileHost - 1) / kDp0RowTileHost > 65535) return fail(BEM_ERR_BAD_ARG, "mesh too large");
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_cbie_dp0(m, k, quad, clamp, parts, (const float2*)psi, (float2*)lhs, ld, (float2*)rhs,
-                                        work, S(stream));
+                                        bem::cbie_work_split(m, 64, work), S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_cbie_dp0 launch");
     return BEM_OK;
 }
@@ -397,6 +397,13 @@ int bem_gmres(const void* A_, int64_t n, int64_t lda, const void* b_, void* x_, 
 int bem_gmres_ex(const void* A_, int64_t n, int64_t lda, int trans, const void* b_, void* x_, float tol, int restart,
                  int maxiter, void* work, bem_solve_info* info, void* stream)
 {
+    return bem_gmres_timed(A_, n, lda, trans, b_, x_, tol, restart, maxiter, work, info, nullptr, nullptr, stream);
+}
+
+int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const void* b_, void* x_, float tol,
+                    int restart, int maxiter, void* work, bem_solve_info* info, double* gemv_ms, int64_t* gemv_count,
+                    void* stream)
+{
     if (!A_ || !b_ || !x_ || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
     if (trans < 0 || trans > 2) return fail(BEM_ERR_BAD_ARG, "trans must be 0, 1 or 2");
     if (n <= 0 || lda < n || restart <= 0 || maxiter < 0 || !(tol > 0.f))
@@ -435,16 +442,53 @@ int bem_gmres_ex(const void* A_, int64_t n, int64_t lda, int trans, const void* 
         return BEM_OK;
     }
     const double target = (double)tol * bnorm;
+    // optional HIP-event timing of the matrix-vector products; each product
+    // is followed by a host read of a reduction, so its events are complete
+    // when the time is read
+    hipEvent_t te0 = nullptr, te1 = nullptr;
+    const bool timed = gemv_ms || gemv_count;
+    if (timed) {
+        HIP_TRY(hipEventCreate(&te0));
+        HIP_TRY(hipEventCreate(&te1));
+        if (gemv_ms) *gemv_ms = 0.0;
+        if (gemv_count) *gemv_count = 0;
+    }
+    auto gemv_timed = [&](const float2* xv, float2* yv, float2 al, float2 be) -> hipError_t {
+        if (timed) {
+            hipError_t e = hipEventRecord(te0, s);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = bem::launch_gemv(A, n, n, lda, trans, xv, yv, al, be, tw, s);
+        if (e != hipSuccess || !timed) return e;
+        return hipEventRecord(te1, s);
+    };
+    auto gemv_account = [&]() -> int {
+        if (!timed) return BEM_OK;
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, te0, te1));
+        if (gemv_ms) *gemv_ms += ms;
+        if (gemv_count) *gemv_count += 1;
+        return BEM_OK;
+    };
+    struct EventFree {
+        hipEvent_t a, b;
+        ~EventFree()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } event_free{te0, te1};
     int its = 0;
     double rnorm = 0.0, implicit = 0.0;
     bool conv = false;
     while (true) {
         // r = b - A x
         HIP_TRY(hipMemcpyAsync(r, b, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
-        HIP_TRY(bem::launch_gemv(A, n, n, lda, trans, x, r, mone, one, tw, s));
+        HIP_TRY(gemv_timed(x, r, mone, one));
         HIP_TRY(bem::launch_norm2(r, n, sc, s));
         double rn2 = 0.0;
         if (int rc = read_double(sc, &rn2, s)) return rc;
+        if (int rc = gemv_account()) return rc;
         rnorm = std::sqrt(rn2);
         if (rnorm <= target) {
             conv = true;
@@ -456,7 +500,7 @@ int bem_gmres_ex(const void* A_, int64_t n, int64_t lda, int trans, const void* 
         int nk = 0;
         for (int j = 0; j < m && its < maxiter; ++j) {
             float2* vj = V + (size_t)j * n;
-            HIP_TRY(bem::launch_gemv(A, n, n, lda, trans, vj, w, one, zero, tw, s));
+            HIP_TRY(gemv_timed(vj, w, one, zero));
             // classical Gram-Schmidt, twice
             HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, s));
             HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));
@@ -468,6 +512,7 @@ int bem_gmres_ex(const void* A_, int64_t n, int64_t lda, int trans, const void* 
             ++its;
             nk = j + 1;
             if (int rc = read_double(res + j, &implicit, s)) return rc;
+            if (int rc = gemv_account()) return rc;
             if (implicit <= target) break;
         }
         HIP_TRY(bem::launch_backsolve(H, m, nk, g, y, s));

@@ -71,15 +71,26 @@ struct BatchItemDev {
     const float2* psi;  // [nt]
     float2* lhs;        // [nt, ld]
     int64_t ld;
-    float2* partial;    // [ceil(nt/256), nt]
+    float2* prow;       // row partials of the fused CBIE pass
+    float2* pcol;       // column partials
     int64_t task0;      // first task of the item in the queue
-    int64_t nbx;        // 256-column blocks
 };
 hipError_t launch_batch_cbie_persistent(const BatchItemDev* items, int nitems, int64_t total, int rt,
                                         unsigned long long* counter, int clamp, hipStream_t s);
-int64_t cbie_work_bytes(const MeshDev& m);
+int64_t batch_cbie_tasks(int64_t nt, int rt);
+// Scratch of the fused DP0 CBIE pass for row tile rt (16, 32 or 64 rows per
+// task): row partials, column partials (64 / rt slots per 64-row block) and
+// the singular V psi terms.
+struct CbieWork {
+    float2* prow;
+    float2* pcol;
+    float2* vpsi;
+    int nsub;
+};
+int64_t cbie_work_bytes(const MeshDev& m, int rt);
+CbieWork cbie_work_split(const MeshDev& m, int rt, void* work);
 hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int parts, const float2* psi, float2* lhs,
-                           int64_t ld, float2* rhs, void* work, hipStream_t s);
+                           int64_t ld, float2* rhs, const CbieWork& w, hipStream_t s);
 int64_t adjoint_work_bytes(const MeshDev& m);
 hipError_t launch_adjoint_pairs(const MeshDev& m, float k, int clamp, const float2* lam, const float2* psi,
                                 const float2* phi, double* gvert, double* gk, double* vhl, void* work, hipStream_t s);

@@ -211,20 +211,40 @@ __global__ __launch_bounds__(256) void multidot_kernel(const float2* __restrict_
     if (threadIdx.x == 0) hout[q] = make_double2(sr, si);
 }
 
-// w[e] -= sum_q h[q] V[q][e]; hacc[q] += h[q] (second Gram-Schmidt pass folds in)
-__global__ __launch_bounds__(256) void gs_update_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n, int nq,
-                                                        const double2* __restrict__ h, float2* __restrict__ w)
+// out[e] += sign * sum_q c[q] V[q][e] over nq basis vectors (the Gram-Schmidt
+// update with sign -1, the solution update with sign +1).  A workgroup owns 64
+// consecutive elements; its 8 wavefronts take every 8th basis vector, so a long
+// Krylov basis keeps 8x more loads in flight than one thread per element, and
+// the 8 FP64 partial sums meet in LDS in a fixed order (deterministic).
+constexpr int kBasisWaves = 8;
+__global__ __launch_bounds__(64 * kBasisWaves) void basis_combine_kernel(const float2* __restrict__ V, int64_t ldv,
+                                                                         int64_t n, int nq,
+                                                                         const double2* __restrict__ c, double sign,
+                                                                         float2* __restrict__ out)
 {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    double wr = w[e].x, wi = w[e].y;
-    for (int q = 0; q < nq; ++q) {
-        const double2 hq = h[q];
-        const float2 v = V[(int64_t)q * ldv + e];
-        wr -= hq.x * v.x - hq.y * v.y;
-        wi -= hq.x * v.y + hq.y * v.x;
+    __shared__ double2 part[kBasisWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+    double ar = 0.0, ai = 0.0;
+    if (e < n) {
+        for (int q = wv; q < nq; q += kBasisWaves) {
+            const double2 cq = c[q];
+            const float2 v = V[(int64_t)q * ldv + e];
+            ar += cq.x * v.x - cq.y * v.y;
+            ai += cq.x * v.y + cq.y * v.x;
+        }
     }
-    w[e] = make_float2((float)wr, (float)wi);
+    part[wv][lane] = make_double2(ar, ai);
+    __syncthreads();
+    if (wv == 0 && e < n) {
+        double sr = 0.0, si = 0.0;
+        for (int q = 0; q < kBasisWaves; ++q) {
+            sr += part[q][lane].x;
+            si += part[q][lane].y;
+        }
+        const float2 o = out[e];
+        out[e] = make_float2((float)((double)o.x + sign * sr), (float)((double)o.y + sign * si));
+    }
 }
 
 // ||w||^2 into out[0] (double), one workgroup of 1024 threads
@@ -328,23 +348,6 @@ __global__ void gmres_backsolve_kernel(const double2* __restrict__ H, int m, int
         y[q] = dd > 0.0 ? make_double2((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd)
                         : make_double2(0.0, 0.0);
     }
-}
-
-// x[e] += sum_q y[q] V[q][e]
-__global__ __launch_bounds__(256) void gmres_update_x_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n,
-                                                             int nk, const double2* __restrict__ y,
-                                                             float2* __restrict__ x)
-{
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    double xr = x[e].x, xi = x[e].y;
-    for (int q = 0; q < nk; ++q) {
-        const double2 yq = y[q];
-        const float2 v = V[(int64_t)q * ldv + e];
-        xr += yq.x * v.x - yq.y * v.y;
-        xi += yq.x * v.y + yq.y * v.x;
-    }
-    x[e] = make_float2((float)xr, (float)xi);
 }
 
 // g = (beta, 0, ...), beta = sqrt(*nrm2)
@@ -668,7 +671,8 @@ hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, cons
 hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL(gs_update_kernel, dim3(g1(n)), dim3(256), 0, s, V, ldv, n, nq, h, w);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, n, nq, h, -1.0,
+                       w);
     return hipGetLastError();
 }
 hipError_t launch_norm2(const float2* w, int64_t n, double* out, hipStream_t s)
@@ -700,7 +704,8 @@ hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, d
 hipError_t launch_update_x(const float2* V, int64_t ldv, int64_t n, int nk, const double2* y, float2* x,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(gmres_update_x_kernel, dim3(g1(n)), dim3(256), 0, s, V, ldv, n, nk, y, x);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, n, nk, y, 1.0,
+                       x);
     return hipGetLastError();
 }
 hipError_t launch_init_g(double2* g, int m, const double* nrm2, hipStream_t s)
