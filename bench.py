@@ -42,8 +42,8 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # HBM bytes per matrix entry per launch (FETCH_SIZE + WRITE_SIZE, scaled by
 # the counters' reading for a copy of known size), from the hardware-counter
 # passes over ``bench.py --traffic-probe`` at level 6 kept under profiles/.
-GEMV_TRAFFIC_PER_ENTRY = None
-CBIE_TRAFFIC_PER_ENTRY = None
+GEMV_TRAFFIC_PER_ENTRY = 8.0007  # 53.69 GB fetched + 0.7 MB written per level-6 launch
+CBIE_TRAFFIC_PER_ENTRY = 10.231  # 14.0 GB fetched (geometry records) + 54.7 GB written
 DLP_FLOP_PER_EVAL = 29  # algorithmic count per Gauss-point pair (SLP: 16)
 SLP_FLOP_PER_EVAL = 16
 
@@ -60,7 +60,7 @@ def parse():
     p.add_argument("--level", type=int, default=6)
     p.add_argument("--ks", type=str, default="10,30,60")
     p.add_argument("--tol", type=float, default=1e-6)
-    p.add_argument("--restart", type=int, default=200)
+    p.add_argument("--restart", type=int, default=1500)
     p.add_argument("--maxiter", type=int, default=3000)
     p.add_argument("--cpu-seconds", type=float, default=15.0)
     p.add_argument("--no-cpu", action="store_true")
@@ -226,7 +226,7 @@ def _monopole_psis(v, t, k, srcs):
     return np.array(out, dtype=np.complex64)
 
 
-def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
+def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500):
     """16 right-hand sides on the assembled level-6 system at k: plain FP32
     (one GMRES solve per right-hand side, timed on one of them, with the fused
     assembly and potentials of the plain path) against 16-bit storage +
@@ -237,8 +237,8 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
     import torch
 
     from neuralat_amd import _lib, mesh
-    from neuralat_amd.bem.precision import LowPrecisionMatrix, matmul, solve_ir
-    from neuralat_amd.bem.solver import _ptr, _stream, gmres, matvec
+    from neuralat_amd.bem.precision import LowPrecisionMatrix, gmres_multi, matmul, solve_ir
+    from neuralat_amd.bem.solver import _ptr, _stream
 
     torch.cuda.synchronize()
     rng = np.random.default_rng(seed)
@@ -254,7 +254,7 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
 
     # plain FP32 path, one right-hand side (the same for each of the 16)
     t0 = time.perf_counter()
-    phi0, p0, info0 = s.solve(k, psi[0], pts_t, 1e-6, 200, 3000)
+    phi0, p0, info0 = s.solve(k, psi[0], pts_t, 1e-6, restart, 3000)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     res["plain_fp32"] = {"seconds_per_solve": t_plain, "solves_per_s": 1.0 / t_plain,
@@ -262,8 +262,10 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
                          "rel_err_vs_analytic": float(np.linalg.norm(p0.cpu().numpy() - fields[0])
                                                       / np.linalg.norm(fields[0]))}
     log(f"bench: precision plain FP32 {t_plain:.2f} s/solve, {info0['iterations']} its")
-    # 16-bit: K (fused pass, lhs) and V assembled once; B = -V Psi on matrix cores
-    for dt in ("bf16", "fp16"):
+    # K (fused pass, lhs) and V assembled once; B = -V Psi on the FP32 matrix
+    # cores; then all 16 systems together: FP32 lockstep GMRES on the complex64
+    # matrix, or 16-bit storage + flexible FP32 refinement
+    for form in ("fp32_multi", "bf16", "fp16"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.assemble(k, psi[0])
@@ -272,27 +274,32 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4):
                                     _ptr(Vm), n, _stream()), "bem_assemble")
         B = matmul(Vm, psi, alpha=-1.0)
         del Vm
-        L = LowPrecisionMatrix(s.lhs, dt)
-        X, info = solve_ir(s.lhs, L, B, tol=1e-6, inner_tol=1e-2 if dt == "bf16" else 1e-3, restart=100,
-                           max_inner=2000, max_outer=20)
+        if form == "fp32_multi":
+            X, infos = gmres_multi(s.lhs, B, tol=1e-6, restart=600, maxiter=3000)
+            info = {"outer": infos[0]["iterations"], "inner": 0, "rel_residual": [i["rel_residual"] for i in infos],
+                    "converged": all(i["converged"] for i in infos)}
+        else:
+            L = LowPrecisionMatrix(s.lhs, form)
+            X, info = solve_ir(s.lhs, L, B, tol=1e-6, inner_tol=1e-2, restart=100, max_inner=300, max_outer=40)
+            del L
         Sp = s.potential(k, pts_t, "single")
         Dp = s.potential(k, pts_t, "double")
         P = matmul(Dp, X)
         matmul(Sp, psi, alpha=-1.0, beta=1.0, out=P)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        del L
         Pc = P.cpu().numpy()
         errs = [float(np.linalg.norm(Pc[q] - fields[q]) / np.linalg.norm(fields[q])) for q in range(nrhs)]
         x0 = X[0]
         rel_fp32 = float((torch.linalg.vector_norm(x0 - phi0) / torch.linalg.vector_norm(phi0)).item())
-        res[dt] = {"seconds": el, "solves_per_s": nrhs / el, "speedup_vs_plain": (nrhs / el) * t_plain,
-                   "outer_iterations": info["outer"], "inner_iterations": info["inner"],
-                   "max_rel_residual": float(max(info["rel_residual"])), "converged": bool(info["converged"]),
-                   "surface_rel_diff_vs_fp32_solution": rel_fp32,
-                   "max_rel_err_vs_analytic": float(max(errs))}
-        log(f"bench: precision {dt}: {el:.2f} s for {nrhs} solves ({nrhs / el:.2f}/s, "
-            f"{(nrhs / el) * t_plain:.1f}x plain), outer {info['outer']}, converged {info['converged']}")
+        res[form] = {"seconds": el, "solves_per_s": nrhs / el, "speedup_vs_plain": (nrhs / el) * t_plain,
+                     "outer_iterations": info["outer"], "inner_iterations": info["inner"],
+                     "max_rel_residual": float(max(info["rel_residual"])), "converged": bool(info["converged"]),
+                     "surface_rel_diff_vs_fp32_solution": rel_fp32,
+                     "max_rel_err_vs_analytic": float(max(errs))}
+        log(f"bench: precision {form}: {el:.2f} s for {nrhs} solves ({nrhs / el:.2f}/s, "
+            f"{(nrhs / el) * t_plain:.1f}x plain), outer {info['outer']}, inner {info['inner']}, "
+            f"residual {max(info['rel_residual']):.1e}")
     if check_level:
         res["fp64_check"] = precision_fp64_check(check_level, k, srcs[:2])
     return res
@@ -321,7 +328,7 @@ def precision_fp64_check(level, k, srcs):
             s.assemble(k, torch.tensor(psis[q], device=s.dev))
             rhs.append(s.rhs.clone())
         L = LowPrecisionMatrix(s.lhs, dt)
-        X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=1e-6, inner_tol=1e-2 if dt == "bf16" else 1e-3)
+        X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=1e-6, inner_tol=1e-2, max_inner=300, max_outer=40)
         Xc = X.cpu().numpy()
         errs = []
         for q in range(len(srcs)):
@@ -408,7 +415,7 @@ def make_batch(nmesh, nk, seed=7):
     return out
 
 
-def batch_form(nmesh=8, nk=8, restart=50, max_cycles=40, seed=7):
+def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
     """Plain per-item loop (fused assembly + GMRES per (mesh, k)) against one
     persistent launch for all items + lockstep graph-captured GMRES.  Neumann
     data: one interior monopole per mesh."""
@@ -447,6 +454,8 @@ def batch_form(nmesh=8, nk=8, restart=50, max_cycles=40, seed=7):
             xs_plain.append(x)
             conv_plain += int(info["converged"])
             q += 1
+            if q % 8 == 0:
+                log(f"bench: batch plain loop {q}/{len(ks)} solves")
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     log(f"bench: batch plain loop {t_plain:.2f} s for {len(ks)} solves")
@@ -618,7 +627,7 @@ def main():
     log(f"bench: matvec {mv['achieved']:.0f} GB/s ({mv['frac']:.3f} of peak)")
     forms = {}
     if not args.no_forms:
-        forms["precision"] = precision_form(s, v, t, ks[-1], pts)
+        forms["precision"] = precision_form(s, v, t, ks[-1], pts, restart=args.restart)
     del s
     torch.cuda.empty_cache()
     if not args.no_forms:

@@ -195,10 +195,13 @@ int64_t bem_gmres_multi_work_size(int64_t n, int nrhs, int restart, int64_t ldlp
 int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp,
                     float inv_scale, int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol,
                     int restart, int maxiter, void* work, bem_solve_info* info, void* stream);
-/* Mixed-precision iterative refinement: x = 0; repeat { solve the 16-bit
- * system for the correction to inner_tol with bem_gmres_multi; x += d;
- * r = b - A x with the complex64 A } until ||r|| <= tol ||b|| for every
- * system.  info[q].iterations = total inner iterations. */
+/* Mixed-precision refinement with a Krylov outer loop: x = 0; flexible GMRES
+ * in FP32 on the complex64 A (residuals and updates in FP32, basis of 20,
+ * restarted beyond it, at most max_outer outer steps), each outer direction
+ * preconditioned by a 16-bit multi-RHS GMRES solve (restart, max_inner) to
+ * inner_tol; until ||b - A x|| <= tol ||b|| for every system.
+ * info[q].iterations = total inner iterations, *outer_iterations = outer
+ * steps. */
 int64_t bem_solve_ir_work_size(int64_t n, int nrhs, int restart, int64_t ldlp);
 int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp, float inv_scale,
                  int64_t n, int nrhs, const void* B, int64_t ldb, void* X, int64_t ldx, float tol, float inner_tol,

@@ -107,10 +107,11 @@ def gmres_multi(A, B: torch.Tensor, tol=1e-6, restart=100, maxiter=1000, raise_o
 
 
 def solve_ir(A: torch.Tensor, Alp: LowPrecisionMatrix, B: torch.Tensor, tol=1e-6, inner_tol=1e-3, restart=100,
-             max_inner=1000, max_outer=10, raise_on_fail=False):
-    """Mixed-precision iterative refinement (FP32 residual on A, inner GMRES on
-    the 16-bit Alp).  Returns (X, info) with info = {"outer", "inner",
-    "rel_residual": [...], "converged"}."""
+             max_inner=1000, max_outer=30, raise_on_fail=False):
+    """Mixed-precision refinement: flexible GMRES in FP32 on A (FP32 residuals
+    and updates) whose directions are preconditioned by inner lockstep GMRES
+    solves on the 16-bit Alp to inner_tol.  Returns (X, info) with info =
+    {"outer", "inner", "rel_residual": [...], "converged"}."""
     check_tensor(A, torch.complex64)
     B = _as_block(B)
     nrhs, n = B.shape

@@ -143,6 +143,62 @@ __global__ __launch_bounds__(256) void gemv_t_partial_kernel(const float2* __res
     part[(int64_t)blockIdx.y * cols + c] = make_float2(ar, ai);
 }
 
+// y = op(A)^T-style product, vector form: a workgroup owns 128 columns (two
+// per lane, 16-byte loads: each wavefront reads a contiguous 1 KB of a row) and
+// a slab of kTVecSlab rows, split over its 4 wavefronts (rows w, w+4, ...);
+// the 4 wave partials meet in LDS in a fixed order.  Needs A and lda such that
+// every row starts 16-byte aligned (checked by the launcher).
+constexpr int kTVecSlab = 128;
+__global__ __launch_bounds__(256) void gemv_t_vec_partial_kernel(const float2* __restrict__ A, int64_t rows,
+                                                                 int64_t cols, int64_t lda,
+                                                                 const float2* __restrict__ x, int conj,
+                                                                 float2* __restrict__ part)
+{
+    __shared__ float4 red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * 128 + 2 * lane;
+    const int64_t r0 = (int64_t)blockIdx.y * kTVecSlab;
+    const int64_t r1 = min(r0 + (int64_t)kTVecSlab, rows);
+    const float sgn = conj ? -1.f : 1.f;
+    float a0r = 0.f, a0i = 0.f, a1r = 0.f, a1i = 0.f;
+    if (c + 1 < cols) {
+#pragma unroll 4
+        for (int64_t r = r0 + wave; r < r1; r += 4) {
+            const float4 a = *reinterpret_cast<const float4*>(A + r * lda + c);
+            const float2 xv = x[r];
+            const float b0 = sgn * a.y, b1 = sgn * a.w;
+            a0r = fmaf(a.x, xv.x, fmaf(-b0, xv.y, a0r));
+            a0i = fmaf(a.x, xv.y, fmaf(b0, xv.x, a0i));
+            a1r = fmaf(a.z, xv.x, fmaf(-b1, xv.y, a1r));
+            a1i = fmaf(a.z, xv.y, fmaf(b1, xv.x, a1i));
+        }
+    } else if (c < cols) {
+        for (int64_t r = r0 + wave; r < r1; r += 4) {
+            const float2 a = A[r * lda + c];
+            const float2 xv = x[r];
+            const float b0 = sgn * a.y;
+            a0r = fmaf(a.x, xv.x, fmaf(-b0, xv.y, a0r));
+            a0i = fmaf(a.x, xv.y, fmaf(b0, xv.x, a0i));
+        }
+    }
+    red[wave][lane] = make_float4(a0r, a0i, a1r, a1i);
+    __syncthreads();
+    if (wave == 0 && c < cols) {
+        float4 t = red[0][lane];
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            const float4 u = red[w][lane];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        float2* out = part + (int64_t)blockIdx.y * cols + c;
+        out[0] = make_float2(t.x, t.y);
+        if (c + 1 < cols) out[1] = make_float2(t.z, t.w);
+    }
+}
+
 __global__ __launch_bounds__(256) void gemv_t_reduce_kernel(const float2* __restrict__ part, int64_t nslab,
                                                             int64_t cols, float2* __restrict__ y, float2 alpha,
                                                             float2 beta, int beta_zero)
@@ -293,42 +349,64 @@ __global__ __launch_bounds__(256) void normalize_kernel(const float2* __restrict
 //   column j of H gets h (length j+1, from CGS2) and ||w||; previous Givens
 //   rotations are applied, a new one zeroes H[j+1][j], g is rotated and
 //   |g[j+1]| is stored in res[j].  H: (m+1) x m column-major (double2).
-__global__ void gmres_givens_kernel(double2* __restrict__ H, int m, int j, const double2* __restrict__ h,
-                                    const double2* __restrict__ h2, const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn,
-                                    double2* __restrict__ g, double* __restrict__ res)
+// New Hessenberg column j = h + h2 (the two Gram-Schmidt passes), rotated by
+// the j previous Givens rotations, then the new rotation and the residual
+// estimate.  The rotations are a serial chain: the workgroup stages them (and
+// the column) in LDS 256 at a time so that thread 0 never waits on a global
+// load inside the chain; the value carried from one rotation to the next
+// stays in registers.
+constexpr int kGivensChunk = 256;
+__global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
+    double2* __restrict__ H, int m, int j, const double2* __restrict__ h, const double2* __restrict__ h2,
+    const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+    double* __restrict__ res)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double2 scs[kGivensChunk], ssn[kGivensChunk], sb[kGivensChunk];
     double2* col = H + (int64_t)j * (m + 1);
-    for (int q = 0; q <= j; ++q) col[q] = make_double2(h[q].x + h2[q].x, h[q].y + h2[q].y);
-    col[j + 1] = make_double2(sqrt(*wn2), 0.0);
-    for (int q = 0; q < j; ++q) {
-        // [c s; -conj(s) conj(c)] applied to (col[q], col[q+1])
-        const double2 c = cs[q], s = sn[q], a = col[q], b = col[q + 1];
-        const double2 t = make_double2(c.x * a.x - c.y * a.y + s.x * b.x - s.y * b.y,
-                                       c.x * a.y + c.y * a.x + s.x * b.y + s.y * b.x);
-        const double2 u = make_double2(-(s.x * a.x + s.y * a.y) + (c.x * b.x + c.y * b.y),
-                                       -(s.x * a.y - s.y * a.x) + (c.x * b.y - c.y * b.x));
-        col[q] = t;
-        col[q + 1] = u;
+    const int t = threadIdx.x;
+    double2 carry = make_double2(h[0].x + h2[0].x, h[0].y + h2[0].y);  // col[q] entering rotation q
+    for (int q0 = 0; q0 < j; q0 += kGivensChunk) {
+        const int cnt = min(kGivensChunk, j - q0);
+        if (t < cnt) {
+            scs[t] = cs[q0 + t];
+            ssn[t] = sn[q0 + t];
+            const int e = q0 + t + 1;
+            sb[t] = make_double2(h[e].x + h2[e].x, h[e].y + h2[e].y);
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int u = 0; u < cnt; ++u) {
+                // [c s; -conj(s) conj(c)] applied to (col[q], col[q+1])
+                const double2 c = scs[u], sv = ssn[u], a = carry, b = sb[u];
+                const double2 top = make_double2(c.x * a.x - c.y * a.y + sv.x * b.x - sv.y * b.y,
+                                                 c.x * a.y + c.y * a.x + sv.x * b.y + sv.y * b.x);
+                carry = make_double2(-(sv.x * a.x + sv.y * a.y) + (c.x * b.x + c.y * b.y),
+                                     -(sv.x * a.y - sv.y * a.x) + (c.x * b.y - c.y * b.x));
+                col[q0 + u] = top;
+            }
+        }
+        __syncthreads();
     }
-    const double2 a = col[j], b = col[j + 1];
+    if (t != 0) return;
+    const double2 a = carry, b = make_double2(sqrt(*wn2), 0.0);
     const double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
-    double2 c, s;
+    double2 c, sv;
     if (den > 0.0) {
-        c = make_double2(a.x / den, -a.y / den);  // conj(a)/den
-        s = make_double2(b.x / den, -b.y / den);  // conj(b)/den
+        c = make_double2(a.x / den, -a.y / den);   // conj(a)/den
+        sv = make_double2(b.x / den, -b.y / den);  // conj(b)/den
     } else {
         c = make_double2(1.0, 0.0);
-        s = make_double2(0.0, 0.0);
+        sv = make_double2(0.0, 0.0);
     }
     cs[j] = c;
-    sn[j] = s;
+    sn[j] = sv;
     col[j] = make_double2(den, 0.0);
     col[j + 1] = make_double2(0.0, 0.0);
     const double2 gj = g[j];
-    g[j + 1] = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));  // -conj(s) g_j
+    const double2 gn = make_double2(-(sv.x * gj.x + sv.y * gj.y), -(sv.x * gj.y - sv.y * gj.x));  // -conj(s) g_j
+    g[j + 1] = gn;
     g[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
-    res[j] = sqrt(g[j + 1].x * g[j + 1].x + g[j + 1].y * g[j + 1].y);
+    res[j] = sqrt(gn.x * gn.x + gn.y * gn.y);
 }
 
 // back substitution y = R^{-1} g[0:nk] (R upper triangular, H layout above)
@@ -620,6 +698,10 @@ __global__ __launch_bounds__(256) void badd_kernel(float2* __restrict__ x, int64
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+static inline bool gemv_t_vec_ok(const float2* A, int64_t lda)
+{
+    return ((uintptr_t)A % 16 == 0) && (lda % 2 == 0);
+}
 static inline unsigned g1(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
 hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda, int trans, const float2* x, float2* y,
@@ -640,6 +722,14 @@ hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda,
         }
         return hipGetLastError();
     }
+    if (gemv_t_vec_ok(A, lda)) {
+        const int64_t nslab = (rows + kTVecSlab - 1) / kTVecSlab;
+        hipLaunchKernelGGL(gemv_t_vec_partial_kernel, dim3(g1(cols, 128), (unsigned)nslab), dim3(256), 0, s, A, rows,
+                           cols, lda, x, trans == 2 ? 1 : 0, work);
+        hipLaunchKernelGGL(gemv_t_reduce_kernel, dim3(g1(cols)), dim3(256), 0, s, work, nslab, cols, y, alpha, beta,
+                           bz);
+        return hipGetLastError();
+    }
     const int64_t nslab = (rows + kTSlabRows - 1) / kTSlabRows;
     hipLaunchKernelGGL(gemv_t_partial_kernel, dim3(g1(cols), (unsigned)nslab), dim3(256), 0, s, A, rows, cols, lda, x,
                        trans == 2 ? 1 : 0, work);
@@ -650,7 +740,8 @@ hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda,
 int64_t gemv_work_elems(int64_t rows, int64_t cols, int trans)
 {
     if (trans == 0) return 0;
-    return ((rows + kTSlabRows - 1) / kTSlabRows) * cols;
+    // enough for either partial layout (the vector form has the smaller slabs)
+    return ((rows + kTVecSlab - 1) / kTVecSlab) * cols;
 }
 
 hipError_t launch_combine(const float2* A, int64_t lda, const float2* B, int64_t ldb, float2* C, int64_t ldc,
@@ -693,7 +784,7 @@ hipError_t launch_normalize(const float2* src, int64_t n, const double* nrm2, fl
 hipError_t launch_givens(double2* H, int m, int j, const double2* h, const double2* h2, const double* wn2,
                          double2* cs, double2* sn, double2* g, double* res, hipStream_t s)
 {
-    hipLaunchKernelGGL(gmres_givens_kernel, dim3(1), dim3(64), 0, s, H, m, j, h, h2, wn2, cs, sn, g, res);
+    hipLaunchKernelGGL(gmres_givens_kernel, dim3(1), dim3(kGivensChunk), 0, s, H, m, j, h, h2, wn2, cs, sn, g, res);
     return hipGetLastError();
 }
 hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s)

@@ -37,10 +37,24 @@ struct IrInfo {
     std::vector<double> rel_residual;
 };
 
+// Flexible right preconditioner of gmres_multi: an inner multi-RHS GMRES on op
+// to tol (restart m, at most maxiter iterations) with its own workspace; the
+// preconditioned directions go to Z ((restart of the outer solve) x ns x ldn).
+struct FlexPrecond {
+    const MultiOp* op = nullptr;
+    double tol = 1e-2;
+    int m = 100;
+    int maxiter = 1000;
+    void* work = nullptr;
+    float2* Z = nullptr;
+    int inner_iterations = 0;
+};
+
 size_t lowp_scratch_bytes(int64_t ldlp);
 size_t gmres_multi_bytes(int64_t n, int ns, int m);
 hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, int64_t ldb, float2* X, int64_t ldx,
-                       double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s);
+                       double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s,
+                       FlexPrecond* prec = nullptr);
 size_t solve_ir_bytes(int64_t n, int ns, int m);
 hipError_t solve_ir(const MultiOp& hi, const MultiOp& lo, int64_t n, int ns, const float2* B, int64_t ldb, float2* X,
                     int64_t ldx, double tol, double inner_tol, int m, int max_inner, int max_outer, void* work,
