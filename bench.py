@@ -226,7 +226,7 @@ def _monopole_psis(v, t, k, srcs):
     return np.array(out, dtype=np.complex64)
 
 
-def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500):
+def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500, tol=1e-5):
     """16 right-hand sides on the assembled level-6 system at k: plain FP32
     (one GMRES solve per right-hand side, timed on one of them, with the fused
     assembly and potentials of the plain path) against 16-bit storage +
@@ -250,11 +250,11 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500
     fields = np.array([mesh.sources_field([q], None, k)[0](pts) for q in srcs])
     n = s.n
     lib = _lib.load()
-    res = {"k": k, "nrhs": nrhs}
+    res = {"k": k, "nrhs": nrhs, "tol": tol}
 
     # plain FP32 path, one right-hand side (the same for each of the 16)
     t0 = time.perf_counter()
-    phi0, p0, info0 = s.solve(k, psi[0], pts_t, 1e-6, restart, 3000)
+    phi0, p0, info0 = s.solve(k, psi[0], pts_t, tol, restart, 3000)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
     res["plain_fp32"] = {"seconds_per_solve": t_plain, "solves_per_s": 1.0 / t_plain,
@@ -275,12 +275,12 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500
         B = matmul(Vm, psi, alpha=-1.0)
         del Vm
         if form == "fp32_multi":
-            X, infos = gmres_multi(s.lhs, B, tol=1e-6, restart=600, maxiter=3000)
+            X, infos = gmres_multi(s.lhs, B, tol=tol, restart=600, maxiter=3000)
             info = {"outer": infos[0]["iterations"], "inner": 0, "rel_residual": [i["rel_residual"] for i in infos],
                     "converged": all(i["converged"] for i in infos)}
         else:
             L = LowPrecisionMatrix(s.lhs, form)
-            X, info = solve_ir(s.lhs, L, B, tol=1e-6, inner_tol=1e-2, restart=100, max_inner=300, max_outer=40)
+            X, info = solve_ir(s.lhs, L, B, tol=tol, inner_tol=1e-2, restart=100, max_inner=300, max_outer=40)
             del L
         Sp = s.potential(k, pts_t, "single")
         Dp = s.potential(k, pts_t, "double")
@@ -301,11 +301,11 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500
             f"{(nrhs / el) * t_plain:.1f}x plain), outer {info['outer']}, inner {info['inner']}, "
             f"residual {max(info['rel_residual']):.1e}")
     if check_level:
-        res["fp64_check"] = precision_fp64_check(check_level, k, srcs[:2])
+        res["fp64_check"] = precision_fp64_check(check_level, k, srcs[:2], tol)
     return res
 
 
-def precision_fp64_check(level, k, srcs):
+def precision_fp64_check(level, k, srcs, tol):
     """The 16-bit + refinement pipeline on a small mesh of the same family
     against the FP64 oracle solution of the same discrete system."""
     import torch
@@ -328,7 +328,7 @@ def precision_fp64_check(level, k, srcs):
             s.assemble(k, torch.tensor(psis[q], device=s.dev))
             rhs.append(s.rhs.clone())
         L = LowPrecisionMatrix(s.lhs, dt)
-        X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=1e-6, inner_tol=1e-2, max_inner=300, max_outer=40)
+        X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=tol, inner_tol=1e-2, max_inner=300, max_outer=40)
         Xc = X.cpu().numpy()
         errs = []
         for q in range(len(srcs)):

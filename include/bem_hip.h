@@ -169,10 +169,14 @@ int bem_bicgstab(const void* A, int64_t n, int64_t lda, const void* b, void* x, 
 
 /* ---- mixed precision (no reference counterpart; the reference is FP32 only) ----
  * A complex64 matrix is stored as two 16-bit planes Ar, Ai (dtype 1 = bf16,
- * 2 = fp16) holding round(scale * A), with a leading dimension ldlp that is a
- * multiple of 32 (bem_lowp_ld); columns beyond cols are zero.  The 16-RHS
- * products run on the matrix cores with FP32 accumulation. */
+ * 2 = fp16) holding round(scale * A), with a column pitch ldlp that is a
+ * multiple of 32 (bem_lowp_ld).  The planes are tiled for the matrix cores
+ * (16 x 32 tiles, each the 1 KB a wavefront reads in one step; zero padding to
+ * 16 rows and ldlp columns): each plane holds bem_lowp_plane_elems(rows, cols)
+ * 16-bit entries.  The products (any number of right-hand sides, 16 per pass)
+ * run on the matrix cores with FP32 accumulation. */
 int64_t bem_lowp_ld(int64_t cols);
+int64_t bem_lowp_plane_elems(int64_t rows, int64_t cols);
 /* scale <= 0 picks it: 1 for bf16, the power of two putting max|A| at 2^14
  * for fp16; the value used is returned in *scale_used. */
 int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int dtype, float scale, void* Ar,

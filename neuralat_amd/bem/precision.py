@@ -18,7 +18,8 @@ DTYPES = {"bf16": 1, "fp16": 2}
 
 
 class LowPrecisionMatrix:
-    """Two 16-bit planes holding round(scale * A) for a complex64 matrix A."""
+    """Two 16-bit planes holding round(scale * A) for a complex64 matrix A,
+    in the library's tiled matrix-core layout."""
 
     def __init__(self, A: torch.Tensor, dtype: str = "bf16", scale: float | None = None):
         check_tensor(A, torch.complex64)
@@ -28,7 +29,9 @@ class LowPrecisionMatrix:
         self.rows, self.cols = A.shape
         self.dtype = dtype
         self.ld = int(lib.bem_lowp_ld(self.cols))
-        self.planes = torch.empty((2, self.rows, self.ld), dtype=torch.int16, device=A.device)
+        # tiled for the matrix cores (16 x 32 tiles), see bem_convert_lowp
+        self.planes = torch.empty((2, int(lib.bem_lowp_plane_elems(self.rows, self.cols))), dtype=torch.int16,
+                                  device=A.device)
         used = ctypes.c_float()
         _lib.check(lib.bem_convert_lowp(_ptr(A), self.rows, self.cols, A.stride(0), DTYPES[dtype],
                                         float(scale or 0.0), _ptr(self.planes[0]), _ptr(self.planes[1]), self.ld,

@@ -119,25 +119,48 @@ __global__ __launch_bounds__(256) void xmultidot_kernel(const BSys* __restrict__
 }
 
 // w_s -= sum_q h[s][q] V_q ; grid (element blocks, items)
-__global__ __launch_bounds__(256) void xgs_update_kernel(const BSys* __restrict__ sys, const float2* __restrict__ V,
-                                                         int64_t ntot, int nq, const double2* __restrict__ h, int hs,
-                                                         float2* __restrict__ w, const int* __restrict__ done)
+// out_s[e] += sign * sum_{q < nq_s} c_s[q] V_q,s[e] (Gram-Schmidt update, sign
+// -1, skipping converged items; solution update, sign +1, nq_s = nk[s]).  A
+// workgroup owns 64 elements of one item; its 8 wavefronts take every 8th
+// basis vector and meet in LDS in a fixed order.  grid (ceil(max n / 64), ns)
+constexpr int kXWaves = 8;
+__global__ __launch_bounds__(64 * kXWaves) void xcombine_kernel(const BSys* __restrict__ sys,
+                                                                const float2* __restrict__ V, int64_t ntot,
+                                                                int nq_fixed, const int* __restrict__ nq_arr,
+                                                                const double2* __restrict__ c, int cstride,
+                                                                double sign, float2* __restrict__ out,
+                                                                const int* __restrict__ done)
 {
     const int s = blockIdx.y;
+    if (done && done[s]) return;  // uniform over the workgroup
     const BSys S = sys[s];
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (done[s] || e >= S.n) return;
-    double wr = w[S.off + e].x, wi = w[S.off + e].y;
-    for (int q = 0; q < nq; ++q) {
-        const double2 hq = h[(int64_t)s * hs + q];
-        const float2 v = V[(int64_t)q * ntot + S.off + e];
-        wr -= hq.x * v.x - hq.y * v.y;
-        wi -= hq.x * v.y + hq.y * v.x;
+    if ((int64_t)blockIdx.x * 64 >= S.n) return;  // uniform: this item is shorter
+    __shared__ double2 part[kXWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+    const int nq = nq_arr ? nq_arr[s] : nq_fixed;
+    const double2* cs = c + (int64_t)s * cstride;
+    double ar = 0.0, ai = 0.0;
+    if (e < S.n) {
+        for (int q = wv; q < nq; q += kXWaves) {
+            const double2 cq = cs[q];
+            const float2 v = V[(int64_t)q * ntot + S.off + e];
+            ar += cq.x * v.x - cq.y * v.y;
+            ai += cq.x * v.y + cq.y * v.x;
+        }
     }
-    w[S.off + e] = make_float2((float)wr, (float)wi);
+    part[wv][lane] = make_double2(ar, ai);
+    __syncthreads();
+    if (wv == 0 && e < S.n) {
+        double sr = 0.0, si = 0.0;
+        for (int q = 0; q < kXWaves; ++q) {
+            sr += part[q][lane].x;
+            si += part[q][lane].y;
+        }
+        const float2 ov = out[S.off + e];
+        out[S.off + e] = make_float2((float)((double)ov.x + sign * sr), (float)((double)ov.y + sign * si));
+    }
 }
-
-// out[s] = ||w_s||^2 ; grid items
 __global__ __launch_bounds__(256) void xnorm2_kernel(const BSys* __restrict__ sys, const float2* __restrict__ w,
                                                      double* __restrict__ out)
 {
@@ -168,24 +191,6 @@ __global__ __launch_bounds__(256) void xnormalize_kernel(const BSys* __restrict_
 }
 
 // x_s += sum_{q < nk_s} y_s[q] V_q ; grid (element blocks, items)
-__global__ __launch_bounds__(256) void xupdate_kernel(const BSys* __restrict__ sys, const float2* __restrict__ V,
-                                                      int64_t ntot, const int* __restrict__ nk,
-                                                      const double2* __restrict__ y, int m, float2* __restrict__ x)
-{
-    const int s = blockIdx.y;
-    const BSys S = sys[s];
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= S.n) return;
-    double xr = x[S.off + e].x, xi = x[S.off + e].y;
-    for (int q = 0; q < nk[s]; ++q) {
-        const double2 yq = y[(int64_t)s * m + q];
-        const float2 v = V[(int64_t)q * ntot + S.off + e];
-        xr += yq.x * v.x - yq.y * v.y;
-        xi += yq.x * v.y + yq.y * v.x;
-    }
-    x[S.off + e] = make_float2((float)xr, (float)xi);
-}
-
 __global__ __launch_bounds__(256) void xcopy_kernel(const float2* __restrict__ src, float2* __restrict__ dst, int64_t n)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -458,6 +463,7 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
 
     const dim3 gx((unsigned)((nmax + 255) / 256), (unsigned)ns);
     const dim3 gmv((unsigned)((nmax + 15) / 16), (unsigned)ns);
+    const dim3 gc((unsigned)((nmax + 63) / 64), (unsigned)ns);
     const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f), mone = make_float2(-1.f, 0.f);
     // one restart cycle: residual, set-up, m Arnoldi steps, back-solve, update,
     // then the true residual and the convergence flags
@@ -473,11 +479,13 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
                                (const int*)done);
             hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, st, dsys, V, ntot, w, h, m + 1,
                                (const int*)done);
-            hipLaunchKernelGGL(xgs_update_kernel, gx, dim3(256), 0, st, dsys, V, ntot, j + 1, h, m + 1, w,
+            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, j + 1,
+                               (const int*)nullptr, h, m + 1, -1.0, w,
                                (const int*)done);
             hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, st, dsys, V, ntot, w, h2, m + 1,
                                (const int*)done);
-            hipLaunchKernelGGL(xgs_update_kernel, gx, dim3(256), 0, st, dsys, V, ntot, j + 1, h2, m + 1, w,
+            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, j + 1,
+                               (const int*)nullptr, h2, m + 1, -1.0, w,
                                (const int*)done);
             hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, st, dsys, w, wn2);
             hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, st, dsys, w, wn2, V + (int64_t)(j + 1) * ntot,
@@ -487,7 +495,8 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
         }
         e = launch_bbacksolve(H, m, nk, g, y, ns, st);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(xupdate_kernel, gx, dim3(256), 0, st, dsys, V, ntot, (const int*)nk, y, m, X);
+        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, 0, (const int*)nk, y, m,
+                           1.0, X, (const int*)nullptr);
         hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, B, r, ntot);
         hipLaunchKernelGGL(xgemv_kernel, gmv, dim3(256), 0, st, dsys, X, r, mone, one, 0, (const int*)nullptr);
         hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, st, dsys, r, rn2);

@@ -624,6 +624,12 @@ int bem_bicgstab(const void* A_, int64_t n, int64_t lda, const void* b_, void* x
 // ---------------------------------------------------------------------------
 int64_t bem_lowp_ld(int64_t cols) { return cols <= 0 ? 0 : (cols + 31) / 32 * 32; }
 
+int64_t bem_lowp_plane_elems(int64_t rows, int64_t cols)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    return bem::lowp_plane_elems(rows, bem_lowp_ld(cols));
+}
+
 int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int dtype, float scale, void* Ar,
                      void* Ai, int64_t ldlp, float* scale_used, void* stream)
 {
@@ -667,7 +673,7 @@ int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const voi
         const int cnt = std::min(16, nrhs - s0);
         HIP_TRY(bem::launch_gemm_c64((const float2*)A, rows, cols, lda, (const float2*)X + (int64_t)s0 * ldx, ldx, cnt,
                                      (float2*)Y + (int64_t)s0 * ldy, ldy, make_float2(alpha_re, alpha_im),
-                                     make_float2(beta_re, beta_im), S(stream)));
+                                     make_float2(beta_re, beta_im), nullptr, S(stream)));
     }
     return BEM_OK;
 }
@@ -684,21 +690,23 @@ int bem_matmul_lowp(const void* Ar, const void* Ai, int dtype, int64_t rows, int
         return fail(BEM_ERR_BAD_ARG, "bad shape");
     uint16_t* Xr = (uint16_t*)work;
     uint16_t* Xi = Xr + 32 * ldlp;
+    float2* part = rows <= ldlp ? bem::lowp_scratch_part(work, ldlp) : nullptr;
     hipStream_t st = S(stream);
-    if (nrhs == 1) {
-        HIP_TRY(bem::launch_gemv_lowp(dtype, (const uint16_t*)Ar, (const uint16_t*)Ai, rows, cols, ldlp, inv_scale,
-                                      (const float2*)X, (float2*)Y, make_float2(alpha_re, alpha_im),
-                                      make_float2(beta_re, beta_im), st));
-        return BEM_OK;
-    }
     for (int s0 = 0; s0 < nrhs; s0 += 16) {
         const int cnt = std::min(16, nrhs - s0);
         HIP_TRY(bem::launch_vec_to_lowp(dtype, (const float2*)X + (int64_t)s0 * ldx, cols, ldx, cnt, Xr, Xi, ldlp, st));
         HIP_TRY(bem::launch_gemm_lowp(dtype, (const uint16_t*)Ar, (const uint16_t*)Ai, rows, ldlp, inv_scale, Xr, Xi,
                                       ldlp, cnt, (float2*)Y + (int64_t)s0 * ldy, ldy, make_float2(alpha_re, alpha_im),
-                                      make_float2(beta_re, beta_im), st));
+                                      make_float2(beta_re, beta_im), part, st));
     }
     return BEM_OK;
+}
+
+// scratch of the multi-RHS operators: 16-bit right-hand-side planes and the
+// split-K partial sums, laid out for max(ldlp, padded n)
+static size_t lowp_work_bytes(int64_t n, int64_t ldlp)
+{
+    return bem::lowp_scratch_bytes(std::max<int64_t>(ldlp, bem_lowp_ld(n)));
 }
 
 static int make_op(bem::MultiOp& op, const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype,
@@ -706,6 +714,8 @@ static int make_op(bem::MultiOp& op, const void* A, int64_t lda, const void* Ar,
 {
     op.n = n;
     op.dtype = dtype;
+    const int64_t ls = std::max<int64_t>(ldlp, bem_lowp_ld(n));  // layout of the scratch (see lowp_work_bytes)
+    if (scratch) op.part = bem::lowp_scratch_part(scratch, ls);
     if (dtype == 0) {
         if (!A || lda < n) return fail(BEM_ERR_BAD_ARG, "complex64 matrix missing or lda < n");
         op.A = (const float2*)A;
@@ -718,15 +728,15 @@ static int make_op(bem::MultiOp& op, const void* A, int64_t lda, const void* Ar,
     op.Ai = (const uint16_t*)Ai;
     op.ldlp = ldlp;
     op.inv_scale = inv_scale;
-    op.Xr = (uint16_t*)scratch;  // Xr (16 ldlp) + its low plane (16 ldlp), then Xi likewise
-    op.Xi = op.Xr + 32 * ldlp;
+    op.Xr = (uint16_t*)scratch;  // Xr (16 ls) + its low plane (16 ls), then Xi likewise
+    op.Xi = op.Xr + 32 * ls;
     return BEM_OK;
 }
 
 int64_t bem_gmres_multi_work_size(int64_t n, int nrhs, int restart, int64_t ldlp)
 {
     if (n <= 0 || nrhs <= 0 || restart <= 0) return 0;
-    return (int64_t)(bem::gmres_multi_bytes(n, nrhs, restart) + bem::lowp_scratch_bytes(std::max<int64_t>(ldlp, 32)));
+    return (int64_t)(bem::gmres_multi_bytes(n, nrhs, restart) + lowp_work_bytes(n, ldlp));
 }
 
 int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp,
@@ -756,7 +766,7 @@ int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, 
 int64_t bem_solve_ir_work_size(int64_t n, int nrhs, int restart, int64_t ldlp)
 {
     if (n <= 0 || nrhs <= 0 || restart <= 0) return 0;
-    return (int64_t)(bem::solve_ir_bytes(n, nrhs, restart) + bem::lowp_scratch_bytes(std::max<int64_t>(ldlp, 32)));
+    return (int64_t)(bem::solve_ir_bytes(n, nrhs, restart) + lowp_work_bytes(n, ldlp));
 }
 
 int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int dtype, int64_t ldlp, float inv_scale,
@@ -771,7 +781,7 @@ int bem_solve_ir(const void* A, int64_t lda, const void* Ar, const void* Ai, int
     if (dtype != bem::kDtBF16 && dtype != bem::kDtFP16) return fail(BEM_ERR_BAD_ARG, "inner dtype must be BF16/FP16");
     bem::MultiOp hi, lo;
     char* scratch = (char*)work + bem::solve_ir_bytes(n, nrhs, restart);
-    if (int rc = make_op(hi, A, lda, nullptr, nullptr, 0, 0, 1.f, n, nullptr)) return rc;
+    if (int rc = make_op(hi, A, lda, nullptr, nullptr, 0, ldlp, 1.f, n, scratch)) return rc;
     if (int rc = make_op(lo, nullptr, 0, Ar, Ai, dtype, ldlp, inv_scale, n, scratch)) return rc;
     bem::IrInfo ir;
     HIP_TRY(bem::solve_ir(hi, lo, n, nrhs, (const float2*)B, ldb, (float2*)X, ldx, tol, inner_tol, restart, max_inner,

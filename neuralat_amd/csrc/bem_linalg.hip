@@ -267,25 +267,33 @@ __global__ __launch_bounds__(256) void multidot_kernel(const float2* __restrict_
     if (threadIdx.x == 0) hout[q] = make_double2(sr, si);
 }
 
-// out[e] += sign * sum_q c[q] V[q][e] over nq basis vectors (the Gram-Schmidt
-// update with sign -1, the solution update with sign +1).  A workgroup owns 64
+// out_s[e] += sign * sum_{q < nq_s} c_s[q] V_s[q][e] for each system s (grid y):
+// V_s[q] at V + q qstride + s sstride, c_s at c + s cstride, out_s at
+// out + s ostride; nq_s = nq_arr[s] (or nq_fixed); systems with done[s] set are
+// skipped.  This is the Gram-Schmidt update (sign -1) and the solution update
+// (sign +1) of the single- and multi-system GMRES.  A workgroup owns 64
 // consecutive elements; its 8 wavefronts take every 8th basis vector, so a long
 // Krylov basis keeps 8x more loads in flight than one thread per element, and
 // the 8 FP64 partial sums meet in LDS in a fixed order (deterministic).
 constexpr int kBasisWaves = 8;
-__global__ __launch_bounds__(64 * kBasisWaves) void basis_combine_kernel(const float2* __restrict__ V, int64_t ldv,
-                                                                         int64_t n, int nq,
-                                                                         const double2* __restrict__ c, double sign,
-                                                                         float2* __restrict__ out)
+__global__ __launch_bounds__(64 * kBasisWaves) void basis_combine_kernel(
+    const float2* __restrict__ V, int64_t qstride, int64_t sstride, int64_t n, int nq_fixed,
+    const int* __restrict__ nq_arr, const double2* __restrict__ c, int64_t cstride, double sign,
+    float2* __restrict__ out, int64_t ostride, const int* __restrict__ done)
 {
+    const int sy = blockIdx.y;
+    if (done && done[sy]) return;  // uniform over the workgroup
     __shared__ double2 part[kBasisWaves][64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int64_t e = (int64_t)blockIdx.x * 64 + lane;
+    const int nq = nq_arr ? nq_arr[sy] : nq_fixed;
+    const float2* Vs = V + (int64_t)sy * sstride;
+    const double2* cs = c + (int64_t)sy * cstride;
     double ar = 0.0, ai = 0.0;
     if (e < n) {
         for (int q = wv; q < nq; q += kBasisWaves) {
-            const double2 cq = c[q];
-            const float2 v = V[(int64_t)q * ldv + e];
+            const double2 cq = cs[q];
+            const float2 v = Vs[(int64_t)q * qstride + e];
             ar += cq.x * v.x - cq.y * v.y;
             ai += cq.x * v.y + cq.y * v.x;
         }
@@ -298,8 +306,9 @@ __global__ __launch_bounds__(64 * kBasisWaves) void basis_combine_kernel(const f
             sr += part[q][lane].x;
             si += part[q][lane].y;
         }
-        const float2 o = out[e];
-        out[e] = make_float2((float)((double)o.x + sign * sr), (float)((double)o.y + sign * si));
+        float2* o = out + (int64_t)sy * ostride;
+        const float2 ov = o[e];
+        o[e] = make_float2((float)((double)ov.x + sign * sr), (float)((double)ov.y + sign * si));
     }
 }
 
@@ -354,15 +363,14 @@ __global__ __launch_bounds__(256) void normalize_kernel(const float2* __restrict
 // estimate.  The rotations are a serial chain: the workgroup stages them (and
 // the column) in LDS 256 at a time so that thread 0 never waits on a global
 // load inside the chain; the value carried from one rotation to the next
-// stays in registers.
+// stays in registers.  Returns, in thread 0, rotated entry j (the value
+// entering the new rotation); called by every thread of the workgroup.
 constexpr int kGivensChunk = 256;
-__global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
-    double2* __restrict__ H, int m, int j, const double2* __restrict__ h, const double2* __restrict__ h2,
-    const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
-    double* __restrict__ res)
+__device__ double2 rotate_column_wg(double2* __restrict__ col, int j, const double2* __restrict__ h,
+                                    const double2* __restrict__ h2, const double2* __restrict__ cs,
+                                    const double2* __restrict__ sn)
 {
     __shared__ double2 scs[kGivensChunk], ssn[kGivensChunk], sb[kGivensChunk];
-    double2* col = H + (int64_t)j * (m + 1);
     const int t = threadIdx.x;
     double2 carry = make_double2(h[0].x + h2[0].x, h[0].y + h2[0].y);  // col[q] entering rotation q
     for (int q0 = 0; q0 < j; q0 += kGivensChunk) {
@@ -387,6 +395,17 @@ __global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
         }
         __syncthreads();
     }
+    return carry;
+}
+
+__global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
+    double2* __restrict__ H, int m, int j, const double2* __restrict__ h, const double2* __restrict__ h2,
+    const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+    double* __restrict__ res)
+{
+    double2* col = H + (int64_t)j * (m + 1);
+    const int t = threadIdx.x;
+    const double2 carry = rotate_column_wg(col, j, h, h2, cs, sn);
     if (t != 0) return;
     const double2 a = carry, b = make_double2(sqrt(*wn2), 0.0);
     const double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
@@ -409,24 +428,6 @@ __global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
     res[j] = sqrt(gn.x * gn.x + gn.y * gn.y);
 }
 
-// back substitution y = R^{-1} g[0:nk] (R upper triangular, H layout above)
-__global__ void gmres_backsolve_kernel(const double2* __restrict__ H, int m, int nk, const double2* __restrict__ g,
-                                       double2* __restrict__ y)
-{
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (int q = nk - 1; q >= 0; --q) {
-        double2 s = g[q];
-        for (int c = q + 1; c < nk; ++c) {
-            const double2 hqc = H[(int64_t)c * (m + 1) + q], yc = y[c];
-            s.x -= hqc.x * yc.x - hqc.y * yc.y;
-            s.y -= hqc.x * yc.y + hqc.y * yc.x;
-        }
-        const double2 d = H[(int64_t)q * (m + 1) + q];
-        const double dd = d.x * d.x + d.y * d.y;
-        y[q] = dd > 0.0 ? make_double2((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd)
-                        : make_double2(0.0, 0.0);
-    }
-}
 
 // g = (beta, 0, ...), beta = sqrt(*nrm2)
 __global__ void gmres_init_g_kernel(double2* __restrict__ g, int m, const double* __restrict__ nrm2)
@@ -522,24 +523,6 @@ __global__ __launch_bounds__(256) void bmultidot_kernel(const float2* __restrict
     if (threadIdx.x == 0) h[(int64_t)sy * hs + q] = make_double2(sr, si);
 }
 
-// w_s -= sum_q h[s][q] V_q,s ; grid (ceil(n/256), ns)
-__global__ __launch_bounds__(256) void bgs_update_kernel(const float2* __restrict__ V, int64_t ldn, int ns, int64_t n,
-                                                         int nq, const double2* __restrict__ h, int hs,
-                                                         float2* __restrict__ w, const int* __restrict__ done)
-{
-    const int sy = blockIdx.y;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (done[sy] || e >= n) return;
-    float2* ww = w + (int64_t)sy * ldn;
-    double wr = ww[e].x, wi = ww[e].y;
-    for (int q = 0; q < nq; ++q) {
-        const double2 hq = h[(int64_t)sy * hs + q];
-        const float2 v = V[((int64_t)q * ns + sy) * ldn + e];
-        wr -= hq.x * v.x - hq.y * v.y;
-        wi -= hq.x * v.y + hq.y * v.x;
-    }
-    ww[e] = make_float2((float)wr, (float)wi);
-}
 
 // out[s] = ||w_s||^2 ; grid ns
 __global__ __launch_bounds__(1024) void bnorm2_kernel(const float2* __restrict__ w, int64_t ldn, int64_t n,
@@ -583,93 +566,95 @@ __global__ void binit_kernel(double2* __restrict__ g, int m, const double* __res
     nk[sy] = 0;
 }
 
-// Per-system Arnoldi bookkeeping (one thread per system): as gmres_givens_kernel,
-// plus the convergence test |g_{j+1}| <= target -> done, nk = j + 1.
-__global__ void bgivens_kernel(double2* __restrict__ H, int m, int j, const double2* __restrict__ h,
-                               const double2* __restrict__ h2, int hs, const double* __restrict__ wn2,
-                               double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
-                               double* __restrict__ res, const double* __restrict__ target, int* __restrict__ done,
-                               int* __restrict__ nk, int ns)
+// Per-system Arnoldi bookkeeping (one workgroup per system): as
+// gmres_givens_kernel, plus the convergence test |g_{j+1}| <= target -> done,
+// nk = j + 1.
+__global__ __launch_bounds__(kGivensChunk) void bgivens_kernel(
+    double2* __restrict__ H, int m, int j, const double2* __restrict__ h, const double2* __restrict__ h2, int hs,
+    const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
+    double* __restrict__ res, const double* __restrict__ target, int* __restrict__ done, int* __restrict__ nk, int ns)
 {
-    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sy >= ns || done[sy]) return;
+    const int sy = blockIdx.x;  // one workgroup per system
+    if (done[sy]) return;       // uniform over the workgroup
     double2* col = H + (int64_t)sy * (m + 1) * m + (int64_t)j * (m + 1);
-    const double2* hh = h + (int64_t)sy * hs;
-    const double2* hh2 = h2 + (int64_t)sy * hs;
     double2* c_ = cs + (int64_t)sy * m;
     double2* s_ = sn + (int64_t)sy * m;
+    const double2 a = rotate_column_wg(col, j, h + (int64_t)sy * hs, h2 + (int64_t)sy * hs, c_, s_);
+    if (threadIdx.x != 0) return;
     double2* gg = g + (int64_t)sy * (m + 1);
-    for (int q = 0; q <= j; ++q) col[q] = make_double2(hh[q].x + hh2[q].x, hh[q].y + hh2[q].y);
-    col[j + 1] = make_double2(sqrt(wn2[sy]), 0.0);
-    for (int q = 0; q < j; ++q) {
-        const double2 c = c_[q], s = s_[q], a = col[q], b = col[q + 1];
-        col[q] = make_double2(c.x * a.x - c.y * a.y + s.x * b.x - s.y * b.y, c.x * a.y + c.y * a.x + s.x * b.y + s.y * b.x);
-        col[q + 1] = make_double2(-(s.x * a.x + s.y * a.y) + (c.x * b.x + c.y * b.y),
-                                  -(s.x * a.y - s.y * a.x) + (c.x * b.y - c.y * b.x));
-    }
-    const double2 a = col[j], b = col[j + 1];
+    const double2 b = make_double2(sqrt(wn2[sy]), 0.0);
     const double den = sqrt(a.x * a.x + a.y * a.y + b.x * b.x + b.y * b.y);
-    double2 c = make_double2(1.0, 0.0), s = make_double2(0.0, 0.0);
+    double2 c = make_double2(1.0, 0.0), sv = make_double2(0.0, 0.0);
     if (den > 0.0) {
         c = make_double2(a.x / den, -a.y / den);
-        s = make_double2(b.x / den, -b.y / den);
+        sv = make_double2(b.x / den, -b.y / den);
     }
     c_[j] = c;
-    s_[j] = s;
+    s_[j] = sv;
     col[j] = make_double2(den, 0.0);
     col[j + 1] = make_double2(0.0, 0.0);
     const double2 gj = gg[j];
-    gg[j + 1] = make_double2(-(s.x * gj.x + s.y * gj.y), -(s.x * gj.y - s.y * gj.x));
+    const double2 gn = make_double2(-(sv.x * gj.x + sv.y * gj.y), -(sv.x * gj.y - sv.y * gj.x));
+    gg[j + 1] = gn;
     gg[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
-    const double rj = sqrt(gg[j + 1].x * gg[j + 1].x + gg[j + 1].y * gg[j + 1].y);
+    const double rj = sqrt(gn.x * gn.x + gn.y * gn.y);
     res[sy] = rj;
     nk[sy] = j + 1;
     if (rj <= target[sy]) done[sy] = 1;
 }
 
-// y_s = R_s^{-1} g_s[0:nk_s] ; one thread per system
-__global__ void bbacksolve_kernel(const double2* __restrict__ H, int m, const int* __restrict__ nk,
-                                  const double2* __restrict__ g, double2* __restrict__ y, int ns)
+// y = R^{-1} g for one system per workgroup (R: leading nk x nk of the
+// rotated Hessenberg, column c at H + c (m + 1)).  Column-oriented: once y_q
+// is known every thread removes R[t][q] y_q from the right-hand side entries
+// t < q (column q is contiguous), one barrier per step; the right-hand side
+// lives in LDS.  Beyond kBackLds unknowns thread 0 does the row-oriented sweep.
+constexpr int kBackLds = 2048;
+__global__ __launch_bounds__(256) void backsolve_wg_kernel(const double2* __restrict__ H, int m, int64_t hstride,
+                                                           const int* __restrict__ nk_arr, int nk_fixed,
+                                                           const double2* __restrict__ g, int64_t gstride,
+                                                           double2* __restrict__ y, int64_t ystride)
 {
-    const int sy = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sy >= ns) return;
-    const double2* Hs = H + (int64_t)sy * (m + 1) * m;
-    const double2* gs = g + (int64_t)sy * (m + 1);
-    double2* ys = y + (int64_t)sy * m;
-    const int n_k = nk[sy];
-    for (int q = n_k - 1; q >= 0; --q) {
-        double2 acc = gs[q];
-        for (int c = q + 1; c < n_k; ++c) {
-            const double2 hqc = Hs[(int64_t)c * (m + 1) + q], yc = ys[c];
-            acc.x -= hqc.x * yc.x - hqc.y * yc.y;
-            acc.y -= hqc.x * yc.y + hqc.y * yc.x;
+    __shared__ double2 sv[kBackLds];
+    const int sy = blockIdx.x;
+    const double2* Hs = H + (int64_t)sy * hstride;
+    const double2* gs = g + (int64_t)sy * gstride;
+    double2* ys = y + (int64_t)sy * ystride;
+    const int n_k = nk_arr ? nk_arr[sy] : nk_fixed;
+    const int t = threadIdx.x;
+    if (n_k > kBackLds) {
+        if (t != 0) return;
+        for (int q = n_k - 1; q >= 0; --q) {
+            double2 acc = gs[q];
+            for (int c = q + 1; c < n_k; ++c) {
+                const double2 hqc = Hs[(int64_t)c * (m + 1) + q], yc = ys[c];
+                acc.x -= hqc.x * yc.x - hqc.y * yc.y;
+                acc.y -= hqc.x * yc.y + hqc.y * yc.x;
+            }
+            const double2 d = Hs[(int64_t)q * (m + 1) + q];
+            const double dd = d.x * d.x + d.y * d.y;
+            ys[q] = dd > 0.0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
+                             : make_double2(0.0, 0.0);
         }
-        const double2 d = Hs[(int64_t)q * (m + 1) + q];
+        return;
+    }
+    for (int e = t; e < n_k; e += 256) sv[e] = gs[e];
+    __syncthreads();
+    for (int q = n_k - 1; q >= 0; --q) {
+        const double2* colq = Hs + (int64_t)q * (m + 1);
+        const double2 d = colq[q], a = sv[q];
         const double dd = d.x * d.x + d.y * d.y;
-        ys[q] = dd > 0.0 ? make_double2((acc.x * d.x + acc.y * d.y) / dd, (acc.y * d.x - acc.x * d.y) / dd)
-                         : make_double2(0.0, 0.0);
+        const double2 yq = dd > 0.0 ? make_double2((a.x * d.x + a.y * d.y) / dd, (a.y * d.x - a.x * d.y) / dd)
+                                    : make_double2(0.0, 0.0);
+        if (t == 0) ys[q] = yq;
+        for (int e = t; e < q; e += 256) {
+            const double2 r = colq[e];
+            sv[e].x -= r.x * yq.x - r.y * yq.y;
+            sv[e].y -= r.x * yq.y + r.y * yq.x;
+        }
+        __syncthreads();
     }
 }
 
-// x_s += sum_{q < nk_s} y_s[q] V_q,s ; grid (ceil(n/256), ns)
-__global__ __launch_bounds__(256) void bupdate_x_kernel(const float2* __restrict__ V, int64_t ldn, int ns, int64_t n,
-                                                        const int* __restrict__ nk, const double2* __restrict__ y,
-                                                        int m, float2* __restrict__ x, int64_t ldx)
-{
-    const int sy = blockIdx.y;
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= n) return;
-    float2* xs = x + (int64_t)sy * ldx;
-    double xr = xs[e].x, xi = xs[e].y;
-    const int n_k = nk[sy];
-    for (int q = 0; q < n_k; ++q) {
-        const double2 yq = y[(int64_t)sy * m + q];
-        const float2 v = V[((int64_t)q * ns + sy) * ldn + e];
-        xr += yq.x * v.x - yq.y * v.y;
-        xi += yq.x * v.y + yq.y * v.x;
-    }
-    xs[e] = make_float2((float)xr, (float)xi);
-}
 
 // dst_s[e] = src_s[e] (strided copies of ns vectors)
 __global__ __launch_bounds__(256) void bcopy_kernel(const float2* __restrict__ src, int64_t lds, float2* __restrict__ dst,
@@ -762,8 +747,8 @@ hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, cons
 hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, n, nq, h, -1.0,
-                       w);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, (int64_t)0, n, nq,
+                       (const int*)nullptr, h, (int64_t)0, -1.0, w, (int64_t)0, (const int*)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_norm2(const float2* w, int64_t n, double* out, hipStream_t s)
@@ -789,14 +774,15 @@ hipError_t launch_givens(double2* H, int m, int j, const double2* h, const doubl
 }
 hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s)
 {
-    hipLaunchKernelGGL(gmres_backsolve_kernel, dim3(1), dim3(64), 0, s, H, m, nk, g, y);
+    hipLaunchKernelGGL(backsolve_wg_kernel, dim3(1), dim3(256), 0, s, H, m, (int64_t)0, (const int*)nullptr, nk, g,
+                       (int64_t)0, y, (int64_t)0);
     return hipGetLastError();
 }
 hipError_t launch_update_x(const float2* V, int64_t ldv, int64_t n, int nk, const double2* y, float2* x,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, n, nk, y, 1.0,
-                       x);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64)), dim3(64 * kBasisWaves), 0, s, V, ldv, (int64_t)0, n, nk,
+                       (const int*)nullptr, y, (int64_t)0, 1.0, x, (int64_t)0, (const int*)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_init_g(double2* g, int m, const double* nrm2, hipStream_t s)
@@ -840,7 +826,8 @@ hipError_t launch_bmultidot(const float2* V, int64_t ldn, int ns, int64_t n, int
 hipError_t launch_bgs_update(const float2* V, int64_t ldn, int ns, int64_t n, int nq, const double2* h, int hs,
                              float2* w, const int* done, hipStream_t s)
 {
-    hipLaunchKernelGGL(bgs_update_kernel, gns(n, ns), dim3(256), 0, s, V, ldn, ns, n, nq, h, hs, w, done);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64), (unsigned)ns), dim3(64 * kBasisWaves), 0, s, V,
+                       ldn * ns, ldn, n, nq, (const int*)nullptr, h, (int64_t)hs, -1.0, w, ldn, done);
     return hipGetLastError();
 }
 hipError_t launch_bnorm2(const float2* w, int64_t ldn, int64_t n, int ns, double* out, hipStream_t s)
@@ -864,20 +851,22 @@ hipError_t launch_bgivens(double2* H, int m, int j, const double2* h, const doub
                           double2* cs, double2* sn, double2* g, double* res, const double* target, int* done, int* nk,
                           int ns, hipStream_t s)
 {
-    hipLaunchKernelGGL(bgivens_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, H, m, j, h, h2, hs, wn2, cs, sn, g, res,
+    hipLaunchKernelGGL(bgivens_kernel, dim3(ns), dim3(kGivensChunk), 0, s, H, m, j, h, h2, hs, wn2, cs, sn, g, res,
                        target, done, nk, ns);
     return hipGetLastError();
 }
 hipError_t launch_bbacksolve(const double2* H, int m, const int* nk, const double2* g, double2* y, int ns,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(bbacksolve_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, H, m, nk, g, y, ns);
+    hipLaunchKernelGGL(backsolve_wg_kernel, dim3(ns), dim3(256), 0, s, H, m, (int64_t)(m + 1) * m, nk, 0, g,
+                       (int64_t)(m + 1), y, (int64_t)m);
     return hipGetLastError();
 }
 hipError_t launch_bupdate_x(const float2* V, int64_t ldn, int ns, int64_t n, const int* nk, const double2* y, int m,
                             float2* x, int64_t ldx, hipStream_t s)
 {
-    hipLaunchKernelGGL(bupdate_x_kernel, gns(n, ns), dim3(256), 0, s, V, ldn, ns, n, nk, y, m, x, ldx);
+    hipLaunchKernelGGL(basis_combine_kernel, dim3(g1(n, 64), (unsigned)ns), dim3(64 * kBasisWaves), 0, s, V,
+                       ldn * ns, ldn, n, 0, nk, y, (int64_t)m, 1.0, x, ldx, (const int*)nullptr);
     return hipGetLastError();
 }
 hipError_t launch_bcopy(const float2* src, int64_t lds, float2* dst, int64_t ldd, int64_t n, int ns, hipStream_t s)

@@ -3,22 +3,28 @@
 //
 // A complex matrix is stored as two planes Ar, Ai of 16-bit values (bf16 or
 // fp16) holding round(scale * A); scale is a power of two, so fp16 keeps the
-// small far-field entries in its normal range.  Columns [cols, ld) of each
-// plane are zero so that kernels may read whole 32-column steps.
+// small far-field entries in its normal range.  The planes are TILED for the
+// matrix cores: a tile covers 16 rows x 32 columns (1 KB per plane), tiles are
+// ordered row-tile major ((T, K) at (T * ld / 32 + K) * 512 elements), and
+// inside a tile lane l = 16 q + i of a wavefront finds its MFMA A operand --
+// the 8 entries of row 16 T + i at columns 32 K + 8 q + [0, 8) -- at
+// elements [8 l, 8 l + 8).  One wavefront step therefore reads one
+// contiguous 1 KB block per plane, and a wavefront streams contiguous memory
+// (row-major planes made each step touch 16 rows 64 bytes apiece, which ran
+// at a third of HBM bandwidth).  Rows are padded to a multiple of 16 and
+// columns to ld (a multiple of 32) with zeros.
 //
-//  * gemv_lowp_kernel: one right-hand side, VALU; 16-byte loads of 8 entries
-//    per plane, 4 rows per wavefront share each x fetch; HBM-bound at 4 B per
-//    complex entry.
-//  * gemm_lowp_mfma_kernel: 16 right-hand sides on v_mfma_f32_16x16x32_{bf16,f16}.
-//    A wavefront owns 32 rows (two 16-row tiles).  Lane l loads 8 consecutive
-//    entries of row (l % 16) at column offset 8 * (l / 16) of the current
-//    32-column step -- exactly one MFMA A operand -- and the matching 8 entries
-//    of right-hand side (l % 16) as the B operand.  Four products per tile:
+//  * gemm_lowp_mfma_kernel: up to 16 right-hand sides on
+//    v_mfma_f32_16x16x32_{bf16,f16}; a wavefront owns two row tiles (32 rows)
+//    and a range of column tiles (split-K); four products per tile:
 //    Ar Xr, Ai Xi, Ar Xi, Ai Xr, accumulated in FP32; Y = (rr - ii) + i (ri + ir).
+//    One right-hand side uses the same kernel (the product is HBM-bound).
 //  * gemm_c64_mfma_kernel: the same 16-RHS product on the complex64 matrix with
-//    v_mfma_f32_16x16x4f32 (the FP32 residual of iterative refinement).
+//    v_mfma_f32_16x16x4f32 (the FP32 residual of the refinement).
 // Right-hand-side blocks are stored per system: X[s * ldx + k].
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "bem_lowp.hpp"
 
@@ -63,13 +69,24 @@ __global__ __launch_bounds__(256) void to_lowp_kernel(const float2* __restrict__
                                                       int64_t lda, float scale, uint16_t* __restrict__ Ar,
                                                       uint16_t* __restrict__ Ai, int64_t ldo)
 {
-    const int64_t total = rows * ldo;
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
-        const int64_t r = e / ldo, c = e - r * ldo;
-        float2 v = make_float2(0.f, 0.f);
-        if (c < cols) v = A[r * lda + c];
-        Ar[e] = to_lp<DT>(v.x * scale);
-        Ai[e] = to_lp<DT>(v.y * scale);
+    // one thread per lane slot (8 entries) of the tiled planes
+    const int64_t nk = ldo / 32;
+    const int64_t total = (rows + 15) / 16 * nk * 64;
+    for (int64_t sl = (int64_t)blockIdx.x * 256 + threadIdx.x; sl < total; sl += (int64_t)gridDim.x * 256) {
+        const int64_t tile = sl >> 6;
+        const int l = (int)(sl & 63);
+        const int64_t T = tile / nk, K = tile - T * nk;
+        const int64_t r = 16 * T + (l & 15), c0 = 32 * K + 8 * (l >> 4);
+        u16x8 hr, hi;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            float2 v = make_float2(0.f, 0.f);
+            if (r < rows && c0 + e < cols) v = A[r * lda + c0 + e];
+            hr[e] = to_lp<DT>(v.x * scale);
+            hi[e] = to_lp<DT>(v.y * scale);
+        }
+        *reinterpret_cast<u16x8*>(Ar + sl * 8) = hr;
+        *reinterpret_cast<u16x8*>(Ai + sl * 8) = hi;
     }
 }
 
@@ -117,65 +134,6 @@ __global__ __launch_bounds__(256) void vec_to_lowp_kernel(const float2* __restri
 }
 
 // ---------------------------------------------------------------------------
-// one right-hand side, VALU
-// ---------------------------------------------------------------------------
-template <int DT, int RW>
-__global__ __launch_bounds__(256) void gemv_lowp_kernel(const uint16_t* __restrict__ Ar, const uint16_t* __restrict__ Ai,
-                                                        int64_t rows, int64_t cols, int64_t ld, float inv_scale,
-                                                        const float2* __restrict__ x, float2* __restrict__ y,
-                                                        float2 alpha, float2 beta, int beta_zero)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RW;
-    if (r0 >= rows) return;
-    float accr[RW], acci[RW];
-#pragma unroll
-    for (int q = 0; q < RW; ++q) accr[q] = acci[q] = 0.f;
-    const int64_t nchunk = (cols + 7) / 8;  // planes are zero beyond cols
-    for (int64_t c8 = lane; c8 < nchunk; c8 += 64) {
-        const int64_t c = c8 * 8;
-        float xr[8], xi[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const float2 v = (c + e < cols) ? x[c + e] : make_float2(0.f, 0.f);
-            xr[e] = v.x;
-            xi[e] = v.y;
-        }
-#pragma unroll
-        for (int q = 0; q < RW; ++q) {
-            const int64_t r = min(r0 + q, rows - 1);
-            const u16x8 ar = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(Ar + r * ld + c));
-            const u16x8 ai = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(Ai + r * ld + c));
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const float a = from_lp<DT>(ar[e]), b = from_lp<DT>(ai[e]);
-                accr[q] = fmaf(a, xr[e], fmaf(-b, xi[e], accr[q]));
-                acci[q] = fmaf(a, xi[e], fmaf(b, xr[e], acci[q]));
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RW; ++q) {
-        float sr = accr[q] * inv_scale, si = acci[q] * inv_scale;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            sr += __shfl_xor(sr, off, 64);
-            si += __shfl_xor(si, off, 64);
-        }
-        const int64_t r = r0 + q;
-        if (lane == q && r < rows) {
-            float2 o = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
-            if (!beta_zero) {
-                const float2 yo = y[r];
-                o.x += beta.x * yo.x - beta.y * yo.y;
-                o.y += beta.x * yo.y + beta.y * yo.x;
-            }
-            y[r] = o;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
 // 16 right-hand sides on the matrix cores
 // ---------------------------------------------------------------------------
 template <int DT>
@@ -196,17 +154,20 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
                                                              const uint16_t* __restrict__ Xr,
                                                              const uint16_t* __restrict__ Xi, int64_t ldx, int nrhs,
                                                              float2* __restrict__ Y, int64_t ldy, float2 alpha,
-                                                             float2 beta, int beta_zero)
+                                                             float2 beta, int beta_zero, int64_t kchunk,
+                                                             float2* __restrict__ part)
 {
     const int lane = threadIdx.x & 63;
     const int li = lane & 15, q = lane >> 4;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
     if (r0 >= rows) return;
-    const int64_t ra = min(r0 + li, rows - 1), rb = min(r0 + 16 + li, rows - 1);
-    const uint16_t* par = Ar + ra * ld + 8 * q;
-    const uint16_t* pai = Ai + ra * ld + 8 * q;
-    const uint16_t* pbr = Ar + rb * ld + 8 * q;
-    const uint16_t* pbi = Ai + rb * ld + 8 * q;
+    const int64_t nk = ld / 32, ntile = (rows + 15) / 16;
+    const int64_t ta = r0 / 16, tb = min(ta + 1, ntile - 1);  // the second tile may not exist
+    // tile (T, K) of a plane at (T nk + K) 512; this lane's 8 entries at 8 lane
+    const uint16_t* par = Ar + ta * nk * 512 + 8 * lane;
+    const uint16_t* pai = Ai + ta * nk * 512 + 8 * lane;
+    const uint16_t* pbr = Ar + tb * nk * 512 + 8 * lane;
+    const uint16_t* pbi = Ai + tb * nk * 512 + 8 * lane;
     const int s = li < nrhs ? li : 0;  // right-hand side of this lane's B operand
     const uint16_t* pxr = Xr + (int64_t)s * ldx + 8 * q;
     const uint16_t* pxi = Xi + (int64_t)s * ldx + 8 * q;
@@ -215,14 +176,18 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
     constexpr bool kSplit = true;  // hi + lo right-hand-side planes
     f32x4 a_rr = {0, 0, 0, 0}, a_ii = a_rr, a_ri = a_rr, a_ir = a_rr;
     f32x4 b_rr = a_rr, b_ii = a_rr, b_ri = a_rr, b_ir = a_rr;
-    const int64_t kend = ld;  // ld is a multiple of 32; padded columns are zero
-    for (int64_t kb = 0; kb < kend; kb += 32) {
+    // columns [k0, k1) of this split (ld and kchunk are multiples of 32; padded columns are zero)
+    const int64_t k0 = (int64_t)blockIdx.y * kchunk;
+    const int64_t kend = min(ld, k0 + kchunk);
+#pragma unroll 2
+    for (int64_t kb = k0; kb < kend; kb += 32) {
         const u16x8 xr = live ? *reinterpret_cast<const u16x8*>(pxr + kb) : zero8;
         const u16x8 xi = live ? *reinterpret_cast<const u16x8*>(pxi + kb) : zero8;
-        const u16x8 ar = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(par + kb));
-        const u16x8 ai = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pai + kb));
-        const u16x8 br = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbr + kb));
-        const u16x8 bi = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbi + kb));
+        const int64_t to = kb * 16;  // (kb / 32) tiles of 512 entries
+        const u16x8 ar = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(par + to));
+        const u16x8 ai = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pai + to));
+        const u16x8 br = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbr + to));
+        const u16x8 bi = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbi + to));
         a_rr = mfma16x16x32<DT>(ar, xr, a_rr);
         a_ii = mfma16x16x32<DT>(ai, xi, a_ii);
         a_ri = mfma16x16x32<DT>(ar, xi, a_ri);
@@ -246,6 +211,20 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
     }
     // D layout: lane l holds column (right-hand side) l % 16, rows 4 (l / 16) + e
     if (!live) return;
+    if (part) {  // split-K partial, combined by splitk_reduce_kernel
+        float2* P = part + ((int64_t)blockIdx.y * nrhs + li) * rows;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int64_t r = r0 + 16 * t + 4 * q + e;
+                if (r < rows)
+                    P[r] = t ? make_float2(b_rr[e] - b_ii[e], b_ri[e] + b_ir[e])
+                             : make_float2(a_rr[e] - a_ii[e], a_ri[e] + a_ir[e]);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
 #pragma unroll
@@ -272,7 +251,8 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
 __global__ __launch_bounds__(256) void gemm_c64_mfma_kernel(const float2* __restrict__ A, int64_t rows, int64_t cols,
                                                             int64_t lda, const float2* __restrict__ X, int64_t ldx,
                                                             int nrhs, float2* __restrict__ Y, int64_t ldy,
-                                                            float2 alpha, float2 beta, int beta_zero)
+                                                            float2 alpha, float2 beta, int beta_zero, int64_t kchunk,
+                                                            float2* __restrict__ part)
 {
     const int lane = threadIdx.x & 63;
     const int li = lane & 15, q = lane >> 4;
@@ -283,12 +263,15 @@ __global__ __launch_bounds__(256) void gemm_c64_mfma_kernel(const float2* __rest
     const bool live = li < nrhs;
     const float2* px = X + (int64_t)(live ? li : 0) * ldx;
     f32x4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
-    for (int64_t kb = 0; kb < cols; kb += 32) {
+    const int64_t k0 = (int64_t)blockIdx.y * kchunk;  // columns [k0, k0 + kchunk) of this split
+    const int64_t kend = min(cols, k0 + kchunk);
+#pragma unroll 2
+    for (int64_t kb = k0; kb < kend; kb += 32) {
         float2 a[8], b[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int64_t c = kb + 8 * q + e;
-            const bool in = c < cols;
+            const bool in = c < kend;
             a[e] = in ? pa[c] : make_float2(0.f, 0.f);
             b[e] = (in && live) ? px[c] : make_float2(0.f, 0.f);
         }
@@ -301,6 +284,15 @@ __global__ __launch_bounds__(256) void gemm_c64_mfma_kernel(const float2* __rest
         }
     }
     if (!live) return;
+    if (part) {  // split-K partial, combined by splitk_reduce_kernel
+        float2* P = part + ((int64_t)blockIdx.y * nrhs + li) * rows;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t r = r0 + 4 * q + e;
+            if (r < rows) P[r] = make_float2(rr[e] - ii[e], ri[e] + ir[e]);
+        }
+        return;
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
         const int64_t r = r0 + 4 * q + e;
@@ -315,6 +307,48 @@ __global__ __launch_bounds__(256) void gemm_c64_mfma_kernel(const float2* __rest
         }
         *dst = o;
     }
+}
+
+// Y[s] = alpha * scale * sum_k part[k][s] + beta * Y[s]; fixed order over the
+// ks column splits (deterministic).  One thread per (row, right-hand side).
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const float2* __restrict__ part, int ks, int nrhs,
+                                                            int64_t rows, float scale, float2* __restrict__ Y,
+                                                            int64_t ldy, float2 alpha, float2 beta, int beta_zero)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int sy = blockIdx.y;
+    if (r >= rows) return;
+    float yr = 0.f, yi = 0.f;
+    for (int k = 0; k < ks; ++k) {
+        const float2 p = part[((int64_t)k * nrhs + sy) * rows + r];
+        yr += p.x;
+        yi += p.y;
+    }
+    yr *= scale;
+    yi *= scale;
+    float2 o = make_float2(alpha.x * yr - alpha.y * yi, alpha.x * yi + alpha.y * yr);
+    float2* dst = &Y[(int64_t)sy * ldy + r];
+    if (!beta_zero) {
+        const float2 yo = *dst;
+        o.x += beta.x * yo.x - beta.y * yo.y;
+        o.y += beta.x * yo.y + beta.y * yo.x;
+    }
+    *dst = o;
+}
+
+// Column splits for a product over `tiles` row tiles and `cols` columns: enough
+// wavefronts to cover the chip about 16 times (a single split leaves 2-5 per
+// SIMD at 81920 rows, too few to hide HBM latency), at most kMaxSplit, each
+// split at least 256 columns, split width a multiple of 32.
+static void choose_split(int64_t tiles, int64_t cols, int& ks, int64_t& kchunk)
+{
+    constexpr int64_t kTargetWaves = 16384;
+    int64_t k = (kTargetWaves + tiles - 1) / tiles;
+    k = std::min<int64_t>(k, kMaxSplit);
+    k = std::min<int64_t>(k, std::max<int64_t>(1, cols / 256));
+    k = std::max<int64_t>(k, 1);
+    kchunk = ((cols + k - 1) / k + 31) / 32 * 32;
+    ks = (int)((cols + kchunk - 1) / kchunk);
 }
 
 // ---------------------------------------------------------------------------
@@ -337,11 +371,11 @@ hipError_t launch_to_lowp(int dt, const float2* A, int64_t rows, int64_t cols, i
                           uint16_t* Ai, int64_t ldo, hipStream_t s)
 {
     if (dt == kDtBF16)
-        hipLaunchKernelGGL(to_lowp_kernel<kDtBF16>, dim3(gcap(rows * ldo)), dim3(256), 0, s, A, rows, cols, lda, scale,
-                           Ar, Ai, ldo);
+        hipLaunchKernelGGL(to_lowp_kernel<kDtBF16>, dim3(gcap(lowp_plane_elems(rows, ldo) / 8)), dim3(256), 0, s, A,
+                           rows, cols, lda, scale, Ar, Ai, ldo);
     else
-        hipLaunchKernelGGL(to_lowp_kernel<kDtFP16>, dim3(gcap(rows * ldo)), dim3(256), 0, s, A, rows, cols, lda, scale,
-                           Ar, Ai, ldo);
+        hipLaunchKernelGGL(to_lowp_kernel<kDtFP16>, dim3(gcap(lowp_plane_elems(rows, ldo) / 8)), dim3(256), 0, s, A,
+                           rows, cols, lda, scale, Ar, Ai, ldo);
     return hipGetLastError();
 }
 
@@ -357,44 +391,49 @@ hipError_t launch_vec_to_lowp(int dt, const float2* X, int64_t n, int64_t ldx_in
     return hipGetLastError();
 }
 
-hipError_t launch_gemv_lowp(int dt, const uint16_t* Ar, const uint16_t* Ai, int64_t rows, int64_t cols, int64_t ld,
-                            float inv_scale, const float2* x, float2* y, float2 alpha, float2 beta, hipStream_t s)
-{
-    constexpr int RW = 4;
-    const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
-    const unsigned g = (unsigned)(((rows + RW - 1) / RW + 3) / 4);
-    if (dt == kDtBF16)
-        hipLaunchKernelGGL((gemv_lowp_kernel<kDtBF16, RW>), dim3(g), dim3(256), 0, s, Ar, Ai, rows, cols, ld, inv_scale,
-                           x, y, alpha, beta, bz);
-    else
-        hipLaunchKernelGGL((gemv_lowp_kernel<kDtFP16, RW>), dim3(g), dim3(256), 0, s, Ar, Ai, rows, cols, ld, inv_scale,
-                           x, y, alpha, beta, bz);
-    return hipGetLastError();
-}
-
 hipError_t launch_gemm_lowp(int dt, const uint16_t* Ar, const uint16_t* Ai, int64_t rows, int64_t ld, float inv_scale,
                             const uint16_t* Xr, const uint16_t* Xi, int64_t ldx, int nrhs, float2* Y, int64_t ldy,
-                            float2 alpha, float2 beta, hipStream_t s)
+                            float2 alpha, float2 beta, float2* part, hipStream_t s)
 {
     const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
-    const unsigned g = (unsigned)(((rows + 31) / 32 + 3) / 4);
+    const int64_t tiles = (rows + 31) / 32;
+    const unsigned g = (unsigned)((tiles + 3) / 4);
+    int ks = 1;
+    int64_t kchunk = ld;
+    if (part) choose_split(tiles, ld, ks, kchunk);
+    float2* pp = ks > 1 ? part : nullptr;
     if (dt == kDtBF16)
-        hipLaunchKernelGGL(gemm_lowp_mfma_kernel<kDtBF16>, dim3(g), dim3(256), 0, s, Ar, Ai, rows, ld, inv_scale, Xr,
-                           Xi, ldx, nrhs, Y, ldy, alpha, beta, bz);
+        hipLaunchKernelGGL(gemm_lowp_mfma_kernel<kDtBF16>, dim3(g, (unsigned)ks), dim3(256), 0, s, Ar, Ai, rows, ld,
+                           inv_scale, Xr, Xi, ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
     else
-        hipLaunchKernelGGL(gemm_lowp_mfma_kernel<kDtFP16>, dim3(g), dim3(256), 0, s, Ar, Ai, rows, ld, inv_scale, Xr,
-                           Xi, ldx, nrhs, Y, ldy, alpha, beta, bz);
+        hipLaunchKernelGGL(gemm_lowp_mfma_kernel<kDtFP16>, dim3(g, (unsigned)ks), dim3(256), 0, s, Ar, Ai, rows, ld,
+                           inv_scale, Xr, Xi, ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
+    if (pp)
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nrhs), dim3(256), 0, s,
+                           pp, ks, nrhs, rows, inv_scale, Y, ldy, alpha, beta, bz);
     return hipGetLastError();
 }
 
 hipError_t launch_gemm_c64(const float2* A, int64_t rows, int64_t cols, int64_t lda, const float2* X, int64_t ldx,
-                           int nrhs, float2* Y, int64_t ldy, float2 alpha, float2 beta, hipStream_t s)
+                           int nrhs, float2* Y, int64_t ldy, float2 alpha, float2 beta, float2* part, hipStream_t s)
 {
     const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
-    const unsigned g = (unsigned)(((rows + 15) / 16 + 3) / 4);
-    hipLaunchKernelGGL(gemm_c64_mfma_kernel, dim3(g), dim3(256), 0, s, A, rows, cols, lda, X, ldx, nrhs, Y, ldy, alpha,
-                       beta, bz);
+    const int64_t tiles = (rows + 15) / 16;
+    const unsigned g = (unsigned)((tiles + 3) / 4);
+    int ks = 1;
+    int64_t kchunk = (cols + 31) / 32 * 32;
+    if (part) choose_split(tiles, cols, ks, kchunk);
+    float2* pp = ks > 1 ? part : nullptr;
+    hipLaunchKernelGGL(gemm_c64_mfma_kernel, dim3(g, (unsigned)ks), dim3(256), 0, s, A, rows, cols, lda, X, ldx, nrhs, Y,
+                       ldy, alpha, beta, bz, kchunk, pp);
+    if (pp)
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nrhs), dim3(256), 0, s,
+                           pp, ks, nrhs, rows, 1.f, Y, ldy, alpha, beta, bz);
     return hipGetLastError();
 }
+
+int64_t splitk_part_elems(int64_t rows) { return (int64_t)kMaxSplit * 16 * rows; }
+
+int64_t lowp_plane_elems(int64_t rows, int64_t ld) { return (rows + 15) / 16 * 16 * ld; }
 
 }  // namespace bem

@@ -34,22 +34,27 @@ hipError_t MultiOp::apply(const float2* X, int64_t ldx, int ns, float2* Y, int64
             if (cnt == 1)
                 e = launch_gemv(A, n, n, lda, 0, Xs, Ys, alpha, beta, nullptr, s);
             else
-                e = launch_gemm_c64(A, n, n, lda, Xs, ldx, cnt, Ys, ldy, alpha, beta, s);
+                e = launch_gemm_c64(A, n, n, lda, Xs, ldx, cnt, Ys, ldy, alpha, beta, part, s);
         } else {
-            if (cnt == 1) {
-                e = launch_gemv_lowp(dtype, Ar, Ai, n, n, ldlp, inv_scale, Xs, Ys, alpha, beta, s);
-            } else {
-                e = launch_vec_to_lowp(dtype, Xs, n, ldx, cnt, Xr, Xi, ldlp, s);
-                if (e == hipSuccess)
-                    e = launch_gemm_lowp(dtype, Ar, Ai, n, ldlp, inv_scale, Xr, Xi, ldlp, cnt, Ys, ldy, alpha, beta, s);
-            }
+            e = launch_vec_to_lowp(dtype, Xs, n, ldx, cnt, Xr, Xi, ldlp, s);
+            if (e == hipSuccess)
+                e = launch_gemm_lowp(dtype, Ar, Ai, n, ldlp, inv_scale, Xr, Xi, ldlp, cnt, Ys, ldy, alpha, beta, part, s);
         }
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
-size_t lowp_scratch_bytes(int64_t ldlp) { return 2 * al(sizeof(uint16_t) * 32 * (size_t)ldlp); }
+// Xr, Xi (hi + lo planes of 16 right-hand sides, 32 x ldlp each), then the
+// split-K partial sums of the multi-RHS products (rows <= ldlp)
+size_t lowp_scratch_bytes(int64_t ldlp)
+{
+    return 2 * al(sizeof(uint16_t) * 32 * (size_t)ldlp) + al(sizeof(float2) * (size_t)splitk_part_elems(ldlp));
+}
+float2* lowp_scratch_part(void* scratch, int64_t ldlp)
+{
+    return (float2*)((char*)scratch + 2 * al(sizeof(uint16_t) * 32 * (size_t)ldlp));
+}
 
 struct MLayout {
     size_t V, w, r, H, h, h2, cs, sn, g, y, res, rn2, wn2, tgt, done, nk, total;

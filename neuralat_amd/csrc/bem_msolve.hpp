@@ -20,6 +20,7 @@ struct MultiOp {
     float inv_scale = 1.f;
     uint16_t* Xr = nullptr;  // scratch for the 16-bit right-hand sides (16 x ldlp each)
     uint16_t* Xi = nullptr;
+    float2* part = nullptr;  // split-K partial sums of the products (null: single pass)
     hipError_t apply(const float2* X, int64_t ldx, int ns, float2* Y, int64_t ldy, float2 alpha, float2 beta,
                      hipStream_t s) const;
 };
@@ -51,6 +52,7 @@ struct FlexPrecond {
 };
 
 size_t lowp_scratch_bytes(int64_t ldlp);
+float2* lowp_scratch_part(void* scratch, int64_t ldlp);
 size_t gmres_multi_bytes(int64_t n, int ns, int m);
 hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, int64_t ldb, float2* X, int64_t ldx,
                        double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s,
