@@ -399,15 +399,16 @@ def grad_form(ndir=4, seed=0):
             "dL_dk_adjoint": float(kt.grad), "dL_dk_fd": fdk}
 
 
-def make_batch(nmesh, nk, seed=7):
-    """Seeded uneven meshes (icosphere levels 2-5 in turn, anisotropic scale
-    in [0.5, 2]^3, SH noise) and nk wavenumbers each, log-uniform in [5, 80]."""
+def make_batch(nmesh, nk, seed=7, levels=(2, 3, 4, 5)):
+    """Seeded uneven meshes (icosphere levels taken in turn from ``levels``,
+    anisotropic scale in [0.5, 2]^3, SH noise) and nk wavenumbers each,
+    log-uniform in [5, 80]."""
     from neuralat_amd import mesh
 
     rng = np.random.default_rng(seed)
     out = []
     for q in range(nmesh):
-        level = 2 + q % 4
+        level = levels[q % len(levels)]
         v, t = mesh.icosphere(level, 1.0)
         v = mesh.anisotropic_scale(mesh.sh_perturb(v, 8, 0.05, seed=seed + q), rng.uniform(0.5, 2.0, 3))
         ks = np.exp(rng.uniform(np.log(5.0), np.log(80.0), size=nk))
@@ -415,10 +416,12 @@ def make_batch(nmesh, nk, seed=7):
     return out
 
 
-def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
+def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7, levels=(2, 3, 4, 5), fp64_level2=True):
     """Plain per-item loop (fused assembly + GMRES per (mesh, k)) against one
     persistent launch for all items + lockstep graph-captured GMRES.  Neumann
-    data: one interior monopole per mesh."""
+    data: one interior monopole per mesh.  Also reports the matrix bytes the
+    two GMRES passes stream (sum over items of iterations x 8 n^2) per second
+    of solve time: the bound that batching cannot move."""
     import torch
 
     import oracle
@@ -427,7 +430,7 @@ def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
     from neuralat_amd.bem.solver import BEM_Solver, gmres
 
     dev = torch.device("cuda:0")
-    items = make_batch(nmesh, nk, seed)
+    items = make_batch(nmesh, nk, seed, levels)
     solvers, meshes, ks, psis, host = [], [], [], [], []
     for v, t, kl in items:
         sv = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="DP0")
@@ -445,7 +448,7 @@ def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
     # plain
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xs_plain, conv_plain = [], 0
+    xs_plain, conv_plain, its_plain = [], 0, []
     q = 0
     for sv, (v, t, kl) in zip(solvers, items):
         for k in kl:
@@ -453,6 +456,7 @@ def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
             x, info = gmres(lhs, rhs, tol=1e-6, restart=restart, maxiter=restart * max_cycles)
             xs_plain.append(x)
             conv_plain += int(info["converged"])
+            its_plain.append(info["iterations"])
             q += 1
             if q % 8 == 0:
                 log(f"bench: batch plain loop {q}/{len(ks)} solves")
@@ -463,6 +467,8 @@ def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     batch_cbie(meshes, ks, psis, outs, row_tile=best)
+    torch.cuda.synchronize()
+    t_asm = time.perf_counter() - t0
     xs, infos = batch_gmres([o[0] for o in outs], [o[1] for o in outs], tol=1e-6, restart=restart,
                             max_cycles=max_cycles, use_graph=True)
     torch.cuda.synchronize()
@@ -474,15 +480,23 @@ def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7):
     # FP64 oracle of the level-2 items (320 triangles)
     errs = []
     for q, (v, t, k) in enumerate(host):
-        if t.shape[0] != 320 or not conv[q]:
+        if not fp64_level2 or t.shape[0] != 320 or not conv[q]:
             continue
         S = oracle.assemble(v, t, k, "single", "DP0")
         D = oracle.assemble(v, t, k, "double", "DP0")
         M = oracle.identity(v, t, "DP0")
         x64 = np.linalg.solve(0.5 * M - D, -S @ psis[q].cpu().numpy().astype(np.complex128))
         errs.append(float(np.linalg.norm(xs[q].cpu().numpy() - x64) / np.linalg.norm(x64)))
-    return {"workload": f"{nmesh} meshes (icosphere L2-L5, anisotropic scale [0.5,2], SH noise) x {nk} k "
+    bytes_plain = float(sum(8.0 * n * n * it for n, it in zip(ntri, its_plain)))
+    bytes_batch = float(sum(8.0 * n * n * i["iterations"] for n, i in zip(ntri, infos)))
+    lv = "-".join(f"L{x}" for x in levels)
+    return {"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
                         f"log-uniform [5,80] = {len(ks)} DP0 CBIE solves, GMRES({restart}) tol 1e-6",
+            "gmres_iterations_total_plain": int(sum(its_plain)),
+            "gmres_iterations_total_batched": int(sum(i["iterations"] for i in infos)),
+            "batched_assembly_seconds": t_asm,
+            "matrix_GBps_plain": bytes_plain / t_plain / 1e9,
+            "matrix_GBps_batched_gmres": bytes_batch / (t_batch - t_asm) / 1e9,
             "triangles": sorted(set(ntri)), "plain_seconds": t_plain, "batched_seconds": t_batch,
             "plain_solves_per_s": len(ks) / t_plain, "batched_solves_per_s": len(ks) / t_batch,
             "speedup": t_plain / t_batch, "entries_per_s_batched": entries / t_batch,
@@ -632,6 +646,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_forms:
         forms["batching"] = batch_form(args.batch_meshes, args.batch_ks)
+        torch.cuda.empty_cache()
+        forms["batching_small_meshes"] = batch_form(16, args.batch_ks, levels=(2, 3), seed=9)
         torch.cuda.empty_cache()
         forms["differentiation"] = grad_form()
         log(f"bench: forms {json.dumps(forms)}")

@@ -520,7 +520,7 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
         }
         XTRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
-    std::vector<int> hc(ns, 0);
+    std::vector<int> hc(ns, 0), hk(ns, 0), its(ns, 0);
     int cycles = 0;
     bool all = false;
     for (; cycles < max_cycles; ++cycles) {
@@ -531,9 +531,13 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
         }
         XTRY(hipMemcpyAsync(hc.data(), conv, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
         XTRY(hipMemcpyAsync(hr.data(), rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipMemcpyAsync(hk.data(), nk, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
         XTRY(hipStreamSynchronize(s));
         all = true;
-        for (int q = 0; q < ns; ++q) all = all && hc[q];
+        for (int q = 0; q < ns; ++q) {
+            all = all && hc[q];
+            its[q] += hk[q];  // Arnoldi steps of this item in the cycle (0 once converged)
+        }
         if (all) {
             ++cycles;
             break;
@@ -546,7 +550,7 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
         XTRY(hipMemcpyAsync(sys[q].x, X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
     if (info)
         for (int q = 0; q < ns; ++q) {
-            info[q].iterations = cycles * m;
+            info[q].iterations = its[q];
             info[q].converged = hc[q];
             info[q].rel_residual = hb[q] > 0 ? std::sqrt(hr[q] / hb[q]) : 0.0;
             info[q].implicit_rel_residual = 0.0;

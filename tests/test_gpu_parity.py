@@ -247,6 +247,29 @@ def test_gmres_and_bicgstab_vs_direct():
     # GMRES reports non-convergence
     x2, info2 = gmres(At, bt, tol=1e-12, restart=5, maxiter=5)
     assert not info2["converged"] and info2["iterations"] == 5
+    # the timed entry point runs the same solve (bit for bit) and counts one
+    # product per Arnoldi step plus one explicit residual per restart cycle and
+    # a final one (a cycle may stop on its implicit residual and be followed by
+    # another when the explicit one is still above the tolerance)
+    x3, info3 = gmres(At, bt, tol=1e-6, restart=10, maxiter=500, time_products=True)
+    x4, info4 = gmres(At, bt, tol=1e-6, restart=10, maxiter=500)
+    assert torch.equal(x3, x4) and info3["iterations"] == info4["iterations"]
+    its3 = info3["iterations"]
+    assert its3 + -(-its3 // 10) + 1 <= info3["gemv_count"] <= 2 * its3 + 1
+    assert info3["gemv_ms"] > 0.0
+    # transposed solves (the adjoint of the differentiation form) against numpy
+    for trans, Aop in ((1, A.T), (2, A.conj().T)):
+        from neuralat_amd import _lib
+        from neuralat_amd.bem.solver import _ptr, _stream
+        import ctypes
+
+        lib = _lib.load()
+        xt = torch.zeros(n, dtype=torch.complex64, device=_dev())
+        wk = torch.empty(lib.bem_gmres_work_size(n, 60), dtype=torch.uint8, device=_dev())
+        inf = _lib.SolveInfo()
+        _lib.check(lib.bem_gmres_ex(_ptr(At), n, n, trans, _ptr(bt), _ptr(xt), 1e-6, 60, 500, _ptr(wk),
+                                    ctypes.byref(inf), _stream()))
+        assert _rel(xt.cpu().numpy(), np.linalg.solve(Aop, b)) < 1e-5, trans
 
 
 def test_solve_linear_equation_matches_reference_semantics():
