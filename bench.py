@@ -399,6 +399,110 @@ def grad_form(ndir=4, seed=0):
             "dL_dk_adjoint": float(kt.grad), "dL_dk_fd": fdk}
 
 
+def p1_form(level=5, k=10.0, bm_level=4, bm_k=20.0, reps=3):
+    """The reference's own basis (P1, vertex unknowns): single- and
+    double-layer assembly at ``level`` (HIP-event time per operator, entries/s
+    and algorithmic FLOP/s: 36 Gauss-point pairs x (16 or 29 + 12 P1 weighting)
+    FLOP per regular pair), and the Burton-Miller solve of the reference's data
+    generation (neumann2dirichlet: bm_lhs / bm_rhs with the approximate rules,
+    GMRES, boundary2potential) at ``bm_level`` against the analytic field."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.solver import BEM_Solver
+
+    dev = torch.device("cuda:0")
+    v, t = mesh.icosphere(level, 1.0)
+    v = mesh.sh_perturb(v, 8, 0.05, seed=3).astype(np.float32)
+    s = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="P1")
+    nv, npairs = v.shape[0], s.mesh.n_neighbor_pairs
+    ntri = t.shape[0]
+    out = {"assembly_level": level, "vertices": nv, "k": k}
+    for op, fl in (("single", 16), ("double", 29)):
+        s.assemble_boundary_matrix(k, op)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record()
+        for r in range(reps):
+            A = s.assemble_boundary_matrix(k, op)
+            ev[r + 1].record()
+            del A
+        torch.cuda.synchronize()
+        ms = min(ev[r].elapsed_time(ev[r + 1]) for r in range(reps))
+        flop = 36 * (fl + 12) * (ntri * ntri - npairs)
+        out[op] = {"ms": ms, "entries_per_s": nv * nv / (ms / 1e3), "tflops_algorithmic": flop / (ms / 1e3) / 1e12}
+        log(f"bench: P1 {op} layer level {level}: {ms:.1f} ms, {flop / (ms / 1e3) / 1e12:.1f} TFLOP/s algorithmic")
+    del s
+    torch.cuda.empty_cache()
+    v, t = mesh.icosphere(bm_level, 1.0)
+    v = mesh.sh_perturb(v, 8, 0.05, seed=4).astype(np.float32)
+    val, dn = mesh.sources_field([[0.1, -0.05, 0.02]], None, bm_k)
+    psi = torch.tensor(mesh.project_p1(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64), device=dev)
+    pts = mesh.random_exterior_points(64, 1.5, 3.0, seed=5)
+    pts_t = torch.tensor(pts.astype(np.float32), device=dev)
+    s = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="P1")
+    s.neumann2dirichlet(bm_k, psi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    phi = s.neumann2dirichlet(bm_k, psi)
+    p = s.boundary2potential(bm_k, psi, phi, pts_t)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    pe = val(pts)
+    out["burton_miller"] = {"level": bm_level, "vertices": v.shape[0], "k": bm_k, "seconds": el,
+                            "gmres_iterations": s.last_solve_info["iterations"],
+                            "rel_err_vs_analytic": float(np.linalg.norm(p.cpu().numpy() - pe) / np.linalg.norm(pe))}
+    log(f"bench: P1 Burton-Miller level {bm_level}: {el * 1e3:.1f} ms, "
+        f"{s.last_solve_info['iterations']} GMRES its, error {out['burton_miller']['rel_err_vs_analytic']:.2e}")
+    return out
+
+
+def datagen_form(n_samples=48, batch=16, level=2, seed=11):
+    """The data-generation path (P1 Burton-Miller per sampled scene, |p| at
+    64 targets): one neumann2dirichlet + boundary2potential per sample against
+    the batched driver (per-sample assembly, one lockstep GMRES per batch).
+    Scene: two level-``level`` spheres, one rotating and resized, one moving."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.solver import BEM_Solver
+    from neuralat_amd.datagen import SceneObject, SceneSampler
+
+    v, t = mesh.icosphere(level, 0.1)
+    objs = [SceneObject(v, t, resize=(1.0, 0.0, 0.0), rot_axis="z", rot_pos=(0.05, 0.0, 0.0), rot_max_deg=90.0,
+                        vibration="x"),
+            SceneObject(v + np.array([0.7, 0.0, 0.0]), t, move=(0.0, 0.2, 0.0), position=(0.0, 0.0, 0.1),
+                        vibration="-z")]
+    sc = SceneSampler(objs, trg_sample_num=64, freq_min=200.0, freq_max=4000.0, trg_pos_min=[-0.5] * 3,
+                      trg_pos_max=[0.5] * 3)
+    dev = torch.device("cuda:0")
+    sc.generate(2, seed=seed, batch=2)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, y, infos = sc.generate(n_samples, seed=seed, batch=batch)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    ref = []
+    for q in range(n_samples):
+        f = sc.sample_factors(rng)
+        vv, tt, neu, k, pts = sc.build(f)
+        s = BEM_Solver(torch.tensor(vv.astype(np.float32), device=dev), torch.tensor(tt, device=dev))
+        psi = torch.tensor(neu, device=dev)
+        phi = s.neumann2dirichlet(k, psi)
+        ref.append(s.boundary2potential(k, psi, phi, torch.tensor(pts.astype(np.float32), device=dev)).abs())
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    yr = torch.stack(ref).float().cpu()
+    diff = float((torch.linalg.vector_norm(y[:, :, 0] - yr) / torch.linalg.vector_norm(yr)).item())
+    log(f"bench: datagen {n_samples} samples: loop {t_loop:.2f} s, batched {t_batch:.2f} s")
+    return {"workload": f"{n_samples} scenes of two level-{level} spheres (P1 Burton-Miller, 64 targets)",
+            "vertices": int(2 * v.shape[0]), "loop_seconds": t_loop, "batched_seconds": t_batch,
+            "speedup": t_loop / t_batch, "samples_per_s_batched": n_samples / t_batch,
+            "converged": int(sum(i["converged"] for i in infos)), "rel_diff_batched_vs_loop": diff}
+
+
 def make_batch(nmesh, nk, seed=7, levels=(2, 3, 4, 5)):
     """Seeded uneven meshes (icosphere levels taken in turn from ``levels``,
     anisotropic scale in [0.5, 2]^3, SH noise) and nk wavenumbers each,
@@ -650,6 +754,10 @@ def main():
         forms["batching_small_meshes"] = batch_form(16, args.batch_ks, levels=(2, 3), seed=9)
         torch.cuda.empty_cache()
         forms["differentiation"] = grad_form()
+        torch.cuda.empty_cache()
+        forms["p1"] = p1_form()
+        torch.cuda.empty_cache()
+        forms["datagen"] = datagen_form()
         log(f"bench: forms {json.dumps(forms)}")
     cpu = None
     if not args.no_cpu:

@@ -471,21 +471,32 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // ---------------------------------------------------------------------------
 // regular pairs, P1 basis (LDS tile accumulation)
 // ---------------------------------------------------------------------------
+// Workgroup = (row tile, column tile) of a locality-ordered triangle list.
+// For each row triangle of the tile, phase 1: lane p evaluates the pair with
+// column triangle p and writes its 3 x 3 block (row-basis a, column-basis b)
+// to an LDS stage, no atomics; phase 2: lane w owns column vertex w of the
+// tile, gathers from the stage the entries of the triangles incident to w
+// (host-built incidence lists) and adds them to its own accumulator column,
+// so every LDS accumulator entry has exactly one writer.  (Adding the blocks
+// with LDS float atomics left the kernel waiting on LDS issue 64 % of its wave
+// cycles.)  Each tile entry is flushed once with a global float atomic.
 template <int OP, int Q, bool CLAMP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                          const float* __restrict__ curls, P1Tiles tl, float k,
                                                          float2* __restrict__ out, int64_t ld)
 {
+    static_assert(kP1ColTris == 256 && kP1ColVerts <= 256, "one lane per column triangle and per column vertex");
     constexpr int S = Rec<Q>::kFloats;
-    __shared__ float acc[kP1RowVerts * kP1ColVerts * 2];
+    __shared__ float2 acc[kP1RowVerts][kP1ColVerts];
+    __shared__ float2 stage[9][kP1ColTris];
     const int ct = blockIdx.x, rt = blockIdx.y;
     const int c0 = tl.col_ptr[ct], c1 = tl.col_ptr[ct + 1];
     const int r0 = tl.row_ptr[rt], r1 = tl.row_ptr[rt + 1];
     const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
     const int cv0 = tl.col_vptr[ct], ncv = tl.col_vptr[ct + 1] - cv0;
-    for (int e = threadIdx.x; e < 2 * nrv * ncv; e += 256) acc[e] = 0.f;
-    __syncthreads();
+    for (int e = threadIdx.x; e < nrv * ncv; e += 256) acc[e / ncv][e % ncv] = make_float2(0.f, 0.f);
 
+    // phase-1 role: column triangle c0 + threadIdx.x
     const int pos = c0 + (int)threadIdx.x;
     const bool active = pos < c1;
     const int j = tl.perm[active ? pos : c0];
@@ -497,110 +508,124 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     for (int b = 0; b < Q; ++b) wy[b] = cr[3 * Q + b] * kInv4Pi;
     const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
     const int4 tj = tris[j];
-    const int4 cs = tl.col_slot[active ? pos : c0];
     float cj[9];
     if constexpr (OP == HYP || OP == BM_LHS) {
 #pragma unroll
         for (int e = 0; e < 9; ++e) cj[e] = curls[(int64_t)j * 12 + e];
     }
     const float krev = k * kInv2Pi;
+    // phase-2 role: column vertex slot w = threadIdx.x
+    const int w = (int)threadIdx.x;
+    const bool owner = w < ncv;
+    const int ib = owner ? tl.col_inc_ptr[cv0 + w] : 0;
+    const int ie = owner ? tl.col_inc_ptr[cv0 + w + 1] : 0;
+    __syncthreads();
 
     for (int rp = r0; rp < r1; ++rp) {
         const int i = tl.perm[rp];  // wave-uniform
         const int4 ti = tris[i];
         const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
                             ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z;
-        if (!active || shared) continue;
-        const float* rr = rec + (int64_t)i * S;
-        const int4 rs = tl.row_slot[rp];
         float br[9], bi[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
-        float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
+        if (active && !shared) {
+            const float* rr = rec + (int64_t)i * S;
+            float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
 #pragma unroll
-        for (int a = 0; a < Q; ++a) {
-            const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
-            float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
-            float tr9 = 0.f, ti9 = 0.f;
+            for (int a = 0; a < Q; ++a) {
+                const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+                float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
+                float tr9 = 0.f, ti9 = 0.f;
 #pragma unroll
-            for (int b = 0; b < Q; ++b) {
-                float vr, vi, gr = 0.f, gi = 0.f;
-                kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
-                                     rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, gr, gi);
-                vr *= wy[b];
-                vi *= wy[b];
-                const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
-                const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
-                sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
-                sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
-                sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
+                for (int b = 0; b < Q; ++b) {
+                    float vr, vi, gr = 0.f, gi = 0.f;
+                    kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
+                                         rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, gr, gi);
+                    vr *= wy[b];
+                    vi *= wy[b];
+                    const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
+                    const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
+                    sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
+                    sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
+                    sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
+                    if constexpr (OP == BM_LHS) {
+                        tr9 = fmaf(wy[b], gr, tr9);
+                        ti9 = fmaf(wy[b], gi, ti9);
+                    }
+                }
+                const float ua = rr[3 * Q + a];
+                const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
+                const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
+#pragma unroll
+                for (int be = 0; be < 3; ++be) {
+                    br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
+                    br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
+                    br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
+                }
                 if constexpr (OP == BM_LHS) {
-                    tr9 = fmaf(wy[b], gr, tr9);
-                    ti9 = fmaf(wy[b], gi, ti9);
+                    gr9 = fmaf(ua, tr9, gr9);
+                    gi9 = fmaf(ua, ti9, gi9);
                 }
             }
-            const float ua = rr[3 * Q + a];
-            const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
-            const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
+            if constexpr (OP == HYP) {
+                // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
+                float sgr = 0.f, sgi = 0.f;
 #pragma unroll
-            for (int be = 0; be < 3; ++be) {
-                br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
-                br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
-                br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
-            }
-            if constexpr (OP == BM_LHS) {
-                gr9 = fmaf(ua, tr9, gr9);
-                gi9 = fmaf(ua, ti9, gi9);
-            }
-        }
-        if constexpr (OP == HYP) {
-            // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
-            float sgr = 0.f, sgi = 0.f;
+                for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
+                float ci[9], cp[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
-            float ci[9], cp[9];
+                for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+                curl_product(ci, cj, cp);
+                const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
+                const float kk = k * k * nn;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
-            curl_product(ci, cj, cp);
-            const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
-            const float kk = k * k * nn;
+                for (int e = 0; e < 9; ++e) {
+                    br[e] = sgr * cp[e] - br[e] * kk;
+                    bi[e] = sgi * cp[e] - bi[e] * kk;
+                }
+            } else if constexpr (OP == BM_LHS) {
+                // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
+                const float invk = 1.0f / k;
+                const float s9r = -gi9 * invk, s9i = gr9 * invk;
+                float ci[9], cp[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                br[e] = sgr * cp[e] - br[e] * kk;
-                bi[e] = sgi * cp[e] - bi[e] * kk;
-            }
-        } else if constexpr (OP == BM_LHS) {
-            // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
-            const float invk = 1.0f / k;
-            const float s9r = -gi9 * invk, s9i = gr9 * invk;
-            float ci[9], cp[9];
+                for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+                curl_product(ci, cj, cp);
 #pragma unroll
-            for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
-            curl_product(ci, cj, cp);
-#pragma unroll
-            for (int e = 0; e < 9; ++e) {
-                br[e] = fmaf(s9r, cp[e], br[e]);
-                bi[e] = fmaf(s9i, cp[e], bi[e]);
+                for (int e = 0; e < 9; ++e) {
+                    br[e] = fmaf(s9r, cp[e], br[e]);
+                    bi[e] = fmaf(s9i, cp[e], bi[e]);
+                }
             }
         }
-        const int rsl[3] = {rs.x, rs.y, rs.z};
-        const int csl[3] = {cs.x, cs.y, cs.z};
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                float* dst = &acc[2 * (rsl[a] * ncv + csl[b])];
-                atomicAdd(dst, br[3 * a + b]);
-                atomicAdd(dst + 1, bi[3 * a + b]);
+        for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
+        __syncthreads();
+        if (owner) {
+            const int4 rs = tl.row_slot[rp];
+            float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
+            for (int q = ib; q < ie; ++q) {
+                const int code = tl.col_inc[q];
+                const int p = code >> 2, b = code & 3;
+                const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
+                s0.x += e0.x; s0.y += e0.y;
+                s1.x += e1.x; s1.y += e1.y;
+                s2.x += e2.x; s2.y += e2.y;
             }
+            acc[rs.x][w].x += s0.x; acc[rs.x][w].y += s0.y;
+            acc[rs.y][w].x += s1.x; acc[rs.y][w].y += s1.y;
+            acc[rs.z][w].x += s2.x; acc[rs.z][w].y += s2.y;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     for (int e = threadIdx.x; e < nrv * ncv; e += 256) {
         const int rvi = e / ncv, cvi = e - rvi * ncv;
         const int64_t gr = tl.row_verts[rv0 + rvi], gc = tl.col_verts[cv0 + cvi];
         float* dst = reinterpret_cast<float*>(&out[gr * ld + gc]);
-        unsafeAtomicAdd(dst, acc[2 * e]);
-        unsafeAtomicAdd(dst + 1, acc[2 * e + 1]);
+        const float2 v = acc[rvi][cvi];
+        unsafeAtomicAdd(dst, v.x);
+        unsafeAtomicAdd(dst + 1, v.y);
     }
 }
 

@@ -14,10 +14,12 @@ constexpr int kDp0RowTile = 64;
 // Potential kernel: points per workgroup.
 constexpr int kPotRowTile = 16;
 // P1 tiles: at most kP1RowTris row triangles touching at most kP1RowVerts
-// vertices, at most 256 column triangles touching at most kP1ColVerts
-// vertices (the LDS accumulator is kP1RowVerts x kP1ColVerts complex).
-constexpr int kP1RowTris = 32;
-constexpr int kP1RowVerts = 40;
+// vertices, at most kP1ColTris column triangles touching at most kP1ColVerts
+// vertices (the LDS accumulator is kP1RowVerts x kP1ColVerts complex, the
+// per-row block stage 9 x kP1ColTris complex; a 256-thread workgroup has one
+// lane per column triangle and one per column vertex).
+constexpr int kP1RowTris = 16;
+constexpr int kP1RowVerts = 20;
 constexpr int kP1ColTris = 256;
 constexpr int kP1ColVerts = 200;
 
@@ -32,6 +34,11 @@ struct P1Tiles {
     const int32_t* col_vptr;
     const int32_t* col_verts;
     const int4* col_slot;
+    // column-vertex incidences: for tile vertex slot g (col_vptr[tile] + w),
+    // col_inc[col_inc_ptr[g] .. col_inc_ptr[g + 1]) = 4 * (position - col_ptr[tile]) + b
+    // for each triangle of the tile whose local vertex b is that vertex
+    const int32_t* col_inc_ptr;
+    const int32_t* col_inc;
     int nrow_tiles, ncol_tiles;
 };
 

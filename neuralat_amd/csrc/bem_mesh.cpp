@@ -277,6 +277,23 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     }
     const Tiling rowT = make_tiling(perm, h_tris, kP1RowTris, kP1RowVerts);
     const Tiling colT = make_tiling(perm, h_tris, kP1ColTris, kP1ColVerts);
+    // incidences of each column-tile vertex slot (CSR over the global slot index)
+    std::vector<int32_t> inc_ptr(colT.verts.size() + 1, 0), inc;
+    {
+        const int ntile = (int)colT.ptr.size() - 1;
+        for (int tl = 0; tl < ntile; ++tl)
+            for (int32_t p = colT.ptr[tl]; p < colT.ptr[tl + 1]; ++p)
+                for (int b = 0; b < 3; ++b) ++inc_ptr[colT.vptr[tl] + colT.slot[4 * (size_t)p + b] + 1];
+        for (size_t g = 0; g + 1 < inc_ptr.size(); ++g) inc_ptr[g + 1] += inc_ptr[g];
+        inc.assign((size_t)inc_ptr.back(), 0);
+        std::vector<int32_t> fill(inc_ptr.begin(), inc_ptr.end() - 1);
+        for (int tl = 0; tl < ntile; ++tl)
+            for (int32_t p = colT.ptr[tl]; p < colT.ptr[tl + 1]; ++p)
+                for (int b = 0; b < 3; ++b) {
+                    const int32_t g = colT.vptr[tl] + colT.slot[4 * (size_t)p + b];
+                    inc[(size_t)fill[g]++] = 4 * (p - colT.ptr[tl]) + b;
+                }
+    }
 
     // ---- one device allocation ----
     struct Part {
@@ -298,7 +315,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
     parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});
-    int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv;
+    int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv, *d_cip, *d_ci;
     int4 *d_rs, *d_cs;
     parts.push_back({perm.data(), perm.size() * 4, (void**)&d_perm});
     parts.push_back({rowT.ptr.data(), rowT.ptr.size() * 4, (void**)&d_rp});
@@ -309,6 +326,8 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({colT.vptr.data(), colT.vptr.size() * 4, (void**)&d_cvp});
     parts.push_back({colT.verts.data(), colT.verts.size() * 4, (void**)&d_cv});
     parts.push_back({colT.slot.data(), colT.slot.size() * 4, (void**)&d_cs});
+    parts.push_back({inc_ptr.data(), inc_ptr.size() * 4, (void**)&d_cip});
+    parts.push_back({inc.data(), inc.size() * 4, (void**)&d_ci});
     size_t total = 0;
     std::vector<size_t> off;
     for (auto& p : parts) {
@@ -344,6 +363,8 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     m.tiles.col_vptr = d_cvp;
     m.tiles.col_verts = d_cv;
     m.tiles.col_slot = d_cs;
+    m.tiles.col_inc_ptr = d_cip;
+    m.tiles.col_inc = d_ci;
     m.tiles.nrow_tiles = (int)rowT.ptr.size() - 1;
     m.tiles.ncol_tiles = (int)colT.ptr.size() - 1;
     return 0;

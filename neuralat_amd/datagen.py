@@ -1,0 +1,184 @@
+"""Batched data generation for the neural acoustic-transfer datasets.
+
+Mirrors the static-object part of the reference's scene sampler and dataset
+writer (src/scene.py:114-291, experiments/neuPAT/generate.py:13-39): each
+sample places the objects (resize, rotation about a pivot, move, position),
+merges them into one mesh with per-vertex Neumann data from the objects'
+vibration masks, draws a frequency log-uniformly in [freq_min, freq_max]
+(k = 2 pi f / 343.2), solves the P1 Burton-Miller system of
+``BEM_Solver.neumann2dirichlet`` and evaluates |p| at target points sampled in
+a spherical shell around the target box.  The samples' systems are assembled
+one by one (as the reference) and solved together by the lockstep batched
+GMRES (one graph-captured restart cycle for all of them), which is where a
+batch of small scenes gains over one solve per sample.
+
+Output layout, as generate.py: x [S, T, 3 + n_rot + n_move + 2] float32 =
+(target factors, rotation factors, move factors, resize factor, frequency
+factor), y [S, T, 1] float32 = |p|.  Meshes are given as arrays (file loading
+and trajectories/object lists of the reference are outside the BEM path).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .bem.batch import batch_gmres
+from .bem.solver import BEM_Solver, combine, matvec
+
+SPEED_OF_SOUND = 343.2  # src/scene.py:275
+
+
+@dataclass
+class SceneObject:
+    """One static object (src/scene.py:114-191).  ``resize``: per-axis scale
+    weights (a zero entry keeps that axis), ``rot_axis``: scipy Euler axis
+    ("x", "y", "z"), ``rot_pos``: pivot (scaled with the object),
+    ``rot_max_deg``, ``move``: displacement at factor 1, ``position``: offset,
+    ``vibration``: "x", "-y", ... marks the vibrating half (Neumann 1)."""
+
+    vertices: np.ndarray
+    triangles: np.ndarray
+    resize: tuple | None = None
+    rot_axis: str | None = None
+    rot_pos: tuple | None = None
+    rot_max_deg: float | None = None
+    move: tuple | None = None
+    position: tuple = (0.0, 0.0, 0.0)
+    vibration: str | None = None
+    neumann: np.ndarray = field(init=False)
+
+    def __post_init__(self):
+        self.vertices = np.asarray(self.vertices, dtype=np.float64)
+        self.triangles = np.asarray(self.triangles, dtype=np.int32)
+        self.neumann = np.zeros(len(self.vertices), dtype=np.complex64)
+        if self.vibration is not None:
+            axis = {"x": 0, "y": 1, "z": 2}[self.vibration.replace("-", "")]
+            coord = self.vertices[:, axis]
+            self.neumann[(coord < 0) if self.vibration.startswith("-") else (coord > 0)] = 1.0
+
+    def place(self, resize_factor, rot_factor, move_factor, moving_scene=True):
+        """Vertices for the given factors: resize, rotation about the scaled
+        pivot, then move and position -- the position offset is applied by the
+        move step, which the reference runs only in scenes that have a moving
+        object (src/scene.py:186-190,243-249)."""
+        from scipy.spatial.transform import Rotation
+
+        base = np.zeros(3) if self.resize is None else np.asarray(self.resize, dtype=np.float64)
+        scale = base * resize_factor
+        scale[scale == 0] = 1.0
+        v = self.vertices * scale
+        if self.rot_axis is not None:
+            pivot = np.asarray(self.rot_pos, dtype=np.float64) * scale
+            v = Rotation.from_euler(self.rot_axis, rot_factor * self.rot_max_deg, degrees=True).apply(v - pivot) + pivot
+        if moving_scene:
+            if self.move is not None:
+                v = v + np.asarray(self.move, dtype=np.float64) * move_factor
+            v = v + np.asarray(self.position, dtype=np.float64)
+        return v
+
+
+class SceneSampler:
+    """Samples scenes and solves them in batches (src/scene.py:193-291)."""
+
+    def __init__(self, objects, trg_sample_num, freq_min, freq_max, trg_pos_min, trg_pos_max, max_resize=2.0):
+        self.objects = list(objects)
+        self.rot_num = sum(o.rot_axis is not None for o in self.objects)
+        self.move_num = sum(o.move is not None for o in self.objects)
+        self.trg_sample_num = int(trg_sample_num)
+        self.freq_min_log, self.freq_max_log = np.log10(freq_min), np.log10(freq_max)
+        lo, hi = np.asarray(trg_pos_min, dtype=np.float64), np.asarray(trg_pos_max, dtype=np.float64)
+        self.bbox_size = float((hi - lo).max())
+        self.bbox_center = (hi + lo) / 2
+        self.max_resize = float(max_resize)
+
+    @property
+    def feature_dim(self):
+        return 3 + self.rot_num + self.move_num + 2
+
+    def sample_factors(self, rng):
+        """Uniform [0, 1) factors of one sample, in the order the reference draws them."""
+        return {"rot": rng.random(self.rot_num), "move": rng.random(self.move_num), "resize": rng.random(),
+                "freq": rng.random(), "trg": rng.random((self.trg_sample_num, 3))}
+
+    def build(self, f):
+        """(vertices, triangles, neumann, k, target points) of one sample (src/scene.py:218-291)."""
+        verts, tris, neu = [], [], []
+        ri = mi = 0
+        nv = 0
+        for o in self.objects:
+            rf = f["rot"][ri] if o.rot_axis is not None else 0.0
+            mf = f["move"][mi] if o.move is not None else 0.0
+            v = o.place(f["resize"] * (self.max_resize - 1) + 1, rf, mf, self.move_num > 0)
+            ri += o.rot_axis is not None and ri < self.rot_num - 1
+            mi += o.move is not None and mi < self.move_num - 1
+            verts.append(v)
+            tris.append(o.triangles + nv)
+            neu.append(o.neumann)
+            nv += len(v)
+        freq = 10 ** (f["freq"] * (self.freq_max_log - self.freq_min_log) + self.freq_min_log)
+        k = 2 * np.pi * freq / SPEED_OF_SOUND
+        u = f["trg"]
+        rs = (u[:, 0] + 1) * self.bbox_size
+        theta = u[:, 1] * 2 * np.pi - np.pi
+        phi = u[:, 2] * np.pi
+        pts = np.stack([rs * np.sin(phi) * np.cos(theta), rs * np.sin(phi) * np.sin(theta), rs * np.cos(phi)], -1)
+        return (np.concatenate(verts), np.concatenate(tris), np.concatenate(neu), float(k),
+                pts + self.bbox_center)
+
+    def features(self, f):
+        x = np.zeros((self.trg_sample_num, self.feature_dim), dtype=np.float32)
+        x[:, :3] = f["trg"]
+        x[:, 3:3 + self.rot_num] = f["rot"]
+        x[:, 3 + self.rot_num:3 + self.rot_num + self.move_num] = f["move"]
+        x[:, -2] = f["resize"]
+        x[:, -1] = f["freq"]
+        return x
+
+    def generate(self, n_samples, seed=0, batch=16, device="cuda", tol=1e-6, restart=100, max_cycles=20):
+        """x [S, T, D], y [S, T, 1] (generate.py:13-39) with ``batch`` samples
+        solved per lockstep GMRES call.  Returns (x, y, infos)."""
+        rng = np.random.default_rng(seed)
+        dev = torch.device(device)
+        S, T = int(n_samples), self.trg_sample_num
+        x = torch.zeros((S, T, self.feature_dim), dtype=torch.float32)
+        y = torch.zeros((S, T, 1), dtype=torch.float32)
+        infos = []
+        for s0 in range(0, S, batch):
+            facs = [self.sample_factors(rng) for _ in range(min(batch, S - s0))]
+            systems = []
+            for f in facs:
+                v, t, neu, k, pts = self.build(f)
+                solver = BEM_Solver(torch.tensor(v.astype(np.float32), device=dev),
+                                    torch.tensor(t, device=dev), basis="P1")
+                lhs, rhs = burton_miller_system(solver, k, torch.tensor(neu, device=dev))
+                systems.append((solver, k, torch.tensor(neu, device=dev), lhs, rhs,
+                                torch.tensor(pts.astype(np.float32), device=dev)))
+            xs, inf = batch_gmres([q[3] for q in systems], [q[4] for q in systems], tol=tol, restart=restart,
+                                  max_cycles=max_cycles)
+            infos.extend(inf)
+            for q, (f, (solver, k, neu, _, _, pts)) in enumerate(zip(facs, systems)):
+                p = solver.boundary2potential(k, neu, xs[q], pts)
+                x[s0 + q] = torch.from_numpy(self.features(f))
+                y[s0 + q, :, 0] = p.abs().float().cpu()
+        return x, y, infos
+
+
+def burton_miller_system(solver, k, neumann):
+    """(bm_lhs + M/2, (bm_rhs - beta M/2) psi) with the approximate rules, the
+    system BEM_Solver.neumann2dirichlet solves (src/bem/solver.py:118-126)."""
+    beta = 1j / k
+    identity = solver.identity_matrix()
+    lhs = combine(solver.assemble_boundary_matrix(k, "bm_lhs", approx=True), 1.0, identity, 0.5)
+    R = combine(solver.assemble_boundary_matrix(k, "bm_rhs", approx=True), 1.0, identity, -0.5 * beta)
+    rhs = matvec(R, neumann.to(torch.complex64).contiguous())
+    return lhs, rhs
+
+
+def save_dataset(path, x, y):
+    """The reference's dataset file: torch.save({"x": x, "y": y}) (generate.py:39)."""
+    torch.save({"x": x, "y": y}, path)
+
+
+__all__ = ["SceneObject", "SceneSampler", "burton_miller_system", "save_dataset"]

@@ -1,0 +1,81 @@
+"""The batched data-generation driver (neuralat_amd/datagen.py): host-side
+sampling semantics of src/scene.py on the CPU, and on the GPU the batched
+solve against one neumann2dirichlet + boundary2potential per sample."""
+import numpy as np
+import pytest
+import torch
+
+from neuralat_amd import mesh
+from neuralat_amd.datagen import SceneObject, SceneSampler
+
+
+def _objects():
+    v, t = mesh.icosphere(1, 0.1)
+    a = SceneObject(v, t, resize=(1.0, 0.0, 0.0), rot_axis="z", rot_pos=(0.05, 0.0, 0.0), rot_max_deg=90.0,
+                    vibration="x")
+    b = SceneObject(v + np.array([0.7, 0.0, 0.0]), t, move=(0.0, 0.2, 0.0), position=(0.0, 0.0, 0.1),
+                    vibration="-z")
+    return a, b
+
+
+def test_object_placement_semantics():
+    a, b = _objects()
+    # zero factors: identity for the rotation, resize weight 0 keeps an axis
+    assert np.allclose(a.place(1.0, 0.0, 0.0), a.vertices)
+    va = a.place(2.0, 0.0, 0.0)
+    assert np.allclose(va[:, 0], 2 * a.vertices[:, 0]) and np.allclose(va[:, 1:], a.vertices[:, 1:])
+    # full rotation factor = rot_max_deg about z through the pivot: distances to the pivot axis are kept
+    vr = a.place(1.0, 1.0, 0.0)
+    piv = np.array([0.05, 0.0])
+    assert np.allclose(np.linalg.norm(vr[:, :2] - piv, axis=1), np.linalg.norm(a.vertices[:, :2] - piv, axis=1))
+    assert np.allclose(vr[:, 2], a.vertices[:, 2])
+    rel = a.vertices[:, :2] - piv
+    assert np.allclose(vr[:, :2] - piv, np.stack([-rel[:, 1], rel[:, 0]], 1), atol=1e-12)  # +90 deg
+    # move + position only in a scene with a moving object
+    assert np.allclose(b.place(1.0, 0.0, 0.5), b.vertices + [0.0, 0.1, 0.1])
+    assert np.allclose(b.place(1.0, 0.0, 0.5, moving_scene=False), b.vertices)
+    # vibration masks
+    assert np.all(a.neumann[a.vertices[:, 0] > 0] == 1) and np.all(a.neumann[a.vertices[:, 0] <= 0] == 0)
+    assert np.all(b.neumann[b.vertices[:, 2] < 0] == 1) and np.all(b.neumann[b.vertices[:, 2] >= 0] == 0)
+
+
+def test_sampler_layout_and_targets():
+    sc = SceneSampler(_objects(), trg_sample_num=7, freq_min=100.0, freq_max=4000.0, trg_pos_min=[-0.5] * 3,
+                      trg_pos_max=[0.5] * 3)
+    assert sc.rot_num == 1 and sc.move_num == 1 and sc.feature_dim == 3 + 1 + 1 + 2
+    rng = np.random.default_rng(0)
+    f = sc.sample_factors(rng)
+    v, t, neu, k, pts = sc.build(f)
+    assert v.shape[0] == 2 * 42 and t.shape[0] == 2 * 80 and t.max() == v.shape[0] - 1 and neu.shape == (84,)
+    # frequency log-uniform in [100, 4000] Hz, k = 2 pi f / 343.2
+    freq = k * 343.2 / (2 * np.pi)
+    assert np.isclose(np.log10(freq), f["freq"] * (np.log10(4000) - 2) + 2)
+    # targets in the shell r in [bbox, 2 bbox] around the box centre
+    r = np.linalg.norm(pts, axis=1)
+    assert np.all(r >= 1.0 - 1e-12) and np.all(r <= 2.0 + 1e-12)
+    x = sc.features(f)
+    assert x.shape == (7, 7) and np.allclose(x[:, :3], f["trg"]) and np.isclose(x[0, 3], f["rot"][0])
+    assert np.isclose(x[0, 4], f["move"][0]) and np.isclose(x[0, -2], f["resize"]) and np.isclose(x[0, -1], f["freq"])
+
+
+@pytest.mark.gpu
+def test_batched_generation_matches_per_sample_solves():
+    from neuralat_amd.bem import BEM_Solver
+
+    sc = SceneSampler(_objects(), trg_sample_num=16, freq_min=200.0, freq_max=2000.0, trg_pos_min=[-0.5] * 3,
+                      trg_pos_max=[0.5] * 3)
+    x, y, infos = sc.generate(5, seed=3, batch=3)
+    assert x.shape == (5, 16, 7) and y.shape == (5, 16, 1)
+    assert all(i["converged"] for i in infos), infos
+    rng = np.random.default_rng(3)
+    dev = torch.device("cuda:0")
+    for q in range(5):
+        f = sc.sample_factors(rng)
+        v, t, neu, k, pts = sc.build(f)
+        s = BEM_Solver(torch.tensor(v.astype(np.float32), device=dev), torch.tensor(t, device=dev))
+        psi = torch.tensor(neu, device=dev)
+        phi = s.neumann2dirichlet(k, psi)
+        p = s.boundary2potential(k, psi, phi, torch.tensor(pts.astype(np.float32), device=dev)).abs().cpu().numpy()
+        assert np.allclose(x[q].numpy(), sc.features(f))
+        err = np.linalg.norm(y[q, :, 0].numpy() - p) / np.linalg.norm(p)
+        assert err < 2e-5, (q, err)
