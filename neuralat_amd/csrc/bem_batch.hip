@@ -65,14 +65,36 @@ __global__ __launch_bounds__(256) void xgemv_kernel(const BSys* __restrict__ sys
     if (r0 >= S.n) return;
     const float2* xs = x + S.off;
     float ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
-    for (int64_t c = lane; c < S.n; c += 64) {
-        const float2 xv = xs[c];
+    // 16-byte non-temporal loads (two entries per lane) when the item's rows
+    // and x are 16-byte aligned; uniform over the workgroup
+    const bool vec = ((S.n | S.lda) & 1) == 0 && (((uintptr_t)S.A | (uintptr_t)xs) & 15) == 0;
+    if (vec) {
+        typedef float v4f __attribute__((ext_vector_type(4)));
+        const int64_t n2 = S.n / 2, ld2 = S.lda / 2;
+        const v4f* A4 = reinterpret_cast<const v4f*>(S.A);
+        const float4* x4 = reinterpret_cast<const float4*>(xs);
+        for (int64_t c = lane; c < n2; c += 64) {
+            const float4 xv = x4[c];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t r = min(r0 + q, S.n - 1);
-            const float2 a = S.A[r * S.lda + c];
-            ar[q] = fmaf(a.x, xv.x, fmaf(-a.y, xv.y, ar[q]));
-            ai[q] = fmaf(a.x, xv.y, fmaf(a.y, xv.x, ai[q]));
+            for (int q = 0; q < 4; ++q) {
+                const int64_t r = min(r0 + q, S.n - 1);
+                const v4f a = __builtin_nontemporal_load(&A4[r * ld2 + c]);
+                ar[q] = fmaf(a.x, xv.x, fmaf(-a.y, xv.y, ar[q]));
+                ai[q] = fmaf(a.x, xv.y, fmaf(a.y, xv.x, ai[q]));
+                ar[q] = fmaf(a.z, xv.z, fmaf(-a.w, xv.w, ar[q]));
+                ai[q] = fmaf(a.z, xv.w, fmaf(a.w, xv.z, ai[q]));
+            }
+        }
+    } else {
+        for (int64_t c = lane; c < S.n; c += 64) {
+            const float2 xv = xs[c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int64_t r = min(r0 + q, S.n - 1);
+                const float2 a = S.A[r * S.lda + c];
+                ar[q] = fmaf(a.x, xv.x, fmaf(-a.y, xv.y, ar[q]));
+                ai[q] = fmaf(a.x, xv.y, fmaf(a.y, xv.x, ai[q]));
+            }
         }
     }
 #pragma unroll
