@@ -532,6 +532,16 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         if (active && !shared) {
             const float* rr = rec + (int64_t)i * S;
             float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
+            // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
+            // triangle's plane, so it is formed once per column point
+            float qb[Q];
+            if constexpr (OP == ADLP) {
+                const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+#pragma unroll
+                for (int b = 0; b < Q; ++b)
+                    qb[b] = wy[b] * ((py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                                     (py[3 * b + 2] - rr[2]) * nx2);
+            }
 #pragma unroll
             for (int a = 0; a < Q; ++a) {
                 const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
@@ -540,10 +550,32 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 #pragma unroll
                 for (int b = 0; b < Q; ++b) {
                     float vr, vi, gr = 0.f, gi = 0.f;
-                    kernel4pi<OP, CLAMP>(py[3 * b] - xa0, py[3 * b + 1] - xa1, py[3 * b + 2] - xa2, rr[4 * Q],
-                                         rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k, krev, vr, vi, gr, gi);
-                    vr *= wy[b];
-                    vi *= wy[b];
+                    const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+                    if constexpr (OP == SLP || OP == DLP || OP == ADLP) {
+                        // specialised evaluations (as the DP0 kernels); the DLP normal
+                        // factor (x_a - y_0).n_y is applied per row point below
+                        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        if (CLAMP) r2 = fmaxf(r2, kEps2);
+                        const float rinv = __builtin_amdgcn_rsqf(r2);
+                        const float r = r2 * rinv;
+                        float sn, cs;
+                        sincos_rev(r * krev, sn, cs);
+                        if constexpr (OP == SLP) {
+                            const float g = rinv * wy[b];
+                            vr = g * cs;
+                            vi = g * sn;
+                        } else {
+                            const float th = r * k;
+                            const float pw = rinv * rinv * (rinv * (OP == DLP ? wy[b] : qb[b]));
+                            vr = pw * fmaf(th, sn, cs);
+                            vi = pw * fmaf(-th, cs, sn);
+                        }
+                    } else {
+                        kernel4pi<OP, CLAMP>(dx, dy, dz, rr[4 * Q], rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k,
+                                             krev, vr, vi, gr, gi);
+                        vr *= wy[b];
+                        vi *= wy[b];
+                    }
                     const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
                     const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
                     sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
@@ -554,7 +586,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                         ti9 = fmaf(wy[b], gi, ti9);
                     }
                 }
-                const float ua = rr[3 * Q + a];
+                float ua = rr[3 * Q + a];
+                if constexpr (OP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
+                    ua *= (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
                 const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
                 const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
 #pragma unroll

@@ -63,7 +63,7 @@ constexpr int64_t kDp0RowTileHost = bem::kDp0RowTile;
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct GmresLayout {
-    size_t V, w, r, H, h, h2, cs, sn, g, y, res, sc, tw, total;
+    size_t V, w, r, H, h, h2, cs, sn, g, y, res, sc, tw, hp, total;
 };
 GmresLayout gmres_layout(int64_t n, int m)
 {
@@ -87,6 +87,7 @@ GmresLayout gmres_layout(int64_t n, int m)
     L.res = take(sizeof(double) * (size_t)m);
     L.sc = take(sizeof(double) * 8);
     L.tw = take(sizeof(float2) * (size_t)bem::gemv_work_elems(n, n, 1));
+    L.hp = take(sizeof(double2) * (size_t)bem::multidot_part_elems(m + 1));
     L.total = o;
     return L;
 }
@@ -424,6 +425,7 @@ int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const voi
     double* res = (double*)(W + L.res);
     double* sc = (double*)(W + L.sc);
     float2* tw = (float2*)(W + L.tw);
+    double2* hp = (double2*)(W + L.hp);
     const float2* A = (const float2*)A_;
     const float2* b = (const float2*)b_;
     float2* x = (float2*)x_;
@@ -502,9 +504,9 @@ int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const voi
             float2* vj = V + (size_t)j * n;
             HIP_TRY(gemv_timed(vj, w, one, zero));
             // classical Gram-Schmidt, twice
-            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, s));
+            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, hp, s));
             HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));
-            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h2, s));
+            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h2, hp, s));
             HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h2, w, s));
             HIP_TRY(bem::launch_norm2(w, n, sc + 1, s));
             HIP_TRY(bem::launch_normalize(w, n, sc + 1, V + (size_t)(j + 1) * n, s));

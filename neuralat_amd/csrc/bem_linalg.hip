@@ -13,6 +13,8 @@
 
 #include "bem_linalg.hpp"
 
+#include <algorithm>
+
 namespace bem {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -265,6 +267,48 @@ __global__ __launch_bounds__(256) void multidot_kernel(const float2* __restrict_
     sr = block_sum_d(sr, sh);
     si = block_sum_d(si, sh);
     if (threadIdx.x == 0) hout[q] = make_double2(sr, si);
+}
+
+// <V_q, w> over element range s of S (grid (nq, S)), 16-byte loads; rows of V
+// and w must be 16-byte aligned and n even (checked by the launcher).  Partial
+// sums part[s * nq + q], combined in a fixed order by multidot_reduce_kernel:
+// a vector split over S workgroups keeps enough loads in flight when the basis
+// is short, and the whole chip busy when it is long.
+constexpr int kDotMaxSplit = 16;
+__global__ __launch_bounds__(256) void multidot_split_kernel(const float2* __restrict__ V, int64_t ldv, int64_t n,
+                                                             const float2* __restrict__ w, int nq,
+                                                             double2* __restrict__ part)
+{
+    __shared__ double sh[8];
+    const int q = blockIdx.x, sp = blockIdx.y, ns = gridDim.y;
+    const int64_t n2 = n / 2;
+    const int64_t chunk = (n2 + ns - 1) / ns;
+    const int64_t e0 = (int64_t)sp * chunk, e1 = min(n2, e0 + chunk);
+    const float4* v = reinterpret_cast<const float4*>(V + (int64_t)q * ldv);
+    const float4* w4 = reinterpret_cast<const float4*>(w);
+    double sr = 0.0, si = 0.0;
+#pragma unroll 4
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+        const float4 a = v[e], b = w4[e];
+        sr += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+        si += (double)a.x * b.y - (double)a.y * b.x + (double)a.z * b.w - (double)a.w * b.z;
+    }
+    sr = block_sum_d(sr, sh);
+    si = block_sum_d(si, sh);
+    if (threadIdx.x == 0) part[(int64_t)sp * nq + q] = make_double2(sr, si);
+}
+__global__ __launch_bounds__(256) void multidot_reduce_kernel(const double2* __restrict__ part, int ns, int nq,
+                                                              double2* __restrict__ hout)
+{
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    double sr = 0.0, si = 0.0;
+    for (int sp = 0; sp < ns; ++sp) {
+        const double2 p = part[(int64_t)sp * nq + q];
+        sr += p.x;
+        si += p.y;
+    }
+    hout[q] = make_double2(sr, si);
 }
 
 // out_s[e] += sign * sum_{q < nq_s} c_s[q] V_s[q][e] for each system s (grid y):
@@ -739,11 +783,23 @@ hipError_t launch_combine(const float2* A, int64_t lda, const float2* B, int64_t
     return hipGetLastError();
 }
 
-hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, const float2* w, double2* h, hipStream_t s)
+hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, const float2* w, double2* h,
+                           double2* part, hipStream_t s)
 {
-    hipLaunchKernelGGL(multidot_kernel, dim3(nq), dim3(256), 0, s, V, ldv, n, w, h);
+    const bool vec = part && (n % 2 == 0) && (ldv % 2 == 0) && (((uintptr_t)V | (uintptr_t)w) & 15) == 0;
+    if (!vec) {
+        hipLaunchKernelGGL(multidot_kernel, dim3(nq), dim3(256), 0, s, V, ldv, n, w, h);
+        return hipGetLastError();
+    }
+    // enough workgroups to fill the chip about four times, each at least 4096 entries
+    int ns = (int)std::min<int64_t>(kDotMaxSplit, std::max<int64_t>(1, (1024 + nq - 1) / nq));
+    ns = (int)std::min<int64_t>(ns, std::max<int64_t>(1, n / 4096));
+    hipLaunchKernelGGL(multidot_split_kernel, dim3(nq, ns), dim3(256), 0, s, V, ldv, n, w, nq, part);
+    hipLaunchKernelGGL(multidot_reduce_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, part, ns, nq, h);
     return hipGetLastError();
 }
+
+int64_t multidot_part_elems(int nq) { return (int64_t)kDotMaxSplit * nq; }
 hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
                             hipStream_t s)
 {

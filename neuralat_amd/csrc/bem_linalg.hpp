@@ -13,7 +13,9 @@ int64_t gemv_work_elems(int64_t rows, int64_t cols, int trans);
 hipError_t launch_combine(const float2* A, int64_t lda, const float2* B, int64_t ldb, float2* C, int64_t ldc,
                           int64_t rows, int64_t cols, float2 a, float2 b, hipStream_t s);
 hipError_t launch_multidot(const float2* V, int64_t ldv, int64_t n, int nq, const float2* w, double2* h,
-                           hipStream_t s);
+                           double2* part, hipStream_t s);
+// scratch (double2 entries) of launch_multidot's split form for nq basis vectors
+int64_t multidot_part_elems(int nq);
 hipError_t launch_gs_update(const float2* V, int64_t ldv, int64_t n, int nq, const double2* h, float2* w,
                             hipStream_t s);
 hipError_t launch_norm2(const float2* w, int64_t n, double* out, hipStream_t s);
