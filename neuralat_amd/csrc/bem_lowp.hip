@@ -20,7 +20,8 @@
 //    Ar Xr, Ai Xi, Ar Xi, Ai Xr, accumulated in FP32; Y = (rr - ii) + i (ri + ir).
 //    One right-hand side uses the same kernel (the product is HBM-bound).
 //  * gemm_c64_mfma_kernel: the same 16-RHS product on the complex64 matrix with
-//    v_mfma_f32_16x16x4f32 (the FP32 residual of the refinement).
+//    v_mfma_f32_16x16x4f32 (the FP32 residual of the refinement), right-hand
+//    sides staged in LDS per workgroup.
 // Right-hand-side blocks are stored per system: X[s * ldx + k].
 #include <hip/hip_runtime.h>
 
@@ -245,45 +246,55 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
     }
 }
 
-// complex64 matrix, 16 right-hand sides, FP32 matrix cores.  Lane l owns
-// row r0 + (l % 16) and the columns kb + 8 (l / 16) + [0, 8) of each 32-wide
-// step; each of the 8 sub-steps is one K = 4 slice over the four lane groups.
-__global__ __launch_bounds__(256) void gemm_c64_mfma_kernel(const float2* __restrict__ A, int64_t rows, int64_t cols,
-                                                            int64_t lda, const float2* __restrict__ X, int64_t ldx,
-                                                            int nrhs, float2* __restrict__ Y, int64_t ldy,
-                                                            float2 alpha, float2 beta, int beta_zero, int64_t kchunk,
-                                                            float2* __restrict__ part)
+// complex64 matrix, up to 16 right-hand sides, FP32 matrix cores.  A
+// workgroup of kC64Waves wavefronts owns kC64Waves row tiles of 16 rows and a
+// column range (split-K); per 32-column step the workgroup stages the 16 x 32
+// slice of the right-hand sides in LDS once (double-buffered, one barrier per
+// step) and every wavefront takes its B operands from there -- read per
+// wavefront from global memory, the right-hand sides cost as many bytes as the
+// matrix and the product ran at 1.8 TB/s.  Lane l owns row r0 + (l % 16) and
+// the columns kb + 8 (l / 16) + [0, 8) of each step; each of the 8 sub-steps
+// is one K = 4 slice over the four lane groups.
+constexpr int kC64Waves = 8;
+__global__ __launch_bounds__(64 * kC64Waves) void gemm_c64_mfma_kernel(
+    const float2* __restrict__ A, int64_t rows, int64_t cols, int64_t lda, const float2* __restrict__ X, int64_t ldx,
+    int nrhs, float2* __restrict__ Y, int64_t ldy, float2 alpha, float2 beta, int beta_zero, int64_t kchunk,
+    float2* __restrict__ part)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ float2 xs[2][16][32];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int li = lane & 15, q = lane >> 4;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 16;
-    if (r0 >= rows) return;
-    const int64_t ra = min(r0 + li, rows - 1);
-    const float2* pa = A + ra * lda;
+    const int64_t r0 = ((int64_t)blockIdx.x * kC64Waves + wv) * 16;
+    const float2* pa = A + min(r0 + li, rows - 1) * lda;
     const bool live = li < nrhs;
-    const float2* px = X + (int64_t)(live ? li : 0) * ldx;
     f32x4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk;  // columns [k0, k0 + kchunk) of this split
     const int64_t kend = min(cols, k0 + kchunk);
-#pragma unroll 2
+    int buf = 0;
     for (int64_t kb = k0; kb < kend; kb += 32) {
-        float2 a[8], b[8];
+        for (int t = threadIdx.x; t < 16 * 32; t += 64 * kC64Waves) {
+            const int sx = t >> 5, c = t & 31;
+            const int64_t cc = kb + c;
+            xs[buf][sx][c] = (sx < nrhs && cc < kend) ? X[(int64_t)sx * ldx + cc] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        float2 a[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int64_t c = kb + 8 * q + e;
-            const bool in = c < kend;
-            a[e] = in ? pa[c] : make_float2(0.f, 0.f);
-            b[e] = (in && live) ? px[c] : make_float2(0.f, 0.f);
+            a[e] = c < kend ? pa[c] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b[e].x, rr, 0, 0, 0);
-            ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b[e].y, ii, 0, 0, 0);
-            ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b[e].y, ri, 0, 0, 0);
-            ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b[e].x, ir, 0, 0, 0);
+            const float2 b = xs[buf][li][8 * q + e];
+            rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b.x, rr, 0, 0, 0);
+            ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b.y, ii, 0, 0, 0);
+            ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b.y, ri, 0, 0, 0);
+            ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b.x, ir, 0, 0, 0);
         }
+        buf ^= 1;  // the next step writes the other buffer: every wave has left it (previous barrier)
     }
-    if (!live) return;
+    if (r0 >= rows || !live) return;  // after the loop: all waves take part in every barrier
     if (part) {  // split-K partial, combined by splitk_reduce_kernel
         float2* P = part + ((int64_t)blockIdx.y * nrhs + li) * rows;
 #pragma unroll
@@ -419,13 +430,13 @@ hipError_t launch_gemm_c64(const float2* A, int64_t rows, int64_t cols, int64_t 
 {
     const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
     const int64_t tiles = (rows + 15) / 16;
-    const unsigned g = (unsigned)((tiles + 3) / 4);
+    const unsigned g = (unsigned)((tiles + kC64Waves - 1) / kC64Waves);
     int ks = 1;
     int64_t kchunk = (cols + 31) / 32 * 32;
     if (part) choose_split(tiles, cols, ks, kchunk);
     float2* pp = ks > 1 ? part : nullptr;
-    hipLaunchKernelGGL(gemm_c64_mfma_kernel, dim3(g, (unsigned)ks), dim3(256), 0, s, A, rows, cols, lda, X, ldx, nrhs, Y,
-                       ldy, alpha, beta, bz, kchunk, pp);
+    hipLaunchKernelGGL(gemm_c64_mfma_kernel, dim3(g, (unsigned)ks), dim3(64 * kC64Waves), 0, s, A, rows, cols, lda, X,
+                       ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
     if (pp)
         hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nrhs), dim3(256), 0, s,
                            pp, ks, nrhs, rows, 1.f, Y, ldy, alpha, beta, bz);
