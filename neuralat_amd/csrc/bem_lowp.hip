@@ -158,37 +158,42 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
                                                              float2 beta, int beta_zero, int64_t kchunk,
                                                              float2* __restrict__ part)
 {
+    // The right-hand-side planes (hi and lo parts of Xr, Xi) of each 32-column
+    // step are staged in LDS once per workgroup (double-buffered, one barrier
+    // per step): read per wavefront they cost as many bytes as the matrix.
+    __shared__ u16x8 xs[2][4][16][4];
     const int lane = threadIdx.x & 63;
     const int li = lane & 15, q = lane >> 4;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 32;
-    if (r0 >= rows) return;
     const int64_t nk = ld / 32, ntile = (rows + 15) / 16;
-    const int64_t ta = r0 / 16, tb = min(ta + 1, ntile - 1);  // the second tile may not exist
+    const int64_t ta = min(r0 / 16, ntile - 1), tb = min(ta + 1, ntile - 1);  // clamped: tiles may not exist
     // tile (T, K) of a plane at (T nk + K) 512; this lane's 8 entries at 8 lane
     const uint16_t* par = Ar + ta * nk * 512 + 8 * lane;
     const uint16_t* pai = Ai + ta * nk * 512 + 8 * lane;
     const uint16_t* pbr = Ar + tb * nk * 512 + 8 * lane;
     const uint16_t* pbi = Ai + tb * nk * 512 + 8 * lane;
-    const int s = li < nrhs ? li : 0;  // right-hand side of this lane's B operand
-    const uint16_t* pxr = Xr + (int64_t)s * ldx + 8 * q;
-    const uint16_t* pxi = Xi + (int64_t)s * ldx + 8 * q;
+    // this thread's share of the staging: plane (t / 64), right-hand side ((t / 4) % 16), 8 columns (t % 4)
+    const int tp = threadIdx.x >> 6, ts = (threadIdx.x >> 2) & 15, tg = threadIdx.x & 3;
+    const uint16_t* px = ((tp & 1) ? Xi : Xr) + (tp >> 1) * 16 * ldx + (int64_t)ts * ldx + 8 * tg;
+    const bool tlive = ts < nrhs;
     const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
     const bool live = li < nrhs;
-    constexpr bool kSplit = true;  // hi + lo right-hand-side planes
     f32x4 a_rr = {0, 0, 0, 0}, a_ii = a_rr, a_ri = a_rr, a_ir = a_rr;
     f32x4 b_rr = a_rr, b_ii = a_rr, b_ri = a_rr, b_ir = a_rr;
     // columns [k0, k1) of this split (ld and kchunk are multiples of 32; padded columns are zero)
     const int64_t k0 = (int64_t)blockIdx.y * kchunk;
     const int64_t kend = min(ld, k0 + kchunk);
-#pragma unroll 2
+    int buf = 0;
     for (int64_t kb = k0; kb < kend; kb += 32) {
-        const u16x8 xr = live ? *reinterpret_cast<const u16x8*>(pxr + kb) : zero8;
-        const u16x8 xi = live ? *reinterpret_cast<const u16x8*>(pxi + kb) : zero8;
-        const int64_t to = kb * 16;  // (kb / 32) tiles of 512 entries
+        const int64_t to = kb * 16;  // (kb / 32) tiles of 512 entries; matrix loads issued before the barrier
         const u16x8 ar = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(par + to));
         const u16x8 ai = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pai + to));
         const u16x8 br = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbr + to));
         const u16x8 bi = __builtin_nontemporal_load(reinterpret_cast<const u16x8*>(pbi + to));
+        xs[buf][tp][ts][tg] = tlive ? *reinterpret_cast<const u16x8*>(px + kb) : zero8;
+        __syncthreads();
+        const u16x8 xr = xs[buf][0][li][q], xi = xs[buf][1][li][q];
+        const u16x8 xrl = xs[buf][2][li][q], xil = xs[buf][3][li][q];
         a_rr = mfma16x16x32<DT>(ar, xr, a_rr);
         a_ii = mfma16x16x32<DT>(ai, xi, a_ii);
         a_ri = mfma16x16x32<DT>(ar, xi, a_ri);
@@ -197,19 +202,17 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
         b_ii = mfma16x16x32<DT>(bi, xi, b_ii);
         b_ri = mfma16x16x32<DT>(br, xi, b_ri);
         b_ir = mfma16x16x32<DT>(bi, xr, b_ir);
-        if constexpr (kSplit) {
-            const u16x8 xrl = live ? *reinterpret_cast<const u16x8*>(pxr + 16 * ldx + kb) : zero8;
-            const u16x8 xil = live ? *reinterpret_cast<const u16x8*>(pxi + 16 * ldx + kb) : zero8;
-            a_rr = mfma16x16x32<DT>(ar, xrl, a_rr);
-            a_ii = mfma16x16x32<DT>(ai, xil, a_ii);
-            a_ri = mfma16x16x32<DT>(ar, xil, a_ri);
-            a_ir = mfma16x16x32<DT>(ai, xrl, a_ir);
-            b_rr = mfma16x16x32<DT>(br, xrl, b_rr);
-            b_ii = mfma16x16x32<DT>(bi, xil, b_ii);
-            b_ri = mfma16x16x32<DT>(br, xil, b_ri);
-            b_ir = mfma16x16x32<DT>(bi, xrl, b_ir);
-        }
+        a_rr = mfma16x16x32<DT>(ar, xrl, a_rr);
+        a_ii = mfma16x16x32<DT>(ai, xil, a_ii);
+        a_ri = mfma16x16x32<DT>(ar, xil, a_ri);
+        a_ir = mfma16x16x32<DT>(ai, xrl, a_ir);
+        b_rr = mfma16x16x32<DT>(br, xrl, b_rr);
+        b_ii = mfma16x16x32<DT>(bi, xil, b_ii);
+        b_ri = mfma16x16x32<DT>(br, xil, b_ri);
+        b_ir = mfma16x16x32<DT>(bi, xrl, b_ir);
+        buf ^= 1;  // the next step writes the other buffer: every wave has left it (previous barrier)
     }
+    if (r0 >= rows) return;  // after the loop: all waves take part in every barrier
     // D layout: lane l holds column (right-hand side) l % 16, rows 4 (l / 16) + e
     if (!live) return;
     if (part) {  // split-K partial, combined by splitk_reduce_kernel
