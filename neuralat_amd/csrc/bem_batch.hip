@@ -130,10 +130,21 @@ __global__ __launch_bounds__(256) void xmultidot_kernel(const BSys* __restrict__
     const float2* v = V + (int64_t)q * ntot + S.off;
     const float2* ww = w + S.off;
     double sr = 0.0, si = 0.0;
-    for (int64_t e = threadIdx.x; e < S.n; e += 256) {
-        const float2 a = v[e], b = ww[e];
-        sr += (double)a.x * b.x + (double)a.y * b.y;
-        si += (double)a.x * b.y - (double)a.y * b.x;
+    if ((S.n & 1) == 0 && (((uintptr_t)v | (uintptr_t)ww) & 15) == 0) {
+        const float4* v4 = reinterpret_cast<const float4*>(v);
+        const float4* w4 = reinterpret_cast<const float4*>(ww);
+#pragma unroll 4
+        for (int64_t e = threadIdx.x; e < S.n / 2; e += 256) {
+            const float4 a = v4[e], b = w4[e];
+            sr += (double)a.x * b.x + (double)a.y * b.y + (double)a.z * b.z + (double)a.w * b.w;
+            si += (double)a.x * b.y - (double)a.y * b.x + (double)a.z * b.w - (double)a.w * b.z;
+        }
+    } else {
+        for (int64_t e = threadIdx.x; e < S.n; e += 256) {
+            const float2 a = v[e], b = ww[e];
+            sr += (double)a.x * b.x + (double)a.y * b.y;
+            si += (double)a.x * b.y - (double)a.y * b.x;
+        }
     }
     sr = bsum_d(sr, sh);
     si = bsum_d(si, sh);
