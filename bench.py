@@ -66,10 +66,13 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--check-level", type=int, default=3, help="mesh level of the FP64 oracle check (0 = skip)")
     p.add_argument("--no-forms", action="store_true", help="skip the precision / batching / differentiation forms")
-    p.add_argument("--batch-meshes", type=int, default=8)
+    p.add_argument("--batch-meshes", type=int, default=64)
     p.add_argument("--batch-ks", type=int, default=8)
     p.add_argument("--traffic-probe", action="store_true",
                    help="run one fused assembly launch, one GEMV and a 4 GiB copy, for hardware-counter passes")
+    p.add_argument("--batch-probe", action="store_true",
+                   help="one persistent assembly, one lockstep complex64 GMRES and one lockstep fp16 mixed-precision "
+                        "solve of the first 8 batching meshes x 8 k, for kernel traces")
     p.add_argument("--p1-probe", action="store_true",
                    help="assemble the level-5 P1 single- and double-layer operators once each, for counter passes")
     return p.parse_args()
@@ -522,94 +525,194 @@ def make_batch(nmesh, nk, seed=7, levels=(2, 3, 4, 5)):
     return out
 
 
-def batch_form(nmesh=8, nk=8, restart=200, max_cycles=10, seed=7, levels=(2, 3, 4, 5), fp64_level2=True):
-    """Plain per-item loop (fused assembly + GMRES per (mesh, k)) against one
-    persistent launch for all items + lockstep graph-captured GMRES.  Neumann
+def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 4, 5), chunk_gb=115.0,
+               mixed_dtype="fp16", fp64_items=8):
+    """The batching workload (nmesh seeded meshes x nk wavenumbers), three ways:
+    a plain per-item loop (fused assembly + GMRES per (mesh, k)); one
+    persistent assembly launch + lockstep graph-captured complex64 GMRES; the
+    same assembly + the lockstep mixed-precision solve (FP32 flexible GMRES,
+    16-bit inner GMRES).  The items' complex64 systems exceed one GPU's HBM
+    for the full workload, so all three run on chunks of consecutive meshes
+    whose systems fit ``chunk_gb`` (one persistent launch per chunk).  Neumann
     data: one interior monopole per mesh.  Also reports the matrix bytes the
-    two GMRES passes stream (sum over items of iterations x 8 n^2) per second
-    of solve time: the bound that batching cannot move."""
+    GMRES passes stream per second of solve time (sum over items of
+    iterations x bytes per entry x n^2)."""
     import torch
 
     import oracle
     from neuralat_amd import mesh
-    from neuralat_amd.bem.batch import allocate_systems, batch_autotune, batch_cbie, batch_gmres
+    from neuralat_amd.bem.batch import (allocate_systems, batch_autotune, batch_cbie, batch_gmres,
+                                        batch_solve_mixed)
     from neuralat_amd.bem.solver import BEM_Solver, gmres
 
     dev = torch.device("cuda:0")
     items = make_batch(nmesh, nk, seed, levels)
-    solvers, meshes, ks, psis, host = [], [], [], [], []
-    for v, t, kl in items:
+    chunks, cur, cur_b = [], [], 0.0
+    for it in items:
+        b = 8.0 * it[1].shape[0] ** 2 * len(it[2])
+        if cur and cur_b + b > chunk_gb * 1e9:
+            chunks.append(cur)
+            cur, cur_b = [], 0.0
+        cur.append(it)
+        cur_b += b
+    chunks.append(cur)
+    max_cycles = max(1, maxiter // restart)
+    acc = {"plain": 0.0, "batched": 0.0, "mixed": 0.0, "batched_asm": 0.0, "mixed_asm": 0.0}
+    its_plain, its_batch, its_mixed, ntri_all = [], [], [], []
+    conv = {"plain": 0, "batched": 0, "mixed": 0}
+    diffs_b, diffs_m, errs_b, errs_m = [], [], [], []
+    best, tune_ms, done = None, None, 0
+    for ci, chunk in enumerate(chunks):
+        solvers, meshes, ks, psis, host = [], [], [], [], []
+        for v, t, kl in chunk:
+            sv = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="DP0")
+            solvers.append(sv)
+            for k in kl:
+                _, dn = mesh.sources_field([[0.05, -0.02, 0.01]], None, k)
+                meshes.append(sv.mesh)
+                ks.append(k)
+                psis.append(torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64),
+                                         device=dev))
+                host.append((v, t, k))
+        outs = allocate_systems(meshes, dev)
+        ntri = [m.nt for m in meshes]
+        ntri_all += ntri
+        # plain per-item loop
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xs_plain = []
+        q = 0
+        for sv, (v, t, kl) in zip(solvers, chunk):
+            for k in kl:
+                lhs, rhs = sv.cbie_system(k, psis[q], out=outs[q][0])
+                x, info = gmres(lhs, rhs, tol=1e-6, restart=restart, maxiter=maxiter)
+                xs_plain.append(x)
+                conv["plain"] += int(info["converged"])
+                its_plain.append(info["iterations"])
+                q += 1
+        torch.cuda.synchronize()
+        acc["plain"] += time.perf_counter() - t0
+        if best is None:  # row tile for the persistent launch, tuned once (not timed)
+            best, tune_ms = batch_autotune(meshes, ks, psis, outs)
+        # persistent assembly + lockstep complex64 GMRES
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch_cbie(meshes, ks, psis, outs, row_tile=best)
+        torch.cuda.synchronize()
+        acc["batched_asm"] += time.perf_counter() - t0
+        xs_b, inf_b = batch_gmres([o[0] for o in outs], [o[1] for o in outs], tol=1e-6, restart=restart,
+                                  max_cycles=max_cycles, use_graph=True)
+        torch.cuda.synchronize()
+        acc["batched"] += time.perf_counter() - t0
+        # persistent assembly + lockstep mixed-precision solve
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch_cbie(meshes, ks, psis, outs, row_tile=best)
+        torch.cuda.synchronize()
+        acc["mixed_asm"] += time.perf_counter() - t0
+        xs_m, inf_m = batch_solve_mixed([o[0] for o in outs], [o[1] for o in outs], dtype=mixed_dtype, tol=1e-6,
+                                        inner_tol=1e-3, outer_restart=10, inner_restart=restart,
+                                        max_inner_cycles=1, max_outer_cycles=3)
+        torch.cuda.synchronize()
+        acc["mixed"] += time.perf_counter() - t0
+        its_batch += [i["iterations"] for i in inf_b]
+        its_mixed += [i["inner_iterations"] for i in inf_m]
+        cb = [bool(i["converged"]) for i in inf_b]
+        cm = [bool(i["converged"]) for i in inf_m]
+        conv["batched"] += sum(cb)
+        conv["mixed"] += sum(cm)
+        for xb, xm, xp, c1, c2 in zip(xs_b, xs_m, xs_plain, cb, cm):
+            nrm = torch.linalg.vector_norm(xp)
+            if c1:
+                diffs_b.append(float((torch.linalg.vector_norm(xb - xp) / nrm).item()))
+            if c2:
+                diffs_m.append(float((torch.linalg.vector_norm(xm - xp) / nrm).item()))
+        # FP64 oracle (the checker) on the first level-2 items (320 triangles)
+        for q, (v, t, k) in enumerate(host):
+            if len(errs_b) >= fp64_items or t.shape[0] != 320 or not (cb[q] and cm[q]):
+                continue
+            S = oracle.assemble(v, t, k, "single", "DP0")
+            D = oracle.assemble(v, t, k, "double", "DP0")
+            M = oracle.identity(v, t, "DP0")
+            x64 = np.linalg.solve(0.5 * M - D, -S @ psis[q].cpu().numpy().astype(np.complex128))
+            n64 = np.linalg.norm(x64)
+            errs_b.append(float(np.linalg.norm(xs_b[q].cpu().numpy() - x64) / n64))
+            errs_m.append(float(np.linalg.norm(xs_m[q].cpu().numpy() - x64) / n64))
+        done += len(ks)
+        log(f"bench: batching chunk {ci + 1}/{len(chunks)} ({done} solves): plain {acc['plain']:.2f} s, "
+            f"batched {acc['batched']:.2f} s, mixed {acc['mixed']:.2f} s")
+        del outs, xs_plain, xs_b, xs_m, solvers, meshes, psis
+        torch.cuda.empty_cache()
+    nsol = len(ntri_all)
+    entries = float(sum(2 * n * n for n in ntri_all))
+    bytes_plain = float(sum(8.0 * n * n * it for n, it in zip(ntri_all, its_plain)))
+    bytes_batch = float(sum(8.0 * n * n * it for n, it in zip(ntri_all, its_batch)))
+    bytes_mixed = float(sum(4.0 * n * n * it for n, it in zip(ntri_all, its_mixed)))
+    lv = "-".join(f"L{x}" for x in levels)
+    tp, tb, tm = acc["plain"], acc["batched"], acc["mixed"]
+    return {"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
+                        f"log-uniform [5,80] = {nsol} DP0 CBIE solves to 1e-6, GMRES({restart}) at most "
+                        f"{maxiter} iterations, in {len(chunks)} chunk(s) of at most {chunk_gb:.0f} GB of systems",
+            "chunks": len(chunks), "triangles": sorted(set(ntri_all)),
+            "plain_seconds": tp, "batched_seconds": tb, "mixed_seconds": tm,
+            "plain_solves_per_s": nsol / tp, "batched_solves_per_s": nsol / tb, "mixed_solves_per_s": nsol / tm,
+            "speedup": tp / tb, "speedup_mixed": tp / tm,
+            "mixed_dtype": mixed_dtype, "mixed_solver": "FP32 flexible GMRES(10), inner 16-bit GMRES(restart) "
+                                                      "to max(1e-3, tol |b| / |r|), diagonal in FP32",
+            "batched_assembly_seconds": acc["batched_asm"], "mixed_assembly_seconds": acc["mixed_asm"],
+            "entries_per_s_batched": entries / tb, "entries_per_s_mixed": entries / tm,
+            "gmres_iterations_total_plain": int(sum(its_plain)),
+            "gmres_iterations_total_batched": int(sum(its_batch)),
+            "inner_iterations_total_mixed": int(sum(its_mixed)),
+            "matrix_GBps_plain": bytes_plain / tp / 1e9,
+            "matrix_GBps_batched_gmres": bytes_batch / (tb - acc["batched_asm"]) / 1e9,
+            "matrix_GBps_mixed_inner": bytes_mixed / (tm - acc["mixed_asm"]) / 1e9,
+            "converged_plain": conv["plain"], "converged_batched": conv["batched"],
+            "converged_mixed": conv["mixed"],
+            "autotune_row_tile": best, "autotune_ms": tune_ms,
+            "max_rel_diff_batched_vs_plain": max(diffs_b) if diffs_b else None,
+            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None,
+            "fp64_items": len(errs_b),
+            "max_surface_rel_err_vs_fp64_level2_batched": max(errs_b) if errs_b else None,
+            "max_surface_rel_err_vs_fp64_level2_mixed": max(errs_m) if errs_m else None}
+
+
+def batch_probe(nmesh=8, nk=8, restart=500):
+    """The first ``nmesh`` batching meshes x ``nk`` wavenumbers: one persistent
+    assembly launch, one lockstep complex64 GMRES, one lockstep fp16
+    mixed-precision solve (the settings of the batching form)."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.batch import allocate_systems, batch_cbie, batch_gmres, batch_solve_mixed
+    from neuralat_amd.bem.solver import BEM_Solver
+
+    dev = torch.device("cuda:0")
+    meshes, ks, psis = [], [], []
+    for v, t, kl in make_batch(nmesh, nk):
         sv = BEM_Solver(torch.tensor(v, device=dev), torch.tensor(t, device=dev), basis="DP0")
-        solvers.append(sv)
         for k in kl:
             _, dn = mesh.sources_field([[0.05, -0.02, 0.01]], None, k)
             meshes.append(sv.mesh)
             ks.append(k)
             psis.append(torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64),
                                      device=dev))
-            host.append((v, t, k))
     outs = allocate_systems(meshes, dev)
-    ntri = [m.nt for m in meshes]
-    entries = float(sum(2 * n * n for n in ntri))
-    # plain
+    batch_cbie(meshes, ks, psis, outs)
+    As, bs = [o[0] for o in outs], [o[1] for o in outs]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    xs_plain, conv_plain, its_plain = [], 0, []
-    q = 0
-    for sv, (v, t, kl) in zip(solvers, items):
-        for k in kl:
-            lhs, rhs = sv.cbie_system(k, psis[q], out=outs[q][0])
-            x, info = gmres(lhs, rhs, tol=1e-6, restart=restart, maxiter=restart * max_cycles)
-            xs_plain.append(x)
-            conv_plain += int(info["converged"])
-            its_plain.append(info["iterations"])
-            q += 1
-            if q % 8 == 0:
-                log(f"bench: batch plain loop {q}/{len(ks)} solves")
+    _, ib = batch_gmres(As, bs, tol=1e-6, restart=restart, max_cycles=max(1, 2000 // restart))
     torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    log(f"bench: batch plain loop {t_plain:.2f} s for {len(ks)} solves")
-    best, tune_ms = batch_autotune(meshes, ks, psis, outs)
+    t1 = time.perf_counter()
+    _, im = batch_solve_mixed(As, bs, dtype="fp16", tol=1e-6, inner_tol=1e-3, outer_restart=10, inner_restart=restart,
+                              max_inner_cycles=1, max_outer_cycles=3)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    batch_cbie(meshes, ks, psis, outs, row_tile=best)
-    torch.cuda.synchronize()
-    t_asm = time.perf_counter() - t0
-    xs, infos = batch_gmres([o[0] for o in outs], [o[1] for o in outs], tol=1e-6, restart=restart,
-                            max_cycles=max_cycles, use_graph=True)
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
-    log(f"bench: batch persistent + graph GMRES {t_batch:.2f} s")
-    conv = [bool(i["converged"]) for i in infos]
-    diffs = [float((torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)).item())
-             for a, b, c in zip(xs, xs_plain, conv) if c]
-    # FP64 oracle of the level-2 items (320 triangles)
-    errs = []
-    for q, (v, t, k) in enumerate(host):
-        if not fp64_level2 or t.shape[0] != 320 or not conv[q]:
-            continue
-        S = oracle.assemble(v, t, k, "single", "DP0")
-        D = oracle.assemble(v, t, k, "double", "DP0")
-        M = oracle.identity(v, t, "DP0")
-        x64 = np.linalg.solve(0.5 * M - D, -S @ psis[q].cpu().numpy().astype(np.complex128))
-        errs.append(float(np.linalg.norm(xs[q].cpu().numpy() - x64) / np.linalg.norm(x64)))
-    bytes_plain = float(sum(8.0 * n * n * it for n, it in zip(ntri, its_plain)))
-    bytes_batch = float(sum(8.0 * n * n * i["iterations"] for n, i in zip(ntri, infos)))
-    lv = "-".join(f"L{x}" for x in levels)
-    return {"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
-                        f"log-uniform [5,80] = {len(ks)} DP0 CBIE solves, GMRES({restart}) tol 1e-6",
-            "gmres_iterations_total_plain": int(sum(its_plain)),
-            "gmres_iterations_total_batched": int(sum(i["iterations"] for i in infos)),
-            "batched_assembly_seconds": t_asm,
-            "matrix_GBps_plain": bytes_plain / t_plain / 1e9,
-            "matrix_GBps_batched_gmres": bytes_batch / (t_batch - t_asm) / 1e9,
-            "triangles": sorted(set(ntri)), "plain_seconds": t_plain, "batched_seconds": t_batch,
-            "plain_solves_per_s": len(ks) / t_plain, "batched_solves_per_s": len(ks) / t_batch,
-            "speedup": t_plain / t_batch, "entries_per_s_batched": entries / t_batch,
-            "converged_batched": int(sum(conv)), "converged_plain": conv_plain,
-            "autotune_row_tile": best, "autotune_ms": tune_ms,
-            "max_rel_diff_batched_vs_plain": max(diffs) if diffs else None,
-            "max_surface_rel_err_vs_fp64_level2": max(errs) if errs else None}
+    t2 = time.perf_counter()
+    log(f"bench: batch probe {len(ks)} items: complex64 GMRES {t1 - t0:.3f} s "
+        f"({sum(i['iterations'] for i in ib)} its, {sum(i['converged'] for i in ib)} converged), "
+        f"fp16 mixed {t2 - t1:.3f} s ({sum(i['inner_iterations'] for i in im)} inner its, "
+        f"{sum(i['converged'] for i in im)} converged)")
 
 
 def cpu_baseline(v, t, k, seconds):
@@ -705,6 +808,9 @@ def main():
         return
     if args.p1_probe:
         p1_probe()
+        return
+    if args.batch_probe:
+        batch_probe()
         return
     log(f"bench: building level-{args.level} workload")
     v, t, src, pts = make_workload(args.level)

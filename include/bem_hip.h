@@ -258,13 +258,30 @@ typedef struct {
     const void* b; /* [n] */
     void* x;       /* [n] initial guess in, solution out */
 } bem_batch_system;
-/* GMRES(restart) on all systems in lockstep (different matrices and sizes),
- * one restart cycle per hipGraph replay when use_graph != 0; stops when every
- * system has ||b - A x|| <= tol ||b|| or after max_cycles cycles.  info[q].
- * iterations = cycles x restart (an upper bound per system). */
+/* GMRES(restart) on all systems in lockstep (different matrices and sizes);
+ * with use_graph != 0 a restart cycle is replayed from hipGraphs (start,
+ * segments of 20 Arnoldi steps, end), and the cycle ends early once every
+ * system meets its target; stops when every system has
+ * ||b - A x|| <= tol ||b|| or after max_cycles cycles.  info[q].iterations =
+ * Arnoldi steps taken by system q. */
 int64_t bem_batch_gmres_work_size(const bem_batch_system* sys, int n_sys, int restart);
 int bem_batch_gmres(const bem_batch_system* sys, int n_sys, float tol, int restart, int max_cycles, int use_graph,
                     void* work, bem_solve_info* info, void* stream);
+/* Batched mixed-precision solve: flexible GMRES(outer_restart) in FP32 on
+ * every system's complex64 matrix (products, residuals and updates in FP32),
+ * whose directions z_j ~ A^-1 v_j come from lockstep GMRES(inner_restart) on
+ * a 16-bit copy of the matrix (dtype 1 = bf16, 2 = fp16 with a per-system
+ * power-of-two scale; the diagonal kept in FP32), made inside the call in the
+ * workspace, run per system to ||v_j - A16 z_j|| <= max(inner_tol,
+ * tol ||b|| / ||r_j||) (r_j: the current outer residual; capped at 0.5) or
+ * for max_inner_cycles cycles.  Stops when every system has
+ * ||b - A x|| <= tol ||b|| (FP32 residual) or after max_outer_cycles cycles.
+ * info[q].iterations = outer steps; inner_iterations[q] (optional) = 16-bit
+ * Arnoldi steps. */
+int64_t bem_batch_fgmres_work_size(const bem_batch_system* sys, int n_sys, int outer_restart, int inner_restart);
+int bem_batch_fgmres(const bem_batch_system* sys, int n_sys, int dtype, float tol, float inner_tol, int outer_restart,
+                     int inner_restart, int max_inner_cycles, int max_outer_cycles, int use_graph, void* work,
+                     bem_solve_info* info, int* inner_iterations, void* stream);
 const char* bem_batch_last_error(void);
 
 #ifdef __cplusplus

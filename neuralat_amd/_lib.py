@@ -98,6 +98,9 @@ SIGNATURES = {
     "bem_batch_autotune": (_I32, [_P, _I32, _I32, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_F), _P]),
     "bem_batch_gmres_work_size": (_I64, [_P, _I32, _I32]),
     "bem_batch_gmres": (_I32, [_P, _I32, _F, _I32, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
+    "bem_batch_fgmres_work_size": (_I64, [_P, _I32, _I32, _I32]),
+    "bem_batch_fgmres": (_I32, [_P, _I32, _I32, _F, _F, _I32, _I32, _I32, _I32, _I32, _P,
+                                ctypes.POINTER(SolveInfo), _P, _P]),
     "bem_batch_last_error": (ctypes.c_char_p, []),
     "bem_gmres_ex": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
     "bem_gmres_timed": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo),

@@ -97,4 +97,41 @@ def batch_gmres(As, bs, xs=None, tol=1e-6, restart=50, max_cycles=40, use_graph=
     return xs, [i.as_dict() for i in info]
 
 
-__all__ = ["Mesh", "allocate_systems", "batch_cbie", "batch_autotune", "batch_gmres"]
+def batch_solve_mixed(As, bs, xs=None, dtype="bf16", tol=1e-6, inner_tol=1e-6, outer_restart=10,
+                      inner_restart=200, max_inner_cycles=10, max_outer_cycles=2, use_graph=True,
+                      raise_on_fail=False):
+    """Lockstep mixed-precision solves of As[q] x = bs[q]: flexible GMRES in
+    FP32 on the complex64 matrices, preconditioned by lockstep GMRES on 16-bit
+    (``dtype`` "bf16" or "fp16") copies with the diagonal kept in FP32.
+    Returns (xs, infos); infos[q]["iterations"] = outer steps,
+    infos[q]["inner_iterations"] = 16-bit Arnoldi steps."""
+    n = len(As)
+    if dtype not in ("bf16", "fp16"):
+        raise ValueError("dtype must be 'bf16' or 'fp16'")
+    xs = xs or [torch.zeros_like(b) for b in bs]
+    arr = (_Sys * n)()
+    for q in range(n):
+        check_tensor(As[q], torch.complex64)
+        check_tensor(bs[q], torch.complex64)
+        arr[q] = _Sys(As[q].data_ptr(), As[q].stride(0), As[q].shape[0], bs[q].data_ptr(), xs[q].data_ptr())
+    lib = _lib.load()
+    inner_restart = int(min(inner_restart, max(A.shape[0] for A in As)))
+    outer_restart = int(outer_restart)
+    work = torch.empty(lib.bem_batch_fgmres_work_size(arr, n, outer_restart, inner_restart), dtype=torch.uint8,
+                       device=bs[0].device)
+    info = (_lib.SolveInfo * n)()
+    inner = (ctypes.c_int32 * n)()
+    rc = lib.bem_batch_fgmres(arr, n, 1 if dtype == "bf16" else 2, float(tol), float(inner_tol), outer_restart,
+                              inner_restart, int(max_inner_cycles), int(max_outer_cycles), int(use_graph),
+                              _ptr(work), info, inner, _stream())
+    if rc != _lib.BEM_ERR_NOT_CONVERGED or raise_on_fail:
+        _check(rc, "bem_batch_fgmres")
+    out = []
+    for q in range(n):
+        d = info[q].as_dict()
+        d["inner_iterations"] = int(inner[q])
+        out.append(d)
+    return xs, out
+
+
+__all__ = ["Mesh", "allocate_systems", "batch_cbie", "batch_autotune", "batch_gmres", "batch_solve_mixed"]

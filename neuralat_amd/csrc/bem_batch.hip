@@ -7,9 +7,13 @@
 //    its own matrix and size; vectors of all items live in one concatenated
 //    space (item s at offset off_s), and every kernel of an Arnoldi step covers
 //    all items (grid y = item).  Per-item Givens rotations and convergence
-//    flags stay on the device; a converged item skips its work.  One restart
-//    cycle (residual, set-up, m Arnoldi steps, update) is captured once as a
-//    hipGraph and replayed; the host reads the items' flags once per cycle.
+//    flags stay on the device; a converged item skips its work.  A restart
+//    cycle (residual and set-up, Arnoldi steps in segments of kSeg, update) is
+//    captured once as hipGraphs and replayed; the host reads the items' flags
+//    after each segment and ends the cycle once every item is done.
+//  * bem_batch_fgmres: the mixed-precision variant -- flexible GMRES in FP32
+//    on the complex64 matrices whose directions come from the same lockstep
+//    GMRES run on 16-bit copies of the matrices (diagonal kept in FP32).
 // No reference counterpart: the reference solves one item at a time
 // (src/scene.py:270-291, experiments/neuPAT/generate.py:23-33).
 #include <hip/hip_runtime.h>
@@ -25,6 +29,8 @@
 #include "../../include/bem_hip.h"
 #include "bem_internal.hpp"
 #include "bem_linalg.hpp"
+#include "bem_lowp.hpp"
+#include "bem_lowp_cvt.hpp"
 
 struct bem_mesh {
     bem::MeshDev dev;
@@ -238,6 +244,181 @@ __global__ void xconv_kernel(const double* __restrict__ rn2, const double* __res
     if (s < ns) conv[s] = sqrt(rn2[s]) <= tgt[s] ? 1 : 0;
 }
 
+// ---- 16-bit copies of the items' matrices (mixed-precision batch) ---------------
+// Entry (r, c) of item s is the pair (re, im) of round(scale_s A_s(r, c)) at
+// L[r * ldl + c] (real part in the low half); columns [n, ldl) are zero.  The
+// diagonal stays in FP32 (diag[r] = A_s(r, r), L(r, r) = 0): it is the
+// largest part of a second-kind operator (the M/2 term), and rounding it
+// would dominate the difference between the 16-bit operator and A.
+struct BLow {
+    uint32_t* L;
+    int64_t ldl;
+    float2* diag;
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int DT>
+__device__ __forceinline__ float2 lp_pair(uint32_t u)
+{
+    if (DT == kDtBF16) return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+    return make_float2(f16_to_f32((uint16_t)(u & 0xffffu)), f16_to_f32((uint16_t)(u >> 16)));
+}
+
+__device__ __forceinline__ void cfma(float2 a, float xr, float xi, float& ar, float& ai)
+{
+    ar = fmaf(a.x, xr, fmaf(-a.y, xi, ar));
+    ai = fmaf(a.x, xi, fmaf(a.y, xr, ai));
+}
+
+// y_s = alpha inv_scale_s Alow_s x_s + beta y_s.  A wavefront owns kLowRows
+// rows; per step each lane reads 16 bytes (four entries) of every row, so a
+// row streams 1 KB per wavefront step; grid (ceil(max n / (4 kLowRows)), items)
+constexpr int kLowRows = 8;
+template <int DT>
+__global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low,
+                                                         const float* __restrict__ inv_scale,
+                                                         const float2* __restrict__ x, float2* __restrict__ y,
+                                                         float2 alpha, float2 beta, int beta_zero,
+                                                         const int* __restrict__ skip)
+{
+    const int s = blockIdx.y;
+    if (skip && skip[s]) return;
+    const BSys S = sys[s];
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kLowRows;
+    if (r0 >= S.n) return;
+    const BLow Lw = low[s];
+    const float2* xs = x + S.off;
+    float ar[kLowRows], ai[kLowRows];
+#pragma unroll
+    for (int q = 0; q < kLowRows; ++q) ar[q] = ai[q] = 0.f;
+    const int64_t n4 = S.n >> 2, ld4 = Lw.ldl >> 2;
+    const u32x4* L4 = reinterpret_cast<const u32x4*>(Lw.L);
+    const float4* x4 = reinterpret_cast<const float4*>(xs);
+    for (int64_t c = lane; c < n4; c += 64) {
+        const float4 xa = x4[2 * c], xb = x4[2 * c + 1];
+#pragma unroll
+        for (int q = 0; q < kLowRows; ++q) {
+            const int64_t r = min(r0 + q, S.n - 1);
+            const u32x4 a = __builtin_nontemporal_load(&L4[r * ld4 + c]);
+            cfma(lp_pair<DT>(a.x), xa.x, xa.y, ar[q], ai[q]);
+            cfma(lp_pair<DT>(a.y), xa.z, xa.w, ar[q], ai[q]);
+            cfma(lp_pair<DT>(a.z), xb.x, xb.y, ar[q], ai[q]);
+            cfma(lp_pair<DT>(a.w), xb.z, xb.w, ar[q], ai[q]);
+        }
+    }
+    for (int64_t c = 4 * n4 + lane; c < S.n; c += 64) {  // the n % 4 last columns
+        const float2 xv = xs[c];
+#pragma unroll
+        for (int q = 0; q < kLowRows; ++q) {
+            const int64_t r = min(r0 + q, S.n - 1);
+            cfma(lp_pair<DT>(Lw.L[r * Lw.ldl + c]), xv.x, xv.y, ar[q], ai[q]);
+        }
+    }
+    const float isc = inv_scale[s];
+#pragma unroll
+    for (int q = 0; q < kLowRows; ++q) {
+        float sr = ar[q], si = ai[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sr += __shfl_xor(sr, o, 64);
+            si += __shfl_xor(si, o, 64);
+        }
+        const int64_t r = r0 + q;
+        if (lane == q && r < S.n) {
+            const float2 d = Lw.diag[r], xv = xs[r];
+            sr = fmaf(d.x, xv.x, fmaf(-d.y, xv.y, sr * isc));
+            si = fmaf(d.x, xv.y, fmaf(d.y, xv.x, si * isc));
+            float2 out = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
+            if (!beta_zero) {
+                const float2 yo = y[S.off + r];
+                out.x += beta.x * yo.x - beta.y * yo.y;
+                out.y += beta.x * yo.y + beta.y * yo.x;
+            }
+            y[S.off + r] = out;
+        }
+    }
+}
+
+// amax_bits[s] = bits of max |Re|, |Im| over A_s (non-negative floats order as
+// their bit patterns); amax_bits zeroed beforehand; grid (row blocks, items)
+__global__ __launch_bounds__(256) void xamax_kernel(const BSys* __restrict__ sys, unsigned int* __restrict__ amax_bits)
+{
+    const int s = blockIdx.y;
+    const BSys S = sys[s];
+    float m = 0.f;
+    for (int64_t r = blockIdx.x; r < S.n; r += gridDim.x)
+        for (int64_t c = threadIdx.x; c < S.n; c += 256) {
+            const float2 v = S.A[r * S.lda + c];
+            m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(&amax_bits[s], __float_as_uint(m));
+}
+
+// scale_s: 1 for bf16; for fp16 the largest power of two with
+// max |A_s| scale_s <= 2^14 (fp16's largest finite value is 65504)
+__global__ void xscale_kernel(const unsigned int* __restrict__ amax_bits, int dt, float* __restrict__ scale,
+                              float* __restrict__ inv_scale, int ns)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    float sc = 1.f;
+    const float m = __uint_as_float(amax_bits[s]);
+    if (dt == kDtFP16 && m > 0.f) {
+        int e = 0;
+        const float f = frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+        sc = ldexpf(1.f, (f == 0.5f ? 15 : 14) - e);
+    }
+    scale[s] = sc;
+    inv_scale[s] = 1.f / sc;
+}
+
+// Alow_s = round(scale_s A_s) off the diagonal, pad columns and the diagonal
+// zero, diag = the FP32 diagonal; grid (row blocks, items)
+template <int DT>
+__global__ __launch_bounds__(256) void xto_lowp_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low,
+                                                       const float* __restrict__ scale)
+{
+    const int s = blockIdx.y;
+    const BSys S = sys[s];
+    const BLow Lw = low[s];
+    const float sc = scale[s];
+    for (int64_t r = blockIdx.x; r < S.n; r += gridDim.x)
+        for (int64_t c = threadIdx.x; c < Lw.ldl; c += 256) {
+            float2 v = c < S.n ? S.A[r * S.lda + c] : make_float2(0.f, 0.f);
+            if (c == r) {
+                Lw.diag[r] = v;
+                v = make_float2(0.f, 0.f);
+            }
+            Lw.L[r * Lw.ldl + c] = (uint32_t)to_lp<DT>(v.x * sc) | ((uint32_t)to_lp<DT>(v.y * sc) << 16);
+        }
+}
+
+__global__ void xfill_kernel(unsigned int* __restrict__ p, int64_t n, unsigned int v)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) p[e] = v;
+}
+
+// Inner-solve targets for the unit-norm outer direction v_j: relaxed as the
+// outer residual falls (inexact Krylov: the outer step only needs the inner
+// error below target / |r_j|), never tighter than tol_min; items whose outer
+// solve is done get a target every residual meets.  cur: |r_0|^2 (first step)
+// or |r_j| (the outer Givens estimate) per item.
+__global__ void xinner_tgt_kernel(const int* __restrict__ outer_done, const double* __restrict__ outer_tgt,
+                                  const double* __restrict__ cur, int cur_is_square, double tol_min,
+                                  double* __restrict__ tgt, int ns)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const double r = cur_is_square ? sqrt(cur[s]) : cur[s];
+    const double rel = r > 0.0 ? outer_tgt[s] / r : 1.0;
+    tgt[s] = outer_done[s] ? 1e300 : fmin(0.5, fmax(tol_min, rel));
+}
+
 namespace {
 
 size_t al(size_t b) { return (b + 255) / 256 * 256; }
@@ -280,6 +461,253 @@ XLayout xlayout(int ns, int64_t ntot, int m)
 }
 
 int64_t padded(int64_t n) { return (n + 63) / 64 * 64; }
+
+// One lockstep GMRES instance over the concatenated space of ns items.
+struct XEng {
+    int ns = 0, m = 0, dt = 0;
+    int64_t ntot = 0, nmax = 0;
+    const BSys* dsys = nullptr;
+    const BLow* dlow = nullptr;  // 16-bit copies of the matrices (null: the complex64 matrices)
+    const float* inv_scale = nullptr;
+    const int* skip = nullptr;   // items left out of the residual products (null: none)
+    bool x0_zero = false;        // cycles start from x = 0 (r = b, no product)
+    bool final_residual = true;  // cycles end with the true residual and the flags conv
+    float2 *V = nullptr, *w = nullptr, *r = nullptr, *B = nullptr, *X = nullptr;
+    double2 *H = nullptr, *h = nullptr, *h2 = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
+    double *res = nullptr, *rn2 = nullptr, *wn2 = nullptr, *tgt = nullptr;
+    int *done = nullptr, *nk = nullptr, *conv = nullptr;
+};
+
+XEng xeng_at(char* W, const XLayout& L, const BSys* dsys, int ns, int64_t ntot, int64_t nmax, int m)
+{
+    XEng E;
+    E.ns = ns;
+    E.m = m;
+    E.ntot = ntot;
+    E.nmax = nmax;
+    E.dsys = dsys;
+    E.V = (float2*)(W + L.V);
+    E.w = (float2*)(W + L.w);
+    E.r = (float2*)(W + L.r);
+    E.B = (float2*)(W + L.b);
+    E.X = (float2*)(W + L.x);
+    E.H = (double2*)(W + L.H);
+    E.h = (double2*)(W + L.h);
+    E.h2 = (double2*)(W + L.h2);
+    E.cs = (double2*)(W + L.cs);
+    E.sn = (double2*)(W + L.sn);
+    E.g = (double2*)(W + L.g);
+    E.y = (double2*)(W + L.y);
+    E.res = (double*)(W + L.res);
+    E.rn2 = (double*)(W + L.rn2);
+    E.wn2 = (double*)(W + L.wn2);
+    E.tgt = (double*)(W + L.tgt);
+    E.done = (int*)(W + L.done);
+    E.nk = (int*)(W + L.nk);
+    E.conv = (int*)(W + L.conv);
+    return E;
+}
+
+const float2 kOne = {1.f, 0.f}, kZero = {0.f, 0.f}, kMinusOne = {-1.f, 0.f};
+
+hipError_t xmatvec(const XEng& E, const float2* x, float2* y, float2 alpha, float2 beta, int beta_zero,
+                   const int* skip, hipStream_t st)
+{
+    if (!E.dlow) {
+        hipLaunchKernelGGL(xgemv_kernel, dim3((unsigned)((E.nmax + 15) / 16), (unsigned)E.ns), dim3(256), 0, st,
+                           E.dsys, x, y, alpha, beta, beta_zero, skip);
+    } else {
+        const dim3 g((unsigned)((E.nmax + 4 * kLowRows - 1) / (4 * kLowRows)), (unsigned)E.ns);
+        if (E.dt == kDtBF16)
+            hipLaunchKernelGGL(xgemv_lowp_kernel<kDtBF16>, g, dim3(256), 0, st, E.dsys, E.dlow, E.inv_scale, x, y,
+                               alpha, beta, beta_zero, skip);
+        else
+            hipLaunchKernelGGL(xgemv_lowp_kernel<kDtFP16>, g, dim3(256), 0, st, E.dsys, E.dlow, E.inv_scale, x, y,
+                               alpha, beta, beta_zero, skip);
+    }
+    return hipGetLastError();
+}
+
+// r = B - A X, rn2 = |r|^2 per item
+hipError_t xresidual(const XEng& E, hipStream_t st)
+{
+    hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((E.ntot + 255) / 256)), dim3(256), 0, st, E.B, E.r, E.ntot);
+    hipError_t e = xmatvec(E, E.X, E.r, kMinusOne, kOne, 0, E.skip, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(xnorm2_kernel, dim3(E.ns), dim3(256), 0, st, E.dsys, E.r, E.rn2);
+    return hipGetLastError();
+}
+
+// cycle start: residual, Givens right-hand side, flags, first basis vector
+hipError_t xcycle_init(const XEng& E, hipStream_t st)
+{
+    hipError_t e;
+    if (E.x0_zero) {
+        hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((E.ntot + 255) / 256)), dim3(256), 0, st, E.B, E.r, E.ntot);
+        hipLaunchKernelGGL(xnorm2_kernel, dim3(E.ns), dim3(256), 0, st, E.dsys, E.r, E.rn2);
+        e = hipGetLastError();
+    } else {
+        e = xresidual(E, st);
+    }
+    if (e != hipSuccess) return e;
+    e = launch_binit(E.g, E.m, E.rn2, E.tgt, E.done, E.nk, E.ns, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(xnormalize_kernel, dim3((unsigned)((E.nmax + 255) / 256), (unsigned)E.ns), dim3(256), 0, st,
+                       E.dsys, E.r, E.rn2, E.V, (const int*)nullptr);
+    return hipGetLastError();
+}
+
+// Arnoldi steps [j0, j1): product, classical Gram-Schmidt with one
+// re-orthogonalisation, normalisation, Givens rotation and convergence flag
+hipError_t xcycle_steps(const XEng& E, int j0, int j1, hipStream_t st)
+{
+    const int m = E.m;
+    const dim3 gx((unsigned)((E.nmax + 255) / 256), (unsigned)E.ns);
+    const dim3 gc((unsigned)((E.nmax + 63) / 64), (unsigned)E.ns);
+    for (int j = j0; j < j1; ++j) {
+        hipError_t e = xmatvec(E, E.V + (int64_t)j * E.ntot, E.w, kOne, kZero, 1, E.done, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, E.ns), dim3(256), 0, st, E.dsys, E.V, E.ntot, E.w, E.h,
+                           m + 1, (const int*)E.done);
+        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,
+                           (const int*)nullptr, E.h, m + 1, -1.0, E.w, (const int*)E.done);
+        hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, E.ns), dim3(256), 0, st, E.dsys, E.V, E.ntot, E.w, E.h2,
+                           m + 1, (const int*)E.done);
+        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,
+                           (const int*)nullptr, E.h2, m + 1, -1.0, E.w, (const int*)E.done);
+        hipLaunchKernelGGL(xnorm2_kernel, dim3(E.ns), dim3(256), 0, st, E.dsys, E.w, E.wn2);
+        hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, st, E.dsys, E.w, E.wn2,
+                           E.V + (int64_t)(j + 1) * E.ntot, (const int*)E.done);
+        e = launch_bgivens(E.H, m, j, E.h, E.h2, m + 1, E.wn2, E.cs, E.sn, E.g, E.res, E.tgt, E.done, E.nk, E.ns,
+                           st);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
+// cycle end: back-solve, x += V y, then (final_residual) the true residual
+// and the flags conv
+hipError_t xcycle_fin(const XEng& E, hipStream_t st)
+{
+    hipError_t e = launch_bbacksolve(E.H, E.m, E.nk, E.g, E.y, E.ns, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(xcombine_kernel, dim3((unsigned)((E.nmax + 63) / 64), (unsigned)E.ns), dim3(64 * kXWaves), 0,
+                       st, E.dsys, E.V, E.ntot, 0, (const int*)E.nk, E.y, E.m, 1.0, E.X, (const int*)nullptr);
+    if (E.final_residual) {
+        e = xresidual(E, st);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(xconv_kernel, dim3((E.ns + 63) / 64), dim3(64), 0, st, E.rn2, E.tgt, E.conv, E.ns);
+    }
+    return hipGetLastError();
+}
+
+constexpr int kSeg = 20;  // Arnoldi steps per graph segment; the host reads the flags between segments
+
+// A restart cycle as graphs: [0] start, [1 .. nseg] step segments, [nseg + 1] end
+struct XGraphs {
+    std::vector<hipGraph_t> g;
+    std::vector<hipGraphExec_t> x;
+    hipStream_t cap = nullptr;
+    XGraphs() = default;
+    XGraphs(const XGraphs&) = delete;
+    XGraphs& operator=(const XGraphs&) = delete;
+    ~XGraphs()
+    {
+        for (auto e : x) (void)hipGraphExecDestroy(e);
+        for (auto q : g) (void)hipGraphDestroy(q);
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
+
+template <class F>
+hipError_t xcapture(XGraphs& G, F&& f)
+{
+    hipError_t e = hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t e1 = f(G.cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(G.cap, &graph);
+    if (graph) G.g.push_back(graph);
+    if (e1 != hipSuccess) return e1;
+    if (e2 != hipSuccess) return e2;
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return e;
+    G.x.push_back(ex);
+    return hipSuccess;
+}
+
+hipError_t xbuild_graphs(const XEng& E, XGraphs& G)
+{
+    hipError_t e = hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = xcapture(G, [&](hipStream_t st) { return xcycle_init(E, st); });
+    for (int j0 = 0; e == hipSuccess && j0 < E.m; j0 += kSeg)
+        e = xcapture(G, [&](hipStream_t st) { return xcycle_steps(E, j0, std::min(E.m, j0 + kSeg), st); });
+    if (e == hipSuccess) e = xcapture(G, [&](hipStream_t st) { return xcycle_fin(E, st); });
+    return e;
+}
+
+// One restart cycle on stream s (graph replays when G is given).  The host
+// reads the items' flags after each segment of Arnoldi steps and ends the
+// cycle early once every item is done.
+hipError_t xrun_cycle(const XEng& E, const XGraphs* G, hipStream_t s, std::vector<int>& hd)
+{
+    const int nseg = (E.m + kSeg - 1) / kSeg;
+    hipError_t e = G ? hipGraphLaunch(G->x[0], s) : xcycle_init(E, s);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < nseg; ++k) {
+        e = G ? hipGraphLaunch(G->x[1 + k], s) : xcycle_steps(E, k * kSeg, std::min(E.m, (k + 1) * kSeg), s);
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync(hd.data(), E.done, sizeof(int) * E.ns, hipMemcpyDeviceToHost, s);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        bool all = true;
+        for (int q = 0; q < E.ns && all; ++q) all = hd[q] != 0;
+        if (all) break;
+    }
+    return G ? hipGraphLaunch(G->x[nseg + 1], s) : xcycle_fin(E, s);
+}
+
+// Workspace of the mixed-precision batch solve
+struct FLayout {
+    XLayout o, in;
+    size_t o_at, z_at, in_at, low_at, scale_at, inv_at, amax_at, diag_at, planes_at, total;
+    std::vector<size_t> plane_at;  // per item, from planes_at
+};
+
+FLayout flayout(const bem_batch_system* sys, int ns, int64_t ntot, int mo, int mi)
+{
+    FLayout F;
+    size_t o = 0;
+    F.o = xlayout(ns, ntot, mo);
+    F.o_at = o;
+    o += F.o.total;
+    F.z_at = o;
+    o += al(sizeof(float2) * (size_t)ntot * mo);
+    F.in = xlayout(ns, ntot, mi);
+    F.in_at = o;
+    o += F.in.total;
+    F.low_at = o;
+    o += al(sizeof(BLow) * ns);
+    F.scale_at = o;
+    o += al(sizeof(float) * ns);
+    F.inv_at = o;
+    o += al(sizeof(float) * ns);
+    F.amax_at = o;
+    o += al(sizeof(unsigned int) * ns);
+    F.diag_at = o;
+    o += al(sizeof(float2) * (size_t)ntot);
+    F.planes_at = o;
+    F.plane_at.resize(ns);
+    for (int q = 0; q < ns; ++q) {
+        F.plane_at[q] = o;
+        o += al(sizeof(uint32_t) * (size_t)sys[q].n * padded(sys[q].n));
+    }
+    F.total = o;
+    return F;
+}
 
 thread_local std::string g_batch_err;
 
@@ -444,143 +872,78 @@ int64_t bem_batch_gmres_work_size(const bem_batch_system* sys, int ns, int resta
     return (int64_t)xlayout(ns, ntot, restart).total;
 }
 
+static int check_systems(const bem_batch_system* sys, int ns, std::vector<BSys>& hs, int64_t& ntot, int64_t& nmax)
+{
+    hs.resize(ns);
+    ntot = 0;
+    nmax = 0;
+    for (int q = 0; q < ns; ++q) {
+        if (!sys[q].A || !sys[q].b || !sys[q].x || sys[q].n <= 0 || sys[q].lda < sys[q].n) {
+            g_batch_err = "bad system " + std::to_string(q);
+            return BEM_ERR_BAD_ARG;
+        }
+        hs[q] = BSys{(const float2*)sys[q].A, sys[q].lda, sys[q].n, ntot};
+        ntot += padded(sys[q].n);
+        nmax = std::max(nmax, sys[q].n);
+    }
+    return BEM_OK;
+}
+
+// b and the initial x into the concatenated space; targets tol |b_s|
+static int gather_systems(const bem_batch_system* sys, const std::vector<BSys>& hs, const XEng& E, float tol,
+                          std::vector<double>& hb, hipStream_t s)
+{
+    const int ns = E.ns;
+    XTRY(hipMemcpyAsync((void*)E.dsys, hs.data(), sizeof(BSys) * ns, hipMemcpyHostToDevice, s));
+    for (int q = 0; q < ns; ++q) {
+        XTRY(hipMemcpyAsync(E.B + hs[q].off, sys[q].b, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+        XTRY(hipMemcpyAsync(E.X + hs[q].off, sys[q].x, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+    }
+    hb.assign(ns, 0.0);
+    std::vector<double> ht(ns);
+    hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, s, E.dsys, E.B, E.rn2);
+    XTRY(hipGetLastError());
+    XTRY(hipMemcpyAsync(hb.data(), E.rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
+    XTRY(hipStreamSynchronize(s));
+    for (int q = 0; q < ns; ++q) ht[q] = (double)tol * std::sqrt(hb[q]);
+    XTRY(hipMemcpyAsync(E.tgt, ht.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
+    return BEM_OK;
+}
+
 int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart, int max_cycles, int use_graph,
                     void* work, bem_solve_info* info, void* stream)
 {
     if (!sys || ns <= 0 || restart <= 0 || max_cycles < 0 || !work || !(tol > 0.f)) return BEM_ERR_BAD_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    const int m = restart;
-    std::vector<BSys> hs(ns);
+    std::vector<BSys> hs;
     int64_t ntot = 0, nmax = 0;
-    for (int q = 0; q < ns; ++q) {
-        if (!sys[q].A || !sys[q].b || !sys[q].x || sys[q].n <= 0 || sys[q].lda < sys[q].n) return BEM_ERR_BAD_ARG;
-        hs[q] = BSys{(const float2*)sys[q].A, sys[q].lda, sys[q].n, ntot};
-        ntot += padded(sys[q].n);
-        nmax = std::max(nmax, sys[q].n);
-    }
-    const XLayout L = xlayout(ns, ntot, m);
+    if (int rc = check_systems(sys, ns, hs, ntot, nmax)) return rc;
+    const XLayout L = xlayout(ns, ntot, restart);
     char* W = (char*)work;
-    BSys* dsys = (BSys*)(W + L.sys);
-    float2* V = (float2*)(W + L.V);
-    float2* w = (float2*)(W + L.w);
-    float2* r = (float2*)(W + L.r);
-    float2* B = (float2*)(W + L.b);
-    float2* X = (float2*)(W + L.x);
-    double2* H = (double2*)(W + L.H);
-    double2* h = (double2*)(W + L.h);
-    double2* h2 = (double2*)(W + L.h2);
-    double2* cs = (double2*)(W + L.cs);
-    double2* sn = (double2*)(W + L.sn);
-    double2* g = (double2*)(W + L.g);
-    double2* y = (double2*)(W + L.y);
-    double* res = (double*)(W + L.res);
-    double* rn2 = (double*)(W + L.rn2);
-    double* wn2 = (double*)(W + L.wn2);
-    double* tgt = (double*)(W + L.tgt);
-    int* done = (int*)(W + L.done);
-    int* nk = (int*)(W + L.nk);
-    int* conv = (int*)(W + L.conv);
-    XTRY(hipMemcpyAsync(dsys, hs.data(), sizeof(BSys) * ns, hipMemcpyHostToDevice, s));
-    // gather b and the initial x into the concatenated space
-    for (int q = 0; q < ns; ++q) {
-        XTRY(hipMemcpyAsync(B + hs[q].off, sys[q].b, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
-        XTRY(hipMemcpyAsync(X + hs[q].off, sys[q].x, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+    const XEng E = xeng_at(W, L, (const BSys*)(W + L.sys), ns, ntot, nmax, restart);
+    std::vector<double> hb, hr(ns);
+    if (int rc = gather_systems(sys, hs, E, tol, hb, s)) return rc;
+    XGraphs G;
+    if (use_graph && xbuild_graphs(E, G) != hipSuccess) {
+        g_batch_err = "graph capture failed";
+        return BEM_ERR_HIP;
     }
-    std::vector<double> hb(ns), ht(ns), hr(ns);
-    hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, s, dsys, B, rn2);
-    XTRY(hipGetLastError());
-    XTRY(hipMemcpyAsync(hb.data(), rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
-    XTRY(hipStreamSynchronize(s));
-    for (int q = 0; q < ns; ++q) ht[q] = (double)tol * std::sqrt(hb[q]);
-    XTRY(hipMemcpyAsync(tgt, ht.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
-
-    const dim3 gx((unsigned)((nmax + 255) / 256), (unsigned)ns);
-    const dim3 gmv((unsigned)((nmax + 15) / 16), (unsigned)ns);
-    const dim3 gc((unsigned)((nmax + 63) / 64), (unsigned)ns);
-    const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f), mone = make_float2(-1.f, 0.f);
-    // one restart cycle: residual, set-up, m Arnoldi steps, back-solve, update,
-    // then the true residual and the convergence flags
-    auto enqueue_cycle = [&](hipStream_t st) -> hipError_t {
-        hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, B, r, ntot);
-        hipLaunchKernelGGL(xgemv_kernel, gmv, dim3(256), 0, st, dsys, X, r, mone, one, 0, (const int*)nullptr);
-        hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, st, dsys, r, rn2);
-        hipError_t e = launch_binit(g, m, rn2, tgt, done, nk, ns, st);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, st, dsys, r, rn2, V, (const int*)nullptr);
-        for (int j = 0; j < m; ++j) {
-            hipLaunchKernelGGL(xgemv_kernel, gmv, dim3(256), 0, st, dsys, V + (int64_t)j * ntot, w, one, zero, 1,
-                               (const int*)done);
-            hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, st, dsys, V, ntot, w, h, m + 1,
-                               (const int*)done);
-            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, j + 1,
-                               (const int*)nullptr, h, m + 1, -1.0, w,
-                               (const int*)done);
-            hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, st, dsys, V, ntot, w, h2, m + 1,
-                               (const int*)done);
-            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, j + 1,
-                               (const int*)nullptr, h2, m + 1, -1.0, w,
-                               (const int*)done);
-            hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, st, dsys, w, wn2);
-            hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, st, dsys, w, wn2, V + (int64_t)(j + 1) * ntot,
-                               (const int*)done);
-            e = launch_bgivens(H, m, j, h, h2, m + 1, wn2, cs, sn, g, res, tgt, done, nk, ns, st);
-            if (e != hipSuccess) return e;
-        }
-        e = launch_bbacksolve(H, m, nk, g, y, ns, st);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, dsys, V, ntot, 0, (const int*)nk, y, m,
-                           1.0, X, (const int*)nullptr);
-        hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, st, B, r, ntot);
-        hipLaunchKernelGGL(xgemv_kernel, gmv, dim3(256), 0, st, dsys, X, r, mone, one, 0, (const int*)nullptr);
-        hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, st, dsys, r, rn2);
-        hipLaunchKernelGGL(xconv_kernel, dim3((ns + 63) / 64), dim3(64), 0, st, rn2, tgt, conv, ns);
-        return hipGetLastError();
-    };
-
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
-    hipStream_t cap = nullptr;
-    if (use_graph) {
-        XTRY(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-        XTRY(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-        hipError_t e = enqueue_cycle(cap);
-        hipError_t e2 = hipStreamEndCapture(cap, &graph);
-        if (e != hipSuccess || e2 != hipSuccess) {
-            if (graph) (void)hipGraphDestroy(graph);
-            (void)hipStreamDestroy(cap);
-            g_batch_err = "graph capture failed";
-            return BEM_ERR_HIP;
-        }
-        XTRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
-    std::vector<int> hc(ns, 0), hk(ns, 0), its(ns, 0);
-    int cycles = 0;
+    std::vector<int> hc(ns, 0), hk(ns, 0), hd(ns, 0), its(ns, 0);
     bool all = false;
-    for (; cycles < max_cycles; ++cycles) {
-        if (use_graph) {
-            XTRY(hipGraphLaunch(exec, s));
-        } else {
-            XTRY(enqueue_cycle(s));
-        }
-        XTRY(hipMemcpyAsync(hc.data(), conv, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
-        XTRY(hipMemcpyAsync(hr.data(), rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
-        XTRY(hipMemcpyAsync(hk.data(), nk, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+    for (int cycles = 0; cycles < max_cycles && !all; ++cycles) {
+        XTRY(xrun_cycle(E, use_graph ? &G : nullptr, s, hd));
+        XTRY(hipMemcpyAsync(hc.data(), E.conv, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipMemcpyAsync(hr.data(), E.rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipMemcpyAsync(hk.data(), E.nk, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
         XTRY(hipStreamSynchronize(s));
         all = true;
         for (int q = 0; q < ns; ++q) {
             all = all && hc[q];
             its[q] += hk[q];  // Arnoldi steps of this item in the cycle (0 once converged)
         }
-        if (all) {
-            ++cycles;
-            break;
-        }
     }
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    if (cap) (void)hipStreamDestroy(cap);
     for (int q = 0; q < ns; ++q)
-        XTRY(hipMemcpyAsync(sys[q].x, X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+        XTRY(hipMemcpyAsync(sys[q].x, E.X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
     if (info)
         for (int q = 0; q < ns; ++q) {
             info[q].iterations = its[q];
@@ -590,6 +953,156 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
         }
     if (!all) {
         g_batch_err = "batched GMRES reached max_cycles";
+        return BEM_ERR_NOT_CONVERGED;
+    }
+    return BEM_OK;
+}
+
+// ---- batched mixed-precision solve ---------------------------------------------
+int64_t bem_batch_fgmres_work_size(const bem_batch_system* sys, int ns, int outer_restart, int inner_restart)
+{
+    if (!sys || ns <= 0 || outer_restart <= 0 || inner_restart <= 0) return 0;
+    int64_t ntot = 0;
+    for (int q = 0; q < ns; ++q) ntot += padded(sys[q].n);
+    return (int64_t)flayout(sys, ns, ntot, outer_restart, inner_restart).total;
+}
+
+int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, float inner_tol, int outer_restart,
+                     int inner_restart, int max_inner_cycles, int max_outer_cycles, int use_graph, void* work,
+                     bem_solve_info* info, int* inner_iterations, void* stream)
+{
+    if (!sys || ns <= 0 || outer_restart <= 0 || inner_restart <= 0 || max_inner_cycles <= 0 ||
+        max_outer_cycles < 0 || !work ||
+        !(tol > 0.f) || !(inner_tol > 0.f) || (dtype != kDtBF16 && dtype != kDtFP16))
+        return BEM_ERR_BAD_ARG;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const int mo = outer_restart, mi = inner_restart;
+    std::vector<BSys> hs;
+    int64_t ntot = 0, nmax = 0;
+    if (int rc = check_systems(sys, ns, hs, ntot, nmax)) return rc;
+    const FLayout F = flayout(sys, ns, ntot, mo, mi);
+    char* W = (char*)work;
+    const BSys* dsys = (const BSys*)(W + F.o_at + F.o.sys);
+    const XEng Eo = xeng_at(W + F.o_at, F.o, dsys, ns, ntot, nmax, mo);  // FP32 flexible GMRES on A
+    XEng Ei = xeng_at(W + F.in_at, F.in, dsys, ns, ntot, nmax, mi);      // 16-bit GMRES per direction
+    float2* Z = (float2*)(W + F.z_at);
+    BLow* dlow = (BLow*)(W + F.low_at);
+    float* scale = (float*)(W + F.scale_at);
+    float* inv = (float*)(W + F.inv_at);
+    unsigned int* amax = (unsigned int*)(W + F.amax_at);
+    Ei.dlow = dlow;
+    Ei.dt = dtype;
+    Ei.inv_scale = inv;
+    Ei.skip = Eo.done;
+
+    std::vector<double> hb, hr(ns);
+    if (int rc = gather_systems(sys, hs, Eo, tol, hb, s)) return rc;
+    // 16-bit copies of the matrices
+    std::vector<BLow> hl(ns);
+    float2* diag = (float2*)(W + F.diag_at);
+    for (int q = 0; q < ns; ++q) hl[q] = BLow{(uint32_t*)(W + F.plane_at[q]), padded(sys[q].n), diag + hs[q].off};
+    XTRY(hipMemcpyAsync(dlow, hl.data(), sizeof(BLow) * ns, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(xfill_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, amax, (int64_t)ns, 0u);
+    const dim3 grow((unsigned)std::min<int64_t>(nmax, 2048), (unsigned)ns);
+    if (dtype == kDtFP16) hipLaunchKernelGGL(xamax_kernel, grow, dim3(256), 0, s, dsys, amax);
+    hipLaunchKernelGGL(xscale_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, amax, dtype, scale, inv, ns);
+    if (dtype == kDtBF16)
+        hipLaunchKernelGGL(xto_lowp_kernel<kDtBF16>, grow, dim3(256), 0, s, dsys, dlow, scale);
+    else
+        hipLaunchKernelGGL(xto_lowp_kernel<kDtFP16>, grow, dim3(256), 0, s, dsys, dlow, scale);
+    XTRY(hipGetLastError());
+
+    XGraphs G;
+    if (use_graph && xbuild_graphs(Ei, G) != hipSuccess) {
+        g_batch_err = "graph capture failed";
+        return BEM_ERR_HIP;
+    }
+    const dim3 gx((unsigned)((nmax + 255) / 256), (unsigned)ns);
+    const dim3 gc((unsigned)((nmax + 63) / 64), (unsigned)ns);
+    std::vector<int> hc(ns, 0), hd(ns, 0), hdi(ns, 0), hci(ns, 0), hko(ns, 0), hki(ns, 0), its_o(ns, 0),
+        its_i(ns, 0);
+    bool all = false;
+    for (int oc = 0; oc < max_outer_cycles && !all; ++oc) {
+        if (oc == 0) XTRY(xresidual(Eo, s));  // later cycles start from the previous cycle's true residual
+        XTRY(launch_binit(Eo.g, mo, Eo.rn2, Eo.tgt, Eo.done, Eo.nk, ns, s));
+        hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, s, dsys, Eo.r, Eo.rn2, Eo.V, (const int*)nullptr);
+        for (int j = 0; j < mo; ++j) {
+            // z_j ~ A^-1 v_j: 16-bit GMRES from zero to inner_tol (at most max_inner_cycles cycles)
+            float2* zj = Z + (int64_t)j * ntot;
+            hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s,
+                               Eo.V + (int64_t)j * ntot, Ei.B, ntot);
+            hipLaunchKernelGGL(xfill_kernel, dim3((unsigned)((2 * ntot + 255) / 256)), dim3(256), 0, s,
+                               (unsigned int*)Ei.X, 2 * ntot, 0u);
+            hipLaunchKernelGGL(xinner_tgt_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, (const int*)Eo.done,
+                               (const double*)Eo.tgt, (const double*)(j == 0 ? Eo.rn2 : Eo.res), j == 0 ? 1 : 0,
+                               (double)inner_tol, Ei.tgt, ns);
+            XTRY(hipGetLastError());
+            for (int ic = 0; ic < max_inner_cycles; ++ic) {
+                XTRY(xrun_cycle(Ei, use_graph ? &G : nullptr, s, hdi));
+                XTRY(hipMemcpyAsync(hki.data(), Ei.nk, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+                XTRY(hipMemcpyAsync(hci.data(), Ei.conv, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+                XTRY(hipStreamSynchronize(s));
+                bool iall = true;
+                for (int q = 0; q < ns; ++q) {
+                    its_i[q] += hki[q];
+                    iall = iall && hci[q];
+                }
+                if (iall) break;
+            }
+            hipLaunchKernelGGL(xcopy_kernel, dim3((unsigned)((ntot + 255) / 256)), dim3(256), 0, s, Ei.X, zj, ntot);
+            // w = A z_j in FP32, orthogonalised against the FP32 basis
+            XTRY(xmatvec(Eo, zj, Eo.w, kOne, kZero, 1, Eo.done, s));
+            hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, s, dsys, Eo.V, ntot, Eo.w, Eo.h,
+                               mo + 1, (const int*)Eo.done);
+            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, s, dsys, Eo.V, ntot, j + 1,
+                               (const int*)nullptr, Eo.h, mo + 1, -1.0, Eo.w, (const int*)Eo.done);
+            hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, ns), dim3(256), 0, s, dsys, Eo.V, ntot, Eo.w, Eo.h2,
+                               mo + 1, (const int*)Eo.done);
+            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, s, dsys, Eo.V, ntot, j + 1,
+                               (const int*)nullptr, Eo.h2, mo + 1, -1.0, Eo.w, (const int*)Eo.done);
+            hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, s, dsys, Eo.w, Eo.wn2);
+            hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, s, dsys, Eo.w, Eo.wn2,
+                               Eo.V + (int64_t)(j + 1) * ntot, (const int*)Eo.done);
+            XTRY(hipGetLastError());
+            XTRY(launch_bgivens(Eo.H, mo, j, Eo.h, Eo.h2, mo + 1, Eo.wn2, Eo.cs, Eo.sn, Eo.g, Eo.res, Eo.tgt,
+                                Eo.done, Eo.nk, ns, s));
+            XTRY(hipMemcpyAsync(hd.data(), Eo.done, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+            XTRY(hipStreamSynchronize(s));
+            bool alldone = true;
+            for (int q = 0; q < ns; ++q) alldone = alldone && hd[q];
+            if (alldone) break;
+        }
+        // x += Z y, then the true FP32 residual
+        XTRY(launch_bbacksolve(Eo.H, mo, Eo.nk, Eo.g, Eo.y, ns, s));
+        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, s, dsys, Z, ntot, 0, (const int*)Eo.nk, Eo.y,
+                           mo, 1.0, Eo.X, (const int*)nullptr);
+        XTRY(hipGetLastError());
+        XTRY(xresidual(Eo, s));
+        hipLaunchKernelGGL(xconv_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, Eo.rn2, Eo.tgt, Eo.conv, ns);
+        XTRY(hipGetLastError());
+        XTRY(hipMemcpyAsync(hc.data(), Eo.conv, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipMemcpyAsync(hr.data(), Eo.rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipMemcpyAsync(hko.data(), Eo.nk, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+        XTRY(hipStreamSynchronize(s));
+        all = true;
+        for (int q = 0; q < ns; ++q) {
+            all = all && hc[q];
+            its_o[q] += hko[q];
+        }
+    }
+    for (int q = 0; q < ns; ++q)
+        XTRY(hipMemcpyAsync(sys[q].x, Eo.X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+    if (info)
+        for (int q = 0; q < ns; ++q) {
+            info[q].iterations = its_o[q];
+            info[q].converged = hc[q];
+            info[q].rel_residual = hb[q] > 0 ? std::sqrt(hr[q] / hb[q]) : 0.0;
+            info[q].implicit_rel_residual = 0.0;
+        }
+    if (inner_iterations)
+        for (int q = 0; q < ns; ++q) inner_iterations[q] = its_i[q];
+    if (!all) {
+        g_batch_err = "batched mixed-precision solve reached max_outer_cycles";
         return BEM_ERR_NOT_CONVERGED;
     }
     return BEM_OK;

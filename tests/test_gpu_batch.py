@@ -73,3 +73,48 @@ def test_batch_gmres_matches_per_item(items, use_graph):
         assert ir["converged"]
         rel = (torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr)).item()
         assert rel < 2e-5, rel
+
+
+def test_batch_gmres_iterations_match_per_item(items):
+    """Per-item Arnoldi step counts of the lockstep solve (cycles ended early
+    once every item is done) agree with one GMRES per item at the same restart."""
+    from neuralat_amd.bem.batch import batch_cbie, batch_gmres
+    from neuralat_amd.bem.solver import gmres
+
+    outs = batch_cbie([s.mesh for s, _, _ in items], [k for _, k, _ in items], [p for _, _, p in items])
+    _, infos = batch_gmres([o[0] for o in outs], [o[1] for o in outs], tol=1e-6, restart=30, max_cycles=60)
+    for (lhs, rhs), inf in zip(outs, infos):
+        _, ir = gmres(lhs, rhs, tol=1e-6, restart=30)
+        assert abs(inf["iterations"] - ir["iterations"]) <= max(2, 0.05 * ir["iterations"]), (inf, ir)
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_mixed_precision_matches_fp64(items, dtype, use_graph):
+    """Flexible FP32 GMRES preconditioned by lockstep 16-bit GMRES reaches the
+    FP32 residual target on every item; the solutions agree with an FP64
+    direct solve of the same systems and with the complex64 batched solve."""
+    from neuralat_amd.bem.batch import batch_cbie, batch_gmres, batch_solve_mixed
+
+    outs = batch_cbie([s.mesh for s, _, _ in items], [k for _, k, _ in items], [p for _, _, p in items])
+    As, bs = [o[0] for o in outs], [o[1] for o in outs]
+    xs, infos = batch_solve_mixed(As, bs, dtype=dtype, tol=1e-6, inner_tol=1e-2, outer_restart=10,
+                                  inner_restart=50, max_outer_cycles=4, use_graph=use_graph)
+    xr, _ = batch_gmres(As, bs, tol=1e-6, restart=30, max_cycles=60)
+    for A, b, x, x32, inf in zip(As, bs, xs, xr, infos):
+        assert inf["converged"] and inf["rel_residual"] <= 1e-6, inf
+        assert inf["inner_iterations"] > 0 and inf["iterations"] <= 40, inf
+        a64 = A.cpu().numpy().astype(np.complex128)
+        x64 = np.linalg.solve(a64, b.cpu().numpy().astype(np.complex128))
+        err = np.linalg.norm(x.cpu().numpy() - x64) / np.linalg.norm(x64)
+        assert err < 1e-4, err
+        rel = (torch.linalg.vector_norm(x - x32) / torch.linalg.vector_norm(x32)).item()
+        assert rel < 1e-4, rel
+
+
+def test_batch_mixed_precision_rejects_bad_dtype(items):
+    from neuralat_amd.bem.batch import batch_solve_mixed
+
+    A = torch.eye(8, dtype=torch.complex64, device=_dev())
+    with pytest.raises(ValueError):
+        batch_solve_mixed([A], [torch.ones(8, dtype=torch.complex64, device=_dev())], dtype="fp8")
