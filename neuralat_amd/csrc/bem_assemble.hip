@@ -480,16 +480,38 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // so every LDS accumulator entry has exactly one writer.  (Adding the blocks
 // with LDS float atomics left the kernel waiting on LDS issue 64 % of its wave
 // cycles.)  Each tile entry is flushed once with a global float atomic.
-template <int OP, int Q, bool CLAMP>
+// SYM (operators whose pair blocks satisfy B(j, i) = B(i, j)^T: single layer,
+// hypersingular): each unordered pair is evaluated once.  Work item w of a
+// 1-D grid pairs row tile r with a column tile c >= the column tile holding r
+// (row tiles nest in column tiles); in that own column tile only columns
+// after the row's position are taken.  The blocks are flushed at (row
+// vertices, column vertices) only; p1_symmetrize_kernel then adds the
+// transpose (flushing the transposed block with atomics as well scattered
+// them over every row of the matrix, and the atomics missed in cache).
+template <int OP, int Q, bool CLAMP, bool SYM>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                          const float* __restrict__ curls, P1Tiles tl, float k,
                                                          float2* __restrict__ out, int64_t ld)
 {
     static_assert(kP1ColTris == 256 && kP1ColVerts <= 256, "one lane per column triangle and per column vertex");
+    static_assert(!SYM || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
     constexpr int S = Rec<Q>::kFloats;
     __shared__ float2 acc[kP1RowVerts][kP1ColVerts];
     __shared__ float2 stage[9][kP1ColTris];
-    const int ct = blockIdx.x, rt = blockIdx.y;
+    int ct = blockIdx.x, rt = blockIdx.y;
+    bool own = false;  // SYM: the column tile holds the row tile
+    if constexpr (SYM) {
+        const int64_t wi = blockIdx.x;
+        int lo = 0, hi = tl.nrow_tiles - 1;  // last r with sym_ptr[r] <= wi
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tl.sym_ptr[mid] <= wi) lo = mid;
+            else hi = mid - 1;
+        }
+        rt = lo;
+        ct = tl.row_ctile[rt] + (int)(wi - tl.sym_ptr[rt]);
+        own = ct == tl.row_ctile[rt];
+    }
     const int c0 = tl.col_ptr[ct], c1 = tl.col_ptr[ct + 1];
     const int r0 = tl.row_ptr[rt], r1 = tl.row_ptr[rt + 1];
     const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
@@ -525,7 +547,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         const int i = tl.perm[rp];  // wave-uniform
         const int4 ti = tris[i];
         const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
-                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z;
+                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z || (SYM && own && pos <= rp);
         float br[9], bi[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
@@ -660,6 +682,36 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         const float2 v = acc[rvi][cvi];
         unsafeAtomicAdd(dst, v.x);
         unsafeAtomicAdd(dst + 1, v.y);
+    }
+}
+
+// out <- out + out^T in place on an n x n matrix (the one-sided regular part
+// of a symmetric P1 operator, before the neighbour pairs are added).  One
+// workgroup per 32 x 32 tile pair (I <= J): both tiles through LDS, written
+// back as A_IJ + A_JI^T and its transpose.
+constexpr int kSymT = 32;
+__global__ __launch_bounds__(256) void p1_symmetrize_kernel(float2* __restrict__ out, int64_t n, int64_t ld)
+{
+    __shared__ float2 a[kSymT][kSymT + 1], b[kSymT][kSymT + 1];
+    int64_t I, J;
+    tri_block_pair(blockIdx.x, I, J);
+    const int64_t r0 = I * kSymT, c0 = J * kSymT;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+    for (int rr = ty; rr < kSymT; rr += 8) {
+        const int64_t ra = r0 + rr, ca = c0 + tx, rb = c0 + rr, cb = r0 + tx;
+        a[rr][tx] = (ra < n && ca < n) ? out[ra * ld + ca] : make_float2(0.f, 0.f);
+        b[rr][tx] = (rb < n && cb < n) ? out[rb * ld + cb] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int rr = ty; rr < kSymT; rr += 8) {
+        const int64_t ra = r0 + rr, ca = c0 + tx, rb = c0 + rr, cb = r0 + tx;
+        const float2 va = a[rr][tx], vb = b[tx][rr];  // A(rr, tx) + B(tx, rr)
+        const float2 s = make_float2(va.x + vb.x, va.y + vb.y);
+        if (ra < n && ca < n) out[ra * ld + ca] = s;
+        if (I != J) {
+            const float2 wb = b[rr][tx], wa = a[tx][rr];  // B(rr, tx) + A(tx, rr)
+            if (rb < n && cb < n) out[rb * ld + cb] = make_float2(wb.x + wa.x, wb.y + wa.y);
+        }
     }
 }
 
@@ -1043,9 +1095,17 @@ template <int OP, int Q, bool CLAMP>
 static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t ld, hipStream_t s)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
-    dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
-    hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP>), grid, dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, k,
-                       out, ld);
+    if constexpr (OP == SLP || OP == HYP) {
+        const int64_t nb = (m.nv + kSymT - 1) / kSymT;
+        if (m.tiles.nsym > 0x7fffffffLL || nb * (nb + 1) / 2 > 0x7fffffffLL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, true>), dim3((unsigned)m.tiles.nsym), dim3(256), 0, s, rec,
+                           m.tris, m.curls, m.tiles, k, out, ld);
+        hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, out, m.nv, ld);
+    } else {
+        dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
+        hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, false>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
+                           m.tiles, k, out, ld);
+    }
     return hipGetLastError();
 }
 

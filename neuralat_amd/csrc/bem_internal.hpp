@@ -39,6 +39,12 @@ struct P1Tiles {
     // for each triangle of the tile whose local vertex b is that vertex
     const int32_t* col_inc_ptr;
     const int32_t* col_inc;
+    // row tiles nest in column tiles: row_ctile[r] = the column tile holding
+    // row tile r; the symmetric sweep pairs row tile r with column tiles
+    // row_ctile[r] .. ncol_tiles - 1, work items sym_ptr[r] .. sym_ptr[r + 1)
+    const int32_t* row_ctile;
+    const int64_t* sym_ptr;
+    int64_t nsym;
     int nrow_tiles, ncol_tiles;
 };
 

@@ -69,8 +69,10 @@ struct Tiling {
 };
 
 // Greedy tiling of the ordered triangle list: a tile closes when it holds
-// max_tris triangles or one more would exceed max_verts distinct vertices.
-Tiling make_tiling(const std::vector<int32_t>& perm, const int32_t* tris, int max_tris, int max_verts)
+// max_tris triangles, one more would exceed max_verts distinct vertices, or
+// the next position is in stops (sorted; tiles never cross one).
+Tiling make_tiling(const std::vector<int32_t>& perm, const int32_t* tris, int max_tris, int max_verts,
+                   const std::vector<int32_t>& stops = {})
 {
     Tiling T;
     const int64_t nt = (int64_t)perm.size();
@@ -79,10 +81,13 @@ Tiling make_tiling(const std::vector<int32_t>& perm, const int32_t* tris, int ma
     T.vptr.push_back(0);
     int64_t pos = 0;
     std::vector<int32_t> local;
+    size_t next_stop = 0;
     while (pos < nt) {
         local.clear();
         int64_t start = pos;
-        while (pos < nt && pos - start < max_tris) {
+        while (next_stop < stops.size() && stops[next_stop] <= start) ++next_stop;
+        const int64_t limit = next_stop < stops.size() ? std::min<int64_t>(nt, stops[next_stop]) : nt;
+        while (pos < limit && pos - start < max_tris) {
             const int32_t* t = tris + 3 * (int64_t)perm[pos];
             int fresh = 0;
             for (int a = 0; a < 3; ++a)
@@ -275,8 +280,18 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
     }
-    const Tiling rowT = make_tiling(perm, h_tris, kP1RowTris, kP1RowVerts);
+    // row tiles nest in column tiles, so the symmetric sweep can pair a row
+    // tile with the column tiles from its own on
     const Tiling colT = make_tiling(perm, h_tris, kP1ColTris, kP1ColVerts);
+    const Tiling rowT = make_tiling(perm, h_tris, kP1RowTris, kP1RowVerts, colT.ptr);
+    const int nrowt = (int)rowT.ptr.size() - 1, ncolt = (int)colT.ptr.size() - 1;
+    std::vector<int32_t> row_ctile((size_t)nrowt);
+    std::vector<int64_t> sym_ptr((size_t)nrowt + 1, 0);
+    for (int r = 0, c = 0; r < nrowt; ++r) {
+        while (colT.ptr[c + 1] <= rowT.ptr[r]) ++c;
+        row_ctile[r] = c;
+        sym_ptr[r + 1] = sym_ptr[r] + (ncolt - c);
+    }
     // incidences of each column-tile vertex slot (CSR over the global slot index)
     std::vector<int32_t> inc_ptr(colT.verts.size() + 1, 0), inc;
     {
@@ -315,7 +330,8 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
     parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});
-    int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv, *d_cip, *d_ci;
+    int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv, *d_cip, *d_ci, *d_rct;
+    int64_t* d_sym;
     int4 *d_rs, *d_cs;
     parts.push_back({perm.data(), perm.size() * 4, (void**)&d_perm});
     parts.push_back({rowT.ptr.data(), rowT.ptr.size() * 4, (void**)&d_rp});
@@ -328,6 +344,8 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({colT.slot.data(), colT.slot.size() * 4, (void**)&d_cs});
     parts.push_back({inc_ptr.data(), inc_ptr.size() * 4, (void**)&d_cip});
     parts.push_back({inc.data(), inc.size() * 4, (void**)&d_ci});
+    parts.push_back({row_ctile.data(), row_ctile.size() * 4, (void**)&d_rct});
+    parts.push_back({sym_ptr.data(), sym_ptr.size() * 8, (void**)&d_sym});
     size_t total = 0;
     std::vector<size_t> off;
     for (auto& p : parts) {
@@ -365,8 +383,11 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     m.tiles.col_slot = d_cs;
     m.tiles.col_inc_ptr = d_cip;
     m.tiles.col_inc = d_ci;
-    m.tiles.nrow_tiles = (int)rowT.ptr.size() - 1;
-    m.tiles.ncol_tiles = (int)colT.ptr.size() - 1;
+    m.tiles.row_ctile = d_rct;
+    m.tiles.sym_ptr = d_sym;
+    m.tiles.nsym = sym_ptr.back();
+    m.tiles.nrow_tiles = nrowt;
+    m.tiles.ncol_tiles = ncolt;
     return 0;
 }
 
