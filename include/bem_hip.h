@@ -203,7 +203,8 @@ int bem_gmres_multi(const void* A, int64_t lda, const void* Ar, const void* Ai, 
  * in FP32 on the complex64 A (residuals and updates in FP32, basis of 20,
  * restarted beyond it, at most max_outer outer steps), each outer direction
  * preconditioned by a 16-bit multi-RHS GMRES solve (restart, max_inner) to
- * inner_tol; until ||b - A x|| <= tol ||b|| for every system.
+ * max(inner_tol, tol ||b|| / ||r_j||) (r_j: the outer residual; capped at
+ * 0.5); until ||b - A x|| <= tol ||b|| for every system.
  * info[q].iterations = total inner iterations, *outer_iterations = outer
  * steps. */
 int64_t bem_solve_ir_work_size(int64_t n, int nrhs, int restart, int64_t ldlp);

@@ -99,7 +99,8 @@ size_t gmres_multi_bytes(int64_t n, int ns, int m) { return mlayout(n, ns, m).to
     } while (0)
 
 hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, int64_t ldb, float2* X, int64_t ldx,
-                       double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s, FlexPrecond* prec)
+                       double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s, FlexPrecond* prec,
+                       const double* rel)
 {
     const MLayout L = mlayout(n, ns, m);
     char* W = (char*)work;
@@ -122,12 +123,12 @@ hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, in
     int* nk = (int*)(W + L.nk);
     const float2 one = make_float2(1.f, 0.f), zero = make_float2(0.f, 0.f), mone = make_float2(-1.f, 0.f);
 
-    std::vector<double> hb(ns), ht(ns), hr(ns);
+    std::vector<double> hb(ns), ht(ns), hr(ns), hres(ns), irel(ns);
     std::vector<int> hd(ns);
     TRY(launch_bnorm2(B, ldb, n, ns, rn2, s));
     TRY(hipMemcpyAsync(hb.data(), rn2, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
     TRY(hipStreamSynchronize(s));
-    for (int q = 0; q < ns; ++q) ht[q] = tol * std::sqrt(hb[q]);
+    for (int q = 0; q < ns; ++q) ht[q] = (rel ? rel[q] : tol) * std::sqrt(hb[q]);
     TRY(hipMemcpyAsync(tgt, ht.data(), sizeof(double) * ns, hipMemcpyHostToDevice, s));
     int its = 0;
     bool all_conv = false;
@@ -146,12 +147,19 @@ hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, in
             const float2* Vj = V + (int64_t)j * ns * ldn;
             const float2* dir = Vj;
             if (prec) {
-                // flexible right preconditioning: z_j ~ M^{-1} v_j by an inner solve
+                // flexible right preconditioning: z_j ~ M^{-1} v_j by an inner
+                // solve whose target relaxes as the outer residual falls
+                // (inexact Krylov: step j only needs the inner error below
+                // target / |r_j|)
                 float2* Zj = prec->Z + (int64_t)j * ns * ldn;
                 TRY(hipMemsetAsync(Zj, 0, sizeof(float2) * (size_t)ldn * ns, s));
+                for (int q = 0; q < ns; ++q) {
+                    const double cur = j == 0 ? std::sqrt(hr[q]) : hres[q];
+                    irel[q] = cur > 0.0 ? std::min(0.5, std::max(prec->tol, ht[q] / cur)) : 0.5;
+                }
                 MultiInfo mi;
                 TRY(gmres_multi(*prec->op, n, ns, Vj, ldn, Zj, ldn, prec->tol, prec->m, prec->maxiter, prec->work, &mi,
-                                s, nullptr));
+                                s, nullptr, irel.data()));
                 prec->inner_iterations += mi.iterations;
                 dir = Zj;
             }
@@ -165,6 +173,7 @@ hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, in
             TRY(launch_bgivens(H, m, j, h, h2, m + 1, wn2, cs, sn, g, res, tgt, done, nk, ns, s));
             ++its;
             TRY(hipMemcpyAsync(hd.data(), done, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+            if (prec) TRY(hipMemcpyAsync(hres.data(), res, sizeof(double) * ns, hipMemcpyDeviceToHost, s));
             TRY(hipStreamSynchronize(s));
             bool all = true;
             for (int q = 0; q < ns; ++q) all = all && hd[q];

@@ -54,9 +54,12 @@ struct FlexPrecond {
 size_t lowp_scratch_bytes(int64_t ldlp);
 float2* lowp_scratch_part(void* scratch, int64_t ldlp);
 size_t gmres_multi_bytes(int64_t n, int ns, int m);
+// rel (optional, ns values): per-system relative tolerances replacing tol.
+// With prec, the inner target of outer step j of system q is relaxed to
+// min(0.5, max(prec->tol, tol |b_q| / |r_j,q|)) (r_j: the outer residual).
 hipError_t gmres_multi(const MultiOp& op, int64_t n, int ns, const float2* B, int64_t ldb, float2* X, int64_t ldx,
                        double tol, int m, int maxiter, void* work, MultiInfo* info, hipStream_t s,
-                       FlexPrecond* prec = nullptr);
+                       FlexPrecond* prec = nullptr, const double* rel = nullptr);
 size_t solve_ir_bytes(int64_t n, int ns, int m);
 hipError_t solve_ir(const MultiOp& hi, const MultiOp& lo, int64_t n, int ns, const float2* B, int64_t ldb, float2* X,
                     int64_t ldx, double tol, double inner_tol, int m, int max_inner, int max_outer, void* work,
