@@ -120,6 +120,23 @@ def test_p1_operators_vs_oracle(l2_sphere, op):
         assert e < TOL, (op, k, e)
 
 
+@pytest.mark.parametrize("op", ["single", "hypersingular", "double"])
+def test_p1_many_column_tiles_vs_oracle(op):
+    """Level 3 (1280 triangles): several column tiles of 256 triangles, so the
+    symmetric sweep of the single layer and hypersingular operators (row tile
+    x column tiles from its own on, then the in-place transpose-add) covers
+    off-diagonal tile pairs, partial tiles and the own-tile half."""
+    v, t = mesh.icosphere(3, 1.0)
+    v = mesh.anisotropic_scale(mesh.sh_perturb(v, 4, 0.05, seed=4), np.array([1.3, 0.8, 1.0])).astype(np.float32)
+    s = _solver(v, t, "P1")
+    k = 7.0
+    _say("P1 L3", op, k)
+    A = s.assemble_boundary_matrix(k, op).cpu().numpy()
+    R = oracle.assemble(v, t, k, op, "P1", nthreads=0)
+    e = _rel(A, R)
+    assert e < TOL, (op, e)
+
+
 @pytest.mark.parametrize("op", ["single", "double", "adjointdouble", "bm_rhs"])
 @pytest.mark.parametrize("clamp", [True, False])
 def test_dp0_operators_vs_oracle(op, clamp):
