@@ -19,9 +19,12 @@ layer (contracted with psi): algorithmic FLOP per launch = (29 + 16) FLOP x
 36 Gauss-point pairs x (N^2 - neighbour pairs), divided by its mean duration
 from HIP events recorded on the stream it runs on.
 
-``cpu_baseline``: the CPU oracle's FP32 mirror of the reference arithmetic
-(OpenMP, all cores of this process's affinity) timed on a row block of the same
-operators; the reference's own CPU path (bempp-cl/numba) is not installed.
+The CPU leg (at the end of the run, on the host cores) is the only part that
+runs the oracle: ``cpu_baseline`` times its FP32 mirror of the reference
+arithmetic (OpenMP, all cores of this process's affinity) on a row block of
+the same operators -- the reference's own CPU path (bempp-cl/numba) is not
+installed -- and the FP64 oracle answers every GPU result the forms recorded
+for comparison (``Fp64Checks``) as the checker.
 """
 from __future__ import annotations
 
@@ -163,9 +166,32 @@ class Solver:
         return out
 
 
+class Fp64Checks:
+    """GPU results of DP0 CBIE solves recorded for comparison with the FP64
+    oracle solution of the same discrete system; the CPU leg computes the
+    oracle answers at the end of the run and writes the relative errors into
+    the result dictionaries (mode "set", or "max" over the items added)."""
+
+    def __init__(self):
+        self.items = []
+
+    def add(self, into, v, t, k, psi, phis, pts=None, ps=None, mode="set"):
+        """phis / ps: {result key: GPU surface solution / pressure at pts}."""
+        self.items.append({"into": into, "v": np.asarray(v, dtype=np.float32), "t": t, "k": float(k),
+                           "psi": np.asarray(psi), "phis": phis, "pts": pts, "ps": ps or {}, "mode": mode})
+        for key in list(phis) + list(ps or {}):
+            if mode == "max":
+                into.setdefault(key, 0.0)
+            else:
+                into[key] = None
+
+
+CHECKS = Fp64Checks()
+
+
 def fp64_check(level, ks):
-    """Same pipeline on a small mesh against the FP64 oracle solution."""
-    import oracle
+    """Same pipeline on a small mesh; the FP64 oracle solution it is compared
+    with comes from the CPU leg."""
     from neuralat_amd import mesh
 
     v, t, src, pts = make_workload(level)
@@ -181,16 +207,9 @@ def fp64_check(level, ks):
         pts_t = torch.tensor(pts.astype(np.float32), device=s.dev)
         phi, p, info = s.solve(k, psi_t, pts_t, 1e-6, 200, 2000)
         torch.cuda.synchronize()
-        S = oracle.assemble(v32, t, k, "single", "DP0")
-        D = oracle.assemble(v32, t, k, "double", "DP0")
-        M = oracle.identity(v32, t, "DP0")
-        phi64 = np.linalg.solve(0.5 * M - D, -S @ psi)
-        p64 = -oracle.potential(v32, t, pts, k, "single", "DP0") @ psi + oracle.potential(v32, t, pts, k, "double",
-                                                                                           "DP0") @ phi64
-        out[str(k)] = {
-            "surface_rel_err": float(np.linalg.norm(phi.cpu().numpy() - phi64) / np.linalg.norm(phi64)),
-            "exterior_rel_err": float(np.linalg.norm(p.cpu().numpy() - p64) / np.linalg.norm(p64)),
-        }
+        out[str(k)] = {}
+        CHECKS.add(out[str(k)], v32, t, k, psi, {"surface_rel_err": phi.cpu().numpy()}, pts,
+                   {"exterior_rel_err": p.cpu().numpy()})
     del s
     return out
 
@@ -311,21 +330,17 @@ def precision_form(s, v, t, k, pts, nrhs=16, seed=5, check_level=4, restart=1500
 
 
 def precision_fp64_check(level, k, srcs, tol):
-    """The 16-bit + refinement pipeline on a small mesh of the same family
-    against the FP64 oracle solution of the same discrete system."""
+    """The 16-bit + refinement pipeline on a small mesh of the same family;
+    compared with the FP64 oracle solution of the same discrete system in the
+    CPU leg."""
     import torch
 
-    import oracle
     from neuralat_amd.bem.precision import LowPrecisionMatrix, solve_ir
 
     v, t, _, pts = make_workload(level)
     v32 = v.astype(np.float32)
     s = Solver(v32, t)
     psis = _monopole_psis(v32, t, k, srcs)
-    S = oracle.assemble(v32, t, k, "single", "DP0")
-    D = oracle.assemble(v32, t, k, "double", "DP0")
-    M = oracle.identity(v32, t, "DP0")
-    A64 = 0.5 * M - D
     out = {"level": level}
     for dt in ("bf16", "fp16"):
         rhs = []
@@ -335,11 +350,9 @@ def precision_fp64_check(level, k, srcs, tol):
         L = LowPrecisionMatrix(s.lhs, dt)
         X, info = solve_ir(s.lhs, L, torch.stack(rhs), tol=tol, inner_tol=1e-2, max_inner=300, max_outer=40)
         Xc = X.cpu().numpy()
-        errs = []
+        out[dt] = {"converged": bool(info["converged"])}
         for q in range(len(srcs)):
-            x64 = np.linalg.solve(A64, -S @ psis[q])
-            errs.append(float(np.linalg.norm(Xc[q] - x64) / np.linalg.norm(x64)))
-        out[dt] = {"max_surface_rel_err": max(errs), "converged": bool(info["converged"])}
+            CHECKS.add(out[dt], v32, t, k, psis[q], {"max_surface_rel_err": Xc[q]}, mode="max")
     del s
     return out
 
@@ -539,7 +552,6 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
     iterations x bytes per entry x n^2)."""
     import torch
 
-    import oracle
     from neuralat_amd import mesh
     from neuralat_amd.bem.batch import (allocate_systems, batch_autotune, batch_cbie, batch_gmres,
                                         batch_solve_mixed)
@@ -560,7 +572,9 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
     acc = {"plain": 0.0, "batched": 0.0, "mixed": 0.0, "batched_asm": 0.0, "mixed_asm": 0.0}
     its_plain, its_batch, its_mixed, ntri_all = [], [], [], []
     conv = {"plain": 0, "batched": 0, "mixed": 0}
-    diffs_b, diffs_m, errs_b, errs_m = [], [], [], []
+    diffs_b, diffs_m = [], []
+    fp64 = {}  # filled by the CPU leg
+    nfp64 = 0
     best, tune_ms, done = None, None, 0
     for ci, chunk in enumerate(chunks):
         solvers, meshes, ks, psis, host = [], [], [], [], []
@@ -627,17 +641,14 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
                 diffs_b.append(float((torch.linalg.vector_norm(xb - xp) / nrm).item()))
             if c2:
                 diffs_m.append(float((torch.linalg.vector_norm(xm - xp) / nrm).item()))
-        # FP64 oracle (the checker) on the first level-2 items (320 triangles)
+        # the first level-2 items (320 triangles), for the FP64 oracle in the CPU leg
         for q, (v, t, k) in enumerate(host):
-            if len(errs_b) >= fp64_items or t.shape[0] != 320 or not (cb[q] and cm[q]):
+            if nfp64 >= fp64_items or t.shape[0] != 320 or not (cb[q] and cm[q]):
                 continue
-            S = oracle.assemble(v, t, k, "single", "DP0")
-            D = oracle.assemble(v, t, k, "double", "DP0")
-            M = oracle.identity(v, t, "DP0")
-            x64 = np.linalg.solve(0.5 * M - D, -S @ psis[q].cpu().numpy().astype(np.complex128))
-            n64 = np.linalg.norm(x64)
-            errs_b.append(float(np.linalg.norm(xs_b[q].cpu().numpy() - x64) / n64))
-            errs_m.append(float(np.linalg.norm(xs_m[q].cpu().numpy() - x64) / n64))
+            CHECKS.add(fp64, v, t, k, psis[q].cpu().numpy(),
+                       {"max_surface_rel_err_vs_fp64_level2_batched": xs_b[q].cpu().numpy(),
+                        "max_surface_rel_err_vs_fp64_level2_mixed": xs_m[q].cpu().numpy()}, mode="max")
+            nfp64 += 1
         done += len(ks)
         log(f"bench: batching chunk {ci + 1}/{len(chunks)} ({done} solves): plain {acc['plain']:.2f} s, "
             f"batched {acc['batched']:.2f} s, mixed {acc['mixed']:.2f} s")
@@ -650,7 +661,8 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
     bytes_mixed = float(sum(4.0 * n * n * it for n, it in zip(ntri_all, its_mixed)))
     lv = "-".join(f"L{x}" for x in levels)
     tp, tb, tm = acc["plain"], acc["batched"], acc["mixed"]
-    return {"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
+    fp64["fp64_items"] = nfp64
+    return {**fp64, "workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
                         f"log-uniform [5,80] = {nsol} DP0 CBIE solves to 1e-6, GMRES({restart}) at most "
                         f"{maxiter} iterations, in {len(chunks)} chunk(s) of at most {chunk_gb:.0f} GB of systems",
             "chunks": len(chunks), "triangles": sorted(set(ntri_all)),
@@ -671,10 +683,7 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
             "converged_mixed": conv["mixed"],
             "autotune_row_tile": best, "autotune_ms": tune_ms,
             "max_rel_diff_batched_vs_plain": max(diffs_b) if diffs_b else None,
-            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None,
-            "fp64_items": len(errs_b),
-            "max_surface_rel_err_vs_fp64_level2_batched": max(errs_b) if errs_b else None,
-            "max_surface_rel_err_vs_fp64_level2_mixed": max(errs_m) if errs_m else None}
+            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None}
 
 
 def batch_probe(nmesh=8, nk=8, restart=500):
@@ -713,6 +722,40 @@ def batch_probe(nmesh=8, nk=8, restart=500):
         f"({sum(i['iterations'] for i in ib)} its, {sum(i['converged'] for i in ib)} converged), "
         f"fp16 mixed {t2 - t1:.3f} s ({sum(i['inner_iterations'] for i in im)} inner its, "
         f"{sum(i['converged'] for i in im)} converged)")
+
+
+def fp64_answers(checks):
+    """The FP64 oracle solution of each recorded system, (M/2 - K) phi = -V psi
+    and p = D_pot phi - S_pot psi, against the recorded GPU results."""
+    import oracle
+
+    for it in checks.items:
+        v, t, k, psi = it["v"], it["t"], it["k"], it["psi"].astype(np.complex128)
+        S = oracle.assemble(v, t, k, "single", "DP0")
+        D = oracle.assemble(v, t, k, "double", "DP0")
+        M = oracle.identity(v, t, "DP0")
+        phi64 = np.linalg.solve(0.5 * M - D, -S @ psi)
+        errs = {key: float(np.linalg.norm(x - phi64) / np.linalg.norm(phi64)) for key, x in it["phis"].items()}
+        if it["ps"]:
+            pts = it["pts"]
+            p64 = -oracle.potential(v, t, pts, k, "single", "DP0") @ psi + oracle.potential(v, t, pts, k, "double",
+                                                                                             "DP0") @ phi64
+            errs.update({key: float(np.linalg.norm(p - p64) / np.linalg.norm(p64)) for key, p in it["ps"].items()})
+        for key, e in errs.items():
+            it["into"][key] = max(it["into"][key] or 0.0, e) if it["mode"] == "max" else e
+
+
+def cpu_leg(args, v, t, k):
+    """The CPU leg, on the host cores after every GPU measurement: the FP64
+    oracle answers for the GPU results the run recorded, then (unless
+    --no-cpu) the CPU baseline.  The only part of the benchmark that runs the
+    oracle."""
+    log(f"bench: CPU leg: FP64 oracle answers for {len(CHECKS.items)} recorded GPU results")
+    fp64_answers(CHECKS)
+    if args.no_cpu:
+        return None
+    log("bench: CPU leg: baseline")
+    return cpu_baseline(v, t, k, args.cpu_seconds)
 
 
 def cpu_baseline(v, t, k, seconds):
@@ -817,9 +860,8 @@ def main():
     n = t.shape[0]
     fp64 = {}
     if args.check_level > 0:
-        log(f"bench: FP64 oracle check on level {args.check_level}")
+        log(f"bench: level-{args.check_level} solves for the FP64 oracle check (answered in the CPU leg)")
         fp64 = fp64_check(args.check_level, ks)
-        log(f"bench: fp64 check {fp64}")
     log(f"bench: allocating {n}x{n} complex64 ({8 * n * n / 1e9:.1f} GB)")
     s = Solver(v.astype(np.float32), t)
     dev = s.dev
@@ -887,11 +929,10 @@ def main():
         forms["p1"] = p1_form()
         torch.cuda.empty_cache()
         forms["datagen"] = datagen_form()
+    cpu = cpu_leg(args, v, t, ks[0])
+    log(f"bench: fp64 check {fp64}")
+    if forms:
         log(f"bench: forms {json.dumps(forms)}")
-    cpu = None
-    if not args.no_cpu:
-        log("bench: CPU baseline")
-        cpu = cpu_baseline(v, t, ks[0], args.cpu_seconds)
     out = {
         "metric": METRIC,
         "value": entries / wall,
