@@ -661,8 +661,9 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
     bytes_mixed = float(sum(4.0 * n * n * it for n, it in zip(ntri_all, its_mixed)))
     lv = "-".join(f"L{x}" for x in levels)
     tp, tb, tm = acc["plain"], acc["batched"], acc["mixed"]
-    fp64["fp64_items"] = nfp64
-    return {**fp64, "workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
+    res = fp64  # the CPU leg fills its FP64 entries in this dictionary
+    res["fp64_items"] = nfp64
+    res.update({"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
                         f"log-uniform [5,80] = {nsol} DP0 CBIE solves to 1e-6, GMRES({restart}) at most "
                         f"{maxiter} iterations, in {len(chunks)} chunk(s) of at most {chunk_gb:.0f} GB of systems",
             "chunks": len(chunks), "triangles": sorted(set(ntri_all)),
@@ -683,7 +684,8 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
             "converged_mixed": conv["mixed"],
             "autotune_row_tile": best, "autotune_ms": tune_ms,
             "max_rel_diff_batched_vs_plain": max(diffs_b) if diffs_b else None,
-            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None}
+            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None})
+    return res
 
 
 def batch_probe(nmesh=8, nk=8, restart=500):
