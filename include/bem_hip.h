@@ -285,6 +285,33 @@ int bem_batch_fgmres(const bem_batch_system* sys, int n_sys, int dtype, float to
                      bem_solve_info* info, int* inner_iterations, void* stream);
 const char* bem_batch_last_error(void);
 
+/* ---- FEM modal analysis (linear tetrahedra, 3 degrees of freedom per vertex,
+ * dof 3 v + c): the mass and stiffness matrices of the solid whose modes give
+ * the BEM's Neumann data.  Replaces the reference's assemble_mass_matrix /
+ * assemble_stiffness_matrix (src/modalobj/cuda/computeMatrix.cu:130-166,
+ * called by FEMmodel, src/modalobj/fem.py:95-166), which write 144 COO
+ * triplets per tetrahedron; here both matrices are 3 x 3 block CSR over the
+ * vertex adjacency (rows and columns sorted), assembled deterministically. */
+typedef struct bem_fem bem_fem;
+/* verts [nv,3] f32, tets [ntet,4] i32, host memory.  Errors: BAD_MESH for an
+ * index out of range, a repeated vertex or a zero-volume tetrahedron. */
+int bem_fem_create(const float* verts, int64_t nv, const int32_t* tets, int64_t ntet, bem_fem** out);
+void bem_fem_free(bem_fem* fem);
+int64_t bem_fem_num_blocks(const bem_fem* fem);
+/* device int32 outputs: row_ptr [nv+1], block_rows [nblocks], cols [nblocks] */
+int bem_fem_pattern(const bem_fem* fem, void* row_ptr, void* block_rows, void* cols, void* stream);
+/* values: device [nblocks, 3, 3], f32 (fp64 = 0, the reference's type) or
+ * f64 (fp64 = 1; summed in FP64 either way); which 0 = mass (consistent,
+ * density), 1 = stiffness (isotropic linear elasticity: Young's modulus,
+ * Poisson's ratio). */
+int bem_fem_assemble(const bem_fem* fem, int which, float density, float youngs, float poisson, int fp64,
+                     void* values, void* stream);
+/* Y = A X for A = values (block CSR of fem), X and Y row-major [3 nv, m]
+ * (leading dimensions ldx, ldy >= m), device, all f32 (fp64 = 0) or f64. */
+int bem_fem_spmm(const bem_fem* fem, int fp64, const void* values, const void* X, int64_t ldx, int m, void* Y,
+                 int64_t ldy, void* stream);
+const char* bem_fem_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,6 +102,13 @@ SIGNATURES = {
     "bem_batch_fgmres": (_I32, [_P, _I32, _I32, _F, _F, _I32, _I32, _I32, _I32, _I32, _P,
                                 ctypes.POINTER(SolveInfo), _P, _P]),
     "bem_batch_last_error": (ctypes.c_char_p, []),
+    "bem_fem_create": (_I32, [_P, _I64, _P, _I64, ctypes.POINTER(ctypes.c_void_p)]),
+    "bem_fem_free": (None, [_P]),
+    "bem_fem_num_blocks": (_I64, [_P]),
+    "bem_fem_pattern": (_I32, [_P, _P, _P, _P, _P]),
+    "bem_fem_assemble": (_I32, [_P, _I32, _F, _F, _F, _I32, _P, _P]),
+    "bem_fem_spmm": (_I32, [_P, _I32, _P, _P, _I64, _I32, _P, _I64, _P]),
+    "bem_fem_last_error": (ctypes.c_char_p, []),
     "bem_gmres_ex": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
     "bem_gmres_timed": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64), _P]),

@@ -191,3 +191,67 @@ def random_exterior_points(n: int, rmin: float, rmax: float, seed: int = 0) -> n
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     r = rng.uniform(rmin, rmax, size=n)
     return d * r[:, None]
+
+
+def tet_box(nx: int, ny: int, nz: int, size=(1.0, 1.0, 1.0), jitter: float = 0.0, seed: int = 0):
+    """Tetrahedral mesh of a box centred at the origin: nx x ny x nz cells,
+    each split into 6 positively oriented tetrahedra around its main diagonal
+    (conforming across cells).  ``jitter`` moves interior vertices by up to
+    that fraction of a cell.  Returns (vertices [V,3] f64, tets [T,4] i32)."""
+    size = np.asarray(size, dtype=np.float64)
+    g = np.stack(np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij"), -1)
+    verts = (g.reshape(-1, 3) / np.array([nx, ny, nz]) - 0.5) * size
+    if jitter > 0:
+        rng = np.random.default_rng(seed)
+        inner = np.all((g.reshape(-1, 3) > 0) & (g.reshape(-1, 3) < np.array([nx, ny, nz])), axis=1)
+        cell = size / np.array([nx, ny, nz])
+        verts[inner] += rng.uniform(-jitter, jitter, (int(inner.sum()), 3)) * cell
+
+    def vid(i, j, k):
+        return (i * (ny + 1) + j) * (nz + 1) + k
+
+    tets = []
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                for p in perms:
+                    pos = np.array([i, j, k])
+                    path = [pos.copy()]
+                    for ax in p:
+                        pos = pos.copy()
+                        pos[ax] += 1
+                        path.append(pos)
+                    tets.append([vid(*q) for q in path])
+    tets = np.array(tets, dtype=np.int32)
+    p = verts[tets]
+    det = np.einsum("ti,ti->t", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]))
+    flip = det < 0
+    tets[flip, 2], tets[flip, 3] = tets[flip, 3].copy(), tets[flip, 2].copy()
+    return verts, tets
+
+
+def tet_boundary(verts: np.ndarray, tets: np.ndarray):
+    """Boundary of a tetrahedral mesh: faces that belong to one tetrahedron,
+    oriented outward (away from the tetrahedron's fourth vertex).  Returns
+    (surface vertices [S,3], surface triangles [F,3] i32 into them)."""
+    faces = {}
+    for t in np.asarray(tets):
+        for skip in range(4):
+            f = tuple(int(t[q]) for q in range(4) if q != skip)
+            key = tuple(sorted(f))
+            faces[key] = None if key in faces else (f, int(t[skip]))
+    tris = []
+    for val in faces.values():
+        if val is None:
+            continue
+        (a, b, c), d = val
+        n = np.cross(verts[b] - verts[a], verts[c] - verts[a])
+        if np.dot(n, verts[d] - verts[a]) > 0:
+            b, c = c, b
+        tris.append((a, b, c))
+    tris = np.array(tris, dtype=np.int64)
+    used = np.unique(tris)
+    remap = -np.ones(len(verts), dtype=np.int64)
+    remap[used] = np.arange(len(used))
+    return verts[used].copy(), remap[tris].astype(np.int32)

@@ -64,3 +64,21 @@ def test_status_mapping():
         _lib.check(_lib.BEM_ERR_NOT_CONVERGED)
     with pytest.raises(_lib.BemError):
         _lib.check(_lib.BEM_ERR_HIP)
+
+
+def test_fem_rejects_bad_meshes():
+    """bem_fem_create validates on the host before any device call."""
+    import ctypes
+
+    import numpy as np
+
+    from neuralat_amd import _lib
+
+    lib = _lib.load()
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0]], dtype=np.float32)
+    h = ctypes.c_void_p()
+    for tets in ([[0, 1, 2, 7]], [[0, 1, 1, 3]], [[0, 1, 2, 4]]):  # out of range, repeated, flat
+        t = np.array(tets, dtype=np.int32)
+        rc = lib.bem_fem_create(v.ctypes.data, len(v), t.ctypes.data, len(t), ctypes.byref(h))
+        assert rc == _lib.BEM_ERR_BAD_MESH, tets
+        assert lib.bem_fem_last_error()
