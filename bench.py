@@ -177,13 +177,22 @@ class Fp64Checks:
 
     def add(self, into, v, t, k, psi, phis, pts=None, ps=None, mode="set"):
         """phis / ps: {result key: GPU surface solution / pressure at pts}."""
-        self.items.append({"into": into, "v": np.asarray(v, dtype=np.float32), "t": t, "k": float(k),
-                           "psi": np.asarray(psi), "phis": phis, "pts": pts, "ps": ps or {}, "mode": mode})
+        self.items.append({"kind": "cbie", "into": into, "v": np.asarray(v, dtype=np.float32), "t": t,
+                           "k": float(k), "psi": np.asarray(psi), "phis": phis, "pts": pts, "ps": ps or {},
+                           "mode": mode})
         for key in list(phis) + list(ps or {}):
             if mode == "max":
                 into.setdefault(key, 0.0)
             else:
                 into[key] = None
+
+    def add_modes(self, into, v, t, material, k, eigenvalues):
+        """eigenvalues: {result key: GPU lowest non-rigid eigenvalues of the
+        solid (v, t)}; the error written is the largest relative one."""
+        self.items.append({"kind": "modes", "into": into, "v": np.asarray(v, dtype=np.float32), "t": t,
+                           "material": material, "k": int(k), "eigs": eigenvalues})
+        for key in eigenvalues:
+            into[key] = None
 
 
 CHECKS = Fp64Checks()
@@ -726,12 +735,90 @@ def batch_probe(nmesh=8, nk=8, restart=500):
         f"{sum(i['converged'] for i in im)} converged)")
 
 
+def modal_form(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=20):
+    """FEM modal analysis of a plastic block (the step upstream of the BEM):
+    HIP assembly of the mass and stiffness block matrices, the block product,
+    the k lowest modes by the block LOBPCG on the library's FP64 products and
+    by torch.lobpcg (the reference's solver, FP32), and the per-triangle
+    Neumann data of every mode.  The FP64 oracle eigenvalues come from the CPU
+    leg."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.modal import FEMmodel, LOBPCG_solver, Material, MatSet, VibrationObj, lobpcg
+
+    v, t = mesh.tet_box(*cells, size, jitter=0.2, seed=1)
+    mat = MatSet.Plastic
+    res = {"workload": f"tetrahedral block {cells[0]}x{cells[1]}x{cells[2]} cells ({len(t)} tetrahedra, "
+                       f"{len(v)} vertices, {3 * len(v)} dofs), {size} m, plastic, k = {k} modes"}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f = FEMmodel(v, t, Material(mat))
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    f._blocks.clear()
+    f.blocks("stiffness")
+    f.blocks("mass")
+    ev1.record()
+    torch.cuda.synchronize()
+    asm_ms = ev0.elapsed_time(ev1)
+    # block product: FP64 matrix and 48 columns (the LOBPCG block), bytes =
+    # 9 * 8 per block + 4 (column index) + one read of X and one write of Y
+    X = torch.randn(f.ndof, 48, dtype=torch.float64, device="cuda")
+    f.matmul("stiffness", X)
+    ev0.record()
+    for _ in range(reps):
+        f.matmul("stiffness", X)
+    ev1.record()
+    torch.cuda.synchronize()
+    spmm_ms = ev0.elapsed_time(ev1) / reps
+    spmm_bytes = f.nblocks * (72 + 4) + 2 * X.numel() * 8
+    t0 = time.perf_counter()
+    w, x, info = lobpcg(f, k)
+    torch.cuda.synchronize()
+    t_lob = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    wt, _ = LOBPCG_solver(f.stiffness_matrix, f.mass_matrix, k)
+    torch.cuda.synchronize()
+    t_torch = time.perf_counter() - t0
+    sv, st = mesh.tet_boundary(v, t)
+    obj = VibrationObj(v, t, sv, st)
+    t0 = time.perf_counter()
+    obj.modal_analysis(k=k, material=Material(mat))
+    neu = np.stack([obj.get_triangle_neumann(q) for q in range(k)])
+    t_modal = time.perf_counter() - t0
+    res.update({"setup_seconds": t_setup, "assembly_ms": asm_ms, "tets_per_s": len(t) / (asm_ms / 1e3),
+                "block_product_ms": spmm_ms, "block_product_GBps": spmm_bytes / (spmm_ms / 1e3) / 1e9,
+                "lobpcg_seconds": t_lob, "lobpcg_iterations": info["iterations"],
+                "lobpcg_max_rel_residual": info["max_rel_residual"], "torch_lobpcg_seconds": t_torch,
+                "max_rel_diff_torch_vs_lobpcg": float(np.max(np.abs(wt - w) / w)),
+                "modal_analysis_seconds": t_modal, "surface_triangles": int(len(st)),
+                "frequencies_hz": [float(x) for x in (np.sqrt(w[:4]) / (2 * np.pi))],
+                "neumann_finite": bool(np.isfinite(neu).all())})
+    CHECKS.add_modes(res, v, t, mat, k, {"max_eig_rel_err_vs_fp64_lobpcg": w,
+                                         "max_eig_rel_err_vs_fp64_torch_lobpcg": wt})
+    log(f"bench: modal {len(t)} tets: assembly {asm_ms:.2f} ms, block product {spmm_ms * 1e3:.0f} us, "
+        f"lobpcg {t_lob:.2f} s ({info['iterations']} its), torch.lobpcg {t_torch:.2f} s")
+    return res
+
+
 def fp64_answers(checks):
     """The FP64 oracle solution of each recorded system, (M/2 - K) phi = -V psi
     and p = D_pot phi - S_pot psi, against the recorded GPU results."""
     import oracle
+    from oracle import fem as ofem
 
     for it in checks.items:
+        if it["kind"] == "modes":
+            rho, E, nu = it["material"][:3]
+            K = ofem.assemble(it["v"], it["t"], "stiffness", youngs=E, poisson=nu)
+            M = ofem.assemble(it["v"], it["t"], "mass", density=rho)
+            w64, _ = ofem.modes(K, M, it["k"])
+            for key, w in it["eigs"].items():
+                it["into"][key] = float(np.max(np.abs(np.asarray(w) - w64) / w64))
+            continue
         v, t, k, psi = it["v"], it["t"], it["k"], it["psi"].astype(np.complex128)
         S = oracle.assemble(v, t, k, "single", "DP0")
         D = oracle.assemble(v, t, k, "double", "DP0")
@@ -931,6 +1018,8 @@ def main():
         forms["p1"] = p1_form()
         torch.cuda.empty_cache()
         forms["datagen"] = datagen_form()
+        torch.cuda.empty_cache()
+        forms["modal"] = modal_form()
     cpu = cpu_leg(args, v, t, ks[0])
     log(f"bench: fp64 check {fp64}")
     if forms:
