@@ -789,6 +789,33 @@ def modal_form(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=20):
     obj.modal_analysis(k=k, material=Material(mat))
     neu = np.stack([obj.get_triangle_neumann(q) for q in range(k)])
     t_modal = time.perf_counter() - t0
+    # modal sound: every mode's Burton-Miller solve and pressure at 256 points,
+    # one mode at a time (the reference's loop) and all modes in lockstep
+    from neuralat_amd.datagen import EditableModalSound, modal_preprocess
+
+    obj.normalize(0.15)
+    data = modal_preprocess(obj, k)
+    ms = EditableModalSound(data, {"vibration_obj": {"size": 0.15},
+                                   "sample": {"freq_rate": 1.0, "size_rate": 2.0, "bbox_rate": 2.0, "sample_num": 1,
+                                              "point_num_per_sample": 256}})
+    ms.sample(freqK_base=torch.tensor([1.0], device="cuda"), sizeK_base=torch.tensor([0.5], device="cuda"))
+    ms.solve()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    p_loop = ms.solve_loop()
+    torch.cuda.synchronize()
+    t_ms_loop = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p_batch = ms.solve()
+    torch.cuda.synchronize()
+    t_ms_batch = time.perf_counter() - t0
+    res["modal_sound"] = {"modes": k, "surface_vertices": int(len(sv)),
+                          "k_range": [float(ms.ks.abs().min()), float(ms.ks.abs().max())],
+                          "loop_seconds": t_ms_loop, "batched_seconds": t_ms_batch, "speedup": t_ms_loop / t_ms_batch,
+                          "converged": int(sum(i["converged"] for i in ms.solve_infos)),
+                          "rel_diff_batched_vs_loop": float((torch.linalg.vector_norm(p_batch - p_loop)
+                                                             / torch.linalg.vector_norm(p_loop)).item())}
+    log(f"bench: modal sound {k} modes: loop {t_ms_loop:.3f} s, batched {t_ms_batch:.3f} s")
     res.update({"setup_seconds": t_setup, "assembly_ms": asm_ms, "tets_per_s": len(t) / (asm_ms / 1e3),
                 "block_product_ms": spmm_ms, "block_product_GBps": spmm_bytes / (spmm_ms / 1e3) / 1e9,
                 "lobpcg_seconds": t_lob, "lobpcg_iterations": info["iterations"],

@@ -176,9 +176,134 @@ def burton_miller_system(solver, k, neumann):
     return lhs, rhs
 
 
+def modal_preprocess(obj, mode_num):
+    """Surface data of a vibrating solid after ``obj.modal_analysis`` (a
+    modal.VibrationObj): per-vertex Neumann data of each mode (the per-triangle
+    normal velocity averaged to the vertices) and the wavenumbers, negated as
+    the reference stores them (experiments/neuPAT_material_scale/
+    preprocess.py:26-60, without the file IO).  Returns the dict
+    EditableModalSound takes."""
+    from .bem.solver import map_triangle2vertex
+
+    verts = torch.tensor(obj.surf_vertices, dtype=torch.float32, device="cuda")
+    tris = torch.tensor(obj.surf_triangles, dtype=torch.int32, device="cuda")
+    neumann_vtx = torch.zeros((mode_num, len(verts)), dtype=torch.complex64, device="cuda")
+    ks = []
+    for i in range(mode_num):
+        neumann_tri = torch.tensor(obj.get_triangle_neumann(i), dtype=torch.complex64, device="cuda")
+        neumann_vtx[i] = map_triangle2vertex(verts, tris, neumann_tri)
+        ks.append(-obj.get_wave_number(i))
+    return {"vertices": verts, "triangles": tris, "neumann_vtx": neumann_vtx,
+            "ks": torch.tensor(ks, dtype=torch.float32, device="cuda"), "eigenvalues": obj.eigenvalues,
+            "modes": obj.modes}
+
+
+class EditableModalSound:
+    """The modal-sound data generator of src/scene.py:301-397: a vibrating
+    object's surface modes (``modal_preprocess``), resampled in size and
+    material frequency, each mode's Burton-Miller solve and exterior pressure
+    at sampled points.  ``data``: the modal_data dict; ``config``: the
+    "sample" and "vibration_obj" sections of the reference's config.json.
+    ``solve`` solves all modes together (one lockstep batched GMRES over the
+    modes' systems); ``solve_loop`` is the reference's one-mode-at-a-time
+    loop."""
+
+    def __init__(self, data, config, ffat_res=(64, 32), uniform=False):
+        sample_config = config.get("sample", {})
+        self.size_base = config.get("vibration_obj", {}).get("size")
+        self.vertices_base = data["vertices"]
+        self.triangles = data["triangles"]
+        self.neumann_vtx = data["neumann_vtx"]
+        self.ks_base = data["ks"]
+        self.mode_num = len(self.ks_base)
+        self.freq_rate = sample_config.get("freq_rate")
+        self.size_rate = sample_config.get("size_rate")
+        self.bbox_rate = sample_config.get("bbox_rate")
+        self.sample_num = sample_config.get("sample_num")
+        self.point_num_per_sample = sample_config.get("point_num_per_sample")
+        self.ffat_res = ffat_res
+        xs = torch.linspace(0, 1, ffat_res[0], device="cuda", dtype=torch.float32)
+        ys = torch.linspace(0, 1, ffat_res[1], device="cuda", dtype=torch.float32)
+        self.gridx, self.gridy = torch.meshgrid(xs, ys, indexing="ij")
+        self.uniform = uniform
+        if uniform:
+            self.point_num_per_sample = ffat_res[0] * ffat_res[1]
+
+    def sample(self, freqK_base=None, sizeK_base=None, generator=None):
+        """New size and frequency factors and target points (scene.py:330-383);
+        returns the network input [P, 5] = (point factors, size, frequency)."""
+        dev = torch.device("cuda")
+        self.freqK_base = torch.rand(1, generator=generator).to(dev) if freqK_base is None else freqK_base
+        self.sizeK_base = torch.rand(1, generator=generator).to(dev) if sizeK_base is None else sizeK_base
+        self.freqK = self.freqK_base * self.freq_rate
+        self.sizeK = 1.0 / (1 + self.sizeK_base * (self.size_rate - 1))
+        self.vertices = self.vertices_base * self.sizeK
+        self.ks = self.ks_base * self.freqK / self.sizeK ** 0.5
+        if self.uniform:
+            base = torch.zeros(self.ffat_res[0] * self.ffat_res[1], 3, device=dev, dtype=torch.float32)
+            base[:, 0] = 0.5
+            base[:, 1] = self.gridx.reshape(-1)
+            base[:, 2] = self.gridy.reshape(-1)
+        else:
+            base = torch.rand(self.point_num_per_sample, 3, generator=generator).to(dev)
+        self.sample_points_base = base
+        rs = (base[:, 0] * (self.bbox_rate - 1) + 1) * self.size_base * 0.7
+        theta = base[:, 1] * 2 * np.pi - np.pi
+        phi = base[:, 2] * np.pi
+        self.trg_points = torch.stack([rs * torch.sin(phi) * torch.cos(theta), rs * torch.sin(phi) * torch.sin(theta),
+                                       rs * torch.cos(phi)], dim=-1)
+        x = torch.zeros(base.shape[0], 5, dtype=torch.float32)
+        x[:, :3] = base.cpu()
+        x[:, 3] = self.sizeK_base.cpu()
+        x[:, 4] = self.freqK_base.cpu()
+        return x
+
+    def solve(self, tol=1e-6, restart=100, max_cycles=10):
+        """Pressure of every mode at the target points [modes, P] complex64:
+        the modes' Burton-Miller systems (one per wavenumber on the same
+        surface) solved together by the lockstep batched GMRES."""
+        solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
+        systems = [burton_miller_system(solver, float(self.ks[i]), self.neumann_vtx[i]) for i in range(self.mode_num)]
+        xs, infos = batch_gmres([q[0] for q in systems], [q[1] for q in systems], tol=tol, restart=restart,
+                                max_cycles=max_cycles)
+        self.solve_infos = infos
+        self.potentials = torch.stack([solver.boundary2potential(float(self.ks[i]), self.neumann_vtx[i], xs[i],
+                                                                 self.trg_points) for i in range(self.mode_num)]).cpu()
+        return self.potentials
+
+    def solve_loop(self):
+        """The reference's loop (scene.py:384-397): one neumann2dirichlet and
+        boundary2potential per mode."""
+        solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
+        out = []
+        for i in range(self.mode_num):
+            k = float(self.ks[i])
+            d = solver.neumann2dirichlet(k, self.neumann_vtx[i])
+            out.append(solver.boundary2potential(k, self.neumann_vtx[i], d, self.trg_points))
+        self.potentials = torch.stack(out).cpu()
+        return self.potentials
+
+
+def generate_modal_dataset(modal_sound, sample_num=None, seed=0):
+    """x [S, P, 5] (point factors, size, frequency), y [S, P, modes]
+    (experiments/neuPAT_material_scale/generate.py:12-37; the reference
+    assigns the complex pressure into a float32 tensor, which keeps its real
+    part -- kept here)."""
+    S = sample_num or modal_sound.sample_num
+    gen = torch.Generator().manual_seed(seed)
+    P, K = modal_sound.point_num_per_sample, modal_sound.mode_num
+    x = torch.zeros(S, P, 5, dtype=torch.float32)
+    y = torch.zeros(S, P, K, dtype=torch.float32)
+    for q in range(S):
+        x[q] = modal_sound.sample(generator=gen)
+        y[q] = modal_sound.solve().T.real
+    return x, y
+
+
 def save_dataset(path, x, y):
     """The reference's dataset file: torch.save({"x": x, "y": y}) (generate.py:39)."""
     torch.save({"x": x, "y": y}, path)
 
 
-__all__ = ["SceneObject", "SceneSampler", "burton_miller_system", "save_dataset"]
+__all__ = ["SceneObject", "SceneSampler", "burton_miller_system", "save_dataset", "modal_preprocess",
+           "EditableModalSound", "generate_modal_dataset"]

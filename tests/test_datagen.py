@@ -79,3 +79,33 @@ def test_batched_generation_matches_per_sample_solves():
         assert np.allclose(x[q].numpy(), sc.features(f))
         err = np.linalg.norm(y[q, :, 0].numpy() - p) / np.linalg.norm(p)
         assert err < 2e-5, (q, err)
+
+
+@pytest.mark.gpu
+def test_modal_sound_pipeline_batched_equals_loop():
+    """Tetrahedral block -> modes (FEM) -> per-vertex modal Neumann data ->
+    every mode's Burton-Miller solve: the lockstep batched solve over the
+    modes equals the reference's one-mode-at-a-time loop."""
+    from neuralat_amd.datagen import EditableModalSound, generate_modal_dataset, modal_preprocess
+    from neuralat_amd.modal import Material, MatSet, VibrationObj
+
+    v, t = mesh.tet_box(6, 4, 3, (0.3, 0.2, 0.15), jitter=0.15, seed=2)
+    sv, st = mesh.tet_boundary(v, t)
+    obj = VibrationObj(v, t, sv, st)
+    obj.normalize(0.1)
+    obj.modal_analysis(k=6, material=Material(MatSet.Ceramic))
+    data = modal_preprocess(obj, 6)
+    assert data["neumann_vtx"].shape == (6, len(sv)) and bool((data["ks"] < 0).all())
+    config = {"vibration_obj": {"size": 0.1},
+              "sample": {"freq_rate": 2.0, "size_rate": 2.0, "bbox_rate": 2.0, "sample_num": 2,
+                         "point_num_per_sample": 32}}
+    ms = EditableModalSound(data, config)
+    x = ms.sample(freqK_base=torch.tensor([0.3], device="cuda"), sizeK_base=torch.tensor([0.6], device="cuda"))
+    assert x.shape == (32, 5)
+    pb = ms.solve()
+    assert all(i["converged"] for i in ms.solve_infos), ms.solve_infos
+    pl = ms.solve_loop()
+    rel = (torch.linalg.vector_norm(pb - pl) / torch.linalg.vector_norm(pl)).item()
+    assert rel < 2e-5, rel
+    xs, ys = generate_modal_dataset(ms, 2, seed=1)
+    assert xs.shape == (2, 32, 5) and ys.shape == (2, 32, 6) and bool(torch.isfinite(ys).all())
