@@ -933,8 +933,9 @@ def traffic_probe(level, k):
 
 
 def p1_probe(level=5, k=10.0):
-    """One assembly of each P1 layer operator on the level-``level`` mesh of
-    the P1 measurement (for hardware-counter passes)."""
+    """One assembly of each P1 layer operator and of the two Burton-Miller
+    operators with the approximate rules (the data-generation path) on the
+    level-``level`` mesh of the P1 measurement (for hardware-counter passes)."""
     import torch
 
     from neuralat_amd import mesh
@@ -943,10 +944,11 @@ def p1_probe(level=5, k=10.0):
     v, t = mesh.icosphere(level, 1.0)
     v = mesh.sh_perturb(v, 8, 0.05, seed=3).astype(np.float32)
     s = BEM_Solver(torch.tensor(v, device="cuda"), torch.tensor(t, device="cuda"), basis="P1")
-    for op in ("single", "double"):
-        A = s.assemble_boundary_matrix(k, op)
+    for op, approx in (("single", False), ("double", False), ("bm_lhs", True), ("bm_rhs", True)):
+        A = s.assemble_boundary_matrix(k, op, approx=approx)
         torch.cuda.synchronize()
-        print(json.dumps({"op": op, "vertices": int(v.shape[0]), "entry_0_5": str(complex(A[0, 5].item()))}))
+        print(json.dumps({"op": op, "approx": approx, "vertices": int(v.shape[0]),
+                          "entry_0_5": str(complex(A[0, 5].item()))}))
         del A
 
 

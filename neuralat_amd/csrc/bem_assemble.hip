@@ -498,6 +498,11 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     constexpr int S = Rec<Q>::kFloats;
     __shared__ float2 acc[kP1RowVerts][kP1ColVerts];
     __shared__ float2 stage[9][kP1ColTris];
+    // the row tile's triangles, records and curls, read by every wavefront at
+    // each row step (from LDS rather than a chain of dependent scalar loads)
+    __shared__ float rrec[kP1RowTris][S];
+    __shared__ int4 rtri[kP1RowTris];
+    __shared__ float rcurl[kP1RowTris][9];
     int ct = blockIdx.x, rt = blockIdx.y;
     bool own = false;  // SYM: the column tile holds the row tile
     if constexpr (SYM) {
@@ -517,6 +522,16 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
     const int cv0 = tl.col_vptr[ct], ncv = tl.col_vptr[ct + 1] - cv0;
     for (int e = threadIdx.x; e < nrv * ncv; e += 256) acc[e / ncv][e % ncv] = make_float2(0.f, 0.f);
+    for (int e = threadIdx.x; e < (r1 - r0) * S; e += 256) {  // r1 - r0 <= kP1RowTris
+        const int q = e / S;
+        rrec[q][e - q * S] = rec[(int64_t)tl.perm[r0 + q] * S + (e - q * S)];
+    }
+    for (int q = threadIdx.x; q < r1 - r0; q += 256) rtri[q] = tris[tl.perm[r0 + q]];
+    if constexpr (OP == HYP || OP == BM_LHS)
+        for (int e = threadIdx.x; e < (r1 - r0) * 9; e += 256) {
+            const int q = e / 9;
+            rcurl[q][e - q * 9] = curls[(int64_t)tl.perm[r0 + q] * 12 + (e - q * 9)];
+        }
 
     // phase-1 role: column triangle c0 + threadIdx.x
     const int pos = c0 + (int)threadIdx.x;
@@ -536,6 +551,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         for (int e = 0; e < 9; ++e) cj[e] = curls[(int64_t)j * 12 + e];
     }
     const float krev = k * kInv2Pi;
+    const float invk = 1.0f / k;
     // phase-2 role: column vertex slot w = threadIdx.x
     const int w = (int)threadIdx.x;
     const bool owner = w < ncv;
@@ -544,31 +560,47 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     __syncthreads();
 
     for (int rp = r0; rp < r1; ++rp) {
-        const int i = tl.perm[rp];  // wave-uniform
-        const int4 ti = tris[i];
+        const int4 ti = rtri[rp - r0];
         const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
                             ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z || (SYM && own && pos <= rp);
         float br[9], bi[9];
 #pragma unroll
         for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
         if (active && !shared) {
-            const float* rr = rec + (int64_t)i * S;
+            const float* rr = rrec[rp - r0];
             float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
             // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
             // triangle's plane, so it is formed once per column point
-            float qb[Q];
-            if constexpr (OP == ADLP) {
+            float qb[Q], qbk[Q];
+            if constexpr (OP == ADLP || OP == BM_RHS) {
                 const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
 #pragma unroll
-                for (int b = 0; b < Q; ++b)
-                    qb[b] = wy[b] * ((py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
-                                     (py[3 * b + 2] - rr[2]) * nx2);
+                for (int b = 0; b < Q; ++b) {
+                    const float yn = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                                     (py[3 * b + 2] - rr[2]) * nx2;
+                    if constexpr (OP == ADLP) {
+                        qb[b] = wy[b] * yn;
+                    } else {  // BM_RHS: (x_0 - y_b).n_x and its 1/k multiple
+                        qb[b] = -yn;
+                        qbk[b] = -yn * invk;
+                    }
+                }
             }
+            // BM_LHS: -k n_x.n_y once per pair; (x_a - y_0).n_y once per row point
+            float mknn = 0.f;
+            if constexpr (OP == BM_LHS)
+                mknn = -k * (rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2);
 #pragma unroll
             for (int a = 0; a < Q; ++a) {
                 const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
                 float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
                 float tr9 = 0.f, ti9 = 0.f;
+                float mca = 0.f, kca = 0.f;
+                if constexpr (OP == BM_LHS) {
+                    const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
+                    mca = -ca;
+                    kca = k * ca;
+                }
 #pragma unroll
                 for (int b = 0; b < Q; ++b) {
                     float vr, vi, gr = 0.f, gi = 0.f;
@@ -592,6 +624,33 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                             vr = pw * fmaf(th, sn, cs);
                             vi = pw * fmaf(-th, cs, sn);
                         }
+                    } else if constexpr (OP == BM_LHS || OP == BM_RHS) {
+                        // specialised Burton-Miller evaluations: value = G (er + i ei)
+                        // with the normal factors constant per row point (BM_LHS:
+                        // (y_b - x_a).n_y = -(x_a - y_0).n_y) or per column point
+                        // (BM_RHS: (x_a - y_b).n_x = (x_0 - y_b).n_x)
+                        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        if (CLAMP) r2 = fmaxf(r2, kEps2);
+                        const float rinv = __builtin_amdgcn_rsqf(r2);
+                        const float r = r2 * rinv;
+                        float sn, cs;
+                        sincos_rev(r * krev, sn, cs);
+                        const float rw = rinv * wy[b];
+                        const float g_r = cs * rw, g_i = sn * rw;  // w_b G
+                        float er, ei;
+                        if constexpr (OP == BM_LHS) {
+                            // (1 - i k r)(y - x).n_y / r^2 - i k n_x.n_y
+                            er = mca * rinv * rinv;
+                            ei = fmaf(kca, rinv, mknn);
+                            gr = g_r;  // w_b G for the curl term
+                            gi = g_i;
+                        } else {
+                            // -1 + (i/k)(1 - i k r)(x - y).n_x / r^2
+                            er = fmaf(qb[b], rinv, -1.0f);
+                            ei = qbk[b] * rinv * rinv;
+                        }
+                        vr = g_r * er - g_i * ei;
+                        vi = g_r * ei + g_i * er;
                     } else {
                         kernel4pi<OP, CLAMP>(dx, dy, dz, rr[4 * Q], rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k,
                                              krev, vr, vi, gr, gi);
@@ -604,8 +663,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                     sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
                     sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
                     if constexpr (OP == BM_LHS) {
-                        tr9 = fmaf(wy[b], gr, tr9);
-                        ti9 = fmaf(wy[b], gi, ti9);
+                        tr9 += gr;  // gr, gi: w_b G
+                        ti9 += gi;
                     }
                 }
                 float ua = rr[3 * Q + a];
@@ -631,7 +690,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                 for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
                 float ci[9], cp[9];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+                for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
                 curl_product(ci, cj, cp);
                 const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
                 const float kk = k * k * nn;
@@ -646,7 +705,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                 const float s9r = -gi9 * invk, s9i = gr9 * invk;
                 float ci[9], cp[9];
 #pragma unroll
-                for (int e = 0; e < 9; ++e) ci[e] = curls[(int64_t)i * 12 + e];
+                for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
                 curl_product(ci, cj, cp);
 #pragma unroll
                 for (int e = 0; e < 9; ++e) {
