@@ -245,6 +245,7 @@ __global__ __launch_bounds__(64 * kC64Waves) void gemm_c64_mfma_kernel(
     const int64_t r0 = ((int64_t)blockIdx.x * kC64Waves + wv) * 16;
     const float2* pa = A + min(r0 + li, rows - 1) * lda;
     const bool live = li < nrhs;
+    const bool vec = (lda & 1) == 0 && ((uintptr_t)A & 15) == 0;  // 16-byte aligned rows
     f32x4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk;  // columns [k0, k0 + kchunk) of this split
     const int64_t kend = min(cols, k0 + kchunk);
@@ -257,10 +258,20 @@ __global__ __launch_bounds__(64 * kC64Waves) void gemm_c64_mfma_kernel(
         }
         __syncthreads();
         float2 a[8];
+        if (vec && kb + 32 <= kend) {  // whole step in range: four 16-byte loads per lane
+            const f32x4* p4 = reinterpret_cast<const f32x4*>(pa + kb + 8 * q);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t c = kb + 8 * q + e;
-            a[e] = c < kend ? pa[c] : make_float2(0.f, 0.f);
+            for (int e = 0; e < 4; ++e) {
+                const f32x4 v = __builtin_nontemporal_load(p4 + e);
+                a[2 * e] = make_float2(v.x, v.y);
+                a[2 * e + 1] = make_float2(v.z, v.w);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t c = kb + 8 * q + e;
+                a[e] = c < kend ? pa[c] : make_float2(0.f, 0.f);
+            }
         }
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
