@@ -483,6 +483,7 @@ int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const voi
     int its = 0;
     double rnorm = 0.0, implicit = 0.0;
     bool conv = false;
+    const int chk = (n <= bem::kArnoldiSmallN && !timed) ? 8 : 1;  // steps per read of the residuals
     while (true) {
         // r = b - A x
         HIP_TRY(hipMemcpyAsync(r, b, sizeof(float2) * (size_t)n, hipMemcpyDeviceToDevice, s));
@@ -499,23 +500,47 @@ int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const voi
         if (its >= maxiter) break;
         HIP_TRY(bem::launch_normalize(r, n, sc, V, s));
         HIP_TRY(bem::launch_init_g(g, m, sc, s));
-        int nk = 0;
+        int nk = 0, checked = 0;
         for (int j = 0; j < m && its < maxiter; ++j) {
             float2* vj = V + (size_t)j * n;
             HIP_TRY(gemv_timed(vj, w, one, zero));
-            // classical Gram-Schmidt, twice
-            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, hp, s));
-            HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));
-            HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h2, hp, s));
-            HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h2, w, s));
-            HIP_TRY(bem::launch_norm2(w, n, sc + 1, s));
-            HIP_TRY(bem::launch_normalize(w, n, sc + 1, V + (size_t)(j + 1) * n, s));
-            HIP_TRY(bem::launch_givens(H, m, j, h, h2, sc + 1, cs, sn, g, res, s));
+            if (n <= bem::kArnoldiSmallN) {  // launch-bound sizes: the step in one workgroup
+                HIP_TRY(bem::launch_arnoldi_small(V, n, j, w, h, h2, H, m, cs, sn, g, res, sc + 1, s));
+            } else {
+                // classical Gram-Schmidt, twice
+                HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h, hp, s));
+                HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h, w, s));
+                HIP_TRY(bem::launch_multidot(V, n, n, j + 1, w, h2, hp, s));
+                HIP_TRY(bem::launch_gs_update(V, n, n, j + 1, h2, w, s));
+                HIP_TRY(bem::launch_norm2(w, n, sc + 1, s));
+                HIP_TRY(bem::launch_normalize(w, n, sc + 1, V + (size_t)(j + 1) * n, s));
+                HIP_TRY(bem::launch_givens(H, m, j, h, h2, sc + 1, cs, sn, g, res, s));
+            }
             ++its;
             nk = j + 1;
-            if (int rc = read_double(res + j, &implicit, s)) return rc;
+            // the implicit residuals are read every `chk` steps (small systems:
+            // the read is most of a step's time); on convergence the steps past
+            // the first converged one are dropped from the update (columns and
+            // g entries of earlier steps are final when a step ends), so the
+            // solution is the one of stopping there
+            const bool last = j + 1 == m || its >= maxiter;
+            if (nk - checked < chk && !last) continue;
+            std::vector<double> rs(nk - checked);
+            HIP_TRY(hipMemcpyAsync(rs.data(), res + checked, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
             if (int rc = gemv_account()) return rc;
-            if (implicit <= target) break;
+            bool hit = false;
+            for (size_t q = 0; q < rs.size(); ++q) {
+                implicit = rs[q];
+                if (implicit <= target) {
+                    its -= (int)(rs.size() - 1 - q);
+                    nk = checked + (int)q + 1;
+                    hit = true;
+                    break;
+                }
+            }
+            checked = nk;
+            if (hit) break;
         }
         HIP_TRY(bem::launch_backsolve(H, m, nk, g, y, s));
         HIP_TRY(bem::launch_update_x(V, n, n, nk, y, x, s));

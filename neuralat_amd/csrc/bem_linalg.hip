@@ -473,6 +473,89 @@ __global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
 }
 
 
+// One Arnoldi step after the product w = A v_j for a small system, in one
+// workgroup: classical Gram-Schmidt with one re-orthogonalisation (h, h2),
+// |w|^2 into *wn2, v_{j+1} = w / |w|, and the Givens update of column j
+// (as gmres_givens_kernel).  One launch instead of nine: for small systems
+// the steps are launch-bound (each kernel does microseconds of work).
+__global__ __launch_bounds__(kGivensChunk) void arnoldi_small_kernel(
+    float2* __restrict__ V, int64_t n, int j, float2* __restrict__ w, double2* __restrict__ h,
+    double2* __restrict__ h2, double2* __restrict__ H, int m, double2* __restrict__ cs, double2* __restrict__ sn,
+    double2* __restrict__ g, double* __restrict__ res, double* __restrict__ wn2)
+{
+    __shared__ double red[kGivensChunk / 64];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int nw = kGivensChunk / 64;
+    for (int pass = 0; pass < 2; ++pass) {
+        double2* hh = pass ? h2 : h;
+        for (int q = wave; q <= j; q += nw) {  // hh[q] = <v_q, w>
+            const float2* vq = V + (int64_t)q * n;
+            double sr = 0.0, si = 0.0;
+            for (int64_t e = lane; e < n; e += 64) {
+                const float2 a = vq[e], b = w[e];
+                sr += (double)a.x * b.x + (double)a.y * b.y;
+                si += (double)a.x * b.y - (double)a.y * b.x;
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                sr += __shfl_xor(sr, o, 64);
+                si += __shfl_xor(si, o, 64);
+            }
+            if (lane == 0) hh[q] = make_double2(sr, si);
+        }
+        __syncthreads();
+        for (int64_t e = t; e < n; e += kGivensChunk) {  // w -= sum_q hh[q] v_q
+            double ar = 0.0, ai = 0.0;
+            for (int q = 0; q <= j; ++q) {
+                const double2 c = hh[q];
+                const float2 v = V[(int64_t)q * n + e];
+                ar += c.x * v.x - c.y * v.y;
+                ai += c.x * v.y + c.y * v.x;
+            }
+            const float2 wo = w[e];
+            w[e] = make_float2((float)((double)wo.x - ar), (float)((double)wo.y - ai));
+        }
+        __syncthreads();
+    }
+    double a2 = 0.0;
+    for (int64_t e = t; e < n; e += kGivensChunk) {
+        const float2 v = w[e];
+        a2 += (double)v.x * v.x + (double)v.y * v.y;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+    if (lane == 0) red[wave] = a2;
+    __syncthreads();
+    double nrm2 = 0.0;
+    for (int q = 0; q < nw; ++q) nrm2 += red[q];
+    const float inv = nrm2 > 0.0 ? (float)(1.0 / sqrt(nrm2)) : 0.f;
+    float2* vn = V + (int64_t)(j + 1) * n;
+    for (int64_t e = t; e < n; e += kGivensChunk) {
+        const float2 v = w[e];
+        vn[e] = make_float2(v.x * inv, v.y * inv);
+    }
+    double2* col = H + (int64_t)j * (m + 1);
+    const double2 carry = rotate_column_wg(col, j, h, h2, cs, sn);
+    if (t != 0) return;
+    *wn2 = nrm2;
+    const double2 aa = carry, bb = make_double2(sqrt(nrm2), 0.0);
+    const double den = sqrt(aa.x * aa.x + aa.y * aa.y + bb.x * bb.x + bb.y * bb.y);
+    double2 c = make_double2(1.0, 0.0), sv = make_double2(0.0, 0.0);
+    if (den > 0.0) {
+        c = make_double2(aa.x / den, -aa.y / den);
+        sv = make_double2(bb.x / den, -bb.y / den);
+    }
+    cs[j] = c;
+    sn[j] = sv;
+    col[j] = make_double2(den, 0.0);
+    col[j + 1] = make_double2(0.0, 0.0);
+    const double2 gj = g[j];
+    const double2 gn = make_double2(-(sv.x * gj.x + sv.y * gj.y), -(sv.x * gj.y - sv.y * gj.x));
+    g[j + 1] = gn;
+    g[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
+    res[j] = sqrt(gn.x * gn.x + gn.y * gn.y);
+}
+
 // g = (beta, 0, ...), beta = sqrt(*nrm2)
 __global__ void gmres_init_g_kernel(double2* __restrict__ g, int m, const double* __restrict__ nrm2)
 {
@@ -838,6 +921,14 @@ hipError_t launch_givens(double2* H, int m, int j, const double2* h, const doubl
                          double2* cs, double2* sn, double2* g, double* res, hipStream_t s)
 {
     hipLaunchKernelGGL(gmres_givens_kernel, dim3(1), dim3(kGivensChunk), 0, s, H, m, j, h, h2, wn2, cs, sn, g, res);
+    return hipGetLastError();
+}
+
+hipError_t launch_arnoldi_small(float2* V, int64_t n, int j, float2* w, double2* h, double2* h2, double2* H, int m,
+                                double2* cs, double2* sn, double2* g, double* res, double* wn2, hipStream_t s)
+{
+    hipLaunchKernelGGL(arnoldi_small_kernel, dim3(1), dim3(kGivensChunk), 0, s, V, n, j, w, h, h2, H, m, cs, sn, g,
+                       res, wn2);
     return hipGetLastError();
 }
 hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s)

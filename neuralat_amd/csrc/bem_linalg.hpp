@@ -23,6 +23,11 @@ hipError_t launch_dot(const float2* a, const float2* b, int64_t n, int conj, dou
 hipError_t launch_normalize(const float2* src, int64_t n, const double* nrm2, float2* out, hipStream_t s);
 hipError_t launch_givens(double2* H, int m, int j, const double2* h, const double2* h2, const double* wn2,
                          double2* cs, double2* sn, double2* g, double* res, hipStream_t s);
+// one Arnoldi step after w = A v_j (Gram-Schmidt twice, norm into *wn2,
+// v_{j+1}, Givens) in one workgroup, for systems of at most kArnoldiSmallN
+constexpr int64_t kArnoldiSmallN = 4096;
+hipError_t launch_arnoldi_small(float2* V, int64_t n, int j, float2* w, double2* h, double2* h2, double2* H, int m,
+                                double2* cs, double2* sn, double2* g, double* res, double* wn2, hipStream_t s);
 hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s);
 hipError_t launch_update_x(const float2* V, int64_t ldv, int64_t n, int nk, const double2* y, float2* x,
                            hipStream_t s);
