@@ -388,3 +388,47 @@ def test_single_triangle_mesh():
         A = s.assemble_boundary_matrix(4.0, "single").cpu().numpy()
         R = oracle.assemble(v, t, 4.0, "single", basis, clamp=False)
         assert _rel(A, R) < TOL
+
+
+@pytest.mark.parametrize("n,restart", [(300, 40), (300, 300), (5000, 60)])
+def test_gmres_windowed_residual_reads(n, restart):
+    """Small systems read the implicit residuals every few steps (the fused
+    one-workgroup step); the iterations, convergence and solution equal the
+    per-step reads of the timed path, also across restarts; larger systems
+    (5000 unknowns) take the multi-kernel step."""
+    from neuralat_amd.bem.solver import gmres
+
+    rng = np.random.default_rng(n + restart)
+    A = (np.eye(n) * (1.5 + 0.5j) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+         * (0.9 / np.sqrt(n))).astype(np.complex64)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    At, bt = torch.tensor(A, device="cuda"), torch.tensor(b, device="cuda")
+    x1, i1 = gmres(At, bt, tol=1e-6, restart=restart, maxiter=2000)
+    x2, i2 = gmres(At, bt, tol=1e-6, restart=restart, maxiter=2000, time_products=True)
+    assert i1["converged"] and i2["converged"]
+    assert i1["iterations"] == i2["iterations"], (i1, i2)
+    assert torch.allclose(x1, x2, rtol=0, atol=1e-6 * float(torch.linalg.vector_norm(x2)))
+    ref = np.linalg.solve(A.astype(np.complex128), b.astype(np.complex128))
+    assert np.linalg.norm(x1.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-5
+
+
+def test_gmres_small_maxiter_and_exact_convergence():
+    """maxiter inside a read window stops at maxiter (not the window's end);
+    an exactly solvable Krylov space (A = I + rank 2) converges in its first
+    steps, the breakdown steps after it dropped."""
+    from neuralat_amd.bem.solver import gmres
+
+    rng = np.random.default_rng(3)
+    n = 200
+    A = (np.eye(n) + 0.9 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+         ).astype(np.complex64)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    At, bt = torch.tensor(A, device="cuda"), torch.tensor(b, device="cuda")
+    _, info = gmres(At, bt, tol=1e-12, restart=50, maxiter=11)
+    assert not info["converged"] and info["iterations"] == 11, info
+    u = rng.standard_normal((n, 2)) + 1j * rng.standard_normal((n, 2))
+    B = (np.eye(n) + u @ u.conj().T / n).astype(np.complex64)
+    x, info = gmres(torch.tensor(B, device="cuda"), bt, tol=1e-6, restart=50, maxiter=100)
+    assert info["converged"] and info["iterations"] <= 4, info
+    ref = np.linalg.solve(B.astype(np.complex128), b.astype(np.complex128))
+    assert np.linalg.norm(x.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-5
