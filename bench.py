@@ -366,10 +366,13 @@ def precision_fp64_check(level, k, srcs, tol):
     return out
 
 
-def grad_form(ndir=4, seed=0):
+def grad_form(ndir=64, h=1e-3, seed=0):
     """Level-4 differentiation workload: loss = sum |p|^2 at 32 listeners from
     one interior monopole at k = 25; adjoint gradient w.r.t. all vertices and
-    k, backward/forward time, and central differences on a few directions."""
+    k, backward/forward time, and central differences along ``ndir`` seeded
+    directions scaled to a root-mean-square vertex displacement h (relative
+    to the unit radius), at h and h/2 with their Richardson combination
+    (4 D(h/2) - D(h)) / 3, which removes the O(h^2) truncation."""
     import torch
 
     from neuralat_amd import mesh
@@ -409,20 +412,26 @@ def grad_form(ndir=4, seed=0):
     L.backward()
     gv = vt.grad.double().cpu().numpy()
     dirs = rng.standard_normal((ndir,) + v.shape)
-    dirs /= np.linalg.norm(dirs.reshape(ndir, -1), axis=1)[:, None, None]
-    ad, fd = [], []
+    dirs *= np.sqrt(v.shape[0]) / np.linalg.norm(dirs.reshape(ndir, -1), axis=1)[:, None, None]
+    quot = {}
     with torch.no_grad():
-        for d in dirs:
-            dp = torch.tensor((v + 1e-3 * d).astype(np.float32), device=dev)
-            dm = torch.tensor((v - 1e-3 * d).astype(np.float32), device=dev)
-            fd.append(float(loss(dp, k0)) - float(loss(dm, k0)))
-            ad.append(float(np.sum(gv * (dp - dm).double().cpu().numpy())))
+        for hh in (h, 0.5 * h):
+            ad, fd = [], []
+            for d in dirs:
+                dp = torch.tensor((v + hh * d).astype(np.float32), device=dev)
+                dm = torch.tensor((v - hh * d).astype(np.float32), device=dev)
+                fd.append((float(loss(dp, k0)) - float(loss(dm, k0))) / (2 * hh))
+                ad.append(float(np.sum(gv * (dp - dm).double().cpu().numpy())) / (2 * hh))
+            quot[hh] = (np.array(ad), np.array(fd))
         hk = 1e-5 * k0
         fdk = (float(loss(vt.detach(), k0 + hk)) - float(loss(vt.detach(), k0 - hk))) / (2 * hk)
-    ad, fd = np.array(ad), np.array(fd)
+    (ad, fd), (ad2, fd2) = quot[h], quot[0.5 * h]
+    adr, fdr = (4 * ad2 - ad) / 3, (4 * fd2 - fd) / 3
     return {"workload": "icosphere L4 (5120 triangles) + SH noise, k=25, 32 listeners, loss sum |p|^2",
             "forward_ms": 1e3 * min(tf), "backward_ms": 1e3 * min(tb), "backward_over_forward": min(tb) / min(tf),
-            "fd_vertex_rel_diff": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)), "fd_directions": ndir,
+            "fd_directions": ndir, "fd_step_rms_per_vertex": h,
+            "fd_vertex_rel_diff": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
+            "fd_vertex_rel_diff_richardson": float(np.linalg.norm(adr - fdr) / np.linalg.norm(fdr)),
             "dL_dk_adjoint": float(kt.grad), "dL_dk_fd": fdk}
 
 

@@ -19,7 +19,8 @@ tetrahedra with numpy:
 The product (neuralat_amd/csrc/bem_fem.hip) forms the stiffness blocks from
 shape-function gradients in the isotropic lambda/mu form instead; the pins in
 tests/test_oracle_fem.py (rigid-body null space, total mass, the linear patch
-test, symmetry, a bar's longitudinal mode) fix this restatement.
+test, symmetry, agreement of ``modes`` with a dense generalized eigensolver)
+fix this restatement.
 """
 from __future__ import annotations
 

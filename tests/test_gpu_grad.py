@@ -1,10 +1,12 @@
 """GPU checks of the differentiation form (adjoint gradients of the DP0 CBIE
 exterior pressure with respect to vertex positions and k).
 
-* central finite differences of the GPU forward pass (h = 1e-3, relative to
-  the unit radius; h = 1e-5 k for k, see the test) along 64 seeded random directions, on the
-  level-4 icosphere with spherical-harmonic noise at k = 25, loss = sum |p|^2
-  at 32 listeners from one interior monopole;
+* central finite differences of the GPU forward pass along 64 seeded random
+  directions, each scaled so that the root-mean-square vertex displacement is
+  h = 1e-3 of the unit radius (h = 1e-5 k for k, see the test), and their
+  Richardson extrapolation from h and h/2, on the level-4 icosphere with
+  spherical-harmonic noise at k = 25, loss = sum |p|^2 at 32 listeners from
+  one interior monopole;
 * an FP64 reference: central differences (h = 1e-6) of the FP64 CPU oracle's
   pipeline (same quadrature, clamp and Neumann projection) on a level-2 mesh.
 """
@@ -42,10 +44,12 @@ def _loss_gpu(vt, tt, k, src, pts_t, tol=1e-7):
 
 
 def _ad_and_fd(level, k0, ndir, h, hk_rel, seed=0):
-    """Adjoint gradient and central differences.  Vertex steps are compared as
+    """Adjoint gradient and central differences at vertex steps h and h/2
+    (h = root-mean-square displacement per vertex).  Each step is compared as
     L(v+) - L(v-) against grad . (v+ - v-) with the float32 vertices actually
     used, so the rounding of the perturbed coordinates does not bias the
-    check."""
+    check; the quotients by the nominal step 2 h are what the Richardson
+    combination takes."""
     v, t, pts, src = _setup(level, seed)
     vt = torch.tensor(v, device=_dev(), requires_grad=True)
     tt = torch.tensor(t, device=_dev())
@@ -57,34 +61,44 @@ def _ad_and_fd(level, k0, ndir, h, hk_rel, seed=0):
     gk = float(kt.grad)
     rng = np.random.default_rng(seed + 20)
     dirs = rng.standard_normal((ndir,) + v.shape)
-    dirs /= np.linalg.norm(dirs.reshape(ndir, -1), axis=1)[:, None, None]
-    ad, fd = [], []
+    dirs *= np.sqrt(v.shape[0]) / np.linalg.norm(dirs.reshape(ndir, -1), axis=1)[:, None, None]
+    out = {}
     with torch.no_grad():
-        for d in dirs:
-            dp = torch.tensor((v + h * d).astype(np.float32), device=_dev())
-            dm = torch.tensor((v - h * d).astype(np.float32), device=_dev())
-            step = (dp - dm).double().cpu().numpy()
-            fd.append(float(_loss_gpu(dp, tt, k0, src, pts_t)) - float(_loss_gpu(dm, tt, k0, src, pts_t)))
-            ad.append(float(np.sum(gv * step)))
+        for hh in (h, 0.5 * h):
+            ad, fd = [], []
+            for d in dirs:
+                dp = torch.tensor((v + hh * d).astype(np.float32), device=_dev())
+                dm = torch.tensor((v - hh * d).astype(np.float32), device=_dev())
+                step = (dp - dm).double().cpu().numpy()
+                fd.append((float(_loss_gpu(dp, tt, k0, src, pts_t)) - float(_loss_gpu(dm, tt, k0, src, pts_t)))
+                          / (2 * hh))
+                ad.append(float(np.sum(gv * step)) / (2 * hh))
+            out[hh] = (np.array(ad), np.array(fd))
         hk = hk_rel * k0
         fdk = (float(_loss_gpu(vt.detach(), tt, k0 + hk, src, pts_t))
                - float(_loss_gpu(vt.detach(), tt, k0 - hk, src, pts_t))) / (2 * hk)
-    return np.array(ad), np.array(fd), gk, fdk, float(L)
+    return out[h], out[0.5 * h], gk, fdk, float(L)
 
 
 def test_gradient_vs_central_differences_level4():
-    # vertices: h = 1e-3 (relative to the unit radius).  k: at k R = 25 the
-    # loss varies on the scale of the interior-resonance spacing (~3e-4 k), so
-    # a 1e-3 k secant crosses resonances; the check uses 1e-5 k, where the
-    # central difference has converged.  The vertex differences carry the
-    # noise of two FP32 forward solves (assembly rounding plus the GMRES
-    # stopping point), ~1e-2 of L(v+) - L(v-) at this step; the tight check of
-    # the same gradient is the FP64 oracle test below.
-    ad, fd, gk, fdk, L = _ad_and_fd(4, 25.0, 64, 1e-3, 1e-5)
+    # vertices: h = 1e-3 root-mean-square displacement per vertex (relative
+    # to the unit radius).  At k R = 25 the central difference at this step
+    # still carries its O(h^2) truncation (~6e-3; ~4e-2 at h = 3e-3), which
+    # the Richardson combination (4 D(h/2) - D(h)) / 3 removes, leaving the
+    # noise of the FP32 forward solves (assembly rounding plus the GMRES
+    # stopping point, ~2e-4 of the h difference).  k: the loss varies on the
+    # scale of the interior-resonance spacing (~3e-4 k), so a 1e-3 k secant
+    # crosses resonances; the check uses 1e-5 k, where the central difference
+    # has converged.  The tight check of the same gradient is the FP64 oracle
+    # test below.
+    (ad, fd), (ad2, fd2), gk, fdk, L = _ad_and_fd(4, 25.0, 64, 1e-3, 1e-5)
     rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
-    print(f"L = {L:.4e}; 64 vertex directions: rel. diff AD vs FD = {rel:.2e}; "
+    fdr, adr = (4 * fd2 - fd) / 3, (4 * ad2 - ad) / 3
+    rel_r = np.linalg.norm(adr - fdr) / np.linalg.norm(fdr)
+    print(f"L = {L:.4e}; 64 vertex directions: rel. diff AD vs FD = {rel:.2e} (h), {rel_r:.2e} (Richardson); "
           f"dL/dk AD {gk:.5e} FD {fdk:.5e}", flush=True)
-    assert rel < 5e-2, rel
+    assert rel < 2e-2, rel
+    assert rel_r < 5e-3, rel_r
     assert abs(gk - fdk) / abs(fdk) < 3e-2
 
 
