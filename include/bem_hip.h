@@ -90,7 +90,14 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
 /* bem_assemble restricted to parts of the pair set: parts = 1 regular pairs
  * only (P1: also zero-fills out first), 2 neighbour (singular) pairs only
  * (added / written on top of out), 3 both (= bem_assemble).  Lets a caller
- * time or overlap the two kernels separately. */
+ * time or overlap the two kernels separately.  Adding 4 makes the P1 regular
+ * pairs one evaluation per ordered triangle pair; without it each unordered
+ * pair is evaluated once where that is faster (single layer and hypersingular
+ * by symmetry; double layer and adjoint double layer with quad 0, on meshes of
+ * at least 4096 vertices, for both orders at once, with an [nv, nv] complex
+ * scratch matrix taken from the
+ * stream-ordered pool and released before return; if none can be had, the
+ * ordered-pair sweep runs).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream);
 /* DP0 rows [row0, row1) only (out has row1-row0 rows); rows are independent,

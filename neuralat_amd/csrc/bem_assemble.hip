@@ -488,13 +488,34 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // vertices, column vertices) only; p1_symmetrize_kernel then adds the
 // transpose (flushing the transposed block with atomics as well scattered
 // them over every row of the matrix, and the atomics missed in cache).
-template <int OP, int Q, bool CLAMP, bool SYM>
+// PAIR (double layer and its adjoint): the same half sweep, and each
+// Gauss-point pair's exp(ikr)(1 - ikr)/r^3 serves both blocks of the
+// unordered pair: B(i, j) with the normal factor (x - y).n_y and B(j, i)^T
+// with (y - x).n_x.  B(i, j) goes through the LDS accumulator to out;
+// B(j, i)^T through the same stage and owner gather into 20 registers per
+// lane (one per row-tile vertex slot: the LDS has no room for a second
+// accumulator at 3 workgroups per CU; each slot is updated by a select on the
+// wave-uniform slot numbers, since a register array indexed by them would be
+// placed in scratch memory) and is flushed to the scratch matrix outT at
+// (row vertices, column vertices), as B(i, j) is; p1_add_transpose_kernel
+// then adds outT^T to out.  The adjoint double layer is the transpose of the
+// double layer, so it evaluates the double layer and swaps the two blocks.
+enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2 };
+
+template <int OP, int Q, bool CLAMP, int SWEEP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                          const float* __restrict__ curls, P1Tiles tl, float k,
-                                                         float2* __restrict__ out, int64_t ld)
+                                                         float2* __restrict__ out, int64_t ld,
+                                                         float2* __restrict__ outT, int64_t ldT)
 {
     static_assert(kP1ColTris == 256 && kP1ColVerts <= 256, "one lane per column triangle and per column vertex");
-    static_assert(!SYM || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
+    static_assert(SWEEP != kSweepSym || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
+    static_assert(SWEEP != kSweepPair || OP == DLP || OP == ADLP, "paired sweep operators");
+    constexpr bool HALF = SWEEP != kSweepAll;  // one evaluation per unordered pair
+    constexpr bool PAIR = SWEEP == kSweepPair;
+    // the evaluated operator: the paired adjoint double layer evaluates the
+    // double layer and swaps the blocks
+    constexpr int EOP = (PAIR && OP == ADLP) ? DLP : OP;
     constexpr int S = Rec<Q>::kFloats;
     __shared__ float2 acc[kP1RowVerts][kP1ColVerts];
     __shared__ float2 stage[9][kP1ColTris];
@@ -504,8 +525,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     __shared__ int4 rtri[kP1RowTris];
     __shared__ float rcurl[kP1RowTris][9];
     int ct = blockIdx.x, rt = blockIdx.y;
-    bool own = false;  // SYM: the column tile holds the row tile
-    if constexpr (SYM) {
+    bool own = false;  // half sweeps: the column tile holds the row tile
+    if constexpr (HALF) {
         const int64_t wi = blockIdx.x;
         int lo = 0, hi = tl.nrow_tiles - 1;  // last r with sym_ptr[r] <= wi
         while (lo < hi) {
@@ -557,34 +578,49 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const bool owner = w < ncv;
     const int ib = owner ? tl.col_inc_ptr[cv0 + w] : 0;
     const int ie = owner ? tl.col_inc_ptr[cv0 + w + 1] : 0;
+    float2 accT[PAIR ? kP1RowVerts : 1];  // PAIR: this lane's column of B(j, i)^T, by row slot
+#pragma unroll
+    for (int q = 0; q < (PAIR ? kP1RowVerts : 1); ++q) accT[q] = make_float2(0.f, 0.f);
     __syncthreads();
 
     for (int rp = r0; rp < r1; ++rp) {
         const int4 ti = rtri[rp - r0];
         const bool shared = ti.x == tj.x || ti.x == tj.y || ti.x == tj.z || ti.y == tj.x || ti.y == tj.y ||
-                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z || (SYM && own && pos <= rp);
+                            ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z || (HALF && own && pos <= rp);
         float br[9], bi[9];
+        float tr[PAIR ? 9 : 1], tii[PAIR ? 9 : 1];  // PAIR: B(j, i)^T
 #pragma unroll
         for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
+#pragma unroll
+        for (int e = 0; e < (PAIR ? 9 : 1); ++e) tr[e] = tii[e] = 0.f;
         if (active && !shared) {
             const float* rr = rrec[rp - r0];
             float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
             // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
             // triangle's plane, so it is formed once per column point
             float qb[Q], qbk[Q];
-            if constexpr (OP == ADLP || OP == BM_RHS) {
+            if constexpr (EOP == ADLP || EOP == BM_RHS) {
                 const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
 #pragma unroll
                 for (int b = 0; b < Q; ++b) {
                     const float yn = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
                                      (py[3 * b + 2] - rr[2]) * nx2;
-                    if constexpr (OP == ADLP) {
+                    if constexpr (EOP == ADLP) {
                         qb[b] = wy[b] * yn;
                     } else {  // BM_RHS: (x_0 - y_b).n_x and its 1/k multiple
                         qb[b] = -yn;
                         qbk[b] = -yn * invk;
                     }
                 }
+            }
+            // PAIR: the transposed block's normal factor (y_b - x_0).n_x per
+            // column point
+            float qt[PAIR ? Q : 1];
+            if constexpr (PAIR) {
+                const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+#pragma unroll
+                for (int b = 0; b < Q; ++b)
+                    qt[b] = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 + (py[3 * b + 2] - rr[2]) * nx2;
             }
             // BM_LHS: -k n_x.n_y once per pair; (x_a - y_0).n_y once per row point
             float mknn = 0.f;
@@ -594,6 +630,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             for (int a = 0; a < Q; ++a) {
                 const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
                 float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
+                float ur[3] = {0.f, 0.f, 0.f}, ui[3] = {0.f, 0.f, 0.f};  // PAIR: transposed sums
                 float tr9 = 0.f, ti9 = 0.f;
                 float mca = 0.f, kca = 0.f;
                 if constexpr (OP == BM_LHS) {
@@ -604,8 +641,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 #pragma unroll
                 for (int b = 0; b < Q; ++b) {
                     float vr, vi, gr = 0.f, gi = 0.f;
+                    float wr = 0.f, wi = 0.f;  // PAIR: the transposed block's value
                     const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
-                    if constexpr (OP == SLP || OP == DLP || OP == ADLP) {
+                    if constexpr (EOP == SLP || EOP == DLP || EOP == ADLP) {
                         // specialised evaluations (as the DP0 kernels); the DLP normal
                         // factor (x_a - y_0).n_y is applied per row point below
                         float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -614,15 +652,19 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                         const float r = r2 * rinv;
                         float sn, cs;
                         sincos_rev(r * krev, sn, cs);
-                        if constexpr (OP == SLP) {
+                        if constexpr (EOP == SLP) {
                             const float g = rinv * wy[b];
                             vr = g * cs;
                             vi = g * sn;
                         } else {
                             const float th = r * k;
-                            const float pw = rinv * rinv * (rinv * (OP == DLP ? wy[b] : qb[b]));
+                            const float pw = rinv * rinv * (rinv * (EOP == DLP ? wy[b] : qb[b]));
                             vr = pw * fmaf(th, sn, cs);
                             vi = pw * fmaf(-th, cs, sn);
+                            if constexpr (PAIR) {  // (y_b - x_0).n_x replaces (x_a - y_0).n_y
+                                wr = vr * qt[b];
+                                wi = vi * qt[b];
+                            }
                         }
                     } else if constexpr (OP == BM_LHS || OP == BM_RHS) {
                         // specialised Burton-Miller evaluations: value = G (er + i ei)
@@ -662,16 +704,32 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                     sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
                     sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
                     sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
+                    if constexpr (PAIR) {
+                        ur[0] = fmaf(wr, p0, ur[0]); ui[0] = fmaf(wi, p0, ui[0]);
+                        ur[1] = fmaf(wr, p1, ur[1]); ui[1] = fmaf(wi, p1, ui[1]);
+                        ur[2] = fmaf(wr, p2, ur[2]); ui[2] = fmaf(wi, p2, ui[2]);
+                    }
                     if constexpr (OP == BM_LHS) {
                         tr9 += gr;  // gr, gi: w_b G
                         ti9 += gi;
                     }
                 }
-                float ua = rr[3 * Q + a];
-                if constexpr (OP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
+                const float ua0 = rr[3 * Q + a];
+                float ua = ua0;
+                if constexpr (EOP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
                     ua *= (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
                 const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
                 const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
+                if constexpr (PAIR) {
+                    const float t0 = ua0 * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
+                    const float t1 = ua0 * TriRule<Q>::x1[a], t2 = ua0 * TriRule<Q>::x2[a];
+#pragma unroll
+                    for (int be = 0; be < 3; ++be) {
+                        tr[be] = fmaf(t0, ur[be], tr[be]); tii[be] = fmaf(t0, ui[be], tii[be]);
+                        tr[3 + be] = fmaf(t1, ur[be], tr[3 + be]); tii[3 + be] = fmaf(t1, ui[be], tii[3 + be]);
+                        tr[6 + be] = fmaf(t2, ur[be], tr[6 + be]); tii[6 + be] = fmaf(t2, ui[be], tii[6 + be]);
+                    }
+                }
 #pragma unroll
                 for (int be = 0; be < 3; ++be) {
                     br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
@@ -713,6 +771,16 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                     bi[e] = fmaf(s9i, cp[e], bi[e]);
                 }
             }
+            if constexpr (PAIR && OP == ADLP) {  // adjoint double layer = double layer transposed
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    const float xr = br[e], xi = bi[e];
+                    br[e] = tr[e];
+                    bi[e] = tii[e];
+                    tr[e] = xr;
+                    tii[e] = xi;
+                }
+            }
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
@@ -733,6 +801,31 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             acc[rs.z][w].x += s2.x; acc[rs.z][w].y += s2.y;
         }
         __syncthreads();
+        if constexpr (PAIR) {  // the transposed block through the same stage
+#pragma unroll
+            for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(tr[e], tii[e]);
+            __syncthreads();
+            const int4 rs = tl.row_slot[rp];
+            float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
+            if (owner) {
+                for (int q = ib; q < ie; ++q) {
+                    const int code = tl.col_inc[q];
+                    const int p = code >> 2, b = code & 3;
+                    const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
+                    s0.x += e0.x; s0.y += e0.y;
+                    s1.x += e1.x; s1.y += e1.y;
+                    s2.x += e2.x; s2.y += e2.y;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < kP1RowVerts; ++q) {
+                const float2 z = make_float2(0.f, 0.f);
+                const float2 add = q == rs.x ? s0 : q == rs.y ? s1 : q == rs.z ? s2 : z;
+                accT[q].x += add.x;
+                accT[q].y += add.y;
+            }
+            __syncthreads();
+        }
     }
     for (int e = threadIdx.x; e < nrv * ncv; e += 256) {
         const int rvi = e / ncv, cvi = e - rvi * ncv;
@@ -741,6 +834,20 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         const float2 v = acc[rvi][cvi];
         unsafeAtomicAdd(dst, v.x);
         unsafeAtomicAdd(dst + 1, v.y);
+    }
+    if constexpr (PAIR) {
+        if (owner) {
+            const int64_t gc = tl.col_verts[cv0 + w];
+#pragma unroll
+            for (int q = 0; q < kP1RowVerts; ++q) {
+                if (q < nrv) {
+                    float* dst = reinterpret_cast<float*>(&outT[(int64_t)tl.row_verts[rv0 + q] * ldT + gc]);
+                    const float2 v = accT[q];
+                    unsafeAtomicAdd(dst, v.x);
+                    unsafeAtomicAdd(dst + 1, v.y);
+                }
+            }
+        }
     }
 }
 
@@ -770,6 +877,32 @@ __global__ __launch_bounds__(256) void p1_symmetrize_kernel(float2* __restrict__
         if (I != J) {
             const float2 wb = b[rr][tx], wa = a[tx][rr];  // B(rr, tx) + A(tx, rr)
             if (rb < n && cb < n) out[rb * ld + cb] = make_float2(wb.x + wa.x, wb.y + wa.y);
+        }
+    }
+}
+
+// out += t^T on an n x n matrix (t: [n, n] with leading dimension n).  One
+// workgroup per 32 x 32 tile of out; the matching tile of t through LDS.
+__global__ __launch_bounds__(256) void p1_add_transpose_kernel(float2* __restrict__ out, int64_t ld,
+                                                               const float2* __restrict__ t, int64_t n, int64_t nb)
+{
+    __shared__ float2 b[kSymT][kSymT + 1];
+    const int64_t I = (int64_t)blockIdx.x / nb, J = (int64_t)blockIdx.x - I * nb;
+    const int64_t r0 = I * kSymT, c0 = J * kSymT;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
+    for (int rr = ty; rr < kSymT; rr += 8) {  // b(rr, tx) = t(c0 + rr, r0 + tx)
+        const int64_t tr = c0 + rr, tc = r0 + tx;
+        b[rr][tx] = (tr < n && tc < n) ? t[tr * n + tc] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    for (int rr = ty; rr < kSymT; rr += 8) {  // out(r0 + rr, c0 + tx) += t(c0 + tx, r0 + rr)
+        const int64_t ra = r0 + rr, ca = c0 + tx;
+        if (ra < n && ca < n) {
+            const float2 v = b[tx][rr];
+            float2 o = out[ra * ld + ca];
+            o.x += v.x;
+            o.y += v.y;
+            out[ra * ld + ca] = o;
         }
     }
 }
@@ -1150,21 +1283,48 @@ static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, fl
     return hipGetLastError();
 }
 
+// Half sweeps where they pay (single layer and hypersingular: symmetric;
+// double layer and its adjoint with the 6-point rule on meshes of at least
+// kP1PairMinVerts vertices: paired, with an [nv, nv] scratch matrix from the
+// stream-ordered pool -- with 9 Gauss-point pairs per triangle pair the
+// second stage pass costs more than the halved evaluations save, and on
+// smaller meshes the halved grid under-fills the GPU while the scratch fill
+// and transpose pass stay); one_sided, or no room for the scratch matrix,
+// evaluates every ordered pair.
+constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
-static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t ld, hipStream_t s)
+static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t ld, bool one_sided, hipStream_t s)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
-    if constexpr (OP == SLP || OP == HYP) {
-        const int64_t nb = (m.nv + kSymT - 1) / kSymT;
-        if (m.tiles.nsym > 0x7fffffffLL || nb * (nb + 1) / 2 > 0x7fffffffLL) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, true>), dim3((unsigned)m.tiles.nsym), dim3(256), 0, s, rec,
-                           m.tris, m.curls, m.tiles, k, out, ld);
-        hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, out, m.nv, ld);
-    } else {
-        dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
-        hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, false>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                           m.tiles, k, out, ld);
+    const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
+    constexpr bool sym = OP == SLP || OP == HYP;
+    constexpr bool pair = (OP == DLP || OP == ADLP) && Q == 6;
+    if (!one_sided && m.tiles.nsym <= 0x7fffffffLL && nb * nb <= 0x7fffffffLL) {
+        if constexpr (sym) {
+            hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym), dim3(256),
+                               0, s, rec, m.tris, m.curls, m.tiles, k, out, ld, nullptr, (int64_t)0);
+            hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, out, n, ld);
+            return hipGetLastError();
+        } else if constexpr (pair) {
+            float2* t = nullptr;
+            if (n >= kP1PairMinVerts && hipMallocAsync((void**)&t, sizeof(float2) * n * n, s) == hipSuccess) {
+                hipError_t e = launch_fill(t, n, n, n, make_float2(0.f, 0.f), s);
+                if (e == hipSuccess) {
+                    hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
+                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, k, out, ld, t, n);
+                    hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
+                                       (const float2*)t, n, nb);
+                    e = hipGetLastError();
+                }
+                const hipError_t ef = hipFreeAsync(t, s);
+                return e != hipSuccess ? e : ef;
+            }
+            (void)hipGetLastError();  // no room for the scratch matrix
+        }
     }
+    dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
+    hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
+                       m.tiles, k, out, ld, nullptr, (int64_t)0);
     return hipGetLastError();
 }
 
@@ -1231,7 +1391,8 @@ hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int p
                  : cbie_t<1, 2, false>(m, k, psi, lhs, ld, rhs, w, parts, s);
 }
 
-// parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour pairs
+// parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour
+// pairs, bit 2 = P1 regular pairs one evaluation per ordered pair
 template <int OP, bool CLAMP>
 static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, int64_t row0, int64_t row1, float k,
                               float2* out, int64_t ld, hipStream_t s)
@@ -1252,9 +1413,10 @@ static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, 
     if (parts & 1) {
         e = launch_fill(out, m.nv, m.nv, ld, make_float2(0.f, 0.f), s);
         if (e != hipSuccess) return e;
-        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, s);
-        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, s);
-        else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, s);
+        const bool one_sided = (parts & 4) != 0;
+        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, one_sided, s);
+        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, one_sided, s);
+        else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, one_sided, s);
         if (e != hipSuccess) return e;
     }
     if (!(parts & 2)) return hipSuccess;

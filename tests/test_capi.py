@@ -66,6 +66,21 @@ def test_status_mapping():
         _lib.check(_lib.BEM_ERR_HIP)
 
 
+def test_assemble_parts_validated_before_device_use():
+    """parts must be 1, 2 or 3, optionally plus 4 (P1 ordered-pair sweep);
+    the check precedes any pointer or device use."""
+    from neuralat_amd import _lib
+
+    lib = _lib.load()
+    for parts in (0, 4, 8, -1):
+        rc = lib.bem_assemble_ex(None, 1.0, 0, 0, 0, 1, parts, None, 1, None)
+        assert rc == _lib.BEM_ERR_BAD_ARG
+        assert b"parts" in lib.bem_last_error()
+    for parts in (1, 5, 7):  # accepted: the null mesh is the error reported
+        assert lib.bem_assemble_ex(None, 1.0, 0, 0, 0, 1, parts, None, 1, None) == _lib.BEM_ERR_BAD_ARG
+        assert b"null" in lib.bem_last_error()
+
+
 def test_fem_rejects_bad_meshes():
     """bem_fem_create validates on the host before any device call."""
     import ctypes
