@@ -100,6 +100,15 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
  * ordered-pair sweep runs).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream);
+/* bem_assemble at nk wavenumbers on one mesh (e.g. the modes of a vibrating
+ * object, each solved by neumann2dirichlet: scene.py:384-397 loops over them
+ * one call per operator): one launch per kernel for up to 64 wavenumbers.
+ * ks: host array [nk]; operator z is written at out + z * stride complex
+ * elements, [n, ld] each (stride >= n * ld).  Same results as nk calls of
+ * bem_assemble (the P1 double layer and its adjoint use the ordered-pair
+ * sweep here). */
+int bem_assemble_batch_k(const bem_mesh* mesh, const float* ks, int nk, int op, int basis, int quad, int clamp,
+                         void* out, int64_t ld, int64_t stride, void* stream);
 /* DP0 rows [row0, row1) only (out has row1-row0 rows); rows are independent,
  * so a large operator can be assembled block by block. */
 int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
@@ -135,6 +144,11 @@ int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert
  * (single/double_boundary_potential, bind.cu:156-176,219-222). */
 int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
                   void* out, int64_t ld, void* stream);
+/* bem_potential at nk wavenumbers (ks: host array [nk]) in one launch per
+ * kernel for up to 64 of them: matrix z at out + z * stride complex elements,
+ * [np, ld] each (stride >= np * ld). */
+int bem_potential_batch_k(const bem_mesh* mesh, const float* points, int64_t np, const float* ks, int nk, int op,
+                          int basis, int clamp, void* out, int64_t ld, int64_t stride, void* stream);
 
 /* y = alpha op(A) x + beta y, A complex64 [rows, cols] row-major, trans:
  * 0 = A, 1 = A^T, 2 = A^H.  work: device scratch of

@@ -64,6 +64,7 @@ SIGNATURES = {
     "bem_mesh_normals_curls": (_I32, [_P, _P, _P, _P]),
     "bem_assemble": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_assemble_ex": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_assemble_batch_k": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_cbie_work_size": (_I64, [_P]),
     "bem_cbie_dp0": (_I32, [_P, _F, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
     "bem_cbie_dp0_ex": (_I32, [_P, _F, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
@@ -73,6 +74,7 @@ SIGNATURES = {
     "bem_identity": (_I32, [_P, _I32, _P, _I64, _P]),
     "bem_triangle2vertex": (_I32, [_P, _P, _P, _P]),
     "bem_potential": (_I32, [_P, _P, _I64, _F, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_potential_batch_k": (_I32, [_P, _P, _I64, _P, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_matvec_work_size": (_I64, [_I64, _I64, _I32]),
     "bem_matvec": (_I32, [_P, _I64, _I64, _I64, _I32, _P, _P, _F, _F, _F, _F, _P, _P]),
     "bem_combine": (_I32, [_P, _I64, _P, _I64, _P, _I64, _I64, _I64, _F, _F, _F, _F, _P]),

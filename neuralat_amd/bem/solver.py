@@ -243,6 +243,23 @@ class BEM_Solver:
                                     _lib.QUADS[quad], int(clamp), _ptr(out), n, _stream()), "bem_assemble")
         return out
 
+    def assemble_boundary_matrix_batch(self, wavenumbers, layer_type, approx=False, basis=None, clamp=None):
+        """The operator at every wavenumber of ``wavenumbers`` (a sequence of
+        floats), complex64 [len, n, n]: one launch per kernel for up to 64 of
+        them (the reference assembles one wavenumber per call)."""
+        basis = basis or self.basis
+        clamp = self.clamp if clamp is None else clamp
+        if layer_type not in _lib.OPS:
+            raise ValueError(f"unknown layer type {layer_type!r}")
+        quad = "approx_1" if approx == "approx_1" else ("approx" if approx else "full")
+        ks = np.ascontiguousarray(np.asarray(wavenumbers, dtype=np.float32).reshape(-1))
+        n = self._n(basis)
+        out = torch.empty((len(ks), n, n), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_assemble_batch_k(self.mesh.handle, ks.ctypes.data, len(ks), _lib.OPS[layer_type],
+                                                    _lib.BASES[basis], _lib.QUADS[quad], int(clamp), _ptr(out), n,
+                                                    n * n, _stream()), "bem_assemble_batch_k")
+        return out
+
     def identity_matrix(self, basis=None):
         basis = basis or self.basis
         n = self._n(basis)
@@ -325,6 +342,20 @@ class BEM_Solver:
         _lib.check(_lib.load().bem_potential(self.mesh.handle, _ptr(points), points.shape[0], float(k),
                                              _lib.OPS[layer_type], _lib.BASES[basis], int(clamp), _ptr(out), n,
                                              _stream()), "bem_potential")
+        return out
+
+    def potential_matrix_batch(self, wavenumbers, points, layer_type, basis=None, clamp=None):
+        """potential_matrix at every wavenumber: complex64 [len, np, n]."""
+        basis = basis or self.basis
+        clamp = self.clamp if clamp is None else clamp
+        points = points.to(torch.float32).contiguous()
+        check_tensor(points, torch.float32)
+        ks = np.ascontiguousarray(np.asarray(wavenumbers, dtype=np.float32).reshape(-1))
+        n, npts = self._n(basis), points.shape[0]
+        out = torch.empty((len(ks), npts, n), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_potential_batch_k(self.mesh.handle, _ptr(points), npts, ks.ctypes.data, len(ks),
+                                                     _lib.OPS[layer_type], _lib.BASES[basis], int(clamp), _ptr(out), n,
+                                                     npts * n, _stream()), "bem_potential_batch_k")
         return out
 
     def single_potential(self, k, neumann, points):

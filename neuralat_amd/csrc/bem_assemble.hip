@@ -22,8 +22,9 @@ namespace bem {
 // utility kernels
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void fill_c64_kernel(float2* __restrict__ out, int64_t rows, int64_t cols,
-                                                       int64_t ld, float2 val)
+                                                       int64_t ld, float2 val, int64_t zstride)
 {
+    out += (int64_t)blockIdx.z * zstride;
     int64_t total = rows * cols;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         int64_t r = e / cols, c = e - r * cols;
@@ -38,9 +39,11 @@ __global__ __launch_bounds__(256) void fill_c64_kernel(float2* __restrict__ out,
 // (rows [row0, row1) of the operator).  out row r holds triangle row0 + r.
 template <int OP, int Q, bool CLAMP>
 __global__ __launch_bounds__(256) void dp0_regular_kernel(const float* __restrict__ rec, int64_t nt, int64_t row0,
-                                                          int64_t row1, float k, float2* __restrict__ out,
+                                                          int64_t row1, const KBatch kb, float2* __restrict__ out,
                                                           int64_t ld)
 {
+    const float k = kb.k[blockIdx.z];
+    out += (int64_t)blockIdx.z * kb.zstride;
     constexpr int S = Rec<Q>::kFloats;
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool active = j < nt;
@@ -504,10 +507,12 @@ enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2 };
 
 template <int OP, int Q, bool CLAMP, int SWEEP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
-                                                         const float* __restrict__ curls, P1Tiles tl, float k,
-                                                         float2* __restrict__ out, int64_t ld,
+                                                         const float* __restrict__ curls, P1Tiles tl,
+                                                         const KBatch kb, float2* __restrict__ out, int64_t ld,
                                                          float2* __restrict__ outT, int64_t ldT)
 {
+    const float k = kb.k[blockIdx.z];
+    out += (int64_t)blockIdx.z * kb.zstride;
     static_assert(kP1ColTris == 256 && kP1ColVerts <= 256, "one lane per column triangle and per column vertex");
     static_assert(SWEEP != kSweepSym || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
     static_assert(SWEEP != kSweepPair || OP == DLP || OP == ADLP, "paired sweep operators");
@@ -856,8 +861,10 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 // workgroup per 32 x 32 tile pair (I <= J): both tiles through LDS, written
 // back as A_IJ + A_JI^T and its transpose.
 constexpr int kSymT = 32;
-__global__ __launch_bounds__(256) void p1_symmetrize_kernel(float2* __restrict__ out, int64_t n, int64_t ld)
+__global__ __launch_bounds__(256) void p1_symmetrize_kernel(float2* __restrict__ out, int64_t n, int64_t ld,
+                                                            int64_t zstride)
 {
+    out += (int64_t)blockIdx.z * zstride;
     __shared__ float2 a[kSymT][kSymT + 1], b[kSymT][kSymT + 1];
     int64_t I, J;
     tri_block_pair(blockIdx.x, I, J);
@@ -923,10 +930,12 @@ template <int OP, int NL, int BASIS, bool CLAMP, int MODE = 0>
 __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__ geo, const int4* __restrict__ tris,
                                                        const float* __restrict__ curls,
                                                        const int4* __restrict__ pairs, int64_t p0, int64_t p1,
-                                                       float k, float2* __restrict__ out, int64_t ld, int64_t row0,
-                                                       float alpha, const float2* __restrict__ psi,
+                                                       const KBatch kb, float2* __restrict__ out, int64_t ld,
+                                                       int64_t row0, float alpha, const float2* __restrict__ psi,
                                                        float2* __restrict__ vpsi)
 {
+    const float k = kb.k[blockIdx.z];
+    out += (int64_t)blockIdx.z * kb.zstride;
     const int lane = threadIdx.x & 63;
     const int64_t p = p0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (p >= p1) return;  // uniform per wavefront
@@ -1195,11 +1204,17 @@ __global__ __launch_bounds__(256) void tri2vert_kernel(const float2* __restrict_
 // Surface -> point operator (bind.cu:156-176, integrand.h:237-255,396-406):
 // row p = evaluation point (n_x = 0), column = triangle (DP0) or its three
 // vertices (P1, summed with atomics into a zeroed output).
-template <int OP, int BASIS, bool CLAMP>
+// P1 with GATHER: the three corner values of each (point, triangle) pair go
+// to a scratch array [np, nt, 3] (out, zstride per wavenumber) for
+// p1_point_gather_kernel; without it they are added to out [np, ld] with
+// float atomics (six per pair).
+template <int OP, int BASIS, bool CLAMP, bool GATHER = false>
 __global__ __launch_bounds__(256) void potential_kernel(const float* __restrict__ rec6, const int4* __restrict__ tris,
-                                                        int64_t nt, const float* __restrict__ pts, int64_t np, float k,
-                                                        float2* __restrict__ out, int64_t ld)
+                                                        int64_t nt, const float* __restrict__ pts, int64_t np,
+                                                        const KBatch kb, float2* __restrict__ out, int64_t ld)
 {
+    const float k = kb.k[blockIdx.z];
+    out += (int64_t)blockIdx.z * kb.zstride;
     constexpr int Q = 6;
     constexpr int S = Rec<Q>::kFloats;
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1242,6 +1257,11 @@ __global__ __launch_bounds__(256) void potential_kernel(const float* __restrict_
         if (!active) continue;
         if constexpr (BASIS == DP0) {
             out[p * ld + j] = make_float2(tr, ti);
+        } else if constexpr (GATHER) {
+            float2* dst = out + (p * nt + j) * 3;
+            dst[0] = make_float2(s[0][0], s[0][1]);
+            dst[1] = make_float2(s[1][0], s[1][1]);
+            dst[2] = make_float2(s[2][0], s[2][1]);
         } else {
             const int vs[3] = {tj.x, tj.y, tj.z};
 #pragma unroll
@@ -1265,21 +1285,29 @@ static inline unsigned grid1d(int64_t n, int64_t per = 256, int64_t cap = 1 << 2
     return (unsigned)g;
 }
 
-hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s)
+// nz matrices at out + z * zstride
+static hipError_t launch_fill_z(float2* out, int64_t rows, int64_t cols, int64_t ld, int64_t zstride, int nz,
+                                float2 val, hipStream_t s)
 {
-    if (rows <= 0 || cols <= 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_c64_kernel, dim3(grid1d(rows * cols, 256, 65536)), dim3(256), 0, s, out, rows, cols, ld,
-                       val);
+    if (rows <= 0 || cols <= 0 || nz <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_c64_kernel, dim3(grid1d(rows * cols, 256, 65536), 1, (unsigned)nz), dim3(256), 0, s,
+                       out, rows, cols, ld, val, zstride);
     return hipGetLastError();
 }
 
+hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s)
+{
+    return launch_fill_z(out, rows, cols, ld, 0, 1, val, s);
+}
+
 template <int OP, int Q, bool CLAMP>
-static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, float k, float2* out, int64_t ld,
-                                hipStream_t s)
+static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, const KBatch& kb, int nz, float2* out,
+                                int64_t ld, hipStream_t s)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
-    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((row1 - row0 + kDp0RowTile - 1) / kDp0RowTile));
-    hipLaunchKernelGGL((dp0_regular_kernel<OP, Q, CLAMP>), grid, dim3(256), 0, s, rec, m.nt, row0, row1, k, out, ld);
+    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((row1 - row0 + kDp0RowTile - 1) / kDp0RowTile),
+              (unsigned)nz);
+    hipLaunchKernelGGL((dp0_regular_kernel<OP, Q, CLAMP>), grid, dim3(256), 0, s, rec, m.nt, row0, row1, kb, out, ld);
     return hipGetLastError();
 }
 
@@ -1290,10 +1318,12 @@ static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, fl
 // second stage pass costs more than the halved evaluations save, and on
 // smaller meshes the halved grid under-fills the GPU while the scratch fill
 // and transpose pass stay); one_sided, or no room for the scratch matrix,
-// evaluates every ordered pair.
+// evaluates every ordered pair.  A batch of wavenumbers (nz > 1) is not
+// paired (it would need one scratch matrix per wavenumber).
 constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
-static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t ld, bool one_sided, hipStream_t s)
+static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float2* out, int64_t ld, bool one_sided,
+                               hipStream_t s)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
     const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
@@ -1301,17 +1331,18 @@ static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t l
     constexpr bool pair = (OP == DLP || OP == ADLP) && Q == 6;
     if (!one_sided && m.tiles.nsym <= 0x7fffffffLL && nb * nb <= 0x7fffffffLL) {
         if constexpr (sym) {
-            hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym), dim3(256),
-                               0, s, rec, m.tris, m.curls, m.tiles, k, out, ld, nullptr, (int64_t)0);
-            hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2)), dim3(256), 0, s, out, n, ld);
+            hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym, 1, nz),
+                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0);
+            hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2), 1, nz), dim3(256), 0, s, out,
+                               n, ld, kb.zstride);
             return hipGetLastError();
         } else if constexpr (pair) {
             float2* t = nullptr;
-            if (n >= kP1PairMinVerts && hipMallocAsync((void**)&t, sizeof(float2) * n * n, s) == hipSuccess) {
+            if (nz == 1 && n >= kP1PairMinVerts && hipMallocAsync((void**)&t, sizeof(float2) * n * n, s) == hipSuccess) {
                 hipError_t e = launch_fill(t, n, n, n, make_float2(0.f, 0.f), s);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
-                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, k, out, ld, t, n);
+                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n);
                     hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
                                        (const float2*)t, n, nb);
                     e = hipGetLastError();
@@ -1322,20 +1353,20 @@ static hipError_t p1_regular_t(const MeshDev& m, float k, float2* out, int64_t l
             (void)hipGetLastError();  // no room for the scratch matrix
         }
     }
-    dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles);
+    dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles, (unsigned)nz);
     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                       m.tiles, k, out, ld, nullptr, (int64_t)0);
+                       m.tiles, kb, out, ld, nullptr, (int64_t)0);
     return hipGetLastError();
 }
 
 template <int OP, int NL, int BASIS, bool CLAMP>
-static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, float k, float2* out, int64_t ld,
-                             int64_t row0, hipStream_t s)
+static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, const KBatch& kb, int nz, float2* out,
+                             int64_t ld, int64_t row0, hipStream_t s)
 {
     if (p1 <= p0) return hipSuccess;
     unsigned g = (unsigned)((p1 - p0 + 3) / 4);
-    hipLaunchKernelGGL((singular_kernel<OP, NL, BASIS, CLAMP>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
-                       m.pairs, p0, p1, k, out, ld, row0, 1.0f, nullptr, nullptr);
+    hipLaunchKernelGGL((singular_kernel<OP, NL, BASIS, CLAMP>), dim3(g, 1, nz), dim3(256), 0, s, m.geo, m.tris,
+                       m.curls, m.pairs, p0, p1, kb, out, ld, row0, 1.0f, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -1372,9 +1403,9 @@ static hipError_t cbie_t(const MeshDev& m, float k, const float2* psi, float2* l
     if (!(parts & 2)) return hipGetLastError();
     const unsigned g = (unsigned)((m.npairs + 3) / 4);
     hipLaunchKernelGGL((singular_kernel<DLP, NL, DP0, CLAMP, 0>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
-                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, -1.0f, nullptr, nullptr);
+                       m.pairs, (int64_t)0, m.npairs, kbatch1(k), lhs, ld, (int64_t)0, -1.0f, nullptr, nullptr);
     hipLaunchKernelGGL((singular_kernel<SLP, NL, DP0, CLAMP, 1>), dim3(g), dim3(256), 0, s, m.geo, m.tris, m.curls,
-                       m.pairs, (int64_t)0, m.npairs, k, lhs, ld, (int64_t)0, 1.0f, psi, w.vpsi);
+                       m.pairs, (int64_t)0, m.npairs, kbatch1(k), lhs, ld, (int64_t)0, 1.0f, psi, w.vpsi);
     hipLaunchKernelGGL(cbie_finalize_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, w.prow, w.pcol, w.nsub, w.vpsi,
                        m.pair_ptr, m.geo, m.nt, lhs, ld, rhs);
     return hipGetLastError();
@@ -1392,45 +1423,46 @@ hipError_t launch_cbie_dp0(const MeshDev& m, float k, int quad, int clamp, int p
 }
 
 // parts: bit 0 = regular pairs (and, for P1, the zero fill), bit 1 = neighbour
-// pairs, bit 2 = P1 regular pairs one evaluation per ordered pair
+// pairs, bit 2 = P1 regular pairs one evaluation per ordered pair; nz
+// wavenumbers of kb at out + z * kb.zstride
 template <int OP, bool CLAMP>
-static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, int64_t row0, int64_t row1, float k,
-                              float2* out, int64_t ld, hipStream_t s)
+static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, int64_t row0, int64_t row1,
+                              const KBatch& kb, int nz, float2* out, int64_t ld, hipStream_t s)
 {
     hipError_t e = hipSuccess;
     if (basis == DP0) {
         if (parts & 1) {
-            if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, k, out, ld, s);
-            else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, k, out, ld, s);
-            else e = dp0_regular_t<OP, 1, CLAMP>(m, row0, row1, k, out, ld, s);
+            if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, kb, nz, out, ld, s);
+            else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, kb, nz, out, ld, s);
+            else e = dp0_regular_t<OP, 1, CLAMP>(m, row0, row1, kb, nz, out, ld, s);
             if (e != hipSuccess) return e;
         }
         if (!(parts & 2)) return hipSuccess;
         const int64_t p0 = m.h_pair_ptr[row0], p1 = m.h_pair_ptr[row1];
-        return (quad == 0) ? singular_t<OP, 4, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s)
-                           : singular_t<OP, 2, DP0, CLAMP>(m, p0, p1, k, out, ld, row0, s);
+        return (quad == 0) ? singular_t<OP, 4, DP0, CLAMP>(m, p0, p1, kb, nz, out, ld, row0, s)
+                           : singular_t<OP, 2, DP0, CLAMP>(m, p0, p1, kb, nz, out, ld, row0, s);
     }
     if (parts & 1) {
-        e = launch_fill(out, m.nv, m.nv, ld, make_float2(0.f, 0.f), s);
+        e = launch_fill_z(out, m.nv, m.nv, ld, kb.zstride, nz, make_float2(0.f, 0.f), s);
         if (e != hipSuccess) return e;
         const bool one_sided = (parts & 4) != 0;
-        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, k, out, ld, one_sided, s);
-        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, k, out, ld, one_sided, s);
-        else e = p1_regular_t<OP, 1, CLAMP>(m, k, out, ld, one_sided, s);
+        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, kb, nz, out, ld, one_sided, s);
+        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, kb, nz, out, ld, one_sided, s);
+        else e = p1_regular_t<OP, 1, CLAMP>(m, kb, nz, out, ld, one_sided, s);
         if (e != hipSuccess) return e;
     }
     if (!(parts & 2)) return hipSuccess;
-    return (quad == 0) ? singular_t<OP, 4, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s)
-                       : singular_t<OP, 2, P1, CLAMP>(m, 0, m.npairs, k, out, ld, 0, s);
+    return (quad == 0) ? singular_t<OP, 4, P1, CLAMP>(m, 0, m.npairs, kb, nz, out, ld, 0, s)
+                       : singular_t<OP, 2, P1, CLAMP>(m, 0, m.npairs, kb, nz, out, ld, 0, s);
 }
 
-hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
-                           int64_t row1, float k, float2* out, int64_t ld, hipStream_t s)
+static hipError_t assemble_dispatch(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
+                                    int64_t row1, const KBatch& kb, int nz, float2* out, int64_t ld, hipStream_t s)
 {
-#define BEM_OP_CASE(OPV)                                                                             \
-    case OPV:                                                                                        \
-        return clamp ? assemble_op<OPV, true>(m, basis, quad, parts, row0, row1, k, out, ld, s)    \
-                     : assemble_op<OPV, false>(m, basis, quad, parts, row0, row1, k, out, ld, s);
+#define BEM_OP_CASE(OPV)                                                                                     \
+    case OPV:                                                                                                \
+        return clamp ? assemble_op<OPV, true>(m, basis, quad, parts, row0, row1, kb, nz, out, ld, s)       \
+                     : assemble_op<OPV, false>(m, basis, quad, parts, row0, row1, kb, nz, out, ld, s);
     switch (op) {
         BEM_OP_CASE(SLP)
         BEM_OP_CASE(DLP)
@@ -1442,6 +1474,28 @@ hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int cl
         return hipErrorInvalidValue;
     }
 #undef BEM_OP_CASE
+}
+
+hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
+                           int64_t row1, float k, float2* out, int64_t ld, hipStream_t s)
+{
+    return assemble_dispatch(m, op, basis, quad, clamp, parts, row0, row1, kbatch1(k), 1, out, ld, s);
+}
+
+hipError_t launch_assemble_batch_k(const MeshDev& m, int op, int basis, int quad, int clamp, const float* ks, int nk,
+                                   float2* out, int64_t ld, int64_t stride, hipStream_t s)
+{
+    const int64_t n = basis == DP0 ? m.nt : m.nv;
+    for (int z0 = 0; z0 < nk; z0 += kMaxBatchK) {
+        const int nz = nk - z0 < kMaxBatchK ? nk - z0 : kMaxBatchK;
+        KBatch kb{};
+        for (int z = 0; z < nz; ++z) kb.k[z] = ks[z0 + z];
+        kb.zstride = stride;
+        const hipError_t e =
+            assemble_dispatch(m, op, basis, quad, clamp, 3, 0, n, kb, nz, out + (int64_t)z0 * stride, ld, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s)
@@ -1508,37 +1562,116 @@ hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipS
     return hipGetLastError();
 }
 
-template <int OP, int BASIS, bool CLAMP>
-static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, float k, float2* out, int64_t ld,
-                              hipStream_t s)
+// out[p, v] = sum of the corner values of v's triangles for point p (the
+// P1 potential from the per-(point, triangle) scratch of potential_kernel):
+// one thread per (point, vertex), every entry written once (no fill, no
+// atomics, a fixed summation order).
+__global__ __launch_bounds__(256) void p1_point_gather_kernel(const float2* __restrict__ t, const int4* __restrict__ tris,
+                                                              const int32_t* __restrict__ vt_ptr,
+                                                              const int32_t* __restrict__ vt_idx, int64_t nv,
+                                                              int64_t nt, int64_t np, int64_t tzstride,
+                                                              float2* __restrict__ out, int64_t ld, int64_t zstride)
 {
-    if (BASIS == P1) {
-        hipError_t e = launch_fill(out, np, m.nv, ld, make_float2(0.f, 0.f), s);
+    t += (int64_t)blockIdx.z * tzstride;
+    out += (int64_t)blockIdx.z * zstride;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < np * nv; e += (int64_t)gridDim.x * 256) {
+        const int64_t p = e / nv, v = e - p * nv;
+        const float2* row = t + p * nt * 3;
+        float sr = 0.f, si = 0.f;
+        for (int q = vt_ptr[v]; q < vt_ptr[v + 1]; ++q) {
+            const int tri = vt_idx[q];
+            const int4 tv = tris[tri];
+            const int c = tv.x == v ? 0 : (tv.y == v ? 1 : 2);
+            const float2 x = row[(int64_t)tri * 3 + c];
+            sr += x.x;
+            si += x.y;
+        }
+        out[p * ld + v] = make_float2(sr, si);
+    }
+}
+
+// Scratch for the P1 point gather, per launch: at most this many bytes
+// (points are taken in chunks that fit).
+constexpr int64_t kPotScratchBytes = (int64_t)512 << 20;
+
+template <int OP, int BASIS, bool CLAMP>
+static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, const KBatch& kb, int nz, float2* out,
+                              int64_t ld, hipStream_t s)
+{
+    if constexpr (BASIS == P1) {
+        // two passes through a scratch array of point chunks; the atomic
+        // kernel only if no scratch can be had
+        const int64_t per_point = (int64_t)sizeof(float2) * 3 * m.nt * nz;
+        int64_t pc = kPotScratchBytes / per_point;
+        pc = pc < np ? pc : np;
+        pc = pc < kPotRowTile ? kPotRowTile : pc;
+        float2* t = nullptr;
+        if (np > 0 && hipMallocAsync((void**)&t, (size_t)(per_point * pc), s) == hipSuccess) {
+            hipError_t e = hipSuccess;
+            for (int64_t c0 = 0; c0 < np && e == hipSuccess; c0 += pc) {
+                const int64_t c1 = c0 + pc < np ? c0 + pc : np, cn = c1 - c0;
+                KBatch kt = kb;
+                kt.zstride = cn * m.nt * 3;
+                dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((cn + kPotRowTile - 1) / kPotRowTile),
+                          (unsigned)nz);
+                hipLaunchKernelGGL((potential_kernel<OP, P1, CLAMP, true>), grid, dim3(256), 0, s, m.rec6, m.tris,
+                                   m.nt, pts + 3 * c0, cn, kt, t, (int64_t)0);
+                hipLaunchKernelGGL(p1_point_gather_kernel, dim3(grid1d(cn * m.nv, 256, 65536), 1, (unsigned)nz),
+                                   dim3(256), 0, s, (const float2*)t, m.tris, m.vt_ptr, m.vt_idx, m.nv, m.nt, cn,
+                                   kt.zstride, out + c0 * ld, ld, kb.zstride);
+                e = hipGetLastError();
+            }
+            const hipError_t ef = hipFreeAsync(t, s);
+            return e != hipSuccess ? e : ef;
+        }
+        (void)hipGetLastError();
+        hipError_t e = launch_fill_z(out, np, m.nv, ld, kb.zstride, nz, make_float2(0.f, 0.f), s);
         if (e != hipSuccess) return e;
     }
     if (np <= 0) return hipSuccess;
-    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((np + kPotRowTile - 1) / kPotRowTile));
-    hipLaunchKernelGGL((potential_kernel<OP, BASIS, CLAMP>), grid, dim3(256), 0, s, m.rec6, m.tris, m.nt, pts, np, k,
+    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((np + kPotRowTile - 1) / kPotRowTile), (unsigned)nz);
+    hipLaunchKernelGGL((potential_kernel<OP, BASIS, CLAMP>), grid, dim3(256), 0, s, m.rec6, m.tris, m.nt, pts, np, kb,
                        out, ld);
     return hipGetLastError();
+}
+
+static hipError_t potential_dispatch(const MeshDev& m, const float* pts, int64_t np, const KBatch& kb, int nz, int op,
+                                     int basis, int clamp, float2* out, int64_t ld, hipStream_t s)
+{
+    if (op == SLP) {
+        if (basis == DP0) return clamp ? potential_t<SLP, DP0, true>(m, pts, np, kb, nz, out, ld, s)
+                                       : potential_t<SLP, DP0, false>(m, pts, np, kb, nz, out, ld, s);
+        return clamp ? potential_t<SLP, P1, true>(m, pts, np, kb, nz, out, ld, s)
+                     : potential_t<SLP, P1, false>(m, pts, np, kb, nz, out, ld, s);
+    }
+    if (op == DLP) {
+        if (basis == DP0) return clamp ? potential_t<DLP, DP0, true>(m, pts, np, kb, nz, out, ld, s)
+                                       : potential_t<DLP, DP0, false>(m, pts, np, kb, nz, out, ld, s);
+        return clamp ? potential_t<DLP, P1, true>(m, pts, np, kb, nz, out, ld, s)
+                     : potential_t<DLP, P1, false>(m, pts, np, kb, nz, out, ld, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, float k, int op, int basis, int clamp,
                             float2* out, int64_t ld, hipStream_t s)
 {
-    if (op == SLP) {
-        if (basis == DP0) return clamp ? potential_t<SLP, DP0, true>(m, pts, np, k, out, ld, s)
-                                       : potential_t<SLP, DP0, false>(m, pts, np, k, out, ld, s);
-        return clamp ? potential_t<SLP, P1, true>(m, pts, np, k, out, ld, s)
-                     : potential_t<SLP, P1, false>(m, pts, np, k, out, ld, s);
+    return potential_dispatch(m, pts, np, kbatch1(k), 1, op, basis, clamp, out, ld, s);
+}
+
+hipError_t launch_potential_batch_k(const MeshDev& m, const float* pts, int64_t np, const float* ks, int nk, int op,
+                                    int basis, int clamp, float2* out, int64_t ld, int64_t stride, hipStream_t s)
+{
+    for (int z0 = 0; z0 < nk; z0 += kMaxBatchK) {
+        const int nz = nk - z0 < kMaxBatchK ? nk - z0 : kMaxBatchK;
+        KBatch kb{};
+        for (int z = 0; z < nz; ++z) kb.k[z] = ks[z0 + z];
+        kb.zstride = stride;
+        const hipError_t e =
+            potential_dispatch(m, pts, np, kb, nz, op, basis, clamp, out + (int64_t)z0 * stride, ld, s);
+        if (e != hipSuccess) return e;
     }
-    if (op == DLP) {
-        if (basis == DP0) return clamp ? potential_t<DLP, DP0, true>(m, pts, np, k, out, ld, s)
-                                       : potential_t<DLP, DP0, false>(m, pts, np, k, out, ld, s);
-        return clamp ? potential_t<DLP, P1, true>(m, pts, np, k, out, ld, s)
-                     : potential_t<DLP, P1, false>(m, pts, np, k, out, ld, s);
-    }
-    return hipErrorInvalidValue;
+    return hipSuccess;
 }
 
 }  // namespace bem

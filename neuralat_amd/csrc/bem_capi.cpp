@@ -207,6 +207,35 @@ int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, 
     return BEM_OK;
 }
 
+static int check_ks(const float* ks, int nk, int op)
+{
+    if (!ks || nk < 1) return fail(BEM_ERR_BAD_ARG, "need at least one wavenumber");
+    for (int z = 0; z < nk; ++z) {
+        if (!std::isfinite(ks[z])) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+        if ((op == BEM_OP_BM_LHS || op == BEM_OP_BM_RHS) && ks[z] == 0.f)
+            return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
+    }
+    return BEM_OK;
+}
+
+int bem_assemble_batch_k(const bem_mesh* mesh, const float* ks, int nk, int op, int basis, int quad, int clamp,
+                         void* out, int64_t ld, int64_t stride, void* stream)
+{
+    if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (int rc = check_op(op, basis, quad)) return rc;
+    if (int rc = check_ks(ks, nk, op)) return rc;
+    const auto& m = mesh->dev;
+    const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
+    if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (nk > 1 && stride < n * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (n * ld)");
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_assemble_batch_k(m, op, basis, quad, clamp, ks, nk, (float2*)out, ld, stride,
+                                                S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_assemble_batch_k launch");
+    return BEM_OK;
+}
+
 int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
                       void* out, int64_t ld, void* stream)
 {
@@ -355,6 +384,26 @@ int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k
 int64_t bem_matvec_work_size(int64_t rows, int64_t cols, int trans)
 {
     return (int64_t)sizeof(float2) * bem::gemv_work_elems(rows, cols, trans);
+}
+
+int bem_potential_batch_k(const bem_mesh* mesh, const float* points, int64_t np, const float* ks, int nk, int op,
+                          int basis, int clamp, void* out, int64_t ld, int64_t stride, void* stream)
+{
+    if (!mesh || !out || (np > 0 && !points)) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (op != BEM_OP_SINGLE && op != BEM_OP_DOUBLE)
+        return fail(BEM_ERR_BAD_ARG, "potential operators are SINGLE and DOUBLE");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    if (int rc = check_ks(ks, nk, op)) return rc;
+    const auto& m = mesh->dev;
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
+    if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    if (nk > 1 && stride < np * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (np * ld)");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_potential_batch_k(m, points, np, ks, nk, op, basis, clamp, (float2*)out, ld, stride,
+                                                 S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_potential_batch_k launch");
+    return BEM_OK;
 }
 
 int bem_matvec(const void* A, int64_t rows, int64_t cols, int64_t lda, int trans, const void* x, void* y,

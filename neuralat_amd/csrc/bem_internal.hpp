@@ -71,9 +71,29 @@ struct MeshDev {
     void* block = nullptr;  // the single device allocation
 };
 
+// Wavenumbers of one launch over blockIdx.z: operator z of a batch on one
+// mesh uses k[z] and is written at out + z * zstride (elements).  Passed by
+// value (kernel arguments), at most kMaxBatchK per launch.
+constexpr int kMaxBatchK = 64;
+struct KBatch {
+    float k[kMaxBatchK];
+    int64_t zstride;
+};
+inline KBatch kbatch1(float k)
+{
+    KBatch b{};
+    b.k[0] = k;
+    b.zstride = 0;
+    return b;
+}
+
 hipError_t launch_fill(float2* out, int64_t rows, int64_t cols, int64_t ld, float2 val, hipStream_t s);
 hipError_t launch_assemble(const MeshDev& m, int op, int basis, int quad, int clamp, int parts, int64_t row0,
                            int64_t row1, float k, float2* out, int64_t ld, hipStream_t s);
+// nk operators of one kind on one mesh (operator z at out + z * stride,
+// [n, ld] each), one launch per kernel for up to kMaxBatchK wavenumbers.
+hipError_t launch_assemble_batch_k(const MeshDev& m, int op, int basis, int quad, int clamp, const float* ks, int nk,
+                                   float2* out, int64_t ld, int64_t stride, hipStream_t s);
 // One (mesh, k) item of a batched fused-CBIE assembly (device-resident table).
 struct BatchItemDev {
     const float* rec;   // 6-point records of the mesh
@@ -115,6 +135,8 @@ hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld,
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
 hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, float k, int op, int basis, int clamp,
                             float2* out, int64_t ld, hipStream_t s);
+hipError_t launch_potential_batch_k(const MeshDev& m, const float* pts, int64_t np, const float* ks, int nk, int op,
+                                    int basis, int clamp, float2* out, int64_t ld, int64_t stride, hipStream_t s);
 
 // Builds all device data from host copies of the mesh; returns 0 or a
 // bem_status code with a message in `err`.

@@ -176,6 +176,23 @@ def burton_miller_system(solver, k, neumann):
     return lhs, rhs
 
 
+def burton_miller_systems(solver, ks, neumanns):
+    """burton_miller_system at every wavenumber of ``ks`` on one surface
+    (``neumanns``: [len(ks), n]): lhs [K, n, n], rhs [K, n].  Each operator kind
+    is assembled for all wavenumbers in one launch per kernel
+    (BEM_Solver.assemble_boundary_matrix_batch); the identity terms and the
+    right-hand-side products are batched tensor operations."""
+    ks = [float(k) for k in ks]
+    identity = solver.identity_matrix()
+    lhs = solver.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
+    lhs.add_(identity, alpha=0.5)
+    R = solver.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
+    half_beta = torch.tensor([0.5j / k for k in ks], dtype=torch.complex64, device=R.device)
+    R.sub_(half_beta[:, None, None] * identity)
+    rhs = torch.matmul(R, neumanns.to(torch.complex64).reshape(len(ks), -1, 1)).squeeze(-1)
+    return lhs, rhs
+
+
 def modal_preprocess(obj, mode_num):
     """Surface data of a vibrating solid after ``obj.modal_analysis`` (a
     modal.VibrationObj): per-vertex Neumann data of each mode (the per-triangle
@@ -263,12 +280,17 @@ class EditableModalSound:
         the modes' Burton-Miller systems (one per wavenumber on the same
         surface) solved together by the lockstep batched GMRES."""
         solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
-        systems = [burton_miller_system(solver, float(self.ks[i]), self.neumann_vtx[i]) for i in range(self.mode_num)]
-        xs, infos = batch_gmres([q[0] for q in systems], [q[1] for q in systems], tol=tol, restart=restart,
-                                max_cycles=max_cycles)
+        ks = [float(k) for k in self.ks]
+        neumann = torch.as_tensor(self.neumann_vtx, device=solver.device).to(torch.complex64).reshape(len(ks), -1)
+        lhs, rhs = burton_miller_systems(solver, ks, neumann)
+        xs, infos = batch_gmres(list(lhs), list(rhs), tol=tol, restart=restart, max_cycles=max_cycles)
         self.solve_infos = infos
-        self.potentials = torch.stack([solver.boundary2potential(float(self.ks[i]), self.neumann_vtx[i], xs[i],
-                                                                 self.trg_points) for i in range(self.mode_num)]).cpu()
+        # p = D phi - S psi for every mode (solver.py:140-147), the potential
+        # operators of all modes in one launch each
+        S = solver.potential_matrix_batch(ks, self.trg_points, "single")
+        D = solver.potential_matrix_batch(ks, self.trg_points, "double")
+        phi = torch.stack(xs)[:, :, None]
+        self.potentials = (torch.matmul(D, phi) - torch.matmul(S, neumann[:, :, None])).squeeze(-1).cpu()
         return self.potentials
 
     def solve_loop(self):
@@ -305,5 +327,5 @@ def save_dataset(path, x, y):
     torch.save({"x": x, "y": y}, path)
 
 
-__all__ = ["SceneObject", "SceneSampler", "burton_miller_system", "save_dataset", "modal_preprocess",
-           "EditableModalSound", "generate_modal_dataset"]
+__all__ = ["SceneObject", "SceneSampler", "burton_miller_system", "burton_miller_systems", "save_dataset",
+           "modal_preprocess", "EditableModalSound", "generate_modal_dataset"]

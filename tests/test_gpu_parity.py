@@ -165,6 +165,61 @@ def test_p1_half_sweeps_match_ordered_pair_sweep(op, quad):
     assert e < 1e-6, (op, quad, e)
 
 
+@pytest.mark.parametrize("basis,op,quad", [("P1", "single", 0), ("P1", "double", 0), ("P1", "hypersingular", 1),
+                                           ("P1", "bm_lhs", 1), ("P1", "bm_rhs", 1), ("DP0", "single", 0),
+                                           ("DP0", "adjointdouble", 1)])
+def test_assemble_batch_k_matches_single_calls(basis, op, quad):
+    """bem_assemble_batch_k (grid z over the wavenumbers) against one
+    bem_assemble per wavenumber, with ld > n and a stride larger than one
+    matrix so that the offsets are exercised (level 3: 642 vertices, 1280
+    triangles)."""
+    from neuralat_amd import _lib
+    from neuralat_amd.bem.solver import _ptr, _stream
+
+    v, t = mesh.icosphere(3, 0.1)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=2).astype(np.float32)
+    s = _solver(v, t, basis)
+    n = len(v) if basis == "P1" else len(t)
+    ld, stride = n + 3, (n + 5) * (n + 3)
+    ks = np.array([3.0, 18.3, -7.5, 40.0], dtype=np.float32)
+    lib = _lib.load()
+    _say("batch_k", basis, op, quad)
+    buf = torch.zeros(len(ks) * stride, dtype=torch.complex64, device=_dev())
+    _lib.check(lib.bem_assemble_batch_k(s.mesh.handle, ks.ctypes.data, len(ks), _lib.OPS[op], _lib.BASES[basis], quad,
+                                        1, _ptr(buf), ld, stride, _stream()), "bem_assemble_batch_k")
+    for z, k in enumerate(ks):
+        one = torch.empty((n, n), dtype=torch.complex64, device=_dev())
+        _lib.check(lib.bem_assemble(s.mesh.handle, float(k), _lib.OPS[op], _lib.BASES[basis], quad, 1, _ptr(one), n,
+                                    _stream()), "bem_assemble")
+        got = buf[z * stride: z * stride + n * ld].reshape(n, ld)[:, :n]
+        e = float(torch.linalg.matrix_norm(got - one) / torch.linalg.matrix_norm(one))
+        assert e < 1e-6, (basis, op, quad, float(k), e)
+        pad = buf[z * stride: (z + 1) * stride]
+        assert float(pad[n * ld:].abs().max()) == 0.0  # nothing written past the matrix
+
+
+def test_assemble_and_potential_batch_k_over_64_wavenumbers():
+    """More wavenumbers than one launch takes (64): the host splits them; the
+    potential operators of every wavenumber match single calls."""
+    from neuralat_amd import _lib
+    from neuralat_amd.bem.solver import _ptr, _stream
+
+    v, t = mesh.icosphere(1, 0.1)
+    v = v.astype(np.float32)
+    s = _solver(v, t, "P1")
+    n = len(v)
+    ks = np.linspace(1.0, 60.0, 70).astype(np.float32)
+    _say("batch_k 70 wavenumbers")
+    A = s.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
+    pts = torch.tensor(mesh.random_exterior_points(9, 0.2, 0.5, seed=3).astype(np.float32), device=_dev())
+    Pd = s.potential_matrix_batch(ks, pts, "double")
+    for z in (0, 31, 63, 64, 69):
+        one = s.assemble_boundary_matrix(float(ks[z]), "bm_lhs", approx=True)
+        assert float(torch.linalg.matrix_norm(A[z] - one) / torch.linalg.matrix_norm(one)) < 1e-6, z
+        pd = s.potential_matrix(float(ks[z]), pts, "double")
+        assert float(torch.linalg.matrix_norm(Pd[z] - pd) / torch.linalg.matrix_norm(pd)) < 1e-6, z
+
+
 @pytest.mark.parametrize("op", ["single", "double", "adjointdouble", "bm_rhs"])
 @pytest.mark.parametrize("clamp", [True, False])
 def test_dp0_operators_vs_oracle(op, clamp):
