@@ -75,6 +75,15 @@ const char* bem_last_error(void);
 int bem_mesh_create(const float* verts, int64_t nv, const int32_t* tris, int64_t nt, int on_device, void* stream,
                     bem_mesh** out);
 int bem_mesh_destroy(bem_mesh* mesh);
+/* New vertex positions [nv,3] for a mesh handle with the same triangles
+ * (rigid or deformed objects in data generation: scene.py:218-291 builds a
+ * new BEM_Solver per sample): the quadrature records, normals, curls and
+ * geometry are recomputed and uploaded in one copy on `stream`, which is
+ * waited for; neighbour pairs, vertex lists and the P1 tiling are kept.
+ * Kernels still reading the old positions on other streams must be done
+ * first.  Results equal those of a new handle built from the positions (P1
+ * sums in another order: the tiling is the first positions'). */
+int bem_mesh_set_vertices(bem_mesh* mesh, const float* verts, int on_device, void* stream);
 int bem_mesh_info(const bem_mesh* mesh, int64_t* nv, int64_t* nt, int64_t* n_neighbor_pairs,
                   int32_t* max_neighbors);
 /* unit normals [nt,3] and P1 surface curls [nt,3,3] (row a = n x grad phi_a)

@@ -74,6 +74,7 @@ SIGNATURES = {
     "bem_identity": (_I32, [_P, _I32, _P, _I64, _P]),
     "bem_triangle2vertex": (_I32, [_P, _P, _P, _P]),
     "bem_potential": (_I32, [_P, _P, _I64, _F, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_mesh_set_vertices": (_I32, [_P, _P, _I32, _P]),
     "bem_potential_batch_k": (_I32, [_P, _P, _I64, _P, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_matvec_work_size": (_I64, [_I64, _I64, _I32]),
     "bem_matvec": (_I32, [_P, _I64, _I64, _I64, _I32, _P, _P, _F, _F, _F, _F, _P, _P]),

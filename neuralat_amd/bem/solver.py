@@ -66,6 +66,14 @@ class Mesh:
     def handle(self):
         return self._h
 
+    def set_vertices(self, vertices: torch.Tensor):
+        """New positions [V,3] for the same triangles (bem_mesh_set_vertices)."""
+        check_tensor(vertices, torch.float32)
+        if tuple(vertices.shape) != (self.nv, 3):
+            raise ValueError(f"vertices must be [{self.nv},3]")
+        vertices = vertices.contiguous()
+        _lib.check(self._lib.bem_mesh_set_vertices(self._h, _ptr(vertices), 1, _stream()), "bem_mesh_set_vertices")
+
     def normals_curls(self):
         n = torch.empty((self.nt, 3), dtype=torch.float32, device=self.device)
         c = torch.empty((self.nt, 3, 3), dtype=torch.float32, device=self.device)
@@ -223,6 +231,15 @@ class BEM_Solver:
         self.mesh = Mesh(vertices, triangles)
         self.normals, self.surface_curls_trans = self.mesh.normals_curls()
         self.last_solve_info = None
+
+    def set_vertices(self, vertices):
+        """Move the surface (same triangles) without rebuilding the mesh
+        handle: the next operators use the new positions."""
+        if isinstance(vertices, np.ndarray):
+            vertices = torch.from_numpy(vertices).to(self.device).float()
+        self.mesh.set_vertices(vertices)
+        self.vertices = vertices
+        self.normals, self.surface_curls_trans = self.mesh.normals_curls()
 
     # -- operators ------------------------------------------------------------
     def _n(self, basis):

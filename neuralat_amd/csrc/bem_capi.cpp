@@ -140,6 +140,26 @@ int bem_mesh_create(const float* verts, int64_t nv, const int32_t* tris, int64_t
     return BEM_OK;
 }
 
+int bem_mesh_set_vertices(bem_mesh* mesh, const float* verts, int on_device, void* stream)
+{
+    if (!mesh || !verts) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    auto& m = mesh->dev;
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    std::vector<float> hv;
+    const float* pv = verts;
+    if (on_device) {
+        hv.resize((size_t)m.nv * 3);
+        HIP_TRY(hipMemcpyAsync(hv.data(), verts, hv.size() * 4, hipMemcpyDeviceToHost, S(stream)));
+        HIP_TRY(hipStreamSynchronize(S(stream)));
+        pv = hv.data();
+    }
+    std::string err;
+    const int rc = bem::mesh_set_vertices(m, pv, S(stream), err);
+    if (rc != 0) return fail(rc == 3 ? BEM_ERR_HIP : (rc == 1 ? BEM_ERR_BAD_ARG : BEM_ERR_BAD_MESH), err);
+    return BEM_OK;
+}
+
 int bem_mesh_destroy(bem_mesh* mesh)
 {
     if (!mesh) return BEM_OK;

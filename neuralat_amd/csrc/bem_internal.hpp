@@ -68,6 +68,8 @@ struct MeshDev {
     std::vector<int64_t> h_pair_ptr;  // [nt+1] host copy: pair range of each row triangle
     int32_t max_neighbors = 0;
     double diameter = 0.0;  // bounding-box diagonal
+    std::vector<int32_t> h_tris;  // [nt*3] host copy (for mesh_set_vertices)
+    size_t geom_bytes = 0;  // the position-dependent arrays: the first geom_bytes of block
     void* block = nullptr;  // the single device allocation
 };
 
@@ -141,6 +143,10 @@ hipError_t launch_potential_batch_k(const MeshDev& m, const float* pts, int64_t 
 // Builds all device data from host copies of the mesh; returns 0 or a
 // bem_status code with a message in `err`.
 int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, MeshDev& m, std::string& err);
+// New vertex positions for a built mesh (same triangles): recomputes the
+// position-dependent arrays on the host and uploads them in one copy on s,
+// then waits for it.  Connectivity data and the tiling are kept.
+int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::string& err);
 void mesh_free(MeshDev& m);
 
 }  // namespace bem

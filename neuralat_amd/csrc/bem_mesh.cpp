@@ -113,41 +113,37 @@ Tiling make_tiling(const std::vector<int32_t>& perm, const int32_t* tris, int ma
 
 }  // namespace
 
-int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, MeshDev& m, std::string& err)
+// Position-dependent host data of a mesh: quadrature records, P1 curls,
+// singular-kernel geometry, centroids, bounding-box diagonal (FP64, rounded
+// once).  Fails on non-finite coordinates and zero-area triangles.
+struct HostGeom {
+    std::vector<float> rec6, rec3, rec1, curls, geo;
+    std::vector<V3> cent;
+    V3 lo{0.0, 0.0, 0.0}, hi{0.0, 0.0, 0.0};  // bounding box
+    double diameter = 0.0;
+};
+
+static int build_geometry(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, HostGeom& g,
+                          std::string& err)
 {
-    if (nv < 3 || nt < 1) {
-        err = "mesh needs at least 3 vertices and 1 triangle";
-        return 1;
-    }
-    if (nt > (int64_t)1 << 30 || nv > (int64_t)1 << 30) {
-        err = "mesh too large (more than 2^30 vertices or triangles)";
-        return 1;
-    }
-    for (int64_t t = 0; t < nt; ++t) {
-        const int32_t* tr = h_tris + 3 * t;
-        for (int a = 0; a < 3; ++a)
-            if (tr[a] < 0 || tr[a] >= nv) {
-                err = "triangle " + std::to_string(t) + " has a vertex index out of range";
-                return 2;
-            }
-        if (tr[0] == tr[1] || tr[1] == tr[2] || tr[0] == tr[2]) {
-            err = "triangle " + std::to_string(t) + " repeats a vertex";
-            return 2;
-        }
-    }
     for (int64_t v = 0; v < 3 * nv; ++v)
         if (!std::isfinite(h_verts[v])) {
             err = "non-finite vertex coordinate";
             return 2;
         }
-    m.nv = nv;
-    m.nt = nt;
-
-    // ---- per-triangle geometry in double ----
     constexpr int S6 = Rec<6>::kFloats, S3 = Rec<3>::kFloats, S1 = Rec<1>::kFloats;
-    std::vector<float> rec6((size_t)nt * S6, 0.f), rec3((size_t)nt * S3, 0.f), rec1((size_t)nt * S1, 0.f);
-    std::vector<float> curls((size_t)nt * 12, 0.f), geo((size_t)nt * 16, 0.f);
-    std::vector<int32_t> tris4((size_t)nt * 4, 0);
+    g.rec6.assign((size_t)nt * S6, 0.f);
+    g.rec3.assign((size_t)nt * S3, 0.f);
+    g.rec1.assign((size_t)nt * S1, 0.f);
+    g.curls.assign((size_t)nt * 12, 0.f);
+    g.geo.assign((size_t)nt * 16, 0.f);
+    g.cent.assign((size_t)nt, V3{0.0, 0.0, 0.0});
+    auto& rec6 = g.rec6;
+    auto& rec3 = g.rec3;
+    auto& rec1 = g.rec1;
+    auto& curls = g.curls;
+    auto& geo = g.geo;
+    auto& cent = g.cent;
     const HostRule R6 = host_rule(6), R3 = host_rule(3), R1 = host_rule(1);
     V3 lo{1e300, 1e300, 1e300}, hi{-1e300, -1e300, -1e300};
     for (int64_t v = 0; v < nv; ++v) {
@@ -158,14 +154,14 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         lo.z = std::min(lo.z, (double)h_verts[3 * v + 2]);
         hi.z = std::max(hi.z, (double)h_verts[3 * v + 2]);
     }
-    m.diameter = std::sqrt(dot(sub(hi, lo), sub(hi, lo)));
-    std::vector<V3> cent((size_t)nt);
+    g.diameter = std::sqrt(dot(sub(hi, lo), sub(hi, lo)));
+    g.lo = lo;
+    g.hi = hi;
     for (int64_t t = 0; t < nt; ++t) {
         const int32_t* tr = h_tris + 3 * t;
         V3 v[3];
         for (int a = 0; a < 3; ++a) {
             v[a] = {h_verts[3 * (int64_t)tr[a]], h_verts[3 * (int64_t)tr[a] + 1], h_verts[3 * (int64_t)tr[a] + 2]};
-            tris4[4 * t + a] = tr[a];
         }
         const V3 e1 = sub(v[1], v[0]), e2 = sub(v[2], v[0]);
         const V3 c = cross(e1, e2);
@@ -204,17 +200,60 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
             curls[(size_t)t * 12 + 3 * a + 1] = (float)cu.y;
             curls[(size_t)t * 12 + 3 * a + 2] = (float)cu.z;
         }
-        float* g = &geo[(size_t)t * 16];
+        float* gr = &geo[(size_t)t * 16];
         for (int a = 0; a < 3; ++a) {
-            g[3 * a] = (float)v[a].x;
-            g[3 * a + 1] = (float)v[a].y;
-            g[3 * a + 2] = (float)v[a].z;
+            gr[3 * a] = (float)v[a].x;
+            gr[3 * a + 1] = (float)v[a].y;
+            gr[3 * a + 2] = (float)v[a].z;
         }
-        g[9] = (float)n.x;
-        g[10] = (float)n.y;
-        g[11] = (float)n.z;
-        g[12] = (float)J;
+        gr[9] = (float)n.x;
+        gr[10] = (float)n.y;
+        gr[11] = (float)n.z;
+        gr[12] = (float)J;
     }
+
+    return 0;
+}
+
+int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t nt, MeshDev& m, std::string& err)
+{
+    if (nv < 3 || nt < 1) {
+        err = "mesh needs at least 3 vertices and 1 triangle";
+        return 1;
+    }
+    if (nt > (int64_t)1 << 30 || nv > (int64_t)1 << 30) {
+        err = "mesh too large (more than 2^30 vertices or triangles)";
+        return 1;
+    }
+    for (int64_t t = 0; t < nt; ++t) {
+        const int32_t* tr = h_tris + 3 * t;
+        for (int a = 0; a < 3; ++a)
+            if (tr[a] < 0 || tr[a] >= nv) {
+                err = "triangle " + std::to_string(t) + " has a vertex index out of range";
+                return 2;
+            }
+        if (tr[0] == tr[1] || tr[1] == tr[2] || tr[0] == tr[2]) {
+            err = "triangle " + std::to_string(t) + " repeats a vertex";
+            return 2;
+        }
+    }
+    m.nv = nv;
+    m.nt = nt;
+
+    HostGeom hg;
+    if (int rc = build_geometry(h_verts, nv, h_tris, nt, hg, err)) return rc;
+    m.diameter = hg.diameter;
+    const auto& rec6 = hg.rec6;
+    const auto& rec3 = hg.rec3;
+    const auto& rec1 = hg.rec1;
+    const auto& curls = hg.curls;
+    const auto& geo = hg.geo;
+    const auto& cent = hg.cent;
+    const V3 lo = hg.lo, hi = hg.hi;
+    std::vector<int32_t> tris4((size_t)nt * 4, 0);
+    for (int64_t t = 0; t < nt; ++t)
+        for (int a = 0; a < 3; ++a) tris4[4 * t + a] = h_tris[3 * t + a];
+    m.h_tris.assign(h_tris, h_tris + 3 * nt);
 
     // ---- vertex -> triangle CSR ----
     std::vector<int32_t> vt_ptr((size_t)nv + 1, 0), vt_idx((size_t)nt * 3);
@@ -319,13 +358,16 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     std::vector<Part> parts;
     float* d_verts = nullptr;
     std::vector<float> vcopy(h_verts, h_verts + 3 * nv);
+    // the position-dependent arrays first: mesh_set_vertices rewrites them as
+    // one contiguous region
     parts.push_back({vcopy.data(), vcopy.size() * 4, (void**)&d_verts});
-    parts.push_back({tris4.data(), tris4.size() * 4, (void**)&m.tris});
     parts.push_back({rec6.data(), rec6.size() * 4, (void**)&m.rec6});
     parts.push_back({rec3.data(), rec3.size() * 4, (void**)&m.rec3});
     parts.push_back({rec1.data(), rec1.size() * 4, (void**)&m.rec1});
     parts.push_back({curls.data(), curls.size() * 4, (void**)&m.curls});
     parts.push_back({geo.data(), geo.size() * 4, (void**)&m.geo});
+    constexpr size_t kGeomParts = 6;
+    parts.push_back({tris4.data(), tris4.size() * 4, (void**)&m.tris});
     parts.push_back({pairs.data(), pairs.size() * 4, (void**)&m.pairs});
     parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
@@ -352,6 +394,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         off.push_back(total);
         total += (p.bytes + 255) / 256 * 256;
     }
+    m.geom_bytes = off[kGeomParts];
     if (hipGetDevice(&m.device) != hipSuccess) {
         err = "no HIP device";
         return 5;
@@ -361,15 +404,18 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         (void)hipGetLastError();
         return 6;
     }
+    // every array through one host staging buffer and one copy (a copy per
+    // array cost more than the rest of the set-up for small meshes)
+    std::vector<char> stage(total);
     for (size_t q = 0; q < parts.size(); ++q) {
-        char* dst = (char*)m.block + off[q];
-        *parts[q].dst = dst;
-        if (parts[q].bytes && hipMemcpy(dst, parts[q].src, parts[q].bytes, hipMemcpyHostToDevice) != hipSuccess) {
-            err = "host to device copy failed";
-            (void)hipFree(m.block);
-            m.block = nullptr;
-            return 3;
-        }
+        *parts[q].dst = (char*)m.block + off[q];
+        if (parts[q].bytes) std::memcpy(stage.data() + off[q], parts[q].src, parts[q].bytes);
+    }
+    if (hipMemcpy(m.block, stage.data(), total, hipMemcpyHostToDevice) != hipSuccess) {
+        err = "host to device copy failed";
+        (void)hipFree(m.block);
+        m.block = nullptr;
+        return 3;
     }
     m.verts = d_verts;
     m.tiles.perm = d_perm;
@@ -388,6 +434,42 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     m.tiles.nsym = sym_ptr.back();
     m.tiles.nrow_tiles = nrowt;
     m.tiles.ncol_tiles = ncolt;
+    return 0;
+}
+
+int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::string& err)
+{
+    if (!m.block || m.h_tris.size() != (size_t)m.nt * 3) {
+        err = "mesh has no device data";
+        return 1;
+    }
+    HostGeom hg;
+    if (int rc = build_geometry(h_verts, m.nv, m.h_tris.data(), m.nt, hg, err)) return rc;
+    // the same 256-byte aligned layout as mesh_build's first six parts
+    const std::pair<const float*, size_t> sec[6] = {{h_verts, (size_t)m.nv * 3},   {hg.rec6.data(), hg.rec6.size()},
+                                                    {hg.rec3.data(), hg.rec3.size()}, {hg.rec1.data(), hg.rec1.size()},
+                                                    {hg.curls.data(), hg.curls.size()}, {hg.geo.data(), hg.geo.size()}};
+    std::vector<char> stage(m.geom_bytes, 0);
+    size_t off = 0;
+    for (const auto& q : sec) {
+        const size_t bytes = q.second * 4;
+        if (off + bytes > stage.size()) {
+            err = "geometry layout mismatch";
+            return 1;
+        }
+        std::memcpy(stage.data() + off, q.first, bytes);
+        off += (bytes + 255) / 256 * 256;
+    }
+    if (off != m.geom_bytes) {
+        err = "geometry layout mismatch";
+        return 1;
+    }
+    if (hipMemcpyAsync(m.block, stage.data(), stage.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        err = "host to device copy failed";
+        return 3;
+    }
+    m.diameter = hg.diameter;
     return 0;
 }
 

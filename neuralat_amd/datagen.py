@@ -145,23 +145,30 @@ class SceneSampler:
         x = torch.zeros((S, T, self.feature_dim), dtype=torch.float32)
         y = torch.zeros((S, T, 1), dtype=torch.float32)
         infos = []
+        pool = []  # one solver per batch slot: every sample has the same triangles, only positions move
         for s0 in range(0, S, batch):
             facs = [self.sample_factors(rng) for _ in range(min(batch, S - s0))]
             systems = []
-            for f in facs:
+            for q, f in enumerate(facs):
                 v, t, neu, k, pts = self.build(f)
-                solver = BEM_Solver(torch.tensor(v.astype(np.float32), device=dev),
-                                    torch.tensor(t, device=dev), basis="P1")
+                vt = torch.tensor(v.astype(np.float32), device=dev)
+                if q < len(pool):
+                    solver = pool[q]
+                    solver.set_vertices(vt)
+                else:
+                    solver = BEM_Solver(vt, torch.tensor(t, device=dev), basis="P1")
+                    pool.append(solver)
                 lhs, rhs = burton_miller_system(solver, k, torch.tensor(neu, device=dev))
                 systems.append((solver, k, torch.tensor(neu, device=dev), lhs, rhs,
                                 torch.tensor(pts.astype(np.float32), device=dev)))
             xs, inf = batch_gmres([q[3] for q in systems], [q[4] for q in systems], tol=tol, restart=restart,
                                   max_cycles=max_cycles)
             infos.extend(inf)
-            for q, (f, (solver, k, neu, _, _, pts)) in enumerate(zip(facs, systems)):
-                p = solver.boundary2potential(k, neu, xs[q], pts)
+            ps = [solver.boundary2potential(k, neu, xs[q], pts).abs()
+                  for q, (solver, k, neu, _, _, pts) in enumerate(systems)]
+            y[s0:s0 + len(facs), :, 0] = torch.stack(ps).float().cpu()  # one copy per batch
+            for q, f in enumerate(facs):
                 x[s0 + q] = torch.from_numpy(self.features(f))
-                y[s0 + q, :, 0] = p.abs().float().cpu()
         return x, y, infos
 
 

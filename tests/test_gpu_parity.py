@@ -220,6 +220,48 @@ def test_assemble_and_potential_batch_k_over_64_wavenumbers():
         assert float(torch.linalg.matrix_norm(Pd[z] - pd) / torch.linalg.matrix_norm(pd)) < 1e-6, z
 
 
+def test_set_vertices_equals_new_handle():
+    """A handle moved with bem_mesh_set_vertices assembles what a handle built
+    from the new positions assembles (P1 and DP0 operators, potentials,
+    normals); invalid positions are rejected and leave the handle as it was."""
+    from neuralat_amd.bem import BEM_Solver
+
+    v, t = mesh.icosphere(3, 0.1)
+    v0 = mesh.sh_perturb(v, 4, 0.05, seed=5).astype(np.float32)
+    rot = np.array([[0.6, -0.8, 0.0], [0.8, 0.6, 0.0], [0.0, 0.0, 1.0]])
+    v1 = (mesh.anisotropic_scale(v0, np.array([1.4, 0.7, 1.1])) @ rot.T + np.array([0.3, -0.1, 0.2]))
+    v1 = v1.astype(np.float32)
+    tt = torch.tensor(t, device=_dev())
+    pts = torch.tensor(mesh.random_exterior_points(7, 0.4, 0.6, seed=1).astype(np.float32) + np.array(
+        [0.3, -0.1, 0.2], dtype=np.float32), device=_dev())
+    for basis in ("P1", "DP0"):
+        moved = BEM_Solver(torch.tensor(v0, device=_dev()), tt, basis=basis)
+        moved.set_vertices(torch.tensor(v1, device=_dev()))
+        fresh = BEM_Solver(torch.tensor(v1, device=_dev()), tt, basis=basis)
+        _say("set_vertices", basis)
+        assert float((moved.normals - fresh.normals).abs().max()) == 0.0
+        ops = [("single", False), ("double", False)] + ([("bm_lhs", True), ("bm_rhs", True)] if basis == "P1" else [])
+        for op, approx in ops:
+            a = moved.assemble_boundary_matrix(11.0, op, approx=approx)
+            b = fresh.assemble_boundary_matrix(11.0, op, approx=approx)
+            assert float(torch.linalg.matrix_norm(a - b) / torch.linalg.matrix_norm(b)) < 1e-6, (basis, op)
+        pa = moved.potential_matrix(11.0, pts, "double")
+        pb = fresh.potential_matrix(11.0, pts, "double")
+        assert float(torch.linalg.matrix_norm(pa - pb) / torch.linalg.matrix_norm(pb)) < 1e-6, basis
+    # rejected positions: a non-finite coordinate, a collapsed triangle
+    s = BEM_Solver(torch.tensor(v0, device=_dev()), tt)
+    before = s.assemble_boundary_matrix(5.0, "single")
+    bad = v0.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(ValueError):
+        s.set_vertices(torch.tensor(bad, device=_dev()))
+    bad = v0.copy()
+    bad[t[0, 1]] = bad[t[0, 0]]
+    with pytest.raises(ValueError):
+        s.set_vertices(torch.tensor(bad, device=_dev()))
+    assert float((s.assemble_boundary_matrix(5.0, "single") - before).abs().max()) <= 1e-6 * float(before.abs().max())
+
+
 @pytest.mark.parametrize("op", ["single", "double", "adjointdouble", "bm_rhs"])
 @pytest.mark.parametrize("clamp", [True, False])
 def test_dp0_operators_vs_oracle(op, clamp):
