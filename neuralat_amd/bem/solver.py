@@ -342,9 +342,8 @@ class BEM_Solver:
         """Burton-Miller with the approximate rules (solver.py:118-126):
         (bm_lhs + M/2) phi = (bm_rhs - beta M/2) psi."""
         beta = 1j / k
-        identity = self.identity_matrix()
-        LHS = combine(self.assemble_boundary_matrix(k, "bm_lhs", approx=True), 1.0, identity, 0.5)
-        R = combine(self.assemble_boundary_matrix(k, "bm_rhs", approx=True), 1.0, identity, -0.5 * beta)
+        LHS = add_identity_(self, self.assemble_boundary_matrix(k, "bm_lhs", approx=True), 0.5)
+        R = add_identity_(self, self.assemble_boundary_matrix(k, "bm_rhs", approx=True), -0.5 * beta)
         RHS = matvec(R, neumann.to(torch.complex64).contiguous())
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 

@@ -25,7 +25,7 @@ import numpy as np
 import torch
 
 from .bem.batch import batch_gmres
-from .bem.solver import BEM_Solver, combine, matvec
+from .bem.solver import BEM_Solver, add_identity_, matvec
 
 SPEED_OF_SOUND = 343.2  # src/scene.py:275
 
@@ -176,9 +176,8 @@ def burton_miller_system(solver, k, neumann):
     """(bm_lhs + M/2, (bm_rhs - beta M/2) psi) with the approximate rules, the
     system BEM_Solver.neumann2dirichlet solves (src/bem/solver.py:118-126)."""
     beta = 1j / k
-    identity = solver.identity_matrix()
-    lhs = combine(solver.assemble_boundary_matrix(k, "bm_lhs", approx=True), 1.0, identity, 0.5)
-    R = combine(solver.assemble_boundary_matrix(k, "bm_rhs", approx=True), 1.0, identity, -0.5 * beta)
+    lhs = add_identity_(solver, solver.assemble_boundary_matrix(k, "bm_lhs", approx=True), 0.5)
+    R = add_identity_(solver, solver.assemble_boundary_matrix(k, "bm_rhs", approx=True), -0.5 * beta)
     rhs = matvec(R, neumann.to(torch.complex64).contiguous())
     return lhs, rhs
 
