@@ -528,6 +528,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     // each row step (from LDS rather than a chain of dependent scalar loads)
     __shared__ float rrec[kP1RowTris][S];
     __shared__ int4 rtri[kP1RowTris];
+    __shared__ int4 rslot[kP1RowTris];  // the row triangles' vertex slots in the row tile
     __shared__ float rcurl[kP1RowTris][9];
     int ct = blockIdx.x, rt = blockIdx.y;
     bool own = false;  // half sweeps: the column tile holds the row tile
@@ -552,7 +553,10 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         const int q = e / S;
         rrec[q][e - q * S] = rec[(int64_t)tl.perm[r0 + q] * S + (e - q * S)];
     }
-    for (int q = threadIdx.x; q < r1 - r0; q += 256) rtri[q] = tris[tl.perm[r0 + q]];
+    for (int q = threadIdx.x; q < r1 - r0; q += 256) {
+        rtri[q] = tris[tl.perm[r0 + q]];
+        rslot[q] = tl.row_slot[r0 + q];
+    }
     if constexpr (OP == HYP || OP == BM_LHS)
         for (int e = threadIdx.x; e < (r1 - r0) * 9; e += 256) {
             const int q = e / 9;
@@ -583,6 +587,36 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const bool owner = w < ncv;
     const int ib = owner ? tl.col_inc_ptr[cv0 + w] : 0;
     const int ie = owner ? tl.col_inc_ptr[cv0 + w + 1] : 0;
+    // the owner's first kIncRegs incidence codes in registers for the whole
+    // workgroup (read from memory at every row triangle they left the 3-point
+    // kernels waiting on the loads)
+    constexpr int kIncRegs = 8;
+    int cinc[kIncRegs];
+#pragma unroll
+    for (int u = 0; u < kIncRegs; ++u) cinc[u] = ib + u < ie ? tl.col_inc[ib + u] : 0;
+    // s_c += the stage entries of row corner c over the owner's incident
+    // column triangles, in incidence order
+    auto gather = [&](float2& s0, float2& s1, float2& s2) {
+#pragma unroll
+        for (int u = 0; u < kIncRegs; ++u) {
+            if (ib + u < ie) {
+                const int code = cinc[u];
+                const int p = code >> 2, b = code & 3;
+                const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
+                s0.x += e0.x; s0.y += e0.y;
+                s1.x += e1.x; s1.y += e1.y;
+                s2.x += e2.x; s2.y += e2.y;
+            }
+        }
+        for (int q = ib + kIncRegs; q < ie; ++q) {
+            const int code = tl.col_inc[q];
+            const int p = code >> 2, b = code & 3;
+            const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
+            s0.x += e0.x; s0.y += e0.y;
+            s1.x += e1.x; s1.y += e1.y;
+            s2.x += e2.x; s2.y += e2.y;
+        }
+    };
     float2 accT[PAIR ? kP1RowVerts : 1];  // PAIR: this lane's column of B(j, i)^T, by row slot
 #pragma unroll
     for (int q = 0; q < (PAIR ? kP1RowVerts : 1); ++q) accT[q] = make_float2(0.f, 0.f);
@@ -791,16 +825,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
         __syncthreads();
         if (owner) {
-            const int4 rs = tl.row_slot[rp];
+            const int4 rs = rslot[rp - r0];
             float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
-            for (int q = ib; q < ie; ++q) {
-                const int code = tl.col_inc[q];
-                const int p = code >> 2, b = code & 3;
-                const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
-                s0.x += e0.x; s0.y += e0.y;
-                s1.x += e1.x; s1.y += e1.y;
-                s2.x += e2.x; s2.y += e2.y;
-            }
+            gather(s0, s1, s2);
             acc[rs.x][w].x += s0.x; acc[rs.x][w].y += s0.y;
             acc[rs.y][w].x += s1.x; acc[rs.y][w].y += s1.y;
             acc[rs.z][w].x += s2.x; acc[rs.z][w].y += s2.y;
@@ -810,18 +837,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 #pragma unroll
             for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(tr[e], tii[e]);
             __syncthreads();
-            const int4 rs = tl.row_slot[rp];
+            const int4 rs = rslot[rp - r0];
             float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
-            if (owner) {
-                for (int q = ib; q < ie; ++q) {
-                    const int code = tl.col_inc[q];
-                    const int p = code >> 2, b = code & 3;
-                    const float2 e0 = stage[b][p], e1 = stage[3 + b][p], e2 = stage[6 + b][p];
-                    s0.x += e0.x; s0.y += e0.y;
-                    s1.x += e1.x; s1.y += e1.y;
-                    s2.x += e2.x; s2.y += e2.y;
-                }
-            }
+            if (owner) gather(s0, s1, s2);
 #pragma unroll
             for (int q = 0; q < kP1RowVerts; ++q) {
                 const float2 z = make_float2(0.f, 0.f);
