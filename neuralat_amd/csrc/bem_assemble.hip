@@ -942,6 +942,15 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// sum over aligned groups of N lanes (N a power of two <= 64), in every lane
+template <int N>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int off = N / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 // MODE 0: write (DP0: out = alpha * entry) or add (P1) the entries;
 // MODE 1 (DP0): vpsi[p] = entry * psi[j], the pair's term of V psi.
 template <int OP, int NL, int BASIS, bool CLAMP, int MODE = 0>
@@ -955,12 +964,20 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
     const int lane = threadIdx.x & 63;
-    const int64_t p = p0 + (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (p >= p1) return;  // uniform per wavefront
-    const int4 pr = pairs[p];
-    const int i = __builtin_amdgcn_readfirstlane(pr.x);
-    const int j = __builtin_amdgcn_readfirstlane(pr.y);
-    const int shared = __builtin_amdgcn_readfirstlane(pr.z);
+    // lanes per pair: a wavefront per pair, except the 2-point line rule of the
+    // P1 operators (16 points): 16 lanes per pair, 4 pairs per wavefront, so
+    // the FP64 reductions run over 16 lanes and serve four pairs
+    constexpr int LPP = (NL == 2 && BASIS == P1 && MODE == 0) ? 16 : 64;
+    constexpr int PPW = 64 / LPP;
+    const int ll = lane % LPP;
+    const int64_t pw = p0 + ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PPW;  // the wavefront's first pair
+    if (pw >= p1) return;  // uniform per wavefront
+    const int64_t p = pw + lane / LPP;
+    const bool valid = p < p1;  // groups past the end compute on the last pair and write nothing
+    const int4 pr = pairs[valid ? p : p1 - 1];
+    const int i = PPW == 1 ? __builtin_amdgcn_readfirstlane(pr.x) : pr.x;
+    const int j = PPW == 1 ? __builtin_amdgcn_readfirstlane(pr.y) : pr.y;
+    const int shared = PPW == 1 ? __builtin_amdgcn_readfirstlane(pr.z) : pr.z;
     const int4 ti = tris[i], tj = tris[j];
     const int ia[3] = {ti.x, ti.y, ti.z}, ja[3] = {tj.x, tj.y, tj.z};
     int pi[3] = {0, 1, 2}, pj[3] = {0, 1, 2};
@@ -1013,7 +1030,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     for (int e = 0; e < 10; ++e) ar[e] = ai[e] = 0.0;
 
     constexpr int NP = NL * NL * NL * NL;
-    for (int q = lane; q < NP; q += 64) {
+    for (int q = ll; q < NP; q += LPP) {
         const int q0 = q / (NL * NL * NL), q1 = (q / (NL * NL)) % NL, q2 = (q / NL) % NL, q3 = q % NL;
         const float xs = LineRule01<NL>::t[q0], e_1 = LineRule01<NL>::t[q1];
         const float e_2 = LineRule01<NL>::t[q2], e_3 = LineRule01<NL>::t[q3];
@@ -1119,8 +1136,8 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     }
 #pragma unroll
     for (int e = 0; e < 10; ++e) {
-        ar[e] = wave_sum(ar[e]) * (double)kInv4Pi;
-        ai[e] = wave_sum(ai[e]) * (double)kInv4Pi;
+        ar[e] = group_sum<LPP>(ar[e]) * (double)kInv4Pi;
+        ai[e] = group_sum<LPP>(ai[e]) * (double)kInv4Pi;
     }
     if constexpr (MODE == 1) {
         if (lane == 0) {
@@ -1131,13 +1148,13 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
         if (lane == 0)
             out[((int64_t)i - row0) * ld + j] = make_float2((float)(alpha * ar[9]), (float)(alpha * ai[9]));
     } else {
-        if (lane < 9) {
-            // lane = reordered (a, b) pair; its original local indices are (pi[a], pj[b])
-            const int a = lane / 3, b = lane % 3;
+        if (valid && ll < 9) {
+            // ll = reordered (a, b) pair; its original local indices are (pi[a], pj[b])
+            const int a = ll / 3, b = ll % 3;
             double vr = 0.0, vi = 0.0;
 #pragma unroll
             for (int e = 0; e < 9; ++e)
-                if (e == lane) { vr = ar[e]; vi = ai[e]; }
+                if (e == ll) { vr = ar[e]; vi = ai[e]; }
             const int oa = pi[a], ob = pj[b];
             if constexpr (OP == HYP || OP == BM_LHS) {
                 float ci[9], cj[9], cp[9];
@@ -1382,9 +1399,13 @@ static hipError_t singular_t(const MeshDev& m, int64_t p0, int64_t p1, const KBa
                              int64_t ld, int64_t row0, hipStream_t s)
 {
     if (p1 <= p0) return hipSuccess;
-    unsigned g = (unsigned)((p1 - p0 + 3) / 4);
+    // pairs per workgroup: 4 wavefronts, 4 pairs each for the 2-point P1 rule
+    const int64_t per = (NL == 2 && BASIS == P1) ? 16 : 4;
+    unsigned g = (unsigned)((p1 - p0 + per - 1) / per);
+    // P1 takes every pair (p0 = 0, p1 = npairs): the list grouped by kind
+    const int4* pl = BASIS == P1 ? m.pairs_by_kind : m.pairs;
     hipLaunchKernelGGL((singular_kernel<OP, NL, BASIS, CLAMP>), dim3(g, 1, nz), dim3(256), 0, s, m.geo, m.tris,
-                       m.curls, m.pairs, p0, p1, kb, out, ld, row0, 1.0f, nullptr, nullptr);
+                       m.curls, pl, p0, p1, kb, out, ld, row0, 1.0f, nullptr, nullptr);
     return hipGetLastError();
 }
 

@@ -60,6 +60,7 @@ struct MeshDev {
     float* curls = nullptr;  // [nt*12]: rows n x grad(phi_a), a = 0..2 (9 used)
     float* geo = nullptr;    // [nt*16]: v0 v1 v2 (9), unit normal (3), |J| (1)
     int4* pairs = nullptr;   // [npairs]: (i, j, shared vertices, 0), sorted by (i, j)
+    int4* pairs_by_kind = nullptr;  // the same pairs, coincident, edge, vertex (P1 launches)
     int64_t* pair_ptr = nullptr;  // [nt+1] pair range of each row triangle
     int64_t npairs = 0;
     int32_t* vt_ptr = nullptr;  // [nv+1] vertex -> triangle CSR (ascending)

@@ -301,6 +301,16 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         m.h_pair_ptr[t + 1] = (int64_t)pairs.size() / 4;
     }
     m.npairs = (int64_t)pairs.size() / 4;
+    // the P1 neighbour-pair launches take every pair in any order: grouped by
+    // kind, the pairs that share a wavefront take the same quadrature branch
+    std::vector<int32_t> pairs_kind(pairs.size());
+    {
+        std::vector<int64_t> idx((size_t)m.npairs);
+        std::iota(idx.begin(), idx.end(), (int64_t)0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int64_t a, int64_t b) { return pairs[4 * a + 2] > pairs[4 * b + 2]; });
+        for (int64_t q = 0; q < m.npairs; ++q)
+            for (int c = 0; c < 4; ++c) pairs_kind[4 * q + c] = pairs[4 * idx[q] + c];
+    }
     m.max_neighbors = maxnb;
 
     // ---- P1 tiling on a Morton order of the centroids ----
@@ -369,6 +379,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     constexpr size_t kGeomParts = 6;
     parts.push_back({tris4.data(), tris4.size() * 4, (void**)&m.tris});
     parts.push_back({pairs.data(), pairs.size() * 4, (void**)&m.pairs});
+    parts.push_back({pairs_kind.data(), pairs_kind.size() * 4, (void**)&m.pairs_by_kind});
     parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
     parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});

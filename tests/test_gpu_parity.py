@@ -85,6 +85,29 @@ def test_golden_approx_rules(quad):
         assert e < TOL, (op, quad, e)
 
 
+@pytest.mark.parametrize("op", OPS)
+def test_p1_neighbour_pairs_approx_vs_oracle(op):
+    """The neighbour pairs alone (parts = 2) with the 2-point line rule, which
+    packs four pairs into a wavefront, grouped by kind: level 2 (320
+    triangles, coincident, edge and vertex pairs in mixed groups at the kind
+    boundaries) against the oracle's neighbour-pair part."""
+    from neuralat_amd import _lib
+    from neuralat_amd.bem.solver import _ptr, _stream
+
+    v, t = mesh.icosphere(2, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.08, seed=6).astype(np.float32)
+    s = _solver(v, t, "P1")
+    n = len(v)
+    k = 6.5
+    _say("P1 neighbour pairs approx", op, s.mesh.n_neighbor_pairs)
+    X = torch.zeros((n, n), dtype=torch.complex64, device=_dev())
+    _lib.check(_lib.load().bem_assemble_ex(s.mesh.handle, k, _lib.OPS[op], _lib.BASES["P1"], _lib.QUADS["approx"], 1, 2,
+                                           _ptr(X), n, _stream()), "bem_assemble_ex")
+    R = oracle.assemble(v, t, k, op, "P1", quad="approx", parts=2, nthreads=0)
+    e = _rel(X.cpu().numpy(), R)
+    assert e < TOL, (op, e)
+
+
 def test_golden_icosphere1_dp0():
     g = np.load(os.path.join(GOLD, "icosphere1_dp0.npz"))
     for clamp in (True, False):
