@@ -78,6 +78,8 @@ def parse():
                         "solve of the first 8 batching meshes x 8 k, for kernel traces")
     p.add_argument("--p1-probe", action="store_true",
                    help="assemble the level-5 P1 single- and double-layer operators once each, for counter passes")
+    p.add_argument("--modal-sound-probe", action="store_true",
+                   help="three batched modal-sound solves (32 modes of the modal form's block), for kernel traces")
     return p.parse_args()
 
 
@@ -941,6 +943,34 @@ def traffic_probe(level, k):
                       "copy_read_bytes": 4 << 30, "copy_write_bytes": 4 << 30, "y0": str(complex(y[0].item()))}))
 
 
+def modal_sound_probe(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=3):
+    """The modal form's modal-sound step alone: the block's k lowest modes,
+    then ``reps`` batched solves (every mode's Burton-Miller system and the
+    pressure at 256 points), for kernel traces."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.datagen import EditableModalSound, modal_preprocess
+    from neuralat_amd.modal import Material, MatSet, VibrationObj
+
+    v, t = mesh.tet_box(*cells, size, jitter=0.2, seed=1)
+    sv, st = mesh.tet_boundary(v, t)
+    obj = VibrationObj(v, t, sv, st)
+    obj.modal_analysis(k=k, material=Material(MatSet.Plastic))
+    obj.normalize(0.15)
+    ms = EditableModalSound(modal_preprocess(obj, k), {"vibration_obj": {"size": 0.15},
+                                                       "sample": {"freq_rate": 1.0, "size_rate": 2.0, "bbox_rate": 2.0,
+                                                                  "sample_num": 1, "point_num_per_sample": 256}})
+    ms.sample(freqK_base=torch.tensor([1.0], device="cuda"), sizeK_base=torch.tensor([0.5], device="cuda"))
+    for q in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms.solve()
+        torch.cuda.synchronize()
+        print(json.dumps({"solve": q, "modes": k, "surface_vertices": int(len(sv)),
+                          "ms": 1e3 * (time.perf_counter() - t0)}))
+
+
 def p1_probe(level=5, k=10.0):
     """One assembly of each P1 layer operator and of the two Burton-Miller
     operators with the approximate rules (the data-generation path) on the
@@ -981,6 +1011,9 @@ def main():
         return
     if args.batch_probe:
         batch_probe()
+        return
+    if args.modal_sound_probe:
+        modal_sound_probe()
         return
     log(f"bench: building level-{args.level} workload")
     v, t, src, pts = make_workload(args.level)
