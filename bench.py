@@ -589,10 +589,11 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
         cur_b += b
     chunks.append(cur)
     max_cycles = max(1, maxiter // restart)
-    acc = {"plain": 0.0, "batched": 0.0, "mixed": 0.0, "batched_asm": 0.0, "mixed_asm": 0.0}
-    its_plain, its_batch, its_mixed, ntri_all = [], [], [], []
-    conv = {"plain": 0, "batched": 0, "mixed": 0}
-    diffs_b, diffs_m = [], []
+    acc = {"plain": 0.0, "batched": 0.0, "mixed": 0.0, "batched_asm": 0.0, "mixed_asm": 0.0, "mixed8": 0.0,
+           "mixed8_asm": 0.0}
+    its_plain, its_batch, its_mixed, its_mixed8, ntri_all = [], [], [], [], []
+    conv = {"plain": 0, "batched": 0, "mixed": 0, "mixed8": 0}
+    diffs_b, diffs_m, diffs_m8 = [], [], []
     fp64 = {}  # filled by the CPU leg
     nfp64 = 0
     best, tune_ms, done = None, None, 0
@@ -649,38 +650,58 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
                                         max_inner_cycles=1, max_outer_cycles=3)
         torch.cuda.synchronize()
         acc["mixed"] += time.perf_counter() - t0
+        # persistent assembly + lockstep mixed-precision solve on 8-bit copies:
+        # half the inner bytes, about 1 % from A, so a looser inner target and
+        # one more outer cycle
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch_cbie(meshes, ks, psis, outs, row_tile=best)
+        torch.cuda.synchronize()
+        acc["mixed8_asm"] += time.perf_counter() - t0
+        xs_m8, inf_m8 = batch_solve_mixed([o[0] for o in outs], [o[1] for o in outs], dtype="fp8", tol=1e-6,
+                                          inner_tol=1e-2, outer_restart=10, inner_restart=restart,
+                                          max_inner_cycles=1, max_outer_cycles=4)
+        torch.cuda.synchronize()
+        acc["mixed8"] += time.perf_counter() - t0
         its_batch += [i["iterations"] for i in inf_b]
         its_mixed += [i["inner_iterations"] for i in inf_m]
+        its_mixed8 += [i["inner_iterations"] for i in inf_m8]
         cb = [bool(i["converged"]) for i in inf_b]
         cm = [bool(i["converged"]) for i in inf_m]
+        cm8 = [bool(i["converged"]) for i in inf_m8]
         conv["batched"] += sum(cb)
         conv["mixed"] += sum(cm)
-        for xb, xm, xp, c1, c2 in zip(xs_b, xs_m, xs_plain, cb, cm):
+        conv["mixed8"] += sum(cm8)
+        for xb, xm, xm8, xp, c1, c2, c3 in zip(xs_b, xs_m, xs_m8, xs_plain, cb, cm, cm8):
             nrm = torch.linalg.vector_norm(xp)
             if c1:
                 diffs_b.append(float((torch.linalg.vector_norm(xb - xp) / nrm).item()))
             if c2:
                 diffs_m.append(float((torch.linalg.vector_norm(xm - xp) / nrm).item()))
+            if c3:
+                diffs_m8.append(float((torch.linalg.vector_norm(xm8 - xp) / nrm).item()))
         # the first level-2 items (320 triangles), for the FP64 oracle in the CPU leg
         for q, (v, t, k) in enumerate(host):
-            if nfp64 >= fp64_items or t.shape[0] != 320 or not (cb[q] and cm[q]):
+            if nfp64 >= fp64_items or t.shape[0] != 320 or not (cb[q] and cm[q] and cm8[q]):
                 continue
             CHECKS.add(fp64, v, t, k, psis[q].cpu().numpy(),
                        {"max_surface_rel_err_vs_fp64_level2_batched": xs_b[q].cpu().numpy(),
-                        "max_surface_rel_err_vs_fp64_level2_mixed": xs_m[q].cpu().numpy()}, mode="max")
+                        "max_surface_rel_err_vs_fp64_level2_mixed": xs_m[q].cpu().numpy(),
+                        "max_surface_rel_err_vs_fp64_level2_mixed_fp8": xs_m8[q].cpu().numpy()}, mode="max")
             nfp64 += 1
         done += len(ks)
         log(f"bench: batching chunk {ci + 1}/{len(chunks)} ({done} solves): plain {acc['plain']:.2f} s, "
-            f"batched {acc['batched']:.2f} s, mixed {acc['mixed']:.2f} s")
-        del outs, xs_plain, xs_b, xs_m, solvers, meshes, psis
+            f"batched {acc['batched']:.2f} s, mixed {acc['mixed']:.2f} s, mixed fp8 {acc['mixed8']:.2f} s")
+        del outs, xs_plain, xs_b, xs_m, xs_m8, solvers, meshes, psis
         torch.cuda.empty_cache()
     nsol = len(ntri_all)
     entries = float(sum(2 * n * n for n in ntri_all))
     bytes_plain = float(sum(8.0 * n * n * it for n, it in zip(ntri_all, its_plain)))
     bytes_batch = float(sum(8.0 * n * n * it for n, it in zip(ntri_all, its_batch)))
     bytes_mixed = float(sum(4.0 * n * n * it for n, it in zip(ntri_all, its_mixed)))
+    bytes_mixed8 = float(sum(2.0 * n * n * it for n, it in zip(ntri_all, its_mixed8)))
     lv = "-".join(f"L{x}" for x in levels)
-    tp, tb, tm = acc["plain"], acc["batched"], acc["mixed"]
+    tp, tb, tm, tm8 = acc["plain"], acc["batched"], acc["mixed"], acc["mixed8"]
     res = fp64  # the CPU leg fills its FP64 entries in this dictionary
     res["fp64_items"] = nfp64
     res.update({"workload": f"{nmesh} meshes (icosphere {lv}, anisotropic scale [0.5,2], SH noise) x {nk} k "
@@ -704,7 +725,15 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
             "converged_mixed": conv["mixed"],
             "autotune_row_tile": best, "autotune_ms": tune_ms,
             "max_rel_diff_batched_vs_plain": max(diffs_b) if diffs_b else None,
-            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None})
+            "max_rel_diff_mixed_vs_plain": max(diffs_m) if diffs_m else None,
+            "mixed_fp8_seconds": tm8, "mixed_fp8_solves_per_s": nsol / tm8, "speedup_mixed_fp8": tp / tm8,
+            "mixed_fp8_solver": "FP32 flexible GMRES(10), inner GMRES(restart) on OCP e4m3 copies with a "
+                                "power-of-two scale per row to max(1e-2, tol |b| / |r|), diagonal in FP32",
+            "mixed_fp8_assembly_seconds": acc["mixed8_asm"],
+            "inner_iterations_total_mixed_fp8": int(sum(its_mixed8)),
+            "matrix_GBps_mixed_fp8_inner": bytes_mixed8 / (tm8 - acc["mixed8_asm"]) / 1e9,
+            "converged_mixed_fp8": conv["mixed8"],
+            "max_rel_diff_mixed_fp8_vs_plain": max(diffs_m8) if diffs_m8 else None})
     return res
 
 

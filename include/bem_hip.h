@@ -302,7 +302,9 @@ int bem_batch_gmres(const bem_batch_system* sys, int n_sys, float tol, int resta
  * every system's complex64 matrix (products, residuals and updates in FP32),
  * whose directions z_j ~ A^-1 v_j come from lockstep GMRES(inner_restart) on
  * a 16-bit copy of the matrix (dtype 1 = bf16, 2 = fp16 with a per-system
- * power-of-two scale; the diagonal kept in FP32), made inside the call in the
+ * power-of-two scale) or an 8-bit one (dtype 3 = OCP e4m3 with a
+ * power-of-two scale per row; half the bytes of the 16-bit copies, about 1 %
+ * from A, so more outer steps); the diagonal kept in FP32, made inside the call in the
  * workspace, run per system to ||v_j - A16 z_j|| <= max(inner_tol,
  * tol ||b|| / ||r_j||) (r_j: the current outer residual; capped at 0.5) or
  * for max_inner_cycles cycles.  Stops when every system has

@@ -102,12 +102,14 @@ def batch_solve_mixed(As, bs, xs=None, dtype="bf16", tol=1e-6, inner_tol=1e-6, o
                       raise_on_fail=False):
     """Lockstep mixed-precision solves of As[q] x = bs[q]: flexible GMRES in
     FP32 on the complex64 matrices, preconditioned by lockstep GMRES on 16-bit
-    (``dtype`` "bf16" or "fp16") copies with the diagonal kept in FP32.
+    (``dtype`` "bf16" or "fp16") or 8-bit ("fp8": OCP e4m3, a power-of-two
+    scale per row) copies with the diagonal kept in FP32.
     Returns (xs, infos); infos[q]["iterations"] = outer steps,
-    infos[q]["inner_iterations"] = 16-bit Arnoldi steps."""
+    infos[q]["inner_iterations"] = low-precision Arnoldi steps."""
     n = len(As)
-    if dtype not in ("bf16", "fp16"):
-        raise ValueError("dtype must be 'bf16' or 'fp16'")
+    codes = {"bf16": 1, "fp16": 2, "fp8": 3}
+    if dtype not in codes:
+        raise ValueError("dtype must be 'bf16', 'fp16' or 'fp8'")
     xs = xs or [torch.zeros_like(b) for b in bs]
     arr = (_Sys * n)()
     for q in range(n):
@@ -121,7 +123,7 @@ def batch_solve_mixed(As, bs, xs=None, dtype="bf16", tol=1e-6, inner_tol=1e-6, o
                        device=bs[0].device)
     info = (_lib.SolveInfo * n)()
     inner = (ctypes.c_int32 * n)()
-    rc = lib.bem_batch_fgmres(arr, n, 1 if dtype == "bf16" else 2, float(tol), float(inner_tol), outer_restart,
+    rc = lib.bem_batch_fgmres(arr, n, codes[dtype], float(tol), float(inner_tol), outer_restart,
                               inner_restart, int(max_inner_cycles), int(max_outer_cycles), int(use_graph),
                               _ptr(work), info, inner, _stream())
     if rc != _lib.BEM_ERR_NOT_CONVERGED or raise_on_fail:

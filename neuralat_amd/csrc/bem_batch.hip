@@ -254,6 +254,7 @@ struct BLow {
     uint32_t* L;
     int64_t ldl;
     float2* diag;
+    float* rinv;  // fp8: per-row 1 / scale (the 16-bit copies scale per item)
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -337,6 +338,128 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
                 out.y += beta.x * yo.y + beta.y * yo.x;
             }
             y[S.off + r] = out;
+        }
+    }
+}
+
+// 8-bit copies (OCP e4m3, two entries per 32-bit word, real part in the low
+// byte of each 16-bit half): entry (r, c) is round(scale_r A_s(r, c)) with a
+// power-of-two scale per row (a row of a second-kind operator spans a few
+// decades off the diagonal, more than e4m3's three mantissa bits keep per
+// item); L holds padded(n) / 2 words per row (ldl in words), diag the FP32
+// diagonal and rinv[r] = 1 / scale_r.  The product: a wavefront owns kLowRows
+// rows, each lane reads 16 bytes (eight entries) of every row per step.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low,
+                                                        const float2* __restrict__ x, float2* __restrict__ y,
+                                                        float2 alpha, float2 beta, int beta_zero,
+                                                        const int* __restrict__ skip)
+{
+    const int s = blockIdx.y;
+    if (skip && skip[s]) return;
+    const BSys S = sys[s];
+    const int lane = threadIdx.x & 63;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kLowRows;
+    if (r0 >= S.n) return;
+    const BLow Lw = low[s];
+    const float2* xs = x + S.off;
+    float ar[kLowRows], ai[kLowRows];
+#pragma unroll
+    for (int q = 0; q < kLowRows; ++q) ar[q] = ai[q] = 0.f;
+    const int64_t n8 = S.n >> 3, ld4 = Lw.ldl >> 2;
+    const u32x4* L4 = reinterpret_cast<const u32x4*>(Lw.L);
+    const float4* x4 = reinterpret_cast<const float4*>(xs);
+    for (int64_t c = lane; c < n8; c += 64) {
+        const float4 xa = x4[4 * c], xb = x4[4 * c + 1], xc = x4[4 * c + 2], xd = x4[4 * c + 3];
+#pragma unroll
+        for (int q = 0; q < kLowRows; ++q) {
+            const int64_t r = min(r0 + q, S.n - 1);
+            const u32x4 a = __builtin_nontemporal_load(&L4[r * ld4 + c]);
+            f32x2 e;
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.x, false); cfma(make_float2(e.x, e.y), xa.x, xa.y, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.x, true);  cfma(make_float2(e.x, e.y), xa.z, xa.w, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.y, false); cfma(make_float2(e.x, e.y), xb.x, xb.y, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.y, true);  cfma(make_float2(e.x, e.y), xb.z, xb.w, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.z, false); cfma(make_float2(e.x, e.y), xc.x, xc.y, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.z, true);  cfma(make_float2(e.x, e.y), xc.z, xc.w, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.w, false); cfma(make_float2(e.x, e.y), xd.x, xd.y, ar[q], ai[q]);
+            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.w, true);  cfma(make_float2(e.x, e.y), xd.z, xd.w, ar[q], ai[q]);
+        }
+    }
+    for (int64_t c = 8 * n8 + lane; c < S.n; c += 64) {  // the n % 8 last columns
+        const float2 xv = xs[c];
+#pragma unroll
+        for (int q = 0; q < kLowRows; ++q) {
+            const int64_t r = min(r0 + q, S.n - 1);
+            const int w = (int)Lw.L[r * Lw.ldl + (c >> 1)];
+            const f32x2 e = (c & 1) ? __builtin_amdgcn_cvt_pk_f32_fp8(w, true) : __builtin_amdgcn_cvt_pk_f32_fp8(w, false);
+            cfma(make_float2(e.x, e.y), xv.x, xv.y, ar[q], ai[q]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kLowRows; ++q) {
+        float sr = ar[q], si = ai[q];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sr += __shfl_xor(sr, o, 64);
+            si += __shfl_xor(si, o, 64);
+        }
+        const int64_t r = r0 + q;
+        if (lane == q && r < S.n) {
+            const float isc = Lw.rinv[r];
+            const float2 d = Lw.diag[r], xv = xs[r];
+            sr = fmaf(d.x, xv.x, fmaf(-d.y, xv.y, sr * isc));
+            si = fmaf(d.x, xv.y, fmaf(d.y, xv.x, si * isc));
+            float2 out = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
+            if (!beta_zero) {
+                const float2 yo = y[S.off + r];
+                out.x += beta.x * yo.x - beta.y * yo.y;
+                out.y += beta.x * yo.y + beta.y * yo.x;
+            }
+            y[S.off + r] = out;
+        }
+    }
+}
+
+// The 8-bit copies: per row, the largest |Re|, |Im| off the diagonal m (a
+// block reduction), scale_r = 2^(8 - e) with m < 2^e (so |scale_r A| < 256,
+// inside e4m3's 448), then round to nearest even; pad columns and the
+// diagonal zero.  grid (row blocks, items)
+__global__ __launch_bounds__(256) void xto_fp8_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low)
+{
+    __shared__ float wmax[4];
+    const int s = blockIdx.y;
+    const BSys S = sys[s];
+    const BLow Lw = low[s];
+    for (int64_t r = blockIdx.x; r < S.n; r += gridDim.x) {
+        const float2* row = S.A + r * S.lda;
+        float m = 0.f;
+        for (int64_t c = threadIdx.x; c < S.n; c += 256)
+            if (c != r) m = fmaxf(m, fmaxf(fabsf(row[c].x), fabsf(row[c].y)));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+        __syncthreads();
+        m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        __syncthreads();  // wmax is rewritten for the next row
+        float sc = 1.f;
+        if (m > 0.f) {
+            int e = 0;
+            (void)frexpf(m, &e);  // m < 2^e
+            sc = ldexpf(1.f, 8 - e);
+        }
+        if (threadIdx.x == 0) {
+            Lw.rinv[r] = 1.f / sc;
+            Lw.diag[r] = row[r];
+        }
+        for (int64_t w = threadIdx.x; w < Lw.ldl; w += 256) {
+            const int64_t c0 = 2 * w, c1 = 2 * w + 1;
+            const float2 v0 = (c0 < S.n && c0 != r) ? row[c0] : make_float2(0.f, 0.f);
+            const float2 v1 = (c1 < S.n && c1 != r) ? row[c1] : make_float2(0.f, 0.f);
+            int u = __builtin_amdgcn_cvt_pk_fp8_f32(v0.x * sc, v0.y * sc, 0, false);
+            u = __builtin_amdgcn_cvt_pk_fp8_f32(v1.x * sc, v1.y * sc, u, true);
+            Lw.L[r * Lw.ldl + w] = (uint32_t)u;
         }
     }
 }
@@ -518,7 +641,10 @@ hipError_t xmatvec(const XEng& E, const float2* x, float2* y, float2 alpha, floa
                            E.dsys, x, y, alpha, beta, beta_zero, skip);
     } else {
         const dim3 g((unsigned)((E.nmax + 4 * kLowRows - 1) / (4 * kLowRows)), (unsigned)E.ns);
-        if (E.dt == kDtBF16)
+        if (E.dt == kDtFP8)
+            hipLaunchKernelGGL(xgemv_fp8_kernel, g, dim3(256), 0, st, E.dsys, E.dlow, x, y, alpha, beta, beta_zero,
+                               skip);
+        else if (E.dt == kDtBF16)
             hipLaunchKernelGGL(xgemv_lowp_kernel<kDtBF16>, g, dim3(256), 0, st, E.dsys, E.dlow, E.inv_scale, x, y,
                                alpha, beta, beta_zero, skip);
         else
@@ -673,7 +799,7 @@ hipError_t xrun_cycle(const XEng& E, const XGraphs* G, hipStream_t s, std::vecto
 // Workspace of the mixed-precision batch solve
 struct FLayout {
     XLayout o, in;
-    size_t o_at, z_at, in_at, low_at, scale_at, inv_at, amax_at, diag_at, planes_at, total;
+    size_t o_at, z_at, in_at, low_at, scale_at, inv_at, amax_at, diag_at, rinv_at, planes_at, total;
     std::vector<size_t> plane_at;  // per item, from planes_at
 };
 
@@ -699,6 +825,8 @@ FLayout flayout(const bem_batch_system* sys, int ns, int64_t ntot, int mo, int m
     o += al(sizeof(unsigned int) * ns);
     F.diag_at = o;
     o += al(sizeof(float2) * (size_t)ntot);
+    F.rinv_at = o;
+    o += al(sizeof(float) * (size_t)ntot);
     F.planes_at = o;
     F.plane_at.resize(ns);
     for (int q = 0; q < ns; ++q) {
@@ -973,7 +1101,7 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
 {
     if (!sys || ns <= 0 || outer_restart <= 0 || inner_restart <= 0 || max_inner_cycles <= 0 ||
         max_outer_cycles < 0 || !work ||
-        !(tol > 0.f) || !(inner_tol > 0.f) || (dtype != kDtBF16 && dtype != kDtFP16))
+        !(tol > 0.f) || !(inner_tol > 0.f) || (dtype != kDtBF16 && dtype != kDtFP16 && dtype != kDtFP8))
         return BEM_ERR_BAD_ARG;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const int mo = outer_restart, mi = inner_restart;
@@ -1000,13 +1128,21 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
     // 16-bit copies of the matrices
     std::vector<BLow> hl(ns);
     float2* diag = (float2*)(W + F.diag_at);
-    for (int q = 0; q < ns; ++q) hl[q] = BLow{(uint32_t*)(W + F.plane_at[q]), padded(sys[q].n), diag + hs[q].off};
+    float* rinv = (float*)(W + F.rinv_at);
+    for (int q = 0; q < ns; ++q) {
+        // 16-bit: one word per entry; fp8: two entries per word (the plane
+        // sized for 16 bits is half used)
+        const int64_t ldl = dtype == kDtFP8 ? padded(sys[q].n) / 2 : padded(sys[q].n);
+        hl[q] = BLow{(uint32_t*)(W + F.plane_at[q]), ldl, diag + hs[q].off, rinv + hs[q].off};
+    }
     XTRY(hipMemcpyAsync(dlow, hl.data(), sizeof(BLow) * ns, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(xfill_kernel, dim3((ns + 255) / 256), dim3(256), 0, s, amax, (int64_t)ns, 0u);
     const dim3 grow((unsigned)std::min<int64_t>(nmax, 2048), (unsigned)ns);
     if (dtype == kDtFP16) hipLaunchKernelGGL(xamax_kernel, grow, dim3(256), 0, s, dsys, amax);
     hipLaunchKernelGGL(xscale_kernel, dim3((ns + 63) / 64), dim3(64), 0, s, amax, dtype, scale, inv, ns);
-    if (dtype == kDtBF16)
+    if (dtype == kDtFP8)
+        hipLaunchKernelGGL(xto_fp8_kernel, grow, dim3(256), 0, s, dsys, dlow);
+    else if (dtype == kDtBF16)
         hipLaunchKernelGGL(xto_lowp_kernel<kDtBF16>, grow, dim3(256), 0, s, dsys, dlow, scale);
     else
         hipLaunchKernelGGL(xto_lowp_kernel<kDtFP16>, grow, dim3(256), 0, s, dsys, dlow, scale);

@@ -7,6 +7,7 @@ namespace bem {
 
 constexpr int kDtBF16 = 1;
 constexpr int kDtFP16 = 2;
+constexpr int kDtFP8 = 3;  // OCP e4m3 with per-row scales (batched mixed-precision solve only)
 constexpr int kMaxSplit = 8;  // column splits of the multi-RHS products
 
 hipError_t launch_absmax(const float2* A, int64_t rows, int64_t cols, int64_t lda, float* partial, int nblocks,

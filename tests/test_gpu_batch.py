@@ -88,12 +88,12 @@ def test_batch_gmres_iterations_match_per_item(items):
         assert abs(inf["iterations"] - ir["iterations"]) <= max(2, 0.05 * ir["iterations"]), (inf, ir)
 
 
-@pytest.mark.parametrize("dtype", ["bf16", "fp16"])
+@pytest.mark.parametrize("dtype", ["bf16", "fp16", "fp8"])
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_mixed_precision_matches_fp64(items, dtype, use_graph):
-    """Flexible FP32 GMRES preconditioned by lockstep 16-bit GMRES reaches the
-    FP32 residual target on every item; the solutions agree with an FP64
-    direct solve of the same systems and with the complex64 batched solve."""
+    """Flexible FP32 GMRES preconditioned by lockstep 16-bit or 8-bit GMRES
+    reaches the FP32 residual target on every item; the solutions agree with an
+    FP64 direct solve of the same systems and with the complex64 batched solve."""
     from neuralat_amd.bem.batch import batch_cbie, batch_gmres, batch_solve_mixed
 
     outs = batch_cbie([s.mesh for s, _, _ in items], [k for _, k, _ in items], [p for _, _, p in items])
@@ -117,4 +117,4 @@ def test_batch_mixed_precision_rejects_bad_dtype(items):
 
     A = torch.eye(8, dtype=torch.complex64, device=_dev())
     with pytest.raises(ValueError):
-        batch_solve_mixed([A], [torch.ones(8, dtype=torch.complex64, device=_dev())], dtype="fp8")
+        batch_solve_mixed([A], [torch.ones(8, dtype=torch.complex64, device=_dev())], dtype="int4")
