@@ -344,9 +344,9 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
 
 // 8-bit copies (OCP e4m3, two entries per 32-bit word, real part in the low
 // byte of each 16-bit half): entry (r, c) is round(scale_r A_s(r, c)) with a
-// power-of-two scale per row (a row of a second-kind operator spans a few
-// decades off the diagonal, more than e4m3's three mantissa bits keep per
-// item); L holds padded(n) / 2 words per row (ldl in words), diag the FP32
+// power-of-two scale per row (rows of meshes with unequal triangle sizes
+// differ in magnitude; e4m3 keeps three mantissa bits and a narrow exponent
+// range); L holds padded(n) / 2 words per row (ldl in words), diag the FP32
 // diagonal and rinv[r] = 1 / scale_r.  The product: a wavefront owns kLowRows
 // rows, each lane reads 16 bytes (eight entries) of every row per step.
 typedef float f32x2 __attribute__((ext_vector_type(2)));
