@@ -739,8 +739,8 @@ def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 
 
 def batch_probe(nmesh=8, nk=8, restart=500):
     """The first ``nmesh`` batching meshes x ``nk`` wavenumbers: one persistent
-    assembly launch, one lockstep complex64 GMRES, one lockstep fp16
-    mixed-precision solve (the settings of the batching form)."""
+    assembly launch, one lockstep complex64 GMRES, one lockstep fp16 and one
+    lockstep fp8 mixed-precision solve (the settings of the batching form)."""
     import torch
 
     from neuralat_amd import mesh
@@ -769,10 +769,15 @@ def batch_probe(nmesh=8, nk=8, restart=500):
                               max_inner_cycles=1, max_outer_cycles=3)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    _, i8 = batch_solve_mixed(As, bs, dtype="fp8", tol=1e-6, inner_tol=1e-2, outer_restart=10, inner_restart=restart,
+                              max_inner_cycles=1, max_outer_cycles=4)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
     log(f"bench: batch probe {len(ks)} items: complex64 GMRES {t1 - t0:.3f} s "
         f"({sum(i['iterations'] for i in ib)} its, {sum(i['converged'] for i in ib)} converged), "
         f"fp16 mixed {t2 - t1:.3f} s ({sum(i['inner_iterations'] for i in im)} inner its, "
-        f"{sum(i['converged'] for i in im)} converged)")
+        f"{sum(i['converged'] for i in im)} converged), fp8 mixed {t3 - t2:.3f} s "
+        f"({sum(i['inner_iterations'] for i in i8)} inner its, {sum(i['converged'] for i in i8)} converged)")
 
 
 def modal_form(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=20):

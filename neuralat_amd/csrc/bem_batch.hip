@@ -351,6 +351,9 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
 // rows, each lane reads 16 bytes (eight entries) of every row per step.
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
+// rows per wavefront of the 8-bit product (4 and 16 were 26 % and 34 % slower
+// on the batching solve: fewer bytes in flight, or fewer waves per SIMD)
+constexpr int kFp8Rows = 8;
 __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low,
                                                         const float2* __restrict__ x, float2* __restrict__ y,
                                                         float2 alpha, float2 beta, int beta_zero,
@@ -360,20 +363,20 @@ __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__
     if (skip && skip[s]) return;
     const BSys S = sys[s];
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kLowRows;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFp8Rows;
     if (r0 >= S.n) return;
     const BLow Lw = low[s];
     const float2* xs = x + S.off;
-    float ar[kLowRows], ai[kLowRows];
+    float ar[kFp8Rows], ai[kFp8Rows];
 #pragma unroll
-    for (int q = 0; q < kLowRows; ++q) ar[q] = ai[q] = 0.f;
+    for (int q = 0; q < kFp8Rows; ++q) ar[q] = ai[q] = 0.f;
     const int64_t n8 = S.n >> 3, ld4 = Lw.ldl >> 2;
     const u32x4* L4 = reinterpret_cast<const u32x4*>(Lw.L);
     const float4* x4 = reinterpret_cast<const float4*>(xs);
     for (int64_t c = lane; c < n8; c += 64) {
         const float4 xa = x4[4 * c], xb = x4[4 * c + 1], xc = x4[4 * c + 2], xd = x4[4 * c + 3];
 #pragma unroll
-        for (int q = 0; q < kLowRows; ++q) {
+        for (int q = 0; q < kFp8Rows; ++q) {
             const int64_t r = min(r0 + q, S.n - 1);
             const u32x4 a = __builtin_nontemporal_load(&L4[r * ld4 + c]);
             f32x2 e;
@@ -390,7 +393,7 @@ __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__
     for (int64_t c = 8 * n8 + lane; c < S.n; c += 64) {  // the n % 8 last columns
         const float2 xv = xs[c];
 #pragma unroll
-        for (int q = 0; q < kLowRows; ++q) {
+        for (int q = 0; q < kFp8Rows; ++q) {
             const int64_t r = min(r0 + q, S.n - 1);
             const int w = (int)Lw.L[r * Lw.ldl + (c >> 1)];
             const f32x2 e = (c & 1) ? __builtin_amdgcn_cvt_pk_f32_fp8(w, true) : __builtin_amdgcn_cvt_pk_f32_fp8(w, false);
@@ -398,7 +401,7 @@ __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__
         }
     }
 #pragma unroll
-    for (int q = 0; q < kLowRows; ++q) {
+    for (int q = 0; q < kFp8Rows; ++q) {
         float sr = ar[q], si = ai[q];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) {
@@ -642,8 +645,8 @@ hipError_t xmatvec(const XEng& E, const float2* x, float2* y, float2 alpha, floa
     } else {
         const dim3 g((unsigned)((E.nmax + 4 * kLowRows - 1) / (4 * kLowRows)), (unsigned)E.ns);
         if (E.dt == kDtFP8)
-            hipLaunchKernelGGL(xgemv_fp8_kernel, g, dim3(256), 0, st, E.dsys, E.dlow, x, y, alpha, beta, beta_zero,
-                               skip);
+            hipLaunchKernelGGL(xgemv_fp8_kernel, dim3((unsigned)((E.nmax + 4 * kFp8Rows - 1) / (4 * kFp8Rows)), g.y),
+                               dim3(256), 0, st, E.dsys, E.dlow, x, y, alpha, beta, beta_zero, skip);
         else if (E.dt == kDtBF16)
             hipLaunchKernelGGL(xgemv_lowp_kernel<kDtBF16>, g, dim3(256), 0, st, E.dsys, E.dlow, E.inv_scale, x, y,
                                alpha, beta, beta_zero, skip);
