@@ -303,14 +303,14 @@ int bem_batch_gmres(const bem_batch_system* sys, int n_sys, float tol, int resta
  * whose directions z_j ~ A^-1 v_j come from lockstep GMRES(inner_restart) on
  * a 16-bit copy of the matrix (dtype 1 = bf16, 2 = fp16 with a per-system
  * power-of-two scale) or an 8-bit one (dtype 3 = OCP e4m3 with a
- * power-of-two scale per row; half the bytes of the 16-bit copies, about 1 %
- * from A, so more outer steps); the diagonal kept in FP32, made inside the call in the
- * workspace, run per system to ||v_j - A16 z_j|| <= max(inner_tol,
- * tol ||b|| / ||r_j||) (r_j: the current outer residual; capped at 0.5) or
- * for max_inner_cycles cycles.  Stops when every system has
+ * power-of-two scale per row: half the bytes of the 16-bit copies, about 1 %
+ * from A, so more outer steps), the diagonal kept in FP32, made inside the
+ * call in the workspace, run per system to ||v_j - Alow z_j|| <=
+ * max(inner_tol, tol ||b|| / ||r_j||) (r_j: the current outer residual;
+ * capped at 0.5) or for max_inner_cycles cycles.  Stops when every system has
  * ||b - A x|| <= tol ||b|| (FP32 residual) or after max_outer_cycles cycles.
- * info[q].iterations = outer steps; inner_iterations[q] (optional) = 16-bit
- * Arnoldi steps. */
+ * info[q].iterations = outer steps; inner_iterations[q] (optional) =
+ * low-precision Arnoldi steps. */
 int64_t bem_batch_fgmres_work_size(const bem_batch_system* sys, int n_sys, int outer_restart, int inner_restart);
 int bem_batch_fgmres(const bem_batch_system* sys, int n_sys, int dtype, float tol, float inner_tol, int outer_restart,
                      int inner_restart, int max_inner_cycles, int max_outer_cycles, int use_graph, void* work,
