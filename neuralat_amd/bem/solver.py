@@ -347,7 +347,46 @@ class BEM_Solver:
         RHS = matvec(R, neumann.to(torch.complex64).contiguous())
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 
+    def burton_miller_systems(self, ks, neumanns):
+        """The systems neumann2dirichlet solves, at every wavenumber of ``ks``
+        (``neumanns``: [K, n]): lhs [K, n, n] = bm_lhs + M/2, rhs [K, n] =
+        (bm_rhs - (i/k) M/2) psi.  Each operator kind is assembled for all
+        wavenumbers in one launch per kernel; the mass terms and the products
+        are batched tensor operations."""
+        ks = [float(k) for k in ks]
+        neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
+        identity = self.identity_matrix()
+        lhs = self.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
+        lhs.add_(identity, alpha=0.5)
+        R = self.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
+        half_beta = torch.tensor([0.5j / k for k in ks], dtype=torch.complex64, device=self.device)
+        R.sub_(half_beta[:, None, None] * identity)
+        rhs = torch.matmul(R, neumanns[:, :, None]).squeeze(-1)
+        return lhs, rhs
+
+    def neumann2dirichlet_batch(self, ks, neumanns, tol=1e-6, restart=100, max_cycles=10):
+        """neumann2dirichlet at every wavenumber of ``ks`` (the per-mode loop of
+        scene.py:384-397 in one pass): the Burton-Miller systems of all
+        wavenumbers (burton_miller_systems) solved by one lockstep batched
+        GMRES.  Returns (dirichlet [K, n], infos)."""
+        from .batch import batch_gmres
+
+        lhs, rhs = self.burton_miller_systems(ks, neumanns)
+        xs, infos = batch_gmres(list(lhs), list(rhs), tol=tol, restart=restart, max_cycles=max_cycles)
+        self.last_solve_info = infos
+        return torch.stack(xs), infos
+
     # -- potentials -----------------------------------------------------------
+    def boundary2potential_batch(self, ks, neumanns, dirichlets, points):
+        """boundary2potential at every wavenumber of ``ks``: p [K, P] = D phi -
+        S psi, the potential operators of all wavenumbers in one launch each."""
+        ks = [float(k) for k in ks]
+        neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
+        dirichlets = torch.as_tensor(dirichlets, device=self.device).to(torch.complex64).reshape(len(ks), -1)
+        S = self.potential_matrix_batch(ks, points, "single")
+        D = self.potential_matrix_batch(ks, points, "double")
+        return (torch.matmul(D, dirichlets[:, :, None]) - torch.matmul(S, neumanns[:, :, None])).squeeze(-1)
+
     def potential_matrix(self, k, points, layer_type, basis=None, clamp=None):
         basis = basis or self.basis
         clamp = self.clamp if clamp is None else clamp

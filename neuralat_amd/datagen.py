@@ -184,19 +184,10 @@ def burton_miller_system(solver, k, neumann):
 
 def burton_miller_systems(solver, ks, neumanns):
     """burton_miller_system at every wavenumber of ``ks`` on one surface
-    (``neumanns``: [len(ks), n]): lhs [K, n, n], rhs [K, n].  Each operator kind
-    is assembled for all wavenumbers in one launch per kernel
-    (BEM_Solver.assemble_boundary_matrix_batch); the identity terms and the
-    right-hand-side products are batched tensor operations."""
-    ks = [float(k) for k in ks]
-    identity = solver.identity_matrix()
-    lhs = solver.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
-    lhs.add_(identity, alpha=0.5)
-    R = solver.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
-    half_beta = torch.tensor([0.5j / k for k in ks], dtype=torch.complex64, device=R.device)
-    R.sub_(half_beta[:, None, None] * identity)
-    rhs = torch.matmul(R, neumanns.to(torch.complex64).reshape(len(ks), -1, 1)).squeeze(-1)
-    return lhs, rhs
+    (``neumanns``: [len(ks), n]): lhs [K, n, n], rhs [K, n]
+    (BEM_Solver.burton_miller_systems: one launch per kernel for all
+    wavenumbers)."""
+    return solver.burton_miller_systems(ks, neumanns)
 
 
 def modal_preprocess(obj, mode_num):
@@ -287,16 +278,9 @@ class EditableModalSound:
         surface) solved together by the lockstep batched GMRES."""
         solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
         ks = [float(k) for k in self.ks]
-        neumann = torch.as_tensor(self.neumann_vtx, device=solver.device).to(torch.complex64).reshape(len(ks), -1)
-        lhs, rhs = burton_miller_systems(solver, ks, neumann)
-        xs, infos = batch_gmres(list(lhs), list(rhs), tol=tol, restart=restart, max_cycles=max_cycles)
-        self.solve_infos = infos
-        # p = D phi - S psi for every mode (solver.py:140-147), the potential
-        # operators of all modes in one launch each
-        S = solver.potential_matrix_batch(ks, self.trg_points, "single")
-        D = solver.potential_matrix_batch(ks, self.trg_points, "double")
-        phi = torch.stack(xs)[:, :, None]
-        self.potentials = (torch.matmul(D, phi) - torch.matmul(S, neumann[:, :, None])).squeeze(-1).cpu()
+        phi, self.solve_infos = solver.neumann2dirichlet_batch(ks, self.neumann_vtx, tol=tol, restart=restart,
+                                                               max_cycles=max_cycles)
+        self.potentials = solver.boundary2potential_batch(ks, self.neumann_vtx, phi, self.trg_points).cpu()
         return self.potentials
 
     def solve_loop(self):

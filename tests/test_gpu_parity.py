@@ -243,6 +243,31 @@ def test_assemble_and_potential_batch_k_over_64_wavenumbers():
         assert float(torch.linalg.matrix_norm(Pd[z] - pd) / torch.linalg.matrix_norm(pd)) < 1e-6, z
 
 
+def test_burton_miller_batch_equals_per_wavenumber_calls():
+    """neumann2dirichlet_batch / boundary2potential_batch (all wavenumbers in
+    one launch per kernel, one lockstep GMRES) against neumann2dirichlet and
+    boundary2potential called once per wavenumber."""
+    from neuralat_amd.bem import BEM_Solver
+
+    v, t = mesh.icosphere(2, 0.1)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=9).astype(np.float32)
+    s = BEM_Solver(torch.tensor(v, device=_dev()), torch.tensor(t, device=_dev()))
+    ks = [3.0, 18.3, 41.0]
+    rng = np.random.default_rng(3)
+    psi = torch.tensor((rng.standard_normal((3, len(v))) + 1j * rng.standard_normal((3, len(v)))).astype(np.complex64),
+                       device=_dev())
+    pts = torch.tensor(mesh.random_exterior_points(20, 0.2, 0.5, seed=4).astype(np.float32), device=_dev())
+    _say("Burton-Miller batch over 3 wavenumbers")
+    phi, infos = s.neumann2dirichlet_batch(ks, psi, tol=1e-6)
+    p = s.boundary2potential_batch(ks, psi, phi, pts)
+    assert all(i["converged"] for i in infos)
+    for z, k in enumerate(ks):
+        d = s.neumann2dirichlet(k, psi[z], tol=1e-6)
+        q = s.boundary2potential(k, psi[z], d, pts)
+        assert float(torch.linalg.vector_norm(phi[z] - d) / torch.linalg.vector_norm(d)) < 1e-5, k
+        assert float(torch.linalg.vector_norm(p[z] - q) / torch.linalg.vector_norm(q)) < 1e-5, k
+
+
 def test_set_vertices_equals_new_handle():
     """A handle moved with bem_mesh_set_vertices assembles what a handle built
     from the new positions assembles (P1 and DP0 operators, potentials,
