@@ -243,6 +243,22 @@ def test_assemble_and_potential_batch_k_over_64_wavenumbers():
         assert float(torch.linalg.matrix_norm(Pd[z] - pd) / torch.linalg.matrix_norm(pd)) < 1e-6, z
 
 
+def test_p1_potential_point_chunks():
+    """The P1 potential goes through a per-(point, triangle) scratch of at most
+    512 MB per launch: 1100 points on the level-5 mesh (540 MB) take two point
+    chunks; each point's entries are computed and summed the same way in any
+    chunking, so the result equals two calls of 550 points bit for bit."""
+    v, t = mesh.icosphere(5, 1.0)
+    v = v.astype(np.float32)
+    s = _solver(v, t, "P1")
+    pts = torch.tensor(mesh.random_exterior_points(1100, 1.5, 3.0, seed=8).astype(np.float32), device=_dev())
+    _say("P1 potential, 1100 points at level 5")
+    whole = s.potential_matrix(9.0, pts, "double")
+    halves = torch.cat([s.potential_matrix(9.0, pts[:550], "double"), s.potential_matrix(9.0, pts[550:], "double")])
+    assert torch.equal(whole, halves)
+    assert float(whole.abs().max()) > 0.0
+
+
 def test_burton_miller_batch_equals_per_wavenumber_calls():
     """neumann2dirichlet_batch / boundary2potential_batch (all wavenumbers in
     one launch per kernel, one lockstep GMRES) against neumann2dirichlet and
