@@ -221,6 +221,23 @@ def test_assemble_batch_k_matches_single_calls(basis, op, quad):
         assert float(pad[n * ld:].abs().max()) == 0.0  # nothing written past the matrix
 
 
+def test_assemble_batch_k_level5_double_layer_vs_paired_single_calls():
+    """Above 4096 vertices a single double-layer call takes the paired sweep;
+    a batch of wavenumbers takes the ordered-pair sweep: the two agree to
+    rounding (level 5, 10242 vertices, differences formed on the device)."""
+    v, t = mesh.icosphere(5, 1.0)
+    v = mesh.sh_perturb(v, 6, 0.05, seed=12).astype(np.float32)
+    s = _solver(v, t, "P1")
+    ks = [4.0, 15.0]
+    _say("batch_k level 5 double layer")
+    A = s.assemble_boundary_matrix_batch(ks, "double")
+    for z, k in enumerate(ks):
+        one = s.assemble_boundary_matrix(k, "double")
+        e = float(torch.linalg.matrix_norm(A[z] - one) / torch.linalg.matrix_norm(one))
+        del one
+        assert e < 1e-6, (k, e)
+
+
 def test_assemble_and_potential_batch_k_over_64_wavenumbers():
     """More wavenumbers than one launch takes (64): the host splits them; the
     potential operators of every wavenumber match single calls."""
