@@ -598,6 +598,7 @@ struct XEng {
     const int* skip = nullptr;   // items left out of the residual products (null: none)
     bool x0_zero = false;        // cycles start from x = 0 (r = b, no product)
     bool final_residual = true;  // cycles end with the true residual and the flags conv
+    bool one_pass = false;       // one Gram-Schmidt pass per step (h2 kept zero)
     float2 *V = nullptr, *w = nullptr, *r = nullptr, *B = nullptr, *X = nullptr;
     double2 *H = nullptr, *h = nullptr, *h2 = nullptr, *cs = nullptr, *sn = nullptr, *g = nullptr, *y = nullptr;
     double *res = nullptr, *rn2 = nullptr, *wn2 = nullptr, *tgt = nullptr;
@@ -700,10 +701,12 @@ hipError_t xcycle_steps(const XEng& E, int j0, int j1, hipStream_t st)
                            m + 1, (const int*)E.done);
         hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,
                            (const int*)nullptr, E.h, m + 1, -1.0, E.w, (const int*)E.done);
-        hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, E.ns), dim3(256), 0, st, E.dsys, E.V, E.ntot, E.w, E.h2,
-                           m + 1, (const int*)E.done);
-        hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,
-                           (const int*)nullptr, E.h2, m + 1, -1.0, E.w, (const int*)E.done);
+        if (!E.one_pass) {
+            hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, E.ns), dim3(256), 0, st, E.dsys, E.V, E.ntot, E.w,
+                               E.h2, m + 1, (const int*)E.done);
+            hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,
+                               (const int*)nullptr, E.h2, m + 1, -1.0, E.w, (const int*)E.done);
+        }
         hipLaunchKernelGGL(xnorm2_kernel, dim3(E.ns), dim3(256), 0, st, E.dsys, E.w, E.wn2);
         hipLaunchKernelGGL(xnormalize_kernel, gx, dim3(256), 0, st, E.dsys, E.w, E.wn2,
                            E.V + (int64_t)(j + 1) * E.ntot, (const int*)E.done);
@@ -1125,6 +1128,12 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
     Ei.dt = dtype;
     Ei.inv_scale = inv;
     Ei.skip = Eo.done;
+    // the inner solve is a preconditioner (the outer loop forms true FP32
+    // residuals): one Gram-Schmidt pass, the second-pass coefficients kept
+    // zero (3-4 % shorter on the batching solves, the same inner step counts)
+    Ei.one_pass = true;
+    hipLaunchKernelGGL(xfill_kernel, dim3((unsigned)((4 * (size_t)ns * (mi + 1) + 255) / 256)), dim3(256), 0, s,
+                       (unsigned int*)Ei.h2, (int64_t)4 * ns * (mi + 1), 0u);
 
     std::vector<double> hb, hr(ns);
     if (int rc = gather_systems(sys, hs, Eo, tol, hb, s)) return rc;
