@@ -560,12 +560,13 @@ def make_batch(nmesh, nk, seed=7, levels=(2, 3, 4, 5)):
 
 def batch_form(nmesh=64, nk=8, restart=500, maxiter=2000, seed=7, levels=(2, 3, 4, 5), chunk_gb=115.0,
                mixed_dtype="fp16", fp64_items=8):
-    """The batching workload (nmesh seeded meshes x nk wavenumbers), three ways:
+    """The batching workload (nmesh seeded meshes x nk wavenumbers), four ways:
     a plain per-item loop (fused assembly + GMRES per (mesh, k)); one
     persistent assembly launch + lockstep graph-captured complex64 GMRES; the
     same assembly + the lockstep mixed-precision solve (FP32 flexible GMRES,
-    16-bit inner GMRES).  The items' complex64 systems exceed one GPU's HBM
-    for the full workload, so all three run on chunks of consecutive meshes
+    16-bit inner GMRES); the same with 8-bit (e4m3) inner copies.  The items'
+    complex64 systems exceed one GPU's HBM for the full workload, so all four
+    run on chunks of consecutive meshes
     whose systems fit ``chunk_gb`` (one persistent launch per chunk).  Neumann
     data: one interior monopole per mesh.  Also reports the matrix bytes the
     GMRES passes stream per second of solve time (sum over items of
