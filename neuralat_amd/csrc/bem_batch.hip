@@ -363,7 +363,8 @@ __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__
     if (skip && skip[s]) return;
     const BSys S = sys[s];
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kFp8Rows;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * kFp8Rows;
     if (r0 >= S.n) return;
     const BLow Lw = low[s];
     const float2* xs = x + S.off;
@@ -373,23 +374,64 @@ __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__
     const int64_t n8 = S.n >> 3, ld4 = Lw.ldl >> 2;
     const u32x4* L4 = reinterpret_cast<const u32x4*>(Lw.L);
     const float4* x4 = reinterpret_cast<const float4*>(xs);
-    for (int64_t c = lane; c < n8; c += 64) {
-        const float4 xa = x4[4 * c], xb = x4[4 * c + 1], xc = x4[4 * c + 2], xd = x4[4 * c + 3];
+    // Software-pipelined over a step count that is uniform across the
+    // wavefront: step t covers column words 64 t + lane; a lane past the last
+    // word reads the last word again with x taken as zero, so no lane leaves
+    // the loop early and the next step's row words and x are requested before
+    // this step's arithmetic (8 KB of matrix reads in flight per wavefront
+    // while it decodes).  Global-address-space pointers keep the loads off
+    // the flat path.  Per lane the summation order is the plain loop's.
+    typedef const u32x4 __attribute__((address_space(1)))* GRow;
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    typedef const f32x4 __attribute__((address_space(1)))* GVec;
+    const GRow Lg = (GRow)L4;
+    const GVec xg = (GVec)x4;
+    int64_t roff[kFp8Rows];
+#pragma unroll
+    for (int q = 0; q < kFp8Rows; ++q) roff[q] = min(r0 + q, S.n - 1) * ld4;
+    const int64_t nst = (n8 + 63) >> 6;
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+    u32x4 a[kFp8Rows], an[kFp8Rows];
+    f32x4 xv[4], xn[4];
+    // x is zeroed past the last word at use, not at the request, so that
+    // nothing waits on the loads before the arithmetic of the step before
+    auto fetch = [&](int64_t t, u32x4 (&m)[kFp8Rows], f32x4 (&v)[4]) {
+        const int64_t cc = min(t * 64 + lane, n8 - 1);
+#pragma unroll
+        for (int q = 0; q < kFp8Rows; ++q) m[q] = __builtin_nontemporal_load(&Lg[roff[q] + cc]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = xg[4 * cc + i];
+    };
+    auto step = [&](int64_t t, const u32x4 (&m)[kFp8Rows], const f32x4 (&vr)[4]) {
+        const bool in = t * 64 + lane < n8;
+        f32x4 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = in ? vr[i] : z4;
 #pragma unroll
         for (int q = 0; q < kFp8Rows; ++q) {
-            const int64_t r = min(r0 + q, S.n - 1);
-            const u32x4 a = __builtin_nontemporal_load(&L4[r * ld4 + c]);
-            f32x2 e;
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.x, false); cfma(make_float2(e.x, e.y), xa.x, xa.y, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.x, true);  cfma(make_float2(e.x, e.y), xa.z, xa.w, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.y, false); cfma(make_float2(e.x, e.y), xb.x, xb.y, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.y, true);  cfma(make_float2(e.x, e.y), xb.z, xb.w, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.z, false); cfma(make_float2(e.x, e.y), xc.x, xc.y, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.z, true);  cfma(make_float2(e.x, e.y), xc.z, xc.w, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.w, false); cfma(make_float2(e.x, e.y), xd.x, xd.y, ar[q], ai[q]);
-            e = __builtin_amdgcn_cvt_pk_f32_fp8((int)a.w, true);  cfma(make_float2(e.x, e.y), xd.z, xd.w, ar[q], ai[q]);
+            const uint32_t w[4] = {m[q].x, m[q].y, m[q].z, m[q].w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                f32x2 e = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[j], false);
+                cfma(make_float2(e.x, e.y), v[j].x, v[j].y, ar[q], ai[q]);
+                e = __builtin_amdgcn_cvt_pk_f32_fp8((int)w[j], true);
+                cfma(make_float2(e.x, e.y), v[j].z, v[j].w, ar[q], ai[q]);
+            }
         }
+    };
+    if (nst > 0) fetch(0, a, xv);
+    int64_t t = 0;
+    for (; t + 2 <= nst; t += 2) {  // ping-pong: no register copies
+        fetch(t + 1, an, xn);
+        __builtin_amdgcn_sched_barrier(0);  // keep the requests ahead of the arithmetic
+        step(t, a, xv);
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(t + 2, a, xv);  // past the end: every lane rereads the last word
+        __builtin_amdgcn_sched_barrier(0);
+        step(t + 1, an, xn);
+        __builtin_amdgcn_sched_barrier(0);
     }
+    if (t < nst) step(t, a, xv);
     for (int64_t c = 8 * n8 + lane; c < S.n; c += 64) {  // the n % 8 last columns
         const float2 xv = xs[c];
 #pragma unroll
