@@ -287,7 +287,8 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
     if (skip && skip[s]) return;
     const BSys S = sys[s];
     const int lane = threadIdx.x & 63;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kLowRows;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * kLowRows;
     if (r0 >= S.n) return;
     const BLow Lw = low[s];
     const float2* xs = x + S.off;
@@ -297,18 +298,53 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
     const int64_t n4 = S.n >> 2, ld4 = Lw.ldl >> 2;
     const u32x4* L4 = reinterpret_cast<const u32x4*>(Lw.L);
     const float4* x4 = reinterpret_cast<const float4*>(xs);
-    for (int64_t c = lane; c < n4; c += 64) {
-        const float4 xa = x4[2 * c], xb = x4[2 * c + 1];
+    // software-pipelined over a wave-uniform step count, as the 8-bit
+    // product below: step t covers column words 64 t + lane, a lane past the
+    // last word rereads it with x zeroed at use, and the next step's loads are
+    // requested before this step's arithmetic
+    typedef const u32x4 __attribute__((address_space(1)))* GRow;
+    typedef float f32x4 __attribute__((ext_vector_type(4)));
+    typedef const f32x4 __attribute__((address_space(1)))* GVec;
+    const GRow Lg = (GRow)L4;
+    const GVec xg = (GVec)x4;
+    int64_t roff[kLowRows];
+#pragma unroll
+    for (int q = 0; q < kLowRows; ++q) roff[q] = min(r0 + q, S.n - 1) * ld4;
+    const int64_t nst = (n4 + 63) >> 6;
+    const f32x4 z4 = {0.f, 0.f, 0.f, 0.f};
+    u32x4 a[kLowRows], an[kLowRows];
+    f32x4 xv[2], xn[2];
+    auto fetch = [&](int64_t t, u32x4 (&m)[kLowRows], f32x4 (&v)[2]) {
+        const int64_t cc = min(t * 64 + lane, n4 - 1);
+#pragma unroll
+        for (int q = 0; q < kLowRows; ++q) m[q] = __builtin_nontemporal_load(&Lg[roff[q] + cc]);
+        v[0] = xg[2 * cc];
+        v[1] = xg[2 * cc + 1];
+    };
+    auto step = [&](int64_t t, const u32x4 (&m)[kLowRows], const f32x4 (&vr)[2]) {
+        const bool in = t * 64 + lane < n4;
+        const f32x4 xa = in ? vr[0] : z4, xb = in ? vr[1] : z4;
 #pragma unroll
         for (int q = 0; q < kLowRows; ++q) {
-            const int64_t r = min(r0 + q, S.n - 1);
-            const u32x4 a = __builtin_nontemporal_load(&L4[r * ld4 + c]);
-            cfma(lp_pair<DT>(a.x), xa.x, xa.y, ar[q], ai[q]);
-            cfma(lp_pair<DT>(a.y), xa.z, xa.w, ar[q], ai[q]);
-            cfma(lp_pair<DT>(a.z), xb.x, xb.y, ar[q], ai[q]);
-            cfma(lp_pair<DT>(a.w), xb.z, xb.w, ar[q], ai[q]);
+            cfma(lp_pair<DT>(m[q].x), xa.x, xa.y, ar[q], ai[q]);
+            cfma(lp_pair<DT>(m[q].y), xa.z, xa.w, ar[q], ai[q]);
+            cfma(lp_pair<DT>(m[q].z), xb.x, xb.y, ar[q], ai[q]);
+            cfma(lp_pair<DT>(m[q].w), xb.z, xb.w, ar[q], ai[q]);
         }
+    };
+    if (nst > 0) fetch(0, a, xv);
+    int64_t t = 0;
+    for (; t + 2 <= nst; t += 2) {
+        fetch(t + 1, an, xn);
+        __builtin_amdgcn_sched_barrier(0);
+        step(t, a, xv);
+        __builtin_amdgcn_sched_barrier(0);
+        fetch(t + 2, a, xv);  // past the end: every lane rereads the last word
+        __builtin_amdgcn_sched_barrier(0);
+        step(t + 1, an, xn);
+        __builtin_amdgcn_sched_barrier(0);
     }
+    if (t < nst) step(t, a, xv);
     for (int64_t c = 4 * n4 + lane; c < S.n; c += 64) {  // the n % 4 last columns
         const float2 xv = xs[c];
 #pragma unroll
