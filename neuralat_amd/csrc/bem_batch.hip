@@ -388,7 +388,8 @@ __global__ __launch_bounds__(256) void xgemv_lowp_kernel(const BSys* __restrict_
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 // rows per wavefront of the 8-bit product (4 and 16 were 26 % and 34 % slower
-// on the batching solve: fewer bytes in flight, or fewer waves per SIMD)
+// on the batching solve: fewer bytes in flight, or fewer waves per SIMD; with
+// the pipelined loop below, 4 rows still take 16 % longer per full call)
 constexpr int kFp8Rows = 8;
 __global__ __launch_bounds__(256) void xgemv_fp8_kernel(const BSys* __restrict__ sys, const BLow* __restrict__ low,
                                                         const float2* __restrict__ x, float2* __restrict__ y,
