@@ -77,7 +77,7 @@ def _ad_and_fd(level, k0, ndir, h, hk_rel, seed=0):
         hk = hk_rel * k0
         fdk = (float(_loss_gpu(vt.detach(), tt, k0 + hk, src, pts_t))
                - float(_loss_gpu(vt.detach(), tt, k0 - hk, src, pts_t))) / (2 * hk)
-    return out[h], out[0.5 * h], gk, fdk, float(L)
+    return out[h], out[0.5 * h], gk, fdk, float(L.detach())
 
 
 def test_gradient_vs_central_differences_level4():
