@@ -13,11 +13,19 @@ One step = the three solves (assembly of K and V, GMRES, exterior potential).
 ``value`` = operator entries assembled (2 N^2 per wavenumber) per second of
 the whole timed region; ``assembly_entries_per_s`` counts assembly time only.
 
-The ``roofline`` object is for the dominant kernel, the fused regular-pair
-DP0 kernel that evaluates the double layer (stored as -K) and the single
-layer (contracted with psi): algorithmic FLOP per launch = (29 + 16) FLOP x
-36 Gauss-point pairs x (N^2 - neighbour pairs), divided by its mean duration
-from HIP events recorded on the stream it runs on.
+The ``roofline`` object is for the dominant kernel, the complex64 GEMV inside
+GMRES (over 90 % of the timed region): algorithmic bytes per launch
+8 N^2 + 16 N, divided by its mean duration from HIP events recorded around
+every product on the stream GMRES launches it on.  ``roofline_assembly`` is
+the fused regular-pair DP0 kernel that evaluates the double layer (stored as
+-K) and the single layer (contracted with psi): algorithmic FLOP per launch =
+(29 + 16) FLOP x 36 Gauss-point pairs x (N^2 - neighbour pairs) over its
+HIP-event duration.
+
+Beside the plain method, the same mesh runs through the reference's own
+production path (P1 Burton-Miller, neumann2dirichlet) at every k, and the
+level-6 systems' rows are checked against FP64 oracle rows in the CPU leg
+(with the FP64 residual of the GPU solutions over those rows).
 
 The CPU leg (at the end of the run, on the host cores) is the only part that
 runs the oracle: ``cpu_baseline`` times its FP32 mirror of the reference
@@ -69,6 +77,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--check-level", type=int, default=3, help="mesh level of the FP64 oracle check (0 = skip)")
     p.add_argument("--no-forms", action="store_true", help="skip the precision / batching / differentiation forms")
+    p.add_argument("--no-bm", action="store_true",
+                   help="skip the Burton-Miller (P1 neumann2dirichlet) solves on the workload mesh")
     p.add_argument("--batch-meshes", type=int, default=64)
     p.add_argument("--batch-ks", type=int, default=8)
     p.add_argument("--traffic-probe", action="store_true",
@@ -187,6 +197,18 @@ class Fp64Checks:
                 into.setdefault(key, 0.0)
             else:
                 into[key] = None
+
+    def add_rows(self, into, kind, v, t, k, rows, a_rows, b_rows, x, psi, clamp=True):
+        """Rows ``rows`` of a GPU system (a_rows [r, n], b_rows [r]) and its GPU
+        solution x: the CPU leg writes the relative error of the rows against
+        FP64 oracle rows and the FP64 residual ||b - A x|| / ||b|| over them.
+        kind "dp0_cbie": (M/2 - K, -V psi); kind "p1_bm": (bm_lhs + M/2,
+        (bm_rhs - (i/k) M/2) psi) with the approximate rules."""
+        self.items.append({"kind": kind, "into": into, "v": np.asarray(v, dtype=np.float32), "t": t, "k": float(k),
+                           "rows": np.asarray(rows), "a": a_rows, "b": b_rows, "x": x, "psi": np.asarray(psi),
+                           "clamp": bool(clamp)})
+        for key in ("rows_rel_err_vs_fp64", "rhs_rel_err_vs_fp64", "fp64_residual_over_rows"):
+            into[key] = None
 
     def add_modes(self, into, v, t, material, k, eigenvalues):
         """eigenvalues: {result key: GPU lowest non-rigid eigenvalues of the
@@ -492,6 +514,69 @@ def p1_form(level=5, k=10.0, bm_level=4, bm_k=20.0, reps=3):
                             "rel_err_vs_analytic": float(np.linalg.norm(p.cpu().numpy() - pe) / np.linalg.norm(pe))}
     log(f"bench: P1 Burton-Miller level {bm_level}: {el * 1e3:.1f} ms, "
         f"{s.last_solve_info['iterations']} GMRES its, error {out['burton_miller']['rel_err_vs_analytic']:.2e}")
+    return out
+
+
+def burton_miller_large_form(v, t, src, pts, ks, restart=1500, tol=1e-6, nrows=16):
+    """The reference's production path on the workload mesh itself:
+    neumann2dirichlet (P1 Burton-Miller, bm_lhs / bm_rhs with the approximate
+    rules, src/bem/solver.py:118-126) + GMRES + boundary2potential at every k,
+    with the r-clamp on (the reference's default) and off.  Per solve: wall
+    time (assembly of both operators + mass terms + right-hand side, GMRES,
+    exterior pressure), entries (2 V^2) per second, the error against the
+    analytic field of the 4 monopoles; with the clamp on, rows of the system
+    and the solution are kept for the FP64 row check of the CPU leg."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.solver import BEM_Solver, gmres
+
+    dev = torch.device("cuda:0")
+    v32 = v.astype(np.float32)
+    s = BEM_Solver(torch.tensor(v32, device=dev), torch.tensor(t, device=dev), basis="P1")
+    nv = v.shape[0]
+    pts_t = torch.tensor(pts.astype(np.float32), device=dev)
+    rng = np.random.default_rng(17)
+    rows = np.sort(np.concatenate([np.arange(8), rng.choice(np.arange(8, nv), nrows - 8, replace=False)]))
+    ridx = torch.tensor(rows, device=dev)
+    psis, fields = {}, {}
+    for k in ks:
+        val, dn = mesh.sources_field(src, None, k)
+        psis[k] = torch.tensor(mesh.project_p1(v32, t, lambda x, nn: dn(x, nn)).astype(np.complex64), device=dev)
+        fields[k] = val(pts)
+    out = {"vertices": nv, "basis": "P1", "entries_per_solve": 2 * nv * nv, "gmres_tol": tol,
+           "gmres_restart": restart, "system": "(bm_lhs + M/2) phi = (bm_rhs - (i/k) M/2) psi, approximate rules"}
+    lhs, rhs = s.burton_miller_system(ks[0], psis[ks[0]])  # warm-up
+    del lhs, rhs
+    for clamp in (True, False):
+        s.clamp = clamp
+        key = "clamp_on" if clamp else "clamp_off"
+        out[key] = {}
+        for k in ks:
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record()
+            lhs, rhs = s.burton_miller_system(k, psis[k])
+            e1.record()
+            phi, info = gmres(lhs, rhs, tol=tol, restart=restart, maxiter=3000)
+            p = s.boundary2potential(k, psis[k], phi, pts_t)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            asm_ms = e0.elapsed_time(e1)
+            err = float(np.linalg.norm(p.cpu().numpy() - fields[k]) / np.linalg.norm(fields[k]))
+            res = {"seconds": el, "solves_per_s": 1.0 / el, "entries_per_s": 2 * nv * nv / el,
+                   "assembly_ms": asm_ms, "assembly_entries_per_s": 2 * nv * nv / (asm_ms / 1e3),
+                   "gmres_iterations": info["iterations"], "converged": bool(info["converged"]),
+                   "rel_err_vs_analytic": err}
+            if clamp:
+                CHECKS.add_rows(res, "p1_bm", v32, t, k, rows, lhs.index_select(0, ridx).cpu().numpy(),
+                                rhs.index_select(0, ridx).cpu().numpy(), phi.cpu().numpy(), psis[k].cpu().numpy())
+            out[key][str(k)] = res
+            log(f"bench: Burton-Miller level 6 ({key}) k={k}: {el:.2f} s (assembly {asm_ms:.0f} ms), "
+                f"{info['iterations']} GMRES its, error vs analytic {err:.2e}")
+            del lhs, rhs, phi
+    del s
     return out
 
 
@@ -892,6 +977,9 @@ def fp64_answers(checks):
             for key, w in it["eigs"].items():
                 it["into"][key] = float(np.max(np.abs(np.asarray(w) - w64) / w64))
             continue
+        if it["kind"] in ("dp0_cbie", "p1_bm"):
+            _fp64_rows_answer(it)
+            continue
         v, t, k, psi = it["v"], it["t"], it["k"], it["psi"].astype(np.complex128)
         S = oracle.assemble(v, t, k, "single", "DP0")
         D = oracle.assemble(v, t, k, "double", "DP0")
@@ -905,6 +993,47 @@ def fp64_answers(checks):
             errs.update({key: float(np.linalg.norm(p - p64) / np.linalg.norm(p64)) for key, p in it["ps"].items()})
         for key, e in errs.items():
             it["into"][key] = max(it["into"][key] or 0.0, e) if it["mode"] == "max" else e
+
+
+def _p1_mass_rows(v, t, rows):
+    import scipy.sparse as sp
+
+    v = np.asarray(v, np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]]), axis=1)
+    r = np.concatenate([t[:, a] for a in range(3) for b in range(3)])
+    c = np.concatenate([t[:, b] for a in range(3) for b in range(3)])
+    d = np.concatenate([area / 12.0 * (1.0 + (a == b)) for a in range(3) for b in range(3)])
+    return sp.csr_matrix((d, (r, c)), shape=(len(v), len(v)))[np.asarray(rows)].toarray()
+
+
+def _fp64_rows_answer(it):
+    """FP64 oracle rows of a recorded level-6 system and the residual of the
+    GPU solution over them (oracle.entries / oracle.p1_rows: no full matrix)."""
+    import oracle
+
+    v, t, k, rows, psi = it["v"], it["t"], it["k"], it["rows"], it["psi"].astype(np.complex128)
+    clamp = it["clamp"]
+    if it["kind"] == "dp0_cbie":
+        nt = t.shape[0]
+        r = np.repeat(rows.astype(np.int32), nt)
+        c = np.tile(np.arange(nt, dtype=np.int32), len(rows))
+        K = oracle.entries(v, t, k, r, c, "double", clamp=clamp).reshape(len(rows), nt)
+        V = oracle.entries(v, t, k, r, c, "single", clamp=clamp).reshape(len(rows), nt)
+        v64 = v.astype(np.float64)
+        area = 0.5 * np.linalg.norm(np.cross(v64[t[:, 1]] - v64[t[:, 0]], v64[t[:, 2]] - v64[t[:, 0]]), axis=1)
+        A = -K
+        A[np.arange(len(rows)), rows] += 0.5 * area[rows]
+        b = -(V @ psi)
+    else:
+        M = _p1_mass_rows(v, t, rows)
+        A = oracle.p1_rows(v, t, k, rows, "bm_lhs", quad="approx", clamp=clamp) + 0.5 * M
+        b = (oracle.p1_rows(v, t, k, rows, "bm_rhs", quad="approx", clamp=clamp) - 0.5j / k * M) @ psi
+    x = np.asarray(it["x"], np.complex128)
+    into = it["into"]
+    into["rows_rel_err_vs_fp64"] = float(np.linalg.norm(it["a"] - A) / np.linalg.norm(A))
+    into["rhs_rel_err_vs_fp64"] = float(np.linalg.norm(it["b"] - b) / np.linalg.norm(b))
+    into["fp64_residual_over_rows"] = float(np.linalg.norm(A @ x - b) / np.linalg.norm(b))
+    into["rows_checked"] = int(len(rows))
 
 
 def cpu_leg(args, v, t, k):
@@ -1068,12 +1197,24 @@ def main():
     pts_t = torch.tensor(pts.astype(np.float32), device=dev)
     npairs = s.mesh.n_neighbor_pairs
 
-    def step(record):
+    # rows of the level-6 system kept from the (untimed) warm-up for the FP64
+    # row check of the CPU leg: the first 32 and 32 seeded others
+    rows = np.sort(np.concatenate([np.arange(32), np.random.default_rng(5).choice(np.arange(32, n), 32,
+                                                                                   replace=False)]))
+    ridx = torch.tensor(rows, device=dev)
+    fp64_rows = {}
+
+    def step(record, capture=False):
         res = []
         for k in ks:
             phi, p, info = s.solve(k, psis[k], pts_t, args.tol, args.restart, args.maxiter)
             reg_ms, asm_ms = s.timings()
             res.append((k, p, info, reg_ms, asm_ms))
+            if capture:
+                fp64_rows[str(k)] = {}
+                CHECKS.add_rows(fp64_rows[str(k)], "dp0_cbie", v, t, k, rows,
+                                s.lhs.index_select(0, ridx).cpu().numpy(), s.rhs.index_select(0, ridx).cpu().numpy(),
+                                phi.cpu().numpy(), psis[k].cpu().numpy(), clamp=bool(s.clamp))
             if record:
                 log(f"bench: k={k} fused regular kernel {reg_ms:.1f} ms, assembly {asm_ms:.1f} ms, "
                     f"GMRES its {info['iterations']} res {info['rel_residual']:.2e}")
@@ -1081,7 +1222,7 @@ def main():
 
     for w in range(args.warmup):
         log(f"bench: warmup step {w}")
-        step(True)
+        step(True, capture=(w == 0))
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
@@ -1114,6 +1255,9 @@ def main():
         forms["precision"] = precision_form(s, v, t, ks[-1], pts, restart=args.restart)
     del s
     torch.cuda.empty_cache()
+    if not args.no_bm:
+        forms["burton_miller_level6"] = burton_miller_large_form(v, t, src, pts, ks, restart=args.restart)
+        torch.cuda.empty_cache()
     if not args.no_forms:
         forms["batching"] = batch_form(args.batch_meshes, args.batch_ks)
         torch.cuda.empty_cache()
@@ -1151,6 +1295,7 @@ def main():
         "gmres_iterations": its,
         "rel_err_vs_analytic": errs,
         "rel_err_vs_fp64_small_mesh": {"level": args.check_level, **fp64},
+        "rel_err_vs_fp64_level6_rows": fp64_rows,
         "roofline": {"bound": "hbm", "kernel": "gemv_n_vec_kernel<4> (complex64 GEMV in GMRES)",
                      "achieved": gemv_bytes / gemv_avg_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": gemv_bytes / gemv_avg_s / HBM_PEAK, "traffic": _per_entry(GEMV_TRAFFIC_PER_ENTRY, n),

@@ -148,6 +148,16 @@ int bem_add_identity(const bem_mesh* mesh, int basis, float alpha_re, float alph
                      void* stream);
 /* Vertex average of per-triangle complex values (bind.cu:23-48). */
 int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream);
+/* The same without a mesh handle, on device triangles [nt,3] (the reference's
+ * triangle2vertex(vertices, triangles, features) call shape, bind.cu:23-25,217,
+ * which reads no positions): the vertex -> triangle lists are built on the
+ * device in work (bem_triangle2vertex_work_size bytes) and each vertex sums
+ * its triangles in ascending order (deterministic).  Waits for the stream;
+ * returns BEM_ERR_BAD_MESH for an index outside [0, nv).  A vertex in no
+ * triangle gets NaN (0/0), as the reference. */
+int64_t bem_triangle2vertex_work_size(int64_t nv, int64_t nt);
+int bem_triangle2vertex_raw(const int32_t* tris, int64_t nt, int64_t nv, const void* tri_values, void* vert_values,
+                            void* work, void* stream);
 /* Potential operator from the surface to np points ([np,3] float32, device):
  * out [np, nv] (P1) or [np, nt] (DP0); op = SINGLE or DOUBLE
  * (single/double_boundary_potential, bind.cu:156-176,219-222). */

@@ -73,6 +73,8 @@ SIGNATURES = {
     "bem_boundary_matrix": (_I32, [_P, _I64, _P, _I64, _F, _I32, _I32, _P, _P]),
     "bem_identity": (_I32, [_P, _I32, _P, _I64, _P]),
     "bem_triangle2vertex": (_I32, [_P, _P, _P, _P]),
+    "bem_triangle2vertex_work_size": (_I64, [_I64, _I64]),
+    "bem_triangle2vertex_raw": (_I32, [_P, _I64, _I64, _P, _P, _P, _P]),
     "bem_potential": (_I32, [_P, _P, _I64, _F, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_mesh_set_vertices": (_I32, [_P, _P, _I32, _P]),
     "bem_potential_batch_k": (_I32, [_P, _P, _I64, _P, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),

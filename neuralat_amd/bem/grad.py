@@ -9,16 +9,37 @@ to the Neumann data psi.  The backward pass solves the adjoint system
 the library's adjoint kernels, which recompute every derivative block on the
 fly.  The reference has no gradients; this is checked against central finite
 differences and the FP64 oracle in tests/test_gpu_grad.py.
+
+A forward or adjoint GMRES solve that stops at ``maxiter`` before reaching
+``tol`` raises ``NotConvergedError`` (``raise_on_fail=True``, the default);
+with ``raise_on_fail=False`` it warns instead.  Either way the two solves'
+reports are kept in ``last_solve_info`` ({"forward": ..., "adjoint": ...}).
 """
 from __future__ import annotations
 
 import ctypes
 import math
+import warnings
 
 import torch
 
 from .. import _lib
 from .solver import Mesh, _ptr, _stream, gmres, matvec
+
+# reports of the most recent forward and adjoint solves (dicts of
+# iterations, converged, rel_residual)
+last_solve_info = {"forward": None, "adjoint": None}
+
+
+def _report(which, info, raise_on_fail):
+    last_solve_info[which] = info
+    if info["converged"]:
+        return
+    msg = (f"exterior_pressure: {which} GMRES stopped after {info['iterations']} iterations at relative "
+           f"residual {info['rel_residual']:.2e}")
+    if raise_on_fail:
+        raise _lib.NotConvergedError(msg, info)
+    warnings.warn(msg, RuntimeWarning, stacklevel=3)
 
 _RULE6 = ((0.445948490915965, 0.445948490915965, 0.223381589678011),
           (0.091576213509771, 0.091576213509771, 0.109951743655322),
@@ -51,7 +72,7 @@ def monopole_neumann_dp0(vertices, triangles, sources, k, amplitudes=None):
 
 class _ExteriorPressure(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, vertices, k, psi, triangles, points, clamp, tol, restart, maxiter):
+    def forward(ctx, vertices, k, psi, triangles, points, clamp, tol, restart, maxiter, raise_on_fail):
         verts = vertices.detach().contiguous().float()
         mesh = Mesh(verts, triangles)
         kf = float(k)
@@ -66,6 +87,8 @@ class _ExteriorPressure(torch.autograd.Function):
                                     _ptr(work), _stream()), "bem_cbie_dp0")
         del work
         phi, info = gmres(lhs, rhs, tol=tol, restart=restart, maxiter=maxiter)
+        last_solve_info["adjoint"] = None
+        _report("forward", info, raise_on_fail)
         pts = points.detach().to(torch.float32).contiguous()
         S = _potential(mesh, pts, kf, 0, clamp)
         D = _potential(mesh, pts, kf, 1, clamp)
@@ -73,6 +96,7 @@ class _ExteriorPressure(torch.autograd.Function):
         matvec(S, psi, alpha=-1.0, beta=1.0, out=p)
         ctx.mesh, ctx.lhs, ctx.S, ctx.D = mesh, lhs, S, D
         ctx.kf, ctx.clamp, ctx.tol, ctx.restart, ctx.maxiter = kf, clamp, tol, restart, maxiter
+        ctx.raise_on_fail = raise_on_fail
         ctx.save_for_backward(phi, psi, pts)
         ctx.info = info
         return p
@@ -96,6 +120,8 @@ class _ExteriorPressure(torch.autograd.Function):
         if rc not in (_lib.BEM_OK, _lib.BEM_ERR_NOT_CONVERGED):
             _lib.check(rc, "bem_gmres_ex")
         del wk
+        ctx.adjoint_info = info.as_dict()
+        _report("adjoint", ctx.adjoint_info, ctx.raise_on_fail)
         gvert = torch.zeros((mesh.nv, 3), dtype=torch.float64, device=dev)
         gk = torch.zeros(1, dtype=torch.float64, device=dev)
         vhl = torch.zeros((nt, 2), dtype=torch.float64, device=dev)
@@ -109,9 +135,8 @@ class _ExteriorPressure(torch.autograd.Function):
         # dL/dpsi (torch convention, 2 dL/dpsi*): -2 (S^H pbar + V^H lambda)
         shp = matvec(S, pbar, trans=2)
         gpsi = -2.0 * (shp + torch.complex(vhl[:, 0], vhl[:, 1]).to(torch.complex64))
-        ctx.adjoint_info = info.as_dict()
         gk_t = gk.reshape(())
-        return (gvert.to(torch.float32), gk_t.to(torch.float32), gpsi, None, None, None, None, None, None)
+        return (gvert.to(torch.float32), gk_t.to(torch.float32), gpsi, None, None, None, None, None, None, None)
 
 
 def _potential(mesh, pts, k, op, clamp):
@@ -121,8 +146,10 @@ def _potential(mesh, pts, k, op, clamp):
     return out
 
 
-def exterior_pressure(vertices, triangles, k, psi, points, clamp=True, tol=1e-6, restart=200, maxiter=2000):
+def exterior_pressure(vertices, triangles, k, psi, points, clamp=True, tol=1e-6, restart=200, maxiter=2000,
+                      raise_on_fail=True):
     """Differentiable p(points) of the DP0 CBIE solve; see the module doc."""
     if not torch.is_tensor(k):
         k = torch.tensor(float(k), dtype=torch.float32, device=vertices.device)
-    return _ExteriorPressure.apply(vertices, k, psi, triangles, points, clamp, tol, restart, maxiter)
+    return _ExteriorPressure.apply(vertices, k, psi, triangles, points, clamp, tol, restart, maxiter,
+                                   bool(raise_on_fail))

@@ -196,14 +196,63 @@ def bicgstab(A: torch.Tensor, b: torch.Tensor, x=None, nsteps=500, tol=1e-10, at
     return x, info.as_dict()
 
 
+def _bicgstab_over(apply, b, x=None, nsteps=None, tol=1e-10, atol=1e-16):
+    """BiCGSTAB (van der Vorst, SIAM J. Sci. Stat. Comput. 13, 1992, Alg. 3)
+    over a caller-supplied operator ``apply(v) -> A v``, with the stopping
+    behaviour of the reference's src/bem/BiCGSTAB.py:30-107:
+
+    * the inner products of the recurrence are bilinear (no conjugation), the
+      shadow residual is the initial residual;
+    * after each step the TRUE residual b - A x is formed and the solve stops
+      when ||b - A x||^2 < tol ||b||^2 or < atol (so one extra product per step);
+    * the recurrence breaks off when its rho vanishes, and running out of steps
+      returns the current iterate without a signal.
+
+    ``nsteps`` None means len(b) steps."""
+    b = b.detach().clone()
+    x = torch.zeros_like(b) if x is None else x
+    limit = tol * torch.vdot(b, b).real
+
+    def done(r):
+        rr = torch.vdot(r, r).real
+        return bool(rr < limit or rr < atol)
+
+    r = b - apply(x)
+    if done(r):
+        return x
+    shadow = r.clone()
+    # state of the two-term recurrence: previous rho, alpha, omega and the
+    # search direction p with its image v = A p
+    state = {"rho": 1.0, "alpha": 1.0, "omega": 1.0}
+    p = torch.zeros_like(b)
+    v = torch.zeros_like(b)
+    budget = b.shape[0] if nsteps is None else nsteps
+    while budget:
+        rho = (r * shadow).sum()
+        p = r + (rho / state["rho"]) * (state["alpha"] / state["omega"]) * (p - state["omega"] * v)
+        v = apply(p)
+        alpha = rho / (shadow * v).sum()
+        s = r - alpha * v
+        t = apply(s)
+        omega = (t * s).sum() / (t * t).sum()
+        x = x + alpha * p + omega * s
+        if done(b - apply(x)):
+            return x
+        r = s - omega * t
+        state.update(rho=rho, alpha=alpha, omega=omega)
+        if rho == 0:
+            break
+        budget -= 1
+    return x
+
+
 def solve_linear_equation(A_func, b, x=None, nsteps=500, tol=1e-10, atol=1e-16):
     """Reference signature and semantics (solver.py:61-66): BiCGSTAB with
     ||r||^2 < tol ||b||^2 (or < atol), returning x silently on failure.  A
     dense complex64 matrix runs on the library's device BiCGSTAB; a callable
-    runs the same iteration over the caller's matvec."""
+    runs the same iteration over the caller's matvec (``_bicgstab_over``)."""
     if callable(A_func):
-        from .BiCGSTAB import BiCGSTAB
-        return BiCGSTAB(A_func).solve(b, x=x, nsteps=nsteps, tol=tol, atol=atol)
+        return _bicgstab_over(A_func, b, x=x, nsteps=nsteps, tol=tol, atol=atol)
     return bicgstab(A_func, b, x=x, nsteps=nsteps, tol=tol, atol=atol)[0]
 
 
@@ -338,30 +387,39 @@ class BEM_Solver:
         RHS = matvec(R, neumann.to(torch.complex64).contiguous())
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 
-    def neumann2dirichlet(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):
-        """Burton-Miller with the approximate rules (solver.py:118-126):
-        (bm_lhs + M/2) phi = (bm_rhs - beta M/2) psi."""
+    def burton_miller_system(self, k, neumann):
+        """The system neumann2dirichlet solves (solver.py:120-125), with the
+        approximate rules: (bm_lhs + M/2, (bm_rhs - (i/k) M/2) psi).  The mass
+        terms are added in place (no dense identity) and the right-hand side
+        is one library GEMV."""
         beta = 1j / k
         LHS = add_identity_(self, self.assemble_boundary_matrix(k, "bm_lhs", approx=True), 0.5)
         R = add_identity_(self, self.assemble_boundary_matrix(k, "bm_rhs", approx=True), -0.5 * beta)
         RHS = matvec(R, neumann.to(torch.complex64).contiguous())
+        return LHS, RHS
+
+    def neumann2dirichlet(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):
+        """Burton-Miller with the approximate rules (solver.py:118-126):
+        (bm_lhs + M/2) phi = (bm_rhs - beta M/2) psi."""
+        LHS, RHS = self.burton_miller_system(k, neumann)
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 
     def burton_miller_systems(self, ks, neumanns):
         """The systems neumann2dirichlet solves, at every wavenumber of ``ks``
         (``neumanns``: [K, n]): lhs [K, n, n] = bm_lhs + M/2, rhs [K, n] =
         (bm_rhs - (i/k) M/2) psi.  Each operator kind is assembled for all
-        wavenumbers in one launch per kernel; the mass terms and the products
-        are batched tensor operations."""
+        wavenumbers in one launch per kernel; the mass terms are added in
+        place and the right-hand sides are library GEMVs."""
         ks = [float(k) for k in ks]
         neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
-        identity = self.identity_matrix()
+        neumanns = neumanns.contiguous()
         lhs = self.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
-        lhs.add_(identity, alpha=0.5)
         R = self.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
-        half_beta = torch.tensor([0.5j / k for k in ks], dtype=torch.complex64, device=self.device)
-        R.sub_(half_beta[:, None, None] * identity)
-        rhs = torch.matmul(R, neumanns[:, :, None]).squeeze(-1)
+        rhs = torch.empty((len(ks), lhs.shape[1]), dtype=torch.complex64, device=self.device)
+        for z, k in enumerate(ks):
+            add_identity_(self, lhs[z], 0.5)
+            add_identity_(self, R[z], -0.5j / k)
+            matvec(R[z], neumanns[z], out=rhs[z])
         return lhs, rhs
 
     def neumann2dirichlet_batch(self, ks, neumanns, tol=1e-6, restart=100, max_cycles=10):
@@ -383,9 +441,14 @@ class BEM_Solver:
         ks = [float(k) for k in ks]
         neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
         dirichlets = torch.as_tensor(dirichlets, device=self.device).to(torch.complex64).reshape(len(ks), -1)
+        neumanns, dirichlets = neumanns.contiguous(), dirichlets.contiguous()
         S = self.potential_matrix_batch(ks, points, "single")
         D = self.potential_matrix_batch(ks, points, "double")
-        return (torch.matmul(D, dirichlets[:, :, None]) - torch.matmul(S, neumanns[:, :, None])).squeeze(-1)
+        p = torch.empty((len(ks), S.shape[1]), dtype=torch.complex64, device=self.device)
+        for z in range(len(ks)):
+            matvec(D[z], dirichlets[z], out=p[z])
+            matvec(S[z], neumanns[z], alpha=-1.0, beta=1.0, out=p[z])
+        return p
 
     def potential_matrix(self, k, points, layer_type, basis=None, clamp=None):
         basis = basis or self.basis
@@ -428,12 +491,19 @@ class BEM_Solver:
 
 
 def map_triangle2vertex(vertices, triangles, features):
-    """Vertex average of per-triangle complex values (solver.py:150-154)."""
+    """Vertex average of per-triangle complex values (solver.py:150-154), on
+    the triangles alone (no mesh handle is built)."""
     check_tensor(vertices, torch.float32)
     check_tensor(triangles, torch.int32)
     check_tensor(features, torch.complex64)
-    mesh = Mesh(vertices, triangles)
-    out = torch.empty(vertices.shape[0], dtype=torch.complex64, device=vertices.device)
-    _lib.check(_lib.load().bem_triangle2vertex(mesh.handle, _ptr(features), _ptr(out), _stream()),
-               "bem_triangle2vertex")
+    nv, nt = vertices.shape[0], triangles.shape[0]
+    if features.shape[0] != nt:
+        raise ValueError("features must hold one value per triangle")
+    lib = _lib.load()
+    out = torch.empty(nv, dtype=torch.complex64, device=vertices.device)
+    work = torch.empty(max(int(lib.bem_triangle2vertex_work_size(nv, nt)), 1), dtype=torch.uint8,
+                       device=vertices.device)
+    with torch.cuda.device(vertices.device):
+        _lib.check(lib.bem_triangle2vertex_raw(_ptr(triangles), nt, nv, _ptr(features), _ptr(out), _ptr(work),
+                                               _stream()), "bem_triangle2vertex_raw")
     return out

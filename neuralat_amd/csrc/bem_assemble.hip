@@ -1236,6 +1236,99 @@ __global__ __launch_bounds__(256) void tri2vert_kernel(const float2* __restrict_
     out[v] = make_float2(sr / n, si / n);
 }
 
+// Stateless triangle -> vertex average (no mesh handle): the vertex ->
+// triangle lists are built here.  work (int32): count/cursor [nv], ptr [nv+1],
+// idx [3 nt], flag [1].  The fill order of a vertex's list depends on the
+// atomics, so the gather sums each list in ascending triangle order (a
+// repeated triangle, from a degenerate one, counts with its multiplicity):
+// the result does not depend on the order.
+__global__ __launch_bounds__(256) void fill_i32_kernel(int32_t* __restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) out[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void t2v_count_kernel(const int32_t* __restrict__ tris, int64_t nt, int64_t nv,
+                                                        int32_t* __restrict__ cnt, int32_t* __restrict__ flag)
+{
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < 3 * nt; e += (int64_t)gridDim.x * 256) {
+        const int32_t v = tris[e];
+        if (v < 0 || v >= nv) {
+            atomicOr(flag, 1);
+            continue;
+        }
+        atomicAdd(&cnt[v], 1);
+    }
+}
+
+// exclusive scan of cnt [nv] into ptr [nv+1] by one workgroup of 1024
+// threads (each owns a contiguous chunk); cnt becomes the fill cursor
+__global__ __launch_bounds__(1024) void t2v_scan_kernel(int32_t* __restrict__ cnt, int64_t nv,
+                                                        int32_t* __restrict__ ptr)
+{
+    __shared__ int32_t part[1024];
+    const int tid = threadIdx.x;
+    const int64_t chunk = (nv + 1023) / 1024;
+    const int64_t b = tid * chunk, e = b + chunk < nv ? b + chunk : nv;
+    int32_t sum = 0;
+    for (int64_t i = b; i < e; ++i) sum += cnt[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t add = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += add;
+        __syncthreads();
+    }
+    int32_t run = part[tid] - sum;
+    for (int64_t i = b; i < e; ++i) {
+        const int32_t c = cnt[i];
+        ptr[i] = run;
+        cnt[i] = run;
+        run += c;
+    }
+    if (tid == 1023) ptr[nv] = part[1023];
+}
+
+__global__ __launch_bounds__(256) void t2v_fill_kernel(const int32_t* __restrict__ tris, int64_t nt, int64_t nv,
+                                                       int32_t* __restrict__ cursor, int32_t* __restrict__ idx)
+{
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < 3 * nt; e += (int64_t)gridDim.x * 256) {
+        const int32_t v = tris[e];
+        if (v < 0 || v >= nv) continue;
+        idx[atomicAdd(&cursor[v], 1)] = (int32_t)(e / 3);
+    }
+}
+
+__global__ __launch_bounds__(256) void t2v_gather_kernel(const float2* __restrict__ tv, const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ idx, int64_t nv,
+                                                         float2* __restrict__ out)
+{
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < nv; v += (int64_t)gridDim.x * 256) {
+        const int e0 = ptr[v], e1 = ptr[v + 1];
+        float sr = 0.f, si = 0.f;
+        int32_t last = -1;
+        for (int done = e0; done < e1;) {
+            // next larger triangle index in the list and its multiplicity
+            int32_t next = 0x7fffffff;
+            for (int e = e0; e < e1; ++e) {
+                const int32_t t = idx[e];
+                if (t > last && t < next) next = t;
+            }
+            int mult = 0;
+            for (int e = e0; e < e1; ++e) mult += (idx[e] == next);
+            const float2 x = tv[next];
+            for (int m = 0; m < mult; ++m) {
+                sr += x.x;
+                si += x.y;
+            }
+            done += mult;
+            last = next;
+        }
+        const float n = (float)(e1 - e0);
+        out[v] = make_float2(sr / n, si / n);
+    }
+}
+
 // Surface -> point operator (bind.cu:156-176, integrand.h:237-255,396-406):
 // row p = evaluation point (n_x = 0), column = triangle (DP0) or its three
 // vertices (P1, summed with atomics into a zeroed output).
@@ -1598,6 +1691,22 @@ hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s)
 {
     hipLaunchKernelGGL(tri2vert_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, tv, m.vt_ptr, m.vt_idx, m.nv, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_tri2vert_raw(const int32_t* tris, int64_t nt, int64_t nv, const float2* tv, float2* out,
+                               int32_t* work, hipStream_t s)
+{
+    int32_t* cnt = work;
+    int32_t* ptr = cnt + nv;
+    int32_t* idx = ptr + nv + 1;
+    int32_t* flag = idx + 3 * nt;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid1d(nv, 256, 65536)), dim3(256), 0, s, cnt, nv);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, s, flag, (int64_t)1);
+    hipLaunchKernelGGL(t2v_count_kernel, dim3(grid1d(3 * nt, 256, 65536)), dim3(256), 0, s, tris, nt, nv, cnt, flag);
+    hipLaunchKernelGGL(t2v_scan_kernel, dim3(1), dim3(1024), 0, s, cnt, nv, ptr);
+    hipLaunchKernelGGL(t2v_fill_kernel, dim3(grid1d(3 * nt, 256, 65536)), dim3(256), 0, s, tris, nt, nv, cnt, idx);
+    hipLaunchKernelGGL(t2v_gather_kernel, dim3(grid1d(nv, 256, 65536)), dim3(256), 0, s, tv, ptr, idx, nv, out);
     return hipGetLastError();
 }
 

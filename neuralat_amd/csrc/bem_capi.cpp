@@ -384,6 +384,30 @@ int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert
     return BEM_OK;
 }
 
+int64_t bem_triangle2vertex_work_size(int64_t nv, int64_t nt)
+{
+    if (nv <= 0 || nt <= 0) return 0;
+    return (int64_t)sizeof(int32_t) * (2 * nv + 3 * nt + 2);
+}
+
+int bem_triangle2vertex_raw(const int32_t* tris, int64_t nt, int64_t nv, const void* tri_values, void* vert_values,
+                            void* work, void* stream)
+{
+    if (!tris || !tri_values || !vert_values || !work) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (nv <= 0 || nt <= 0) return fail(BEM_ERR_BAD_ARG, "empty mesh");
+    if (3 * nt >= INT32_MAX || nv >= INT32_MAX) return fail(BEM_ERR_BAD_ARG, "mesh too large for 32-bit lists");
+    int32_t* w = (int32_t*)work;
+    hipError_t e = bem::launch_tri2vert_raw(tris, nt, nv, (const float2*)tri_values, (float2*)vert_values, w,
+                                           S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_triangle2vertex_raw launch");
+    int32_t flag = 0;
+    e = hipMemcpyAsync(&flag, w + 2 * nv + 1 + 3 * nt, sizeof(flag), hipMemcpyDeviceToHost, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_triangle2vertex_raw sync");
+    if (flag) return fail(BEM_ERR_BAD_MESH, "triangle vertex index out of range");
+    return BEM_OK;
+}
+
 int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
                   void* out, int64_t ld, void* stream)
 {

@@ -13,6 +13,9 @@ Contents and the reference code each part restates:
   ``src/bem/cuda/include/integrand.h:63-394`` (regular and Sauter-Schwab
   quadrature), ``potential.h:18-65`` (kernels, r-clamp).  Implemented in C
   (``bem_oracle.c``), FP64 and an FP32 mirror.
+* ``entries``    -- DP0 entries at listed (row, column) pairs, and
+  ``p1_rows`` -- P1 rows of listed vertices: the same values as ``assemble``
+  (pinned against it in tests/test_oracle_invariants.py).
 * ``potential``  -- surface -> point operators: ``bind.cu:156-176``,
   ``integrand.h:237-255,396-406``.
 * ``identity``   -- mass matrix: ``bind.cu:10-21``, ``integrand.h:257-278``.
@@ -109,6 +112,12 @@ def _load():
             f = getattr(lib, "oracle_identity" + sfx)
             f.argtypes = [P, i64, P, i64, i32, P, i32, P]
             f.restype = i32
+            f = getattr(lib, "oracle_entries" + sfx)
+            f.argtypes = [P, i64, P, i64, real, i32, P, i32, P, i32, i32, P, P, i64, P, i32]
+            f.restype = i32
+            f = getattr(lib, "oracle_p1_rows" + sfx)
+            f.argtypes = [P, i64, P, i64, real, i32, P, i32, P, i32, i32, P, i64, P, i32]
+            f.restype = i32
         lib.oracle_neighbors.argtypes = [P, i64, i64, P, P, P, i64, P]
         lib.oracle_neighbors.restype = ctypes.c_int64
         lib.oracle_max_threads.restype = i32
@@ -165,6 +174,43 @@ def assemble(verts, tris, k, op="single", basis="P1", quad="full", precision="f6
               lr.shape[0], int(clamp), r0, r1, parts, _ptr(out), nthreads)
     _check(code)
     return (out[..., 0].astype(np.float64) + 1j * out[..., 1].astype(np.float64))
+
+
+def entries(verts, tris, k, rows, cols, op="single", quad="full", precision="f64", clamp=True, nthreads=0):
+    """DP0 matrix entries at the (rows[e], cols[e]) triangle pairs as
+    complex128 [len(rows)]: the same values ``assemble(..., basis="DP0")``
+    puts there, without building the whole matrix."""
+    lib = _load()
+    v, t, dt = _prep(verts, tris, precision)
+    r = np.ascontiguousarray(np.asarray(rows, dtype=np.int32).reshape(-1))
+    c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32).reshape(-1))
+    if r.shape != c.shape:
+        raise ValueError("rows and cols must have the same length")
+    nl, nq = QUADS[quad]
+    tr = np.ascontiguousarray(TRI_RULES[nq], dtype=dt)
+    lr = np.ascontiguousarray(LINE_RULES[nl], dtype=dt)
+    out = np.zeros((r.shape[0], 2), dtype=dt)
+    fn = getattr(lib, "oracle_entries_" + precision)
+    _check(fn(_ptr(v), v.shape[0], _ptr(t), t.shape[0], float(k), OPS[op], _ptr(tr), tr.shape[0], _ptr(lr),
+              lr.shape[0], int(clamp), _ptr(r), _ptr(c), r.shape[0], _ptr(out), nthreads))
+    return out[:, 0].astype(np.float64) + 1j * out[:, 1].astype(np.float64)
+
+
+def p1_rows(verts, tris, k, row_vertices, op="single", quad="full", precision="f64", clamp=True, nthreads=0):
+    """Rows ``row_vertices`` of the P1 matrix as complex128 [len, V]: the same
+    values ``assemble(..., basis="P1")`` puts in those rows, without building
+    the whole matrix (cost ~ 6 N pair evaluations per row)."""
+    lib = _load()
+    v, t, dt = _prep(verts, tris, precision)
+    rv = np.ascontiguousarray(np.asarray(row_vertices, dtype=np.int32).reshape(-1))
+    nl, nq = QUADS[quad]
+    tr = np.ascontiguousarray(TRI_RULES[nq], dtype=dt)
+    lr = np.ascontiguousarray(LINE_RULES[nl], dtype=dt)
+    out = np.zeros((rv.shape[0], v.shape[0], 2), dtype=dt)
+    fn = getattr(lib, "oracle_p1_rows_" + precision)
+    _check(fn(_ptr(v), v.shape[0], _ptr(t), t.shape[0], float(k), OPS[op], _ptr(tr), tr.shape[0], _ptr(lr),
+              lr.shape[0], int(clamp), _ptr(rv), rv.shape[0], _ptr(out), nthreads))
+    return out[..., 0].astype(np.float64) + 1j * out[..., 1].astype(np.float64)
 
 
 def potential(verts, tris, points, k, op="single", basis="P1", precision="f64", clamp=True, nthreads=0,

@@ -338,6 +338,91 @@ int FN(oracle_assemble)(const R *verts, int64_t nv, const int32_t *tris, int64_t
     return ORACLE_OK;
 }
 
+/* DP0 entries at listed (row, column) triangle pairs: out[2e], out[2e+1] =
+ * entry (rows[e], cols[e]) of the same matrix oracle_assemble builds for the
+ * DP0 basis (the pair is regular when the triangles share no vertex,
+ * bind.cu:67-79, else Sauter-Schwab, bind.cu:81-152; the entry is the
+ * unweighted sum result[9] of integrand.h:234,392).  For checks of a few
+ * rows or columns of a large mesh without assembling it whole. */
+int FN(oracle_entries)(const R *verts, int64_t nv, const int32_t *tris, int64_t nt, R k, int op,
+                       const R *tri_rule, int nq, const R *line_rule, int nl, int clamp, const int32_t *rows,
+                       const int32_t *cols, int64_t n, R *out, int nthreads)
+{
+    if (nv <= 0 || nt <= 0 || n < 0) return ORACLE_BAD_ARG;
+    if (op == ORACLE_HYP || op == ORACLE_BM_LHS) return ORACLE_BAD_ARG;
+    for (int64_t t = 0; t < 3 * nt; ++t)
+        if (tris[t] < 0 || tris[t] >= nv) return ORACLE_BAD_MESH;
+    for (int64_t e = 0; e < n; ++e)
+        if (rows[e] < 0 || rows[e] >= nt || cols[e] < 0 || cols[e] >= nt) return ORACLE_BAD_ARG;
+    FN(tri_t) *T = FN(make_tris)(verts, tris, nt);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t e = 0; e < n; ++e) {
+        const FN(tri_t) *Ti = &T[rows[e]], *Tj = &T[cols[e]];
+        int sh = 0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) sh += (Ti->idx[a] == Tj->idx[b]);
+        FN(blk_t) B;
+        if (sh) FN(pair_singular)(Ti, Tj, k, op, line_rule, nl, clamp, &B);
+        else FN(pair_regular)(Ti, Tj, k, op, tri_rule, nq, clamp, &B);
+        FN(post_combine)(Ti, Tj, k, op, &B);
+        out[2 * e] = B.re[9];
+        out[2 * e + 1] = B.im[9];
+    }
+    free(T);
+    return ORACLE_OK;
+}
+
+/* P1 rows of listed vertices: out [nrows, nv] complex = rows row_vertices[e]
+ * of the matrix oracle_assemble builds for the P1 basis.  Row a of the P1
+ * matrix only receives the blocks of pairs whose row triangle holds vertex a
+ * (bind.cu:67-79,144-151 scatter B[alpha][beta] to (T_i.alpha, T_j.beta)), so
+ * each row is the sum over the triangles around a of their pairs with every
+ * column triangle.  For checks of a few rows of a large mesh. */
+int FN(oracle_p1_rows)(const R *verts, int64_t nv, const int32_t *tris, int64_t nt, R k, int op,
+                       const R *tri_rule, int nq, const R *line_rule, int nl, int clamp, const int32_t *row_vertices,
+                       int64_t nrows, R *out, int nthreads)
+{
+    if (nv <= 0 || nt <= 0 || nrows < 0) return ORACLE_BAD_ARG;
+    for (int64_t t = 0; t < 3 * nt; ++t)
+        if (tris[t] < 0 || tris[t] >= nv) return ORACLE_BAD_MESH;
+    for (int64_t e = 0; e < nrows; ++e)
+        if (row_vertices[e] < 0 || row_vertices[e] >= nv) return ORACLE_BAD_ARG;
+    FN(tri_t) *T = FN(make_tris)(verts, tris, nt);
+    memset(out, 0, sizeof(R) * 2 * (size_t)nrows * (size_t)nv);
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    /* one thread per listed row: its sum runs over its own triangles in
+     * ascending order, so the result does not depend on the thread count */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t e = 0; e < nrows; ++e) {
+        int32_t a = row_vertices[e];
+        R *row = out + 2 * e * nv;
+        for (int64_t i = 0; i < nt; ++i) {
+            const FN(tri_t) *Ti = &T[i];
+            int alpha = -1;
+            for (int c = 0; c < 3; ++c)
+                if (Ti->idx[c] == a) alpha = c;
+            if (alpha < 0) continue;
+            for (int64_t j = 0; j < nt; ++j) {
+                const FN(tri_t) *Tj = &T[j];
+                int sh = 0;
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d) sh += (Ti->idx[c] == Tj->idx[d]);
+                FN(blk_t) B;
+                if (sh) FN(pair_singular)(Ti, Tj, k, op, line_rule, nl, clamp, &B);
+                else FN(pair_regular)(Ti, Tj, k, op, tri_rule, nq, clamp, &B);
+                FN(post_combine)(Ti, Tj, k, op, &B);
+                for (int beta = 0; beta < 3; ++beta) {
+                    row[2 * Tj->idx[beta]] += B.re[3 * alpha + beta];
+                    row[2 * Tj->idx[beta] + 1] += B.im[3 * alpha + beta];
+                }
+            }
+        }
+    }
+    free(T);
+    return ORACLE_OK;
+}
+
 /* Potential operator from the surface to points (integrand.h:237-255,
  * 396-406; bind.cu:156-176).  out: [np, nv] (P1) or [np, nt] (DP0). */
 int FN(oracle_potential)(const R *verts, int64_t nv, const int32_t *tris, int64_t nt, const R *pts, int64_t np, R k,

@@ -41,8 +41,11 @@ def test_library_built_for_gfx950_only():
     from neuralat_amd import _lib
 
     data = open(_lib.LIB_PATH, "rb").read()
-    assert b"gfx950" in data
-    assert b"sm_" not in data[:0]  # no CUDA targets exist in a hipcc build
+    # every offload bundle in the fat binary targets gfx950, and no NVIDIA
+    # target (nvptx / sm_NN) is embedded anywhere in the file
+    targets = set(re.findall(rb"amdgcn-amd-amdhsa-[-a-z0-9:+]*", data))
+    assert targets == {b"amdgcn-amd-amdhsa--gfx950"}, targets
+    assert re.search(rb"nvptx|sm_[0-9]{2}", data) is None
 
 
 def test_missing_library_fails_loudly(monkeypatch):

@@ -168,3 +168,32 @@ def test_gradient_cost_ratio():
     ratio = min(ts_b) / min(ts_f)
     print(f"forward {min(ts_f) * 1e3:.1f} ms, backward {min(ts_b) * 1e3:.1f} ms, ratio {ratio:.2f}", flush=True)
     assert ratio < 6.0
+
+
+def test_nonconverged_solves_are_signalled():
+    """A forward or adjoint GMRES that stops at maxiter before tol raises
+    NotConvergedError, or with raise_on_fail=False warns, and both solves'
+    reports are kept in grad.last_solve_info."""
+    from neuralat_amd import _lib
+    from neuralat_amd.bem import grad
+    from neuralat_amd.bem.grad import exterior_pressure, monopole_neumann_dp0
+
+    v, t, pts, src = _setup(2)
+    vt = torch.tensor(v, device=_dev(), requires_grad=True)
+    tt = torch.tensor(t, device=_dev())
+    pts_t = torch.tensor(pts, device=_dev())
+    psi = monopole_neumann_dp0(vt, tt, src, 9.0)
+    with pytest.raises(_lib.NotConvergedError):
+        exterior_pressure(vt, tt, 9.0, psi, pts_t, maxiter=2)
+    assert grad.last_solve_info["forward"]["iterations"] == 2
+    with pytest.warns(RuntimeWarning, match="forward GMRES"):
+        p = exterior_pressure(vt, tt, 9.0, psi, pts_t, maxiter=2, raise_on_fail=False)
+    with pytest.warns(RuntimeWarning, match="adjoint GMRES"):
+        (p.real ** 2 + p.imag ** 2).sum().backward()
+    assert not grad.last_solve_info["forward"]["converged"]
+    assert not grad.last_solve_info["adjoint"]["converged"]
+    # a converged pair of solves reports both as converged
+    vt.grad = None
+    p = exterior_pressure(vt, tt, 9.0, monopole_neumann_dp0(vt, tt, src, 9.0), pts_t)
+    (p.real ** 2 + p.imag ** 2).sum().backward()
+    assert grad.last_solve_info["forward"]["converged"] and grad.last_solve_info["adjoint"]["converged"]

@@ -418,6 +418,39 @@ def test_identity_potential_t2v_vs_oracle(l2_sphere):
     assert _rel(out, oracle.triangle2vertex(len(v), t, f)) < 1e-6
 
 
+def test_triangle2vertex_without_mesh_handle():
+    """map_triangle2vertex builds its vertex lists on the device: equal to the
+    handle-based kernel bit for bit and to the oracle's average; repeated
+    calls give the same bits (the sum order does not follow the atomics);
+    a vertex in no triangle gets NaN (0/0, as the reference); an index out of
+    range raises instead of reading out of bounds."""
+    from neuralat_amd.bem import Mesh, map_triangle2vertex
+    from neuralat_amd.bem.solver import _ptr, _stream
+    from neuralat_amd import _lib
+
+    v, t = mesh.icosphere(4, 1.0)
+    v = v.astype(np.float32)
+    rng = np.random.default_rng(2)
+    f = (rng.standard_normal(len(t)) + 1j * rng.standard_normal(len(t))).astype(np.complex64)
+    vt, tt, ft = (torch.tensor(x, device=_dev()) for x in (v, t, f))
+    out = map_triangle2vertex(vt, tt, ft)
+    again = map_triangle2vertex(vt, tt, ft)
+    assert torch.equal(out, again)
+    m = Mesh(vt, tt)
+    ref = torch.empty_like(out)
+    _lib.check(_lib.load().bem_triangle2vertex(m.handle, _ptr(ft), _ptr(ref), _stream()))
+    assert torch.equal(out, ref)
+    assert _rel(out.cpu().numpy(), oracle.triangle2vertex(len(v), t, f)) < 1e-6
+    # an isolated extra vertex, and a bad index
+    v2 = torch.cat([vt, torch.zeros((1, 3), device=_dev())])
+    out2 = map_triangle2vertex(v2, tt, ft).cpu().numpy()
+    assert np.isnan(out2[-1]) and np.array_equal(out2[:-1], out.cpu().numpy())
+    bad = tt.clone()
+    bad[5, 1] = len(v) + 3
+    with pytest.raises(ValueError):
+        map_triangle2vertex(vt, bad, ft)
+
+
 def test_negative_wavenumber_is_conjugate(l2_sphere):
     v, t = l2_sphere
     s = _solver(v, t, "P1")

@@ -213,3 +213,89 @@ def test_neighbors_icosphere():
         s = sh[ptr[tri]:ptr[tri + 1]]
         assert (s == 3).sum() == 1 and (s == 2).sum() == 3
     assert 4 <= ncol <= 16
+
+
+def test_entries_equal_assembled_matrix():
+    """oracle.entries returns exactly the DP0 entries oracle.assemble builds,
+    for regular and neighbour pairs alike."""
+    v, t = mesh.icosphere(2, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=3)
+    rng = np.random.default_rng(0)
+    r = rng.integers(0, len(t), 400)
+    c = rng.integers(0, len(t), 400)
+    ptr, idx, _, _ = oracle.neighbors(t, len(v))
+    r[:40] = np.repeat(np.arange(4), 10)[:40]
+    c[:40] = np.concatenate([idx[ptr[i]:ptr[i] + 10] for i in range(4)])[:40]
+    for op in ("single", "double", "adjointdouble", "bm_rhs"):
+        A = oracle.assemble(v, t, 7.0, op, "DP0")
+        assert np.array_equal(oracle.entries(v, t, 7.0, r, c, op), A[r, c]), op
+    with pytest.raises(ValueError):
+        oracle.entries(v, t, 7.0, [0], [len(t)], "single")
+    with pytest.raises(ValueError):
+        oracle.entries(v, t, 7.0, [0], [1], "hypersingular")
+
+
+@pytest.mark.parametrize("level", [0, 1])
+def test_hypersingular_normal_product_at_neighbour_pairs(level):
+    """The k^2 term of the Maue form (bind.cu:103-108,127-134) multiplies each
+    pair's single-layer block by n_i . n_j of the two triangles of the pair
+    (bind.cu:105-106 takes the neighbour's normal, normals[adj[j]]).  The
+    surface curls of the three P1 hat functions of a triangle sum to zero and
+    the hat functions sum to one, so for the neighbour (Sauter-Schwab) pairs
+    alone, with the same rule on both sides,
+        1^T W_sing 1 = -k^2 sum_(i,j) (n_i . n_j) V^DP0_sing[i, j]
+    exactly, with the normals taken here from numpy.  On the octahedron
+    sphere (level 0) the neighbour pairs have n_i . n_j in {1, 1/3, -1/3}, and
+    at level 1 they spread from 1/3 to 1, so a normal product taken from any
+    other triangle fails it."""
+    v, t = mesh.octasphere(level, 1.0)
+    k = 10.0
+    W = oracle.assemble(v, t, k, "hypersingular", "P1", parts=2)
+    V = oracle.assemble(v, t, k, "single", "DP0", parts=2)
+    n = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    lhs = W.sum()
+    rhs = -k * k * np.sum((n @ n.T) * V)
+    assert abs(lhs - rhs) <= 1e-12 * abs(rhs), (lhs, rhs)
+    # the identity is not vacuous: the pairs' normal products differ
+    nn = (n @ n.T)[np.abs(V) > 0]
+    assert np.ptp(nn) > 0.5, np.ptp(nn)
+
+
+def test_callable_bicgstab_matches_restatement():
+    """The product's BiCGSTAB over a caller's matvec (solve_linear_equation
+    with a callable) follows the restated iteration of BiCGSTAB.py step for
+    step; it is plain torch, so it runs here on CPU tensors."""
+    import torch
+
+    from neuralat_amd.bem.solver import _bicgstab_over
+
+    rng = np.random.default_rng(1)
+    n = 80
+    A = np.eye(n) * 3 + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n) * 0.4
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    At, bt = torch.tensor(A), torch.tensor(b)
+    for nsteps in (1, 2, 5, 500):
+        x_or, steps, conv = oracle.bicgstab(lambda z: A @ z, b, nsteps=nsteps)
+        x = _bicgstab_over(lambda z: At @ z, bt, nsteps=nsteps).numpy()
+        assert np.linalg.norm(x - x_or) <= 1e-12 * np.linalg.norm(x_or), nsteps
+    # b = 0 returns the zero start without iterating; an exact start returns itself
+    assert torch.count_nonzero(_bicgstab_over(lambda z: At @ z, torch.zeros_like(bt))) == 0
+    xs = torch.tensor(np.linalg.solve(A, b))
+    assert _bicgstab_over(lambda z: At @ z, bt, x=xs) is xs
+
+
+@pytest.mark.parametrize("quad", ["full", "approx"])
+def test_p1_rows_equal_assembled_matrix(quad):
+    """oracle.p1_rows gives the rows oracle.assemble builds for the P1 basis
+    (to rounding: the pair blocks of a row are summed in another order), for
+    every operator including the curl post-combinations."""
+    v, t = mesh.icosphere(2, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=3)
+    rows = [0, 5, 77, len(v) - 1]
+    for op in ("single", "double", "adjointdouble", "hypersingular", "bm_lhs", "bm_rhs"):
+        A = oracle.assemble(v, t, 7.0, op, "P1", quad=quad)
+        R = oracle.p1_rows(v, t, 7.0, rows, op, quad=quad)
+        assert np.abs(R - A[rows]).max() <= 1e-14 * np.abs(A).max(), op
+    with pytest.raises(ValueError):
+        oracle.p1_rows(v, t, 7.0, [len(v)], "single")
