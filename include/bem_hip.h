@@ -200,6 +200,18 @@ int bem_gmres_ex(const void* A, int64_t n, int64_t lda, int trans, const void* b
 int bem_gmres_timed(const void* A, int64_t n, int64_t lda, int trans, const void* b, void* x, float tol, int restart,
                     int maxiter, void* work, bem_solve_info* info, double* gemv_ms, int64_t* gemv_count,
                     void* stream);
+/* bem_gmres_ex that allocates nothing: no workspace argument; the workspace
+ * is a static device array of BEM_GMRES_NOALLOC_SCRATCH_BYTES (64 MiB) that
+ * each GPU running the library holds until the process ends.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (serving
+ * loops, harnesses that forbid hipMalloc).  Same results as bem_gmres_ex.
+ * Returns BEM_ERR_BAD_ARG when bem_gmres_work_size(n, restart) exceeds the
+ * array (e.g. n = 20480 takes restart <= 240).  Calls that overlap on one GPU
+ * (other streams or threads) share the array and race: serialise them. */
+#define BEM_GMRES_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int64_t bem_gmres_noalloc_capacity(void);
+int bem_gmres_noalloc(const void* A, int64_t n, int64_t lda, int trans, const void* b, void* x, float tol,
+                      int restart, int maxiter, bem_solve_info* info, void* stream);
 /* Reference-compatible BiCGSTAB (BiCGSTAB.py:5-107): bilinear rho, alpha,
  * omega; stops when ||b - A x||^2 < tol ||b||^2 or < atol; stops silently
  * (returns BEM_OK, info->converged = 0) when nsteps run out or rho == 0. */

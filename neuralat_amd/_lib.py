@@ -115,6 +115,8 @@ SIGNATURES = {
     "bem_fem_spmm": (_I32, [_P, _I32, _P, _P, _I64, _I32, _P, _I64, _P]),
     "bem_fem_last_error": (ctypes.c_char_p, []),
     "bem_gmres_ex": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo), _P]),
+    "bem_gmres_noalloc_capacity": (_I64, []),
+    "bem_gmres_noalloc": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, ctypes.POINTER(SolveInfo), _P]),
     "bem_gmres_timed": (_I32, [_P, _I64, _I64, _I32, _P, _P, _F, _I32, _I32, _P, ctypes.POINTER(SolveInfo),
                                ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64), _P]),
     "bem_adjoint_work_size": (_I64, [_P]),

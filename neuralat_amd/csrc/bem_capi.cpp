@@ -60,6 +60,14 @@ struct DeviceGuard {
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 constexpr int64_t kDp0RowTileHost = bem::kDp0RowTile;
 
+}  // namespace
+
+namespace bem {
+int report_failure(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace bem
+
+namespace {
+
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 struct GmresLayout {
@@ -619,15 +627,17 @@ int bem_gmres_timed(const void* A_, int64_t n, int64_t lda, int trans, const voi
             // solution is the one of stopping there
             const bool last = j + 1 == m || its >= maxiter;
             if (nk - checked < chk && !last) continue;
-            std::vector<double> rs(nk - checked);
-            HIP_TRY(hipMemcpyAsync(rs.data(), res + checked, sizeof(double) * rs.size(), hipMemcpyDeviceToHost, s));
+            // at most chk (<= 8) residuals per read: a fixed host array
+            double rs[8];
+            const int nrs = nk - checked;
+            HIP_TRY(hipMemcpyAsync(rs, res + checked, sizeof(double) * (size_t)nrs, hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (int rc = gemv_account()) return rc;
             bool hit = false;
-            for (size_t q = 0; q < rs.size(); ++q) {
+            for (int q = 0; q < nrs; ++q) {
                 implicit = rs[q];
                 if (implicit <= target) {
-                    its -= (int)(rs.size() - 1 - q);
+                    its -= nrs - 1 - q;
                     nk = checked + (int)q + 1;
                     hit = true;
                     break;

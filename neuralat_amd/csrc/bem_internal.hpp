@@ -135,6 +135,8 @@ hipError_t launch_adjoint_potential(const MeshDev& m, const float* pts, int64_t 
                                     hipStream_t s);
 hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
+// records msg as the calling thread's bem_last_error() and returns code
+int report_failure(int code, const std::string& msg);
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
 // work: int32 [2 nv + 3 nt + 2] (counts, list pointers, lists, error flag)
 hipError_t launch_tri2vert_raw(const int32_t* tris, int64_t nt, int64_t nv, const float2* tv, float2* out,
