@@ -88,6 +88,9 @@ def parse():
                         "solve of the first 8 batching meshes x 8 k, for kernel traces")
     p.add_argument("--p1-probe", action="store_true",
                    help="assemble the level-5 P1 single- and double-layer operators once each, for counter passes")
+    p.add_argument("--only-form", type=str, default="",
+                   help="run one form alone and print its JSON (p1, batching, batching_small_meshes, "
+                        "differentiation, datagen, modal, burton_miller_level6)")
     p.add_argument("--modal-sound-probe", action="store_true",
                    help="three batched modal-sound solves (32 modes of the modal form's block), for kernel traces")
     return p.parse_args()
@@ -414,7 +417,7 @@ def grad_form(ndir=64, h=1e-3, seed=0):
 
     def loss(vt, k, tol=1e-7):
         psi = monopole_neumann_dp0(vt, tt, src, k)
-        p = exterior_pressure(vt, tt, k, psi, pts_t, tol=tol)
+        p = exterior_pressure(vt, tt, k, psi, pts_t, tol=tol, adjoint_tol=max(tol, 1e-6))
         return (p.real ** 2 + p.imag ** 2).sum()
 
     tf, tb = [], []
@@ -492,6 +495,36 @@ def p1_form(level=5, k=10.0, bm_level=4, bm_k=20.0, reps=3):
         flop = 36 * (fl + 12) * (ntri * ntri - npairs)
         out[op] = {"ms": ms, "entries_per_s": nv * nv / (ms / 1e3), "tflops_algorithmic": flop / (ms / 1e3) / 1e12}
         log(f"bench: P1 {op} layer level {level}: {ms:.1f} ms, {flop / (ms / 1e3) / 1e12:.1f} TFLOP/s algorithmic")
+    # the Burton-Miller operators of the data generation (approximate rules:
+    # 3-point triangle rule, 9 Gauss-point pairs per regular pair): the
+    # regular-pair kernel alone (HIP events around bem_assemble_ex parts = 1)
+    # and the whole operator
+    from neuralat_amd import _lib
+    from neuralat_amd.bem.solver import _ptr, _stream
+
+    lib = _lib.load()
+    A = torch.empty((nv, nv), dtype=torch.complex64, device=dev)
+    for op in ("bm_lhs", "bm_rhs"):
+        ms_reg, ms_all = [], []
+        for r in range(reps + 1):
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            _lib.check(lib.bem_assemble_ex(s.mesh.handle, float(k), _lib.OPS[op], 0, 1, 1, 1, _ptr(A), nv,
+                                           _stream()))
+            e1.record()
+            _lib.check(lib.bem_assemble_ex(s.mesh.handle, float(k), _lib.OPS[op], 0, 1, 1, 3, _ptr(A), nv,
+                                           _stream()))
+            e2.record()
+            torch.cuda.synchronize()
+            if r:
+                ms_reg.append(e0.elapsed_time(e1))
+                ms_all.append(e1.elapsed_time(e2))
+        evals = 9 * (ntri * ntri - npairs)  # Gauss-point pairs of the regular pairs
+        out[op] = {"rule": "approx (3-point triangle, 2-point line)", "regular_ms": min(ms_reg), "ms": min(ms_all),
+                   "entries_per_s": nv * nv / (min(ms_all) / 1e3),
+                   "regular_gauss_point_pairs_per_s": evals / (min(ms_reg) / 1e3)}
+        log(f"bench: P1 {op} level {level}: regular pairs {min(ms_reg):.2f} ms, whole operator {min(ms_all):.2f} ms")
+    del A
     del s
     torch.cuda.empty_cache()
     v, t = mesh.icosphere(bm_level, 1.0)
@@ -1178,6 +1211,20 @@ def main():
         return
     if args.modal_sound_probe:
         modal_sound_probe()
+        return
+    if args.only_form:
+        forms = {"p1": lambda: p1_form(), "batching": lambda: batch_form(args.batch_meshes, args.batch_ks),
+                 "batching_small_meshes": lambda: batch_form(16, args.batch_ks, levels=(2, 3), seed=9),
+                 "differentiation": lambda: grad_form(), "datagen": lambda: datagen_form(),
+                 "modal": lambda: modal_form()}
+        if args.only_form == "burton_miller_level6":
+            v, t, src, pts = make_workload(args.level)
+            res = burton_miller_large_form(v, t, src, pts, ks, restart=args.restart)
+            fp64_answers(CHECKS)
+        else:
+            res = forms[args.only_form]()
+            fp64_answers(CHECKS)
+        print(json.dumps({args.only_form: res}))
         return
     log(f"bench: building level-{args.level} workload")
     v, t, src, pts = make_workload(args.level)

@@ -93,7 +93,10 @@ int bem_mesh_normals_curls(const bem_mesh* mesh, float* normals, float* curls, v
 /* Boundary operator matrix.  Replaces <op>_boundary_matrix[_approx[_1]]
  * (bind.cu:49-154, exports bind.cu:180-215).  P1: out is [nv, nv]; DP0:
  * [nt, nt].  clamp != 0 applies the reference's r >= 1e-3 clamp
- * (potential.h:16-23). */
+ * (potential.h:16-23).  P1 accumulates the regular pairs in an [nv, nv]
+ * complex scratch matrix (vertices renumbered along the tiles) taken from
+ * the stream-ordered pool and released before return, then moves it to out;
+ * without room for it, directly in out (see bem_assemble_ex bit 8). */
 int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, void* out, int64_t ld,
                  void* stream);
 /* bem_assemble restricted to parts of the pair set: parts = 1 regular pairs
@@ -106,7 +109,9 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
  * at least 4096 vertices, for both orders at once, with an [nv, nv] complex
  * scratch matrix taken from the
  * stream-ordered pool and released before return; if none can be had, the
- * ordered-pair sweep runs).  The results agree to rounding. */
+ * ordered-pair sweep runs).  Adding 8 makes the P1 regular pairs accumulate
+ * directly in out (no [nv, nv] scratch matrix per wavenumber: the kernels then
+ * flush to scattered addresses, slower).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream);
 /* bem_assemble at nk wavenumbers on one mesh (e.g. the modes of a vibrating

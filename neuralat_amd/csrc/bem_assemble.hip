@@ -505,15 +505,220 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // double layer, so it evaluates the double layer and swaps the two blocks.
 enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2 };
 
+// The 3 x 3 block B(i, j) (row-basis a, column-basis b) of one regular pair
+// of the P1 operator OP with the Q-point rule: rr = the row triangle's
+// quadrature record, rci its curls (HYP, BM_LHS); py, wy, ny, cj the column
+// triangle's points, weights / 4 pi, normal and curls.  br, bi are added to
+// (zero them first); with PAIR also tr, tii = B(j, i)^T of the paired sweep.
+// EOP is the evaluated operator (the paired adjoint double layer evaluates
+// the double layer).
+template <int OP, int Q, bool CLAMP, bool PAIR, int EOP>
+__device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, const float* __restrict__ rci,
+                                              const float* py, const float* wy, float ny0, float ny1, float ny2,
+                                              const float* cj, float k, float krev, float invk, float* br, float* bi,
+                                              float* tr, float* tii)
+{
+    float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
+    // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
+    // triangle's plane, so it is formed once per column point
+    float qb[Q], qbk[Q];
+    if constexpr (EOP == ADLP || EOP == BM_RHS) {
+        const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            const float yn = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                             (py[3 * b + 2] - rr[2]) * nx2;
+            if constexpr (EOP == ADLP) {
+                qb[b] = wy[b] * yn;
+            } else {  // BM_RHS: (x_0 - y_b).n_x and its 1/k multiple
+                qb[b] = -yn;
+                qbk[b] = -yn * invk;
+            }
+        }
+    }
+    // PAIR: the transposed block's normal factor (y_b - x_0).n_x per
+    // column point
+    float qt[PAIR ? Q : 1];
+    if constexpr (PAIR) {
+        const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+#pragma unroll
+        for (int b = 0; b < Q; ++b)
+            qt[b] = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 + (py[3 * b + 2] - rr[2]) * nx2;
+    }
+    // BM_LHS: -k n_x.n_y once per pair; (x_a - y_0).n_y once per row point
+    float mknn = 0.f;
+    if constexpr (OP == BM_LHS)
+        mknn = -k * (rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2);
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+        float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
+        float ur[3] = {0.f, 0.f, 0.f}, ui[3] = {0.f, 0.f, 0.f};  // PAIR: transposed sums
+        float tr9 = 0.f, ti9 = 0.f;
+        float mca = 0.f, kca = 0.f;
+        if constexpr (OP == BM_LHS) {
+            const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
+            mca = -ca;
+            kca = k * ca;
+        }
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            float vr, vi, gr = 0.f, gi = 0.f;
+            float wr = 0.f, wi = 0.f;  // PAIR: the transposed block's value
+            const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+            if constexpr (EOP == SLP || EOP == DLP || EOP == ADLP) {
+                // specialised evaluations (as the DP0 kernels); the DLP normal
+                // factor (x_a - y_0).n_y is applied per row point below
+                float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (CLAMP) r2 = fmaxf(r2, kEps2);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                float sn, cs;
+                sincos_rev(r * krev, sn, cs);
+                if constexpr (EOP == SLP) {
+                    const float g = rinv * wy[b];
+                    vr = g * cs;
+                    vi = g * sn;
+                } else {
+                    const float th = r * k;
+                    const float pw = rinv * rinv * (rinv * (EOP == DLP ? wy[b] : qb[b]));
+                    vr = pw * fmaf(th, sn, cs);
+                    vi = pw * fmaf(-th, cs, sn);
+                    if constexpr (PAIR) {  // (y_b - x_0).n_x replaces (x_a - y_0).n_y
+                        wr = vr * qt[b];
+                        wi = vi * qt[b];
+                    }
+                }
+            } else if constexpr (OP == BM_LHS || OP == BM_RHS) {
+                // specialised Burton-Miller evaluations: value = G (er + i ei)
+                // with the normal factors constant per row point (BM_LHS:
+                // (y_b - x_a).n_y = -(x_a - y_0).n_y) or per column point
+                // (BM_RHS: (x_a - y_b).n_x = (x_0 - y_b).n_x)
+                float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                if (CLAMP) r2 = fmaxf(r2, kEps2);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                float sn, cs;
+                sincos_rev(r * krev, sn, cs);
+                const float rw = rinv * wy[b];
+                const float g_r = cs * rw, g_i = sn * rw;  // w_b G
+                float er, ei;
+                if constexpr (OP == BM_LHS) {
+                    // (1 - i k r)(y - x).n_y / r^2 - i k n_x.n_y
+                    er = mca * rinv * rinv;
+                    ei = fmaf(kca, rinv, mknn);
+                    gr = g_r;  // w_b G for the curl term
+                    gi = g_i;
+                } else {
+                    // -1 + (i/k)(1 - i k r)(x - y).n_x / r^2
+                    er = fmaf(qb[b], rinv, -1.0f);
+                    ei = qbk[b] * rinv * rinv;
+                }
+                vr = g_r * er - g_i * ei;
+                vi = g_r * ei + g_i * er;
+            } else {
+                kernel4pi<OP, CLAMP>(dx, dy, dz, rr[4 * Q], rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k,
+                                     krev, vr, vi, gr, gi);
+                vr *= wy[b];
+                vi *= wy[b];
+            }
+            const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
+            const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
+            sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
+            sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
+            sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
+            if constexpr (PAIR) {
+                ur[0] = fmaf(wr, p0, ur[0]); ui[0] = fmaf(wi, p0, ui[0]);
+                ur[1] = fmaf(wr, p1, ur[1]); ui[1] = fmaf(wi, p1, ui[1]);
+                ur[2] = fmaf(wr, p2, ur[2]); ui[2] = fmaf(wi, p2, ui[2]);
+            }
+            if constexpr (OP == BM_LHS) {
+                tr9 += gr;  // gr, gi: w_b G
+                ti9 += gi;
+            }
+        }
+        const float ua0 = rr[3 * Q + a];
+        float ua = ua0;
+        if constexpr (EOP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
+            ua *= (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
+        const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
+        const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
+        if constexpr (PAIR) {
+            const float t0 = ua0 * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
+            const float t1 = ua0 * TriRule<Q>::x1[a], t2 = ua0 * TriRule<Q>::x2[a];
+#pragma unroll
+            for (int be = 0; be < 3; ++be) {
+                tr[be] = fmaf(t0, ur[be], tr[be]); tii[be] = fmaf(t0, ui[be], tii[be]);
+                tr[3 + be] = fmaf(t1, ur[be], tr[3 + be]); tii[3 + be] = fmaf(t1, ui[be], tii[3 + be]);
+                tr[6 + be] = fmaf(t2, ur[be], tr[6 + be]); tii[6 + be] = fmaf(t2, ui[be], tii[6 + be]);
+            }
+        }
+#pragma unroll
+        for (int be = 0; be < 3; ++be) {
+            br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
+            br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
+            br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
+        }
+        if constexpr (OP == BM_LHS) {
+            gr9 = fmaf(ua, tr9, gr9);
+            gi9 = fmaf(ua, ti9, gi9);
+        }
+    }
+    if constexpr (OP == HYP) {
+        // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
+        float sgr = 0.f, sgi = 0.f;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
+        float ci[9], cp[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) ci[e] = rci[e];
+        curl_product(ci, cj, cp);
+        const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
+        const float kk = k * k * nn;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            br[e] = sgr * cp[e] - br[e] * kk;
+            bi[e] = sgi * cp[e] - bi[e] * kk;
+        }
+    } else if constexpr (OP == BM_LHS) {
+        // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
+        const float invk = 1.0f / k;
+        const float s9r = -gi9 * invk, s9i = gr9 * invk;
+        float ci[9], cp[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) ci[e] = rci[e];
+        curl_product(ci, cj, cp);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            br[e] = fmaf(s9r, cp[e], br[e]);
+            bi[e] = fmaf(s9i, cp[e], bi[e]);
+        }
+    }
+    if constexpr (PAIR && OP == ADLP) {  // adjoint double layer = double layer transposed
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            const float xr = br[e], xi = bi[e];
+            br[e] = tr[e];
+            bi[e] = tii[e];
+            tr[e] = xr;
+            tii[e] = xi;
+        }
+    }
+}
+
+
 template <int OP, int Q, bool CLAMP, int SWEEP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                          const float* __restrict__ curls, P1Tiles tl,
                                                          const KBatch kb, float2* __restrict__ out, int64_t ld,
-                                                         float2* __restrict__ outT, int64_t ldT)
+                                                         float2* __restrict__ outT, int64_t ldT,
+                                                         const int32_t* __restrict__ vmap)
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
-    static_assert(kP1ColTris == 256 && kP1ColVerts <= 256, "one lane per column triangle and per column vertex");
+    constexpr int CT = kP1ColTris;  // one lane per column triangle and per column vertex
+    constexpr int NCV = kP1ColVerts;
+    static_assert(CT == 256 && NCV <= CT, "one lane per column triangle and per column vertex");
     static_assert(SWEEP != kSweepSym || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
     static_assert(SWEEP != kSweepPair || OP == DLP || OP == ADLP, "paired sweep operators");
     constexpr bool HALF = SWEEP != kSweepAll;  // one evaluation per unordered pair
@@ -522,8 +727,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     // double layer and swaps the blocks
     constexpr int EOP = (PAIR && OP == ADLP) ? DLP : OP;
     constexpr int S = Rec<Q>::kFloats;
-    __shared__ float2 acc[kP1RowVerts][kP1ColVerts];
-    __shared__ float2 stage[9][kP1ColTris];
+    __shared__ float2 acc[kP1RowVerts][NCV];
+    __shared__ float2 stage[9][CT];
     // the row tile's triangles, records and curls, read by every wavefront at
     // each row step (from LDS rather than a chain of dependent scalar loads)
     __shared__ float rrec[kP1RowTris][S];
@@ -548,17 +753,17 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const int r0 = tl.row_ptr[rt], r1 = tl.row_ptr[rt + 1];
     const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
     const int cv0 = tl.col_vptr[ct], ncv = tl.col_vptr[ct + 1] - cv0;
-    for (int e = threadIdx.x; e < nrv * ncv; e += 256) acc[e / ncv][e % ncv] = make_float2(0.f, 0.f);
-    for (int e = threadIdx.x; e < (r1 - r0) * S; e += 256) {  // r1 - r0 <= kP1RowTris
+    for (int e = threadIdx.x; e < nrv * ncv; e += CT) acc[e / ncv][e % ncv] = make_float2(0.f, 0.f);
+    for (int e = threadIdx.x; e < (r1 - r0) * S; e += CT) {  // r1 - r0 <= kP1RowTris
         const int q = e / S;
         rrec[q][e - q * S] = rec[(int64_t)tl.perm[r0 + q] * S + (e - q * S)];
     }
-    for (int q = threadIdx.x; q < r1 - r0; q += 256) {
+    for (int q = threadIdx.x; q < r1 - r0; q += CT) {
         rtri[q] = tris[tl.perm[r0 + q]];
         rslot[q] = tl.row_slot[r0 + q];
     }
     if constexpr (OP == HYP || OP == BM_LHS)
-        for (int e = threadIdx.x; e < (r1 - r0) * 9; e += 256) {
+        for (int e = threadIdx.x; e < (r1 - r0) * 9; e += CT) {
             const int q = e / 9;
             rcurl[q][e - q * 9] = curls[(int64_t)tl.perm[r0 + q] * 12 + (e - q * 9)];
         }
@@ -633,193 +838,13 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 #pragma unroll
         for (int e = 0; e < (PAIR ? 9 : 1); ++e) tr[e] = tii[e] = 0.f;
         if (active && !shared) {
-            const float* rr = rrec[rp - r0];
-            float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
-            // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
-            // triangle's plane, so it is formed once per column point
-            float qb[Q], qbk[Q];
-            if constexpr (EOP == ADLP || EOP == BM_RHS) {
-                const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
-#pragma unroll
-                for (int b = 0; b < Q; ++b) {
-                    const float yn = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
-                                     (py[3 * b + 2] - rr[2]) * nx2;
-                    if constexpr (EOP == ADLP) {
-                        qb[b] = wy[b] * yn;
-                    } else {  // BM_RHS: (x_0 - y_b).n_x and its 1/k multiple
-                        qb[b] = -yn;
-                        qbk[b] = -yn * invk;
-                    }
-                }
-            }
-            // PAIR: the transposed block's normal factor (y_b - x_0).n_x per
-            // column point
-            float qt[PAIR ? Q : 1];
-            if constexpr (PAIR) {
-                const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
-#pragma unroll
-                for (int b = 0; b < Q; ++b)
-                    qt[b] = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 + (py[3 * b + 2] - rr[2]) * nx2;
-            }
-            // BM_LHS: -k n_x.n_y once per pair; (x_a - y_0).n_y once per row point
-            float mknn = 0.f;
-            if constexpr (OP == BM_LHS)
-                mknn = -k * (rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2);
-#pragma unroll
-            for (int a = 0; a < Q; ++a) {
-                const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
-                float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
-                float ur[3] = {0.f, 0.f, 0.f}, ui[3] = {0.f, 0.f, 0.f};  // PAIR: transposed sums
-                float tr9 = 0.f, ti9 = 0.f;
-                float mca = 0.f, kca = 0.f;
-                if constexpr (OP == BM_LHS) {
-                    const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
-                    mca = -ca;
-                    kca = k * ca;
-                }
-#pragma unroll
-                for (int b = 0; b < Q; ++b) {
-                    float vr, vi, gr = 0.f, gi = 0.f;
-                    float wr = 0.f, wi = 0.f;  // PAIR: the transposed block's value
-                    const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
-                    if constexpr (EOP == SLP || EOP == DLP || EOP == ADLP) {
-                        // specialised evaluations (as the DP0 kernels); the DLP normal
-                        // factor (x_a - y_0).n_y is applied per row point below
-                        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        if (CLAMP) r2 = fmaxf(r2, kEps2);
-                        const float rinv = __builtin_amdgcn_rsqf(r2);
-                        const float r = r2 * rinv;
-                        float sn, cs;
-                        sincos_rev(r * krev, sn, cs);
-                        if constexpr (EOP == SLP) {
-                            const float g = rinv * wy[b];
-                            vr = g * cs;
-                            vi = g * sn;
-                        } else {
-                            const float th = r * k;
-                            const float pw = rinv * rinv * (rinv * (EOP == DLP ? wy[b] : qb[b]));
-                            vr = pw * fmaf(th, sn, cs);
-                            vi = pw * fmaf(-th, cs, sn);
-                            if constexpr (PAIR) {  // (y_b - x_0).n_x replaces (x_a - y_0).n_y
-                                wr = vr * qt[b];
-                                wi = vi * qt[b];
-                            }
-                        }
-                    } else if constexpr (OP == BM_LHS || OP == BM_RHS) {
-                        // specialised Burton-Miller evaluations: value = G (er + i ei)
-                        // with the normal factors constant per row point (BM_LHS:
-                        // (y_b - x_a).n_y = -(x_a - y_0).n_y) or per column point
-                        // (BM_RHS: (x_a - y_b).n_x = (x_0 - y_b).n_x)
-                        float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                        if (CLAMP) r2 = fmaxf(r2, kEps2);
-                        const float rinv = __builtin_amdgcn_rsqf(r2);
-                        const float r = r2 * rinv;
-                        float sn, cs;
-                        sincos_rev(r * krev, sn, cs);
-                        const float rw = rinv * wy[b];
-                        const float g_r = cs * rw, g_i = sn * rw;  // w_b G
-                        float er, ei;
-                        if constexpr (OP == BM_LHS) {
-                            // (1 - i k r)(y - x).n_y / r^2 - i k n_x.n_y
-                            er = mca * rinv * rinv;
-                            ei = fmaf(kca, rinv, mknn);
-                            gr = g_r;  // w_b G for the curl term
-                            gi = g_i;
-                        } else {
-                            // -1 + (i/k)(1 - i k r)(x - y).n_x / r^2
-                            er = fmaf(qb[b], rinv, -1.0f);
-                            ei = qbk[b] * rinv * rinv;
-                        }
-                        vr = g_r * er - g_i * ei;
-                        vi = g_r * ei + g_i * er;
-                    } else {
-                        kernel4pi<OP, CLAMP>(dx, dy, dz, rr[4 * Q], rr[4 * Q + 1], rr[4 * Q + 2], ny0, ny1, ny2, k,
-                                             krev, vr, vi, gr, gi);
-                        vr *= wy[b];
-                        vi *= wy[b];
-                    }
-                    const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
-                    const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
-                    sr[0] = fmaf(vr, p0, sr[0]); si[0] = fmaf(vi, p0, si[0]);
-                    sr[1] = fmaf(vr, p1, sr[1]); si[1] = fmaf(vi, p1, si[1]);
-                    sr[2] = fmaf(vr, p2, sr[2]); si[2] = fmaf(vi, p2, si[2]);
-                    if constexpr (PAIR) {
-                        ur[0] = fmaf(wr, p0, ur[0]); ui[0] = fmaf(wi, p0, ui[0]);
-                        ur[1] = fmaf(wr, p1, ur[1]); ui[1] = fmaf(wi, p1, ui[1]);
-                        ur[2] = fmaf(wr, p2, ur[2]); ui[2] = fmaf(wi, p2, ui[2]);
-                    }
-                    if constexpr (OP == BM_LHS) {
-                        tr9 += gr;  // gr, gi: w_b G
-                        ti9 += gi;
-                    }
-                }
-                const float ua0 = rr[3 * Q + a];
-                float ua = ua0;
-                if constexpr (EOP == DLP)  // -(y - x).n_y = (x_a - y_0).n_y
-                    ua *= (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
-                const float q0 = ua * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
-                const float q1 = ua * TriRule<Q>::x1[a], q2 = ua * TriRule<Q>::x2[a];
-                if constexpr (PAIR) {
-                    const float t0 = ua0 * (1.0f - TriRule<Q>::x1[a] - TriRule<Q>::x2[a]);
-                    const float t1 = ua0 * TriRule<Q>::x1[a], t2 = ua0 * TriRule<Q>::x2[a];
-#pragma unroll
-                    for (int be = 0; be < 3; ++be) {
-                        tr[be] = fmaf(t0, ur[be], tr[be]); tii[be] = fmaf(t0, ui[be], tii[be]);
-                        tr[3 + be] = fmaf(t1, ur[be], tr[3 + be]); tii[3 + be] = fmaf(t1, ui[be], tii[3 + be]);
-                        tr[6 + be] = fmaf(t2, ur[be], tr[6 + be]); tii[6 + be] = fmaf(t2, ui[be], tii[6 + be]);
-                    }
-                }
-#pragma unroll
-                for (int be = 0; be < 3; ++be) {
-                    br[be] = fmaf(q0, sr[be], br[be]); bi[be] = fmaf(q0, si[be], bi[be]);
-                    br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
-                    br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
-                }
-                if constexpr (OP == BM_LHS) {
-                    gr9 = fmaf(ua, tr9, gr9);
-                    gi9 = fmaf(ua, ti9, gi9);
-                }
-            }
-            if constexpr (OP == HYP) {
-                // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
-                float sgr = 0.f, sgi = 0.f;
-#pragma unroll
-                for (int e = 0; e < 9; ++e) { sgr += br[e]; sgi += bi[e]; }
-                float ci[9], cp[9];
+            float ci[9];
+            if constexpr (OP == HYP || OP == BM_LHS) {
 #pragma unroll
                 for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
-                curl_product(ci, cj, cp);
-                const float nn = rr[4 * Q] * ny0 + rr[4 * Q + 1] * ny1 + rr[4 * Q + 2] * ny2;
-                const float kk = k * k * nn;
-#pragma unroll
-                for (int e = 0; e < 9; ++e) {
-                    br[e] = sgr * cp[e] - br[e] * kk;
-                    bi[e] = sgi * cp[e] - bi[e] * kk;
-                }
-            } else if constexpr (OP == BM_LHS) {
-                // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
-                const float invk = 1.0f / k;
-                const float s9r = -gi9 * invk, s9i = gr9 * invk;
-                float ci[9], cp[9];
-#pragma unroll
-                for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
-                curl_product(ci, cj, cp);
-#pragma unroll
-                for (int e = 0; e < 9; ++e) {
-                    br[e] = fmaf(s9r, cp[e], br[e]);
-                    bi[e] = fmaf(s9i, cp[e], bi[e]);
-                }
             }
-            if constexpr (PAIR && OP == ADLP) {  // adjoint double layer = double layer transposed
-#pragma unroll
-                for (int e = 0; e < 9; ++e) {
-                    const float xr = br[e], xi = bi[e];
-                    br[e] = tr[e];
-                    bi[e] = tii[e];
-                    tr[e] = xr;
-                    tii[e] = xi;
-                }
-            }
+            p1_pair_block<OP, Q, CLAMP, PAIR, EOP>(rrec[rp - r0], ci, py, wy, ny0, ny1, ny2, cj, k, krev, invk, br,
+                                                    bi, tr, tii);
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
@@ -850,9 +875,14 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             __syncthreads();
         }
     }
-    for (int e = threadIdx.x; e < nrv * ncv; e += 256) {
+    for (int e = threadIdx.x; e < nrv * ncv; e += CT) {
         const int rvi = e / ncv, cvi = e - rvi * ncv;
-        const int64_t gr = tl.row_verts[rv0 + rvi], gc = tl.col_verts[cv0 + cvi];
+        // out is in the tiles' P1 numbering, or (vmap) in the caller's
+        int64_t gr = tl.row_verts[rv0 + rvi], gc = tl.col_verts[cv0 + cvi];
+        if (vmap) {
+            gr = vmap[gr];
+            gc = vmap[gc];
+        }
         float* dst = reinterpret_cast<float*>(&out[gr * ld + gc]);
         const float2 v = acc[rvi][cvi];
         unsafeAtomicAdd(dst, v.x);
@@ -860,11 +890,12 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     }
     if constexpr (PAIR) {
         if (owner) {
-            const int64_t gc = tl.col_verts[cv0 + w];
+            const int64_t gc = vmap ? vmap[tl.col_verts[cv0 + w]] : tl.col_verts[cv0 + w];
 #pragma unroll
             for (int q = 0; q < kP1RowVerts; ++q) {
                 if (q < nrv) {
-                    float* dst = reinterpret_cast<float*>(&outT[(int64_t)tl.row_verts[rv0 + q] * ldT + gc]);
+                    const int64_t gr = vmap ? vmap[tl.row_verts[rv0 + q]] : tl.row_verts[rv0 + q];
+                    float* dst = reinterpret_cast<float*>(&outT[gr * ldT + gc]);
                     const float2 v = accT[q];
                     unsafeAtomicAdd(dst, v.x);
                     unsafeAtomicAdd(dst + 1, v.y);
@@ -1439,6 +1470,20 @@ static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, co
     return hipGetLastError();
 }
 
+// out[c, d] = S[vnew[c], vnew[d]] for each wavenumber z = blockIdx.y: the
+// P1 regular part accumulated in the tiles' numbering (S: [n, n] per z,
+// contiguous) moved to the caller's.  One workgroup per output row: the
+// source row is gathered (it stays in L2), the output row written whole.
+__global__ __launch_bounds__(256) void p1_permute_kernel(const float2* __restrict__ S, int64_t n,
+                                                         const int32_t* __restrict__ vnew, float2* __restrict__ out,
+                                                         int64_t ld, int64_t zstride)
+{
+    const int64_t c = blockIdx.x;
+    const float2* src = S + (int64_t)blockIdx.y * n * n + (int64_t)vnew[c] * n;
+    float2* dst = out + (int64_t)blockIdx.y * zstride + c * ld;
+    for (int64_t d = threadIdx.x; d < n; d += 256) dst[d] = src[vnew[d]];
+}
+
 // Half sweeps where they pay (single layer and hypersingular: symmetric;
 // double layer and its adjoint with the 6-point rule on meshes of at least
 // kP1PairMinVerts vertices: paired, with an [nv, nv] scratch matrix from the
@@ -1451,7 +1496,7 @@ static hipError_t dp0_regular_t(const MeshDev& m, int64_t row0, int64_t row1, co
 constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
 static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float2* out, int64_t ld, bool one_sided,
-                               hipStream_t s)
+                               const int32_t* vmap, hipStream_t s)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
     const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
@@ -1460,7 +1505,7 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
     if (!one_sided && m.tiles.nsym <= 0x7fffffffLL && nb * nb <= 0x7fffffffLL) {
         if constexpr (sym) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym, 1, nz),
-                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0);
+                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap);
             hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2), 1, nz), dim3(256), 0, s, out,
                                n, ld, kb.zstride);
             return hipGetLastError();
@@ -1470,7 +1515,7 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
                 hipError_t e = launch_fill(t, n, n, n, make_float2(0.f, 0.f), s);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
-                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n);
+                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n, vmap);
                     hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
                                        (const float2*)t, n, nb);
                     e = hipGetLastError();
@@ -1478,12 +1523,12 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
                 const hipError_t ef = hipFreeAsync(t, s);
                 return e != hipSuccess ? e : ef;
             }
-            (void)hipGetLastError();  // no room for the scratch matrix
+            (void)hipGetLastError();  // no room for the scratch, or none asked for (parts & 8) matrix
         }
     }
     dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles, (unsigned)nz);
     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                       m.tiles, kb, out, ld, nullptr, (int64_t)0);
+                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap);
     return hipGetLastError();
 }
 
@@ -1575,12 +1620,40 @@ static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, 
                            : singular_t<OP, 2, DP0, CLAMP>(m, p0, p1, kb, nz, out, ld, row0, s);
     }
     if (parts & 1) {
-        e = launch_fill_z(out, m.nv, m.nv, ld, kb.zstride, nz, make_float2(0.f, 0.f), s);
-        if (e != hipSuccess) return e;
+        // The regular pairs accumulate in the tiles' P1 numbering, where a
+        // tile's flush atomics land in few cache lines (scattered over the
+        // caller's numbering the flush took half of the level-5
+        // Burton-Miller kernels' 10 ms), in a scratch matrix per wavenumber
+        // from the stream-ordered pool; one pass then moves them to out in
+        // the caller's numbering.  Without room for the scratch the kernels
+        // flush to out through the index map (slower: the slots of a tile
+        // are in P1 order, so the mapped addresses are scattered).
         const bool one_sided = (parts & 4) != 0;
-        if (quad == 0) e = p1_regular_t<OP, 6, CLAMP>(m, kb, nz, out, ld, one_sided, s);
-        else if (quad == 1) e = p1_regular_t<OP, 3, CLAMP>(m, kb, nz, out, ld, one_sided, s);
-        else e = p1_regular_t<OP, 1, CLAMP>(m, kb, nz, out, ld, one_sided, s);
+        const int64_t n = m.nv;
+        auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap) -> hipError_t {
+            hipError_t er = launch_fill_z(dst, n, n, ldd, kbd.zstride, nz, make_float2(0.f, 0.f), s);
+            if (er != hipSuccess) return er;
+            if (quad == 0) return p1_regular_t<OP, 6, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
+            if (quad == 1) return p1_regular_t<OP, 3, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
+            return p1_regular_t<OP, 1, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
+        };
+        float2* S = nullptr;
+        if (!(parts & 8) &&
+            hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n * (size_t)nz, s) == hipSuccess) {
+            KBatch kbs = kb;
+            kbs.zstride = n * n;
+            e = regular(S, n, kbs, nullptr);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n,
+                                   m.p1_vnew, out, ld, kb.zstride);
+                e = hipGetLastError();
+            }
+            const hipError_t ef = hipFreeAsync(S, s);
+            if (e == hipSuccess) e = ef;
+        } else {
+            (void)hipGetLastError();  // no room for the scratch, or none asked for (parts & 8)
+            e = regular(out, ld, kb, m.p1_vold);
+        }
         if (e != hipSuccess) return e;
     }
     if (!(parts & 2)) return hipSuccess;

@@ -218,8 +218,8 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream)
 {
-    if (parts < 1 || parts > 7 || (parts & 3) == 0)
-        return fail(BEM_ERR_BAD_ARG, "parts must be 1, 2 or 3, optionally with bit 4 set");
+    if (parts < 1 || parts > 15 || (parts & 3) == 0)
+        return fail(BEM_ERR_BAD_ARG, "parts must be 1, 2 or 3, optionally with bits 4 and 8 set");
     if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
     if (int rc = check_op(op, basis, quad)) return rc;
     const auto& m = mesh->dev;

@@ -66,6 +66,11 @@ struct MeshDev {
     int32_t* vt_ptr = nullptr;  // [nv+1] vertex -> triangle CSR (ascending)
     int32_t* vt_idx = nullptr;
     P1Tiles tiles{};
+    // the P1 numbering of the tiles (vertices in order of first appearance
+    // along the tiles' triangle order): p1_vnew [nv] caller id -> P1 id,
+    // p1_vold [nv] P1 id -> caller id; tiles.row_verts / col_verts hold P1 ids
+    int32_t* p1_vnew = nullptr;
+    int32_t* p1_vold = nullptr;
     std::vector<int64_t> h_pair_ptr;  // [nt+1] host copy: pair range of each row triangle
     int32_t max_neighbors = 0;
     double diameter = 0.0;  // bounding-box diagonal

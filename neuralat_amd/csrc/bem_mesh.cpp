@@ -329,10 +329,31 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
         std::iota(perm.begin(), perm.end(), 0);
         std::stable_sort(perm.begin(), perm.end(), [&](int32_t a, int32_t b) { return code[a] < code[b]; });
     }
+    // P1 numbering: vertices in the order they first appear along the Morton
+    // order of the triangles, so the vertices of a tile are (nearly)
+    // consecutive and its accumulator flush lands in few cache lines; the
+    // P1 kernels accumulate in this numbering (see p1_vnew / p1_vold)
+    std::vector<int32_t> vnew((size_t)nv, -1), vold;
+    vold.reserve((size_t)nv);
+    for (int64_t q = 0; q < nt; ++q)
+        for (int a = 0; a < 3; ++a) {
+            const int32_t v = h_tris[3 * (int64_t)perm[q] + a];
+            if (vnew[v] < 0) {
+                vnew[v] = (int32_t)vold.size();
+                vold.push_back(v);
+            }
+        }
+    for (int64_t v = 0; v < nv; ++v)  // vertices in no triangle
+        if (vnew[v] < 0) {
+            vnew[v] = (int32_t)vold.size();
+            vold.push_back((int32_t)v);
+        }
+    std::vector<int32_t> tris_p1((size_t)nt * 3);
+    for (int64_t e = 0; e < 3 * nt; ++e) tris_p1[e] = vnew[h_tris[e]];
     // row tiles nest in column tiles, so the symmetric sweep can pair a row
     // tile with the column tiles from its own on
-    const Tiling colT = make_tiling(perm, h_tris, kP1ColTris, kP1ColVerts);
-    const Tiling rowT = make_tiling(perm, h_tris, kP1RowTris, kP1RowVerts, colT.ptr);
+    const Tiling colT = make_tiling(perm, tris_p1.data(), kP1ColTris, kP1ColVerts);
+    const Tiling rowT = make_tiling(perm, tris_p1.data(), kP1RowTris, kP1RowVerts, colT.ptr);
     const int nrowt = (int)rowT.ptr.size() - 1, ncolt = (int)colT.ptr.size() - 1;
     std::vector<int32_t> row_ctile((size_t)nrowt);
     std::vector<int64_t> sym_ptr((size_t)nrowt + 1, 0);
@@ -399,6 +420,8 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({inc.data(), inc.size() * 4, (void**)&d_ci});
     parts.push_back({row_ctile.data(), row_ctile.size() * 4, (void**)&d_rct});
     parts.push_back({sym_ptr.data(), sym_ptr.size() * 8, (void**)&d_sym});
+    parts.push_back({vnew.data(), vnew.size() * 4, (void**)&m.p1_vnew});
+    parts.push_back({vold.data(), vold.size() * 4, (void**)&m.p1_vold});
     size_t total = 0;
     std::vector<size_t> off;
     for (auto& p : parts) {

@@ -70,16 +70,16 @@ def test_status_mapping():
 
 
 def test_assemble_parts_validated_before_device_use():
-    """parts must be 1, 2 or 3, optionally plus 4 (P1 ordered-pair sweep);
-    the check precedes any pointer or device use."""
+    """parts must be 1, 2 or 3, optionally plus 4 (P1 ordered-pair sweep) and
+    8 (no scratch matrix); the check precedes any pointer or device use."""
     from neuralat_amd import _lib
 
     lib = _lib.load()
-    for parts in (0, 4, 8, -1):
+    for parts in (0, 4, 8, 12, 16, -1):
         rc = lib.bem_assemble_ex(None, 1.0, 0, 0, 0, 1, parts, None, 1, None)
         assert rc == _lib.BEM_ERR_BAD_ARG
         assert b"parts" in lib.bem_last_error()
-    for parts in (1, 5, 7):  # accepted: the null mesh is the error reported
+    for parts in (1, 5, 7, 11, 15):  # accepted: the null mesh is the error reported
         assert lib.bem_assemble_ex(None, 1.0, 0, 0, 0, 1, parts, None, 1, None) == _lib.BEM_ERR_BAD_ARG
         assert b"null" in lib.bem_last_error()
 

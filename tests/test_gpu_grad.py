@@ -8,7 +8,11 @@ exterior pressure with respect to vertex positions and k).
   spherical-harmonic noise at k = 25, loss = sum |p|^2 at 32 listeners from
   one interior monopole;
 * an FP64 reference: central differences (h = 1e-6) of the FP64 CPU oracle's
-  pipeline (same quadrature, clamp and Neumann projection) on a level-2 mesh.
+  pipeline (same quadrature, clamp and Neumann projection) on a level-2 mesh,
+  and at the configuration's own size (level 4, k = 25) the FP64 oracle's
+  derivative along single-vertex directions (tests/fp64_grad.py, itself
+  checked against whole-pipeline central differences in
+  tests/test_oracle_gradient.py).
 """
 import time
 
@@ -16,6 +20,7 @@ import numpy as np
 import pytest
 import torch
 
+import fp64_grad
 import oracle
 from neuralat_amd import mesh
 
@@ -38,8 +43,10 @@ def _setup(level, seed=0):
 def _loss_gpu(vt, tt, k, src, pts_t, tol=1e-7):
     from neuralat_amd.bem.grad import exterior_pressure, monopole_neumann_dp0
 
+    # a tight forward target for the differences of two forward solves; the
+    # adjoint (A^H) solve stops at 1e-6, above its FP32 floor (2-4e-7)
     psi = monopole_neumann_dp0(vt, tt, src, k)
-    p = exterior_pressure(vt, tt, k, psi, pts_t, tol=tol)
+    p = exterior_pressure(vt, tt, k, psi, pts_t, tol=tol, adjoint_tol=1e-6)
     return (p.real ** 2 + p.imag ** 2).sum()
 
 
@@ -102,18 +109,6 @@ def test_gradient_vs_central_differences_level4():
     assert abs(gk - fdk) / abs(fdk) < 3e-2
 
 
-def _loss_fp64(v, t, k, src, pts):
-    val, dn = mesh.sources_field([src], None, k)
-    psi = mesh.project_dp0(v, t, lambda x, n: dn(x, n))
-    S = oracle.assemble(v, t, k, "single", "DP0")
-    D = oracle.assemble(v, t, k, "double", "DP0")
-    M = oracle.identity(v, t, "DP0")
-    phi = np.linalg.solve(0.5 * M - D, -S @ psi)
-    p = (-oracle.potential(v, t, pts, k, "single", "DP0") @ psi
-         + oracle.potential(v, t, pts, k, "double", "DP0") @ phi)
-    return float(np.sum(np.abs(p) ** 2))
-
-
 def test_gradient_vs_fp64_oracle_level2():
     v, t, pts, src = _setup(2, seed=1)
     k0 = 9.0
@@ -129,12 +124,12 @@ def test_gradient_vs_fp64_oracle_level2():
     for _ in range(8):
         d = rng.standard_normal(v.shape)
         d /= np.linalg.norm(d)
-        fd.append((_loss_fp64(v64 + h * d, t, k0, src, pts) - _loss_fp64(v64 - h * d, t, k0, src, pts)) / (2 * h))
+        fd.append((fp64_grad.loss(v64 + h * d, t, k0, src, pts) - fp64_grad.loss(v64 - h * d, t, k0, src, pts)) / (2 * h))
         ad.append(np.sum(gv * d))
     ad, fd = np.array(ad), np.array(fd)
     rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
     hk = 1e-6 * k0
-    fdk = (_loss_fp64(v64, t, k0 + hk, src, pts) - _loss_fp64(v64, t, k0 - hk, src, pts)) / (2 * hk)
+    fdk = (fp64_grad.loss(v64, t, k0 + hk, src, pts) - fp64_grad.loss(v64, t, k0 - hk, src, pts)) / (2 * hk)
     relk = abs(float(kt.grad) - fdk) / abs(fdk)
     print(f"level 2 vs FP64 oracle: vertex rel {rel:.2e}, k rel {relk:.2e}", flush=True)
     assert rel < 2e-3, rel
@@ -197,3 +192,25 @@ def test_nonconverged_solves_are_signalled():
     p = exterior_pressure(vt, tt, 9.0, monopole_neumann_dp0(vt, tt, src, 9.0), pts_t)
     (p.real ** 2 + p.imag ** 2).sum().backward()
     assert grad.last_solve_info["forward"]["converged"] and grad.last_solve_info["adjoint"]["converged"]
+
+
+def test_gradient_vs_fp64_oracle_level4():
+    """The differentiation configuration (level 4 with spherical-harmonic
+    noise, k = 25, 32 listeners): the GPU adjoint gradient along 8 seeded
+    single-vertex directions against the FP64 oracle's derivative of the same
+    discrete pipeline."""
+    v, t, pts, src = _setup(4)
+    k0 = 25.0
+    vt = torch.tensor(v, device=_dev(), requires_grad=True)
+    L = _loss_gpu(vt, torch.tensor(t, device=_dev()), k0, src, torch.tensor(pts, device=_dev()))
+    L.backward()
+    gv = vt.grad.detach().cpu().numpy().astype(np.float64)
+    rng = np.random.default_rng(11)
+    ws = rng.choice(len(v), 8, replace=False)
+    dirs = [(int(w), rng.standard_normal(3) / np.sqrt(3)) for w in ws]
+    fd = fp64_grad.directional_derivatives(v, t, k0, src, pts, dirs)
+    ad = np.array([float(gv[w] @ d) for w, d in dirs])
+    rel = np.linalg.norm(ad - fd) / np.linalg.norm(fd)
+    print(f"level 4, k = {k0}: 8 single-vertex directions, adjoint vs FP64 oracle derivative {rel:.2e}; "
+          f"per direction {np.abs(ad - fd) / np.abs(fd)}", flush=True)
+    assert rel < 2e-3, (rel, ad, fd)

@@ -167,7 +167,9 @@ def test_p1_half_sweeps_match_ordered_pair_sweep(op, quad):
     """parts bit 4 of bem_assemble_ex evaluates every ordered triangle pair;
     the default sweeps (symmetric for the single layer and hypersingular,
     paired for the double layer and its adjoint with the 6-point rule from
-    4096 vertices on) must agree with it to rounding.  Level 5 (10242
+    4096 vertices on) must agree with it to rounding.  Bit 8 accumulates
+    directly in the output through the vertex map instead of the scratch
+    matrix in the tiles' numbering: both sweeps again agree.  Level 5 (10242
     vertices, 80 column tiles); the difference is formed on the device."""
     from neuralat_amd import _lib
     from neuralat_amd.bem.solver import _ptr, _stream
@@ -178,14 +180,15 @@ def test_p1_half_sweeps_match_ordered_pair_sweep(op, quad):
     n = len(v)
     lib = _lib.load()
     out = {}
-    for parts in (3, 7):
-        _say("P1 L4 sweep", op, quad, parts)
+    for parts in (3, 7, 11, 15):
+        _say("P1 L5 sweep", op, quad, parts)
         X = torch.empty((n, n), dtype=torch.complex64, device=_dev())
         _lib.check(lib.bem_assemble_ex(s.mesh.handle, 11.0, _lib.OPS[op], _lib.BASES["P1"], quad, 1, parts, _ptr(X),
                                        n, _stream()), "bem_assemble_ex")
         out[parts] = X
-    e = float(torch.linalg.matrix_norm(out[3] - out[7]) / torch.linalg.matrix_norm(out[7]))
-    assert e < 1e-6, (op, quad, e)
+    for parts in (3, 11, 15):
+        e = float(torch.linalg.matrix_norm(out[parts] - out[7]) / torch.linalg.matrix_norm(out[7]))
+        assert e < 1e-6, (op, quad, parts, e)
 
 
 @pytest.mark.parametrize("basis,op,quad", [("P1", "single", 0), ("P1", "double", 0), ("P1", "hypersingular", 1),
