@@ -123,6 +123,39 @@ int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, 
  * sweep here). */
 int bem_assemble_batch_k(const bem_mesh* mesh, const float* ks, int nk, int op, int basis, int quad, int clamp,
                          void* out, int64_t ld, int64_t stride, void* stream);
+/* Geometry sets: ns placements of one mesh's triangles, e.g. the samples of
+ * a scene whose objects are resized, rotated and moved (scene.py:218-291,
+ * where the reference builds a new BEM_Solver and assembles each sample on
+ * its own).  verts: host [ns, nv, 3] float32.  Each sample's
+ * position-dependent data (quadrature records, normals, curls, singular-pair
+ * geometry) is built on the host and uploaded in one copy (the stream is
+ * waited for); the mesh's connectivity (P1 tiling, neighbour pairs, vertex
+ * lists) serves every sample, so the mesh must outlive the set.  Errors:
+ * BAD_MESH for a degenerate triangle in any sample. */
+typedef struct bem_geomset bem_geomset;
+int bem_geomset_create(const bem_mesh* mesh, const float* verts, int ns, void* stream, bem_geomset** out);
+int bem_geomset_destroy(bem_geomset* set);
+int bem_geomset_count(const bem_geomset* set);
+/* bem_assemble for every sample z of the set at its own wavenumber ks[z]
+ * (host [ns]), in one launch per kernel for up to 64 samples: operator z at
+ * out + z * stride complex elements, [n, ld] each (stride >= n * ld). */
+int bem_assemble_samples(const bem_geomset* set, const float* ks, int op, int basis, int quad, int clamp, void* out,
+                         int64_t ld, int64_t stride, void* stream);
+/* out_z += alpha_z M_z (M_z the mass matrix of sample z), alphas host [ns]
+ * complex as (re, im) pairs. */
+int bem_add_identity_samples(const bem_geomset* set, int basis, const float* alphas, void* out, int64_t ld,
+                             int64_t stride, void* stream);
+/* bem_potential for every sample z at its own points (device [ns, np, 3])
+ * and wavenumber ks[z]: matrix z at out + z * stride ([np, ld] each). */
+int bem_potential_samples(const bem_geomset* set, const float* points, int64_t np, const float* ks, int op, int basis,
+                          int clamp, void* out, int64_t ld, int64_t stride, void* stream);
+/* y_z = alpha A_z x_z + beta y_z for nz complex64 matrices [rows, cols] (A_z at
+ * A + z * stride_a, x_z at x + z * stride_x, y_z at y + z * stride_y, complex
+ * elements), in one launch; y is not read when beta = 0 (e.g. the right-hand
+ * sides and pressures of a batch of samples, solver.py:125,147). */
+int bem_matvec_strided(const void* A, int64_t rows, int64_t cols, int64_t lda, int64_t stride_a, const void* x,
+                       int64_t stride_x, void* y, int64_t stride_y, int nz, float alpha_re, float alpha_im,
+                       float beta_re, float beta_im, void* stream);
 /* DP0 rows [row0, row1) only (out has row1-row0 rows); rows are independent,
  * so a large operator can be assembled block by block. */
 int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,

@@ -70,6 +70,13 @@ SIGNATURES = {
     "bem_cbie_dp0_ex": (_I32, [_P, _F, _I32, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
     "bem_add_identity": (_I32, [_P, _I32, _F, _F, _P, _I64, _P]),
     "bem_assemble_rows": (_I32, [_P, _F, _I32, _I32, _I32, _I64, _I64, _P, _I64, _P]),
+    "bem_geomset_create": (_I32, [_P, _P, _I32, _P, ctypes.POINTER(_P)]),
+    "bem_geomset_destroy": (_I32, [_P]),
+    "bem_geomset_count": (_I32, [_P]),
+    "bem_assemble_samples": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
+    "bem_add_identity_samples": (_I32, [_P, _I32, _P, _P, _I64, _I64, _P]),
+    "bem_potential_samples": (_I32, [_P, _P, _I64, _P, _I32, _I32, _I32, _P, _I64, _I64, _P]),
+    "bem_matvec_strided": (_I32, [_P, _I64, _I64, _I64, _I64, _P, _I64, _P, _I64, _I32, _F, _F, _F, _F, _P]),
     "bem_boundary_matrix": (_I32, [_P, _I64, _P, _I64, _F, _I32, _I32, _P, _P]),
     "bem_identity": (_I32, [_P, _I32, _P, _I64, _P]),
     "bem_triangle2vertex": (_I32, [_P, _P, _P, _P]),

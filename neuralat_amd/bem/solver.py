@@ -94,6 +94,122 @@ class Mesh:
             self._h = None
 
 
+class GeometrySet:
+    """Several placements ("samples") of one mesh's triangles (a
+    ``bem_geomset``): each sample's position-dependent data on the device, the
+    mesh's connectivity (tiling, neighbour pairs) shared.  ``vertices``:
+    [ns, V, 3].  Operators of all samples come from one launch per kernel
+    (the reference builds one BEM_Solver per sample, scene.py:218-291)."""
+
+    def __init__(self, mesh: Mesh, vertices):
+        if torch.is_tensor(vertices):
+            vertices = vertices.detach().cpu().numpy()
+        v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(-1, mesh.nv, 3))
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        with torch.cuda.device(mesh.device):
+            rc = lib.bem_geomset_create(mesh.handle, v.ctypes.data, v.shape[0], _stream(), ctypes.byref(h))
+        _lib.check(rc, "bem_geomset_create")
+        self._h, self._lib, self.mesh, self.ns = h, lib, mesh, int(v.shape[0])
+        self.device = mesh.device
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _ks(self, ks):
+        ks = np.ascontiguousarray(np.asarray(ks, dtype=np.float32).reshape(-1))
+        if ks.shape[0] != self.ns:
+            raise ValueError(f"need one wavenumber per sample ({self.ns})")
+        return ks
+
+    def assemble(self, ks, layer_type, approx=False, basis="P1", clamp=True):
+        """Operator of every sample at its wavenumber: complex64 [ns, n, n]."""
+        if layer_type not in _lib.OPS:
+            raise ValueError(f"unknown layer type {layer_type!r}")
+        quad = "approx_1" if approx == "approx_1" else ("approx" if approx else "full")
+        ks = self._ks(ks)
+        n = self.mesh.size(basis)
+        out = torch.empty((self.ns, n, n), dtype=torch.complex64, device=self.device)
+        _lib.check(self._lib.bem_assemble_samples(self._h, ks.ctypes.data, _lib.OPS[layer_type], _lib.BASES[basis],
+                                                  _lib.QUADS[quad], int(clamp), _ptr(out), n, n * n, _stream()),
+                   "bem_assemble_samples")
+        return out
+
+    def add_identity_(self, A, alphas, basis="P1"):
+        """A[z] += alphas[z] M_z in place."""
+        al = np.ascontiguousarray(np.asarray(alphas, dtype=np.complex64).reshape(-1))
+        if al.shape[0] != self.ns:
+            raise ValueError(f"need one coefficient per sample ({self.ns})")
+        al = al.view(np.float32)
+        n = A.shape[-1]
+        _lib.check(self._lib.bem_add_identity_samples(self._h, _lib.BASES[basis], al.ctypes.data, _ptr(A), A.stride(1),
+                                                      A.stride(0), _stream()), "bem_add_identity_samples")
+        return A
+
+    def potential(self, ks, points, layer_type, basis="P1", clamp=True):
+        """Potential operator of every sample at its own points [ns, np, 3]:
+        complex64 [ns, np, n]."""
+        ks = self._ks(ks)
+        pts = torch.as_tensor(points, dtype=torch.float32, device=self.device).reshape(self.ns, -1, 3).contiguous()
+        npts, n = pts.shape[1], self.mesh.size(basis)
+        out = torch.empty((self.ns, npts, n), dtype=torch.complex64, device=self.device)
+        _lib.check(self._lib.bem_potential_samples(self._h, _ptr(pts), npts, ks.ctypes.data, _lib.OPS[layer_type],
+                                                   _lib.BASES[basis], int(clamp), _ptr(out), n, npts * n, _stream()),
+                   "bem_potential_samples")
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.current_stream(self.device).synchronize()
+            except Exception:
+                pass
+            self._lib.bem_geomset_destroy(h)
+            self._h = None
+
+
+def matvec_strided(A: torch.Tensor, x: torch.Tensor, alpha=1.0, beta=0.0, out=None):
+    """out[z] = alpha A[z] x[z] + beta out[z] for A [nz, rows, cols], x [nz, cols]
+    (one launch)."""
+    check_tensor(A, torch.complex64)
+    check_tensor(x, torch.complex64)
+    nz, rows, cols = A.shape
+    if out is None:
+        out = torch.empty((nz, rows), dtype=torch.complex64, device=A.device)
+    a, b = complex(alpha), complex(beta)
+    _lib.check(_lib.load().bem_matvec_strided(_ptr(A), rows, cols, A.stride(1), A.stride(0), _ptr(x), x.stride(0),
+                                              _ptr(out), out.stride(0), nz, a.real, a.imag, b.real, b.imag,
+                                              _stream()), "bem_matvec_strided")
+    return out
+
+
+def burton_miller_samples(gs: GeometrySet, ks, neumanns, clamp=True):
+    """The neumann2dirichlet systems (solver.py:120-125, approximate rules) of
+    every sample of a geometry set at its wavenumber: lhs [ns, V, V] =
+    bm_lhs + M/2, rhs [ns, V] = (bm_rhs - (i/k) M/2) psi; one launch per
+    kernel for all samples."""
+    ks = [float(k) for k in ks]
+    psi = torch.as_tensor(neumanns, device=gs.device).to(torch.complex64).reshape(gs.ns, -1).contiguous()
+    lhs = gs.assemble(ks, "bm_lhs", approx=True, clamp=clamp)
+    gs.add_identity_(lhs, [0.5] * gs.ns)
+    R = gs.assemble(ks, "bm_rhs", approx=True, clamp=clamp)
+    gs.add_identity_(R, [-0.5j / k for k in ks])
+    return lhs, matvec_strided(R, psi)
+
+
+def boundary2potential_samples(gs: GeometrySet, ks, neumanns, dirichlets, points, clamp=True):
+    """p[z] = D_z phi_z - S_z psi_z at every sample's own points [ns, np, 3]
+    (solver.py:140-147): [ns, np]."""
+    psi = torch.as_tensor(neumanns, device=gs.device).to(torch.complex64).reshape(gs.ns, -1).contiguous()
+    phi = torch.as_tensor(dirichlets, device=gs.device).to(torch.complex64).reshape(gs.ns, -1).contiguous()
+    S = gs.potential(ks, points, "single", clamp=clamp)
+    D = gs.potential(ks, points, "double", clamp=clamp)
+    p = matvec_strided(D, phi)
+    return matvec_strided(S, psi, alpha=-1.0, beta=1.0, out=p)
+
+
 def preprocess(vertices, triangles):
     """Unit normals [N,3] and P1 surface curls [N,3,3] (row a = n x grad phi_a),
     as the reference's preprocess (solver.py:40-58), computed by the library."""
@@ -409,18 +525,16 @@ class BEM_Solver:
         (``neumanns``: [K, n]): lhs [K, n, n] = bm_lhs + M/2, rhs [K, n] =
         (bm_rhs - (i/k) M/2) psi.  Each operator kind is assembled for all
         wavenumbers in one launch per kernel; the mass terms are added in
-        place and the right-hand sides are library GEMVs."""
+        place and the right-hand sides come from one strided GEMV launch."""
         ks = [float(k) for k in ks]
         neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
         neumanns = neumanns.contiguous()
         lhs = self.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
         R = self.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
-        rhs = torch.empty((len(ks), lhs.shape[1]), dtype=torch.complex64, device=self.device)
         for z, k in enumerate(ks):
             add_identity_(self, lhs[z], 0.5)
             add_identity_(self, R[z], -0.5j / k)
-            matvec(R[z], neumanns[z], out=rhs[z])
-        return lhs, rhs
+        return lhs, matvec_strided(R, neumanns)
 
     def neumann2dirichlet_batch(self, ks, neumanns, tol=1e-6, restart=100, max_cycles=10):
         """neumann2dirichlet at every wavenumber of ``ks`` (the per-mode loop of
@@ -444,11 +558,8 @@ class BEM_Solver:
         neumanns, dirichlets = neumanns.contiguous(), dirichlets.contiguous()
         S = self.potential_matrix_batch(ks, points, "single")
         D = self.potential_matrix_batch(ks, points, "double")
-        p = torch.empty((len(ks), S.shape[1]), dtype=torch.complex64, device=self.device)
-        for z in range(len(ks)):
-            matvec(D[z], dirichlets[z], out=p[z])
-            matvec(S[z], neumanns[z], alpha=-1.0, beta=1.0, out=p[z])
-        return p
+        p = matvec_strided(D, dirichlets)
+        return matvec_strided(S, neumanns, alpha=-1.0, beta=1.0, out=p)
 
     def potential_matrix(self, k, points, layer_type, basis=None, clamp=None):
         basis = basis or self.basis

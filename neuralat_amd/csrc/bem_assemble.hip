@@ -44,6 +44,7 @@ __global__ __launch_bounds__(256) void dp0_regular_kernel(const float* __restric
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
+    rec += (int64_t)blockIdx.z * kb.gstride;
     constexpr int S = Rec<Q>::kFloats;
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const bool active = j < nt;
@@ -716,6 +717,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
+    rec += (int64_t)blockIdx.z * kb.gstride;
+    curls += (int64_t)blockIdx.z * kb.gstride;
     constexpr int CT = kP1ColTris;  // one lane per column triangle and per column vertex
     constexpr int NCV = kP1ColVerts;
     static_assert(CT == 256 && NCV <= CT, "one lane per column triangle and per column vertex");
@@ -994,6 +997,8 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
+    geo += (int64_t)blockIdx.z * kb.gstride;
+    curls += (int64_t)blockIdx.z * kb.gstride;
     const int lane = threadIdx.x & 63;
     // lanes per pair: a wavefront per pair, except the 2-point line rule of the
     // P1 operators (16 points): 16 lanes per pair, 4 pairs per wavefront, so
@@ -1374,6 +1379,8 @@ __global__ __launch_bounds__(256) void potential_kernel(const float* __restrict_
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
+    rec6 += (int64_t)blockIdx.z * kb.gstride;
+    pts += (int64_t)blockIdx.z * kb.pstride;
     constexpr int Q = 6;
     constexpr int S = Rec<Q>::kFloats;
     const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1716,26 +1723,39 @@ hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld,
     return hipGetLastError();
 }
 
-// out += alpha * M (mass matrix) in place: DP0 diagonal, P1 one thread per row
-__global__ __launch_bounds__(256) void add_identity_dp0_kernel(const float* __restrict__ geo, int64_t nt, float2 alpha,
-                                                               float2* __restrict__ out, int64_t ld)
+// out_z += alpha_z * M_z (mass matrix of geometry z) in place, z = blockIdx.y
+// (one geometry: gstride 0): DP0 diagonal, P1 one thread per row
+struct AlphaBatch {
+    float2 a[kMaxBatchK];
+};
+
+__global__ __launch_bounds__(256) void add_identity_dp0_kernel(const float* __restrict__ geo, int64_t gstride,
+                                                               int64_t nt, const AlphaBatch ab,
+                                                               float2* __restrict__ out, int64_t ld, int64_t zstride)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nt) return;
+    geo += (int64_t)blockIdx.y * gstride;
+    out += (int64_t)blockIdx.y * zstride;
+    const float2 alpha = ab.a[blockIdx.y];
     const float a = 0.5f * geo[t * 16 + 12];
     float2* d = &out[t * ld + t];
     d->x += alpha.x * a;
     d->y += alpha.y * a;
 }
 
-__global__ __launch_bounds__(256) void add_identity_p1_kernel(const float* __restrict__ geo,
+__global__ __launch_bounds__(256) void add_identity_p1_kernel(const float* __restrict__ geo, int64_t gstride,
                                                               const int4* __restrict__ tris,
                                                               const int32_t* __restrict__ vt_ptr,
                                                               const int32_t* __restrict__ vt_idx, int64_t nv,
-                                                              float2 alpha, float2* __restrict__ out, int64_t ld)
+                                                              const AlphaBatch ab, float2* __restrict__ out,
+                                                              int64_t ld, int64_t zstride)
 {
     const int64_t va = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (va >= nv) return;
+    geo += (int64_t)blockIdx.y * gstride;
+    out += (int64_t)blockIdx.y * zstride;
+    const float2 alpha = ab.a[blockIdx.y];
     for (int e = vt_ptr[va]; e < vt_ptr[va + 1]; ++e) {
         const int t = vt_idx[e];
         const float area = 0.5f * geo[(int64_t)t * 16 + 12];
@@ -1751,14 +1771,68 @@ __global__ __launch_bounds__(256) void add_identity_p1_kernel(const float* __res
     }
 }
 
-hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s)
+static hipError_t add_identity_z(const MeshDev& m, int64_t gstride, int basis, const AlphaBatch& ab, int nz,
+                                 float2* out, int64_t ld, int64_t zstride, hipStream_t s)
 {
     if (basis == DP0)
-        hipLaunchKernelGGL(add_identity_dp0_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, m.geo, m.nt, alpha, out, ld);
+        hipLaunchKernelGGL(add_identity_dp0_kernel, dim3(grid1d(m.nt), (unsigned)nz), dim3(256), 0, s, m.geo, gstride,
+                           m.nt, ab, out, ld, zstride);
     else
-        hipLaunchKernelGGL(add_identity_p1_kernel, dim3(grid1d(m.nv)), dim3(256), 0, s, m.geo, m.tris, m.vt_ptr,
-                           m.vt_idx, m.nv, alpha, out, ld);
+        hipLaunchKernelGGL(add_identity_p1_kernel, dim3(grid1d(m.nv), (unsigned)nz), dim3(256), 0, s, m.geo, gstride,
+                           m.tris, m.vt_ptr, m.vt_idx, m.nv, ab, out, ld, zstride);
     return hipGetLastError();
+}
+
+hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s)
+{
+    AlphaBatch ab{};
+    ab.a[0] = alpha;
+    return add_identity_z(m, 0, basis, ab, 1, out, ld, 0, s);
+}
+
+// The position-dependent pointers of m moved by off floats (the geometry of
+// sample z0 of a geometry set whose region view is m).
+static MeshDev geometry_shift(const MeshDev& m, int64_t off)
+{
+    MeshDev g = m;
+    g.verts += off;
+    g.rec6 += off;
+    g.rec3 += off;
+    g.rec1 += off;
+    g.curls += off;
+    g.geo += off;
+    return g;
+}
+
+hipError_t launch_assemble_samples(const MeshDev& mg, int64_t gstride, const float* ks, int ns, int op, int basis,
+                                   int quad, int clamp, float2* out, int64_t ld, int64_t stride, hipStream_t s)
+{
+    const int64_t n = basis == DP0 ? mg.nt : mg.nv;
+    for (int z0 = 0; z0 < ns; z0 += kMaxBatchK) {
+        const int nz = ns - z0 < kMaxBatchK ? ns - z0 : kMaxBatchK;
+        KBatch kb{};
+        for (int z = 0; z < nz; ++z) kb.k[z] = ks[z0 + z];
+        kb.zstride = stride;
+        kb.gstride = gstride;
+        const hipError_t e = assemble_dispatch(geometry_shift(mg, (int64_t)z0 * gstride), op, basis, quad, clamp, 3, 0,
+                                               n, kb, nz, out + (int64_t)z0 * stride, ld, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_add_identity_samples(const MeshDev& mg, int64_t gstride, int basis, const float* alphas, int ns,
+                                       float2* out, int64_t ld, int64_t stride, hipStream_t s)
+{
+    for (int z0 = 0; z0 < ns; z0 += kMaxBatchK) {
+        const int nz = ns - z0 < kMaxBatchK ? ns - z0 : kMaxBatchK;
+        AlphaBatch ab{};
+        for (int z = 0; z < nz; ++z) ab.a[z] = make_float2(alphas[2 * (z0 + z)], alphas[2 * (z0 + z) + 1]);
+        const hipError_t e = add_identity_z(geometry_shift(mg, (int64_t)z0 * gstride), gstride, basis, ab, nz,
+                                            out + (int64_t)z0 * stride, ld, stride, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s)
@@ -1878,6 +1952,24 @@ hipError_t launch_potential(const MeshDev& m, const float* pts, int64_t np, floa
                             float2* out, int64_t ld, hipStream_t s)
 {
     return potential_dispatch(m, pts, np, kbatch1(k), 1, op, basis, clamp, out, ld, s);
+}
+
+hipError_t launch_potential_samples(const MeshDev& mg, int64_t gstride, const float* pts, int64_t np, const float* ks,
+                                    int ns, int op, int basis, int clamp, float2* out, int64_t ld, int64_t stride,
+                                    hipStream_t s)
+{
+    for (int z0 = 0; z0 < ns; z0 += kMaxBatchK) {
+        const int nz = ns - z0 < kMaxBatchK ? ns - z0 : kMaxBatchK;
+        KBatch kb{};
+        for (int z = 0; z < nz; ++z) kb.k[z] = ks[z0 + z];
+        kb.zstride = stride;
+        kb.gstride = gstride;
+        kb.pstride = 3 * np;
+        const hipError_t e = potential_dispatch(geometry_shift(mg, (int64_t)z0 * gstride), pts + 3 * np * z0, np, kb,
+                                                nz, op, basis, clamp, out + (int64_t)z0 * stride, ld, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_potential_batch_k(const MeshDev& m, const float* pts, int64_t np, const float* ks, int nk, int op,

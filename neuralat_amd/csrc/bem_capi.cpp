@@ -264,6 +264,140 @@ int bem_assemble_batch_k(const bem_mesh* mesh, const float* ks, int nk, int op, 
     return BEM_OK;
 }
 
+// ---- geometry sets: several placements of one mesh's triangles ----------------
+
+struct bem_geomset {
+    const bem_mesh* mesh = nullptr;
+    bem::MeshDev view;  // the mesh's connectivity, position-dependent pointers at sample 0 of block
+    int64_t gstride = 0;  // floats from one sample's region to the next
+    int ns = 0;
+    void* block = nullptr;
+};
+
+int bem_geomset_create(const bem_mesh* mesh, const float* verts, int ns, void* stream, bem_geomset** out)
+{
+    if (!mesh || !verts || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    *out = nullptr;
+    if (ns < 1) return fail(BEM_ERR_BAD_ARG, "need at least one sample");
+    const auto& m = mesh->dev;
+    const size_t gb = m.geom_bytes;
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    std::vector<char> stage(gb * (size_t)ns);
+    std::string err;
+    for (int z = 0; z < ns; ++z) {
+        const int rc = bem::mesh_geometry_region(m, verts + (size_t)z * 3 * (size_t)m.nv, stage.data() + z * gb, err);
+        if (rc) return fail(rc == 2 ? BEM_ERR_BAD_MESH : BEM_ERR_BAD_ARG, "sample " + std::to_string(z) + ": " + err);
+    }
+    auto* gs = new bem_geomset();
+    if (hipMalloc(&gs->block, stage.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        delete gs;
+        return fail(BEM_ERR_OOM, "device allocation of " + std::to_string(stage.size()) + " bytes failed");
+    }
+    hipError_t e = hipMemcpyAsync(gs->block, stage.data(), stage.size(), hipMemcpyHostToDevice, S(stream));
+    if (e == hipSuccess) e = hipStreamSynchronize(S(stream));
+    if (e != hipSuccess) {
+        (void)hipFree(gs->block);
+        delete gs;
+        return hip_fail(e, "bem_geomset_create copy");
+    }
+    gs->mesh = mesh;
+    gs->ns = ns;
+    gs->gstride = (int64_t)(gb / sizeof(float));
+    gs->view = m;
+    auto rebase = [&](float*& p) { p = (float*)((char*)gs->block + ((char*)p - (char*)m.block)); };
+    rebase(gs->view.verts);
+    rebase(gs->view.rec6);
+    rebase(gs->view.rec3);
+    rebase(gs->view.rec1);
+    rebase(gs->view.curls);
+    rebase(gs->view.geo);
+    *out = gs;
+    return BEM_OK;
+}
+
+int bem_geomset_destroy(bem_geomset* gs)
+{
+    if (!gs) return BEM_OK;
+    DeviceGuard g(gs->view.device);
+    if (gs->block) (void)hipFree(gs->block);
+    delete gs;
+    return BEM_OK;
+}
+
+int bem_geomset_count(const bem_geomset* gs) { return gs ? gs->ns : 0; }
+
+int bem_assemble_samples(const bem_geomset* gs, const float* ks, int op, int basis, int quad, int clamp, void* out,
+                         int64_t ld, int64_t stride, void* stream)
+{
+    if (!gs || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (int rc = check_op(op, basis, quad)) return rc;
+    if (int rc = check_ks(ks, gs->ns, op)) return rc;
+    const auto& m = gs->view;
+    const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
+    if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (gs->ns > 1 && stride < n * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (n * ld)");
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_assemble_samples(m, gs->gstride, ks, gs->ns, op, basis, quad, clamp, (float2*)out, ld,
+                                                stride, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_assemble_samples launch");
+    return BEM_OK;
+}
+
+int bem_add_identity_samples(const bem_geomset* gs, int basis, const float* alphas, void* out, int64_t ld,
+                             int64_t stride, void* stream)
+{
+    if (!gs || !alphas || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    const auto& m = gs->view;
+    const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
+    if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (gs->ns > 1 && stride < n * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (n * ld)");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_add_identity_samples(m, gs->gstride, basis, alphas, gs->ns, (float2*)out, ld, stride,
+                                                    S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_add_identity_samples launch");
+    return BEM_OK;
+}
+
+int bem_potential_samples(const bem_geomset* gs, const float* points, int64_t np, const float* ks, int op, int basis,
+                          int clamp, void* out, int64_t ld, int64_t stride, void* stream)
+{
+    if (!gs || !out || (np > 0 && !points)) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (op != BEM_OP_SINGLE && op != BEM_OP_DOUBLE)
+        return fail(BEM_ERR_BAD_ARG, "potential operators are SINGLE and DOUBLE");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    if (int rc = check_ks(ks, gs->ns, op)) return rc;
+    const auto& m = gs->view;
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
+    const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
+    if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    if (gs->ns > 1 && stride < np * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (np * ld)");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_potential_samples(m, gs->gstride, points, np, ks, gs->ns, op, basis, clamp,
+                                                 (float2*)out, ld, stride, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_potential_samples launch");
+    return BEM_OK;
+}
+
+int bem_matvec_strided(const void* A, int64_t rows, int64_t cols, int64_t lda, int64_t stride_a, const void* x,
+                       int64_t stride_x, void* y, int64_t stride_y, int nz, float alpha_re, float alpha_im,
+                       float beta_re, float beta_im, void* stream)
+{
+    if (!A || !x || !y) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (rows < 0 || cols < 0 || lda < cols || nz < 0) return fail(BEM_ERR_BAD_ARG, "bad shape");
+    if (nz > 1 && (stride_a < rows * lda || stride_x < cols || stride_y < rows))
+        return fail(BEM_ERR_BAD_ARG, "strides smaller than one matrix / vector");
+    hipError_t e = bem::launch_gemv_strided((const float2*)A, rows, cols, lda, stride_a, (const float2*)x, stride_x,
+                                            (float2*)y, stride_y, nz, make_float2(alpha_re, alpha_im),
+                                            make_float2(beta_re, beta_im), S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_matvec_strided launch");
+    return BEM_OK;
+}
+
 int bem_assemble_rows(const bem_mesh* mesh, float k, int op, int quad, int clamp, int64_t row0, int64_t row1,
                       void* out, int64_t ld, void* stream)
 {

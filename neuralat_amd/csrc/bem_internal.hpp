@@ -82,10 +82,16 @@ struct MeshDev {
 // Wavenumbers of one launch over blockIdx.z: operator z of a batch on one
 // mesh uses k[z] and is written at out + z * zstride (elements).  Passed by
 // value (kernel arguments), at most kMaxBatchK per launch.
+// With a geometry set (several placements of one mesh's triangles) launch z
+// also reads the position-dependent arrays (records, curls, singular-pair
+// geometry) gstride floats further on, and the evaluation points pstride
+// floats further on; both are 0 for one geometry and one point set.
 constexpr int kMaxBatchK = 64;
 struct KBatch {
     float k[kMaxBatchK];
     int64_t zstride;
+    int64_t gstride = 0;
+    int64_t pstride = 0;
 };
 inline KBatch kbatch1(float k)
 {
@@ -143,6 +149,18 @@ hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld,
 // records msg as the calling thread's bem_last_error() and returns code
 int report_failure(int code, const std::string& msg);
 hipError_t launch_tri2vert(const MeshDev& m, const float2* tv, float2* out, hipStream_t s);
+// Geometry sets: mg is the view of sample 0 (the mesh's connectivity with
+// position-dependent pointers into the set), sample z's arrays lie gstride
+// floats further on.  One wavenumber per sample (ks host [ns]); operator z
+// at out + z * stride.  Potentials: points device [ns, np, 3].  Mass terms:
+// alphas host [ns] complex (re, im).
+hipError_t launch_assemble_samples(const MeshDev& mg, int64_t gstride, const float* ks, int ns, int op, int basis,
+                                   int quad, int clamp, float2* out, int64_t ld, int64_t stride, hipStream_t s);
+hipError_t launch_add_identity_samples(const MeshDev& mg, int64_t gstride, int basis, const float* alphas, int ns,
+                                       float2* out, int64_t ld, int64_t stride, hipStream_t s);
+hipError_t launch_potential_samples(const MeshDev& mg, int64_t gstride, const float* pts, int64_t np, const float* ks,
+                                    int ns, int op, int basis, int clamp, float2* out, int64_t ld, int64_t stride,
+                                    hipStream_t s);
 // work: int32 [2 nv + 3 nt + 2] (counts, list pointers, lists, error flag)
 hipError_t launch_tri2vert_raw(const int32_t* tris, int64_t nt, int64_t nv, const float2* tv, float2* out,
                                int32_t* work, hipStream_t s);
@@ -158,6 +176,10 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
 // position-dependent arrays on the host and uploads them in one copy on s,
 // then waits for it.  Connectivity data and the tiling are kept.
 int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::string& err);
+// The position-dependent region of m (the first m.geom_bytes of its block:
+// vertices, quadrature records, curls, singular-pair geometry) for new
+// positions h_verts [nv, 3] of the same triangles, written to region (host).
+int mesh_geometry_region(const MeshDev& m, const float* h_verts, char* region, std::string& err);
 void mesh_free(MeshDev& m);
 
 }  // namespace bem

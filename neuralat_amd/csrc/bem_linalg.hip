@@ -828,6 +828,56 @@ static inline bool gemv_t_vec_ok(const float2* A, int64_t lda)
 }
 static inline unsigned g1(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// y_z = alpha A_z x_z + beta y_z for nz matrices (z = blockIdx.y, each at
+// its own strides): one wavefront per row, four rows per workgroup, the
+// row's products summed in a fixed order (lane partials, then a butterfly).
+// y is not read when beta = 0.
+__global__ __launch_bounds__(256) void gemv_strided_kernel(const float2* __restrict__ A, int64_t rows, int64_t cols,
+                                                           int64_t lda, int64_t sa, const float2* __restrict__ x,
+                                                           int64_t sx, float2* __restrict__ y, int64_t sy,
+                                                           float2 alpha, float2 beta)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const float2* a = A + (int64_t)blockIdx.y * sa + r * lda;
+    const float2* xz = x + (int64_t)blockIdx.y * sx;
+    float sr = 0.f, si = 0.f;
+    for (int64_t c = lane; c < cols; c += 64) {
+        const float2 av = a[c], xv = xz[c];
+        sr = fmaf(av.x, xv.x, fmaf(-av.y, xv.y, sr));
+        si = fmaf(av.x, xv.y, fmaf(av.y, xv.x, si));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, 64);
+        si += __shfl_xor(si, off, 64);
+    }
+    if (lane == 0) {
+        float2* yz = y + (int64_t)blockIdx.y * sy + r;
+        float2 v = make_float2(alpha.x * sr - alpha.y * si, alpha.x * si + alpha.y * sr);
+        if (beta.x != 0.f || beta.y != 0.f) {
+            const float2 o = *yz;
+            v.x += beta.x * o.x - beta.y * o.y;
+            v.y += beta.x * o.y + beta.y * o.x;
+        }
+        *yz = v;
+    }
+}
+
+hipError_t launch_gemv_strided(const float2* A, int64_t rows, int64_t cols, int64_t lda, int64_t sa, const float2* x,
+                               int64_t sx, float2* y, int64_t sy, int nz, float2 alpha, float2 beta, hipStream_t s)
+{
+    if (rows <= 0 || nz <= 0) return hipSuccess;
+    for (int z0 = 0; z0 < nz; z0 += 65535) {  // grid y limit
+        const int cz = nz - z0 < 65535 ? nz - z0 : 65535;
+        hipLaunchKernelGGL(gemv_strided_kernel, dim3((unsigned)((rows + 3) / 4), (unsigned)cz), dim3(256), 0, s,
+                           A + (int64_t)z0 * sa, rows, cols, lda, sa, x + (int64_t)z0 * sx, sx, y + (int64_t)z0 * sy,
+                           sy, alpha, beta);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda, int trans, const float2* x, float2* y,
                        float2 alpha, float2 beta, float2* work, hipStream_t s)
 {

@@ -7,6 +7,10 @@ namespace bem {
 
 constexpr int kGemvRows = 4;  // rows per wavefront in the streaming GEMV
 
+// y_z = alpha A_z x_z + beta y_z for nz matrices [rows, cols] (A_z at A + z sa,
+// x_z at x + z sx, y_z at y + z sy; elements)
+hipError_t launch_gemv_strided(const float2* A, int64_t rows, int64_t cols, int64_t lda, int64_t sa, const float2* x,
+                               int64_t sx, float2* y, int64_t sy, int nz, float2 alpha, float2 beta, hipStream_t s);
 hipError_t launch_gemv(const float2* A, int64_t rows, int64_t cols, int64_t lda, int trans, const float2* x, float2* y,
                        float2 alpha, float2 beta, float2* work, hipStream_t s);
 int64_t gemv_work_elems(int64_t rows, int64_t cols, int trans);

@@ -471,7 +471,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     return 0;
 }
 
-int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::string& err)
+int mesh_geometry_region(const MeshDev& m, const float* h_verts, char* region, std::string& err)
 {
     if (!m.block || m.h_tris.size() != (size_t)m.nt * 3) {
         err = "mesh has no device data";
@@ -483,27 +483,33 @@ int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::stri
     const std::pair<const float*, size_t> sec[6] = {{h_verts, (size_t)m.nv * 3},   {hg.rec6.data(), hg.rec6.size()},
                                                     {hg.rec3.data(), hg.rec3.size()}, {hg.rec1.data(), hg.rec1.size()},
                                                     {hg.curls.data(), hg.curls.size()}, {hg.geo.data(), hg.geo.size()}};
-    std::vector<char> stage(m.geom_bytes, 0);
+    std::memset(region, 0, m.geom_bytes);
     size_t off = 0;
     for (const auto& q : sec) {
         const size_t bytes = q.second * 4;
-        if (off + bytes > stage.size()) {
+        if (off + bytes > m.geom_bytes) {
             err = "geometry layout mismatch";
             return 1;
         }
-        std::memcpy(stage.data() + off, q.first, bytes);
+        std::memcpy(region + off, q.first, bytes);
         off += (bytes + 255) / 256 * 256;
     }
     if (off != m.geom_bytes) {
         err = "geometry layout mismatch";
         return 1;
     }
+    return 0;
+}
+
+int mesh_set_vertices(MeshDev& m, const float* h_verts, hipStream_t s, std::string& err)
+{
+    std::vector<char> stage(m.geom_bytes, 0);
+    if (int rc = mesh_geometry_region(m, h_verts, stage.data(), err)) return rc;
     if (hipMemcpyAsync(m.block, stage.data(), stage.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess) {
         err = "host to device copy failed";
         return 3;
     }
-    m.diameter = hg.diameter;
     return 0;
 }
 

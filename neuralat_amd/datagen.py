@@ -7,10 +7,12 @@ merges them into one mesh with per-vertex Neumann data from the objects'
 vibration masks, draws a frequency log-uniformly in [freq_min, freq_max]
 (k = 2 pi f / 343.2), solves the P1 Burton-Miller system of
 ``BEM_Solver.neumann2dirichlet`` and evaluates |p| at target points sampled in
-a spherical shell around the target box.  The samples' systems are assembled
-one by one (as the reference) and solved together by the lockstep batched
-GMRES (one graph-captured restart cycle for all of them), which is where a
-batch of small scenes gains over one solve per sample.
+a spherical shell around the target box.  The samples of a batch share the
+scene's triangles, so they form one geometry set: each operator kind of
+every sample is assembled in one launch per kernel, the systems are solved
+together by the lockstep batched GMRES (one graph-captured restart cycle for
+all of them), and the pressures of all samples come from one launch per
+kernel as well (the reference solves sample by sample, src/scene.py:270-291).
 
 Output layout, as generate.py: x [S, T, 3 + n_rot + n_move + 2] float32 =
 (target factors, rotation factors, move factors, resize factor, frequency
@@ -25,7 +27,8 @@ import numpy as np
 import torch
 
 from .bem.batch import batch_gmres
-from .bem.solver import BEM_Solver, add_identity_, matvec
+from .bem.solver import (BEM_Solver, GeometrySet, Mesh, add_identity_, boundary2potential_samples,
+                         burton_miller_samples, matvec)
 
 SPEED_OF_SOUND = 343.2  # src/scene.py:275
 
@@ -137,36 +140,30 @@ class SceneSampler:
         return x
 
     def generate(self, n_samples, seed=0, batch=16, device="cuda", tol=1e-6, restart=100, max_cycles=20):
-        """x [S, T, D], y [S, T, 1] (generate.py:13-39) with ``batch`` samples
-        solved per lockstep GMRES call.  Returns (x, y, infos)."""
+        """x [S, T, D], y [S, T, 1] (generate.py:13-39), ``batch`` samples per
+        geometry set and lockstep GMRES call.  Returns (x, y, infos)."""
         rng = np.random.default_rng(seed)
         dev = torch.device(device)
         S, T = int(n_samples), self.trg_sample_num
         x = torch.zeros((S, T, self.feature_dim), dtype=torch.float32)
         y = torch.zeros((S, T, 1), dtype=torch.float32)
         infos = []
-        pool = []  # one solver per batch slot: every sample has the same triangles, only positions move
+        mesh = None  # the scene's triangles (every sample has the same ones)
         for s0 in range(0, S, batch):
             facs = [self.sample_factors(rng) for _ in range(min(batch, S - s0))]
-            systems = []
-            for q, f in enumerate(facs):
-                v, t, neu, k, pts = self.build(f)
-                vt = torch.tensor(v.astype(np.float32), device=dev)
-                if q < len(pool):
-                    solver = pool[q]
-                    solver.set_vertices(vt)
-                else:
-                    solver = BEM_Solver(vt, torch.tensor(t, device=dev), basis="P1")
-                    pool.append(solver)
-                lhs, rhs = burton_miller_system(solver, k, torch.tensor(neu, device=dev))
-                systems.append((solver, k, torch.tensor(neu, device=dev), lhs, rhs,
-                                torch.tensor(pts.astype(np.float32), device=dev)))
-            xs, inf = batch_gmres([q[3] for q in systems], [q[4] for q in systems], tol=tol, restart=restart,
-                                  max_cycles=max_cycles)
+            built = [self.build(f) for f in facs]
+            verts = np.stack([b[0] for b in built]).astype(np.float32)
+            if mesh is None:
+                mesh = Mesh(torch.tensor(verts[0], device=dev), torch.tensor(built[0][1], device=dev))
+            gs = GeometrySet(mesh, verts)
+            ks = [b[3] for b in built]
+            neu = torch.tensor(np.stack([b[2] for b in built]), device=dev)
+            pts = torch.tensor(np.stack([b[4] for b in built]).astype(np.float32), device=dev)
+            lhs, rhs = burton_miller_samples(gs, ks, neu)
+            xs, inf = batch_gmres(list(lhs), list(rhs), tol=tol, restart=restart, max_cycles=max_cycles)
             infos.extend(inf)
-            ps = [solver.boundary2potential(k, neu, xs[q], pts).abs()
-                  for q, (solver, k, neu, _, _, pts) in enumerate(systems)]
-            y[s0:s0 + len(facs), :, 0] = torch.stack(ps).float().cpu()  # one copy per batch
+            p = boundary2potential_samples(gs, ks, neu, torch.stack(xs), pts)
+            y[s0:s0 + len(facs), :, 0] = p.abs().float().cpu()  # one copy per batch
             for q, f in enumerate(facs):
                 x[s0 + q] = torch.from_numpy(self.features(f))
         return x, y, infos

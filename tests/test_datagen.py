@@ -109,3 +109,84 @@ def test_modal_sound_pipeline_batched_equals_loop():
     assert rel < 2e-5, rel
     xs, ys = generate_modal_dataset(ms, 2, seed=1)
     assert xs.shape == (2, 32, 5) and ys.shape == (2, 32, 6) and bool(torch.isfinite(ys).all())
+
+
+def _placements(ns, level=2, seed=4):
+    """ns placements of one perturbed sphere: random rotation, anisotropic
+    scale and offset per sample (same triangles)."""
+    from scipy.spatial.transform import Rotation
+
+    v, t = mesh.icosphere(level, 0.1)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=seed)
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(ns):
+        R = Rotation.from_rotvec(rng.normal(size=3)).as_matrix()
+        out.append((v * rng.uniform(0.7, 1.4, 3)) @ R.T + rng.uniform(-0.2, 0.2, 3))
+    return np.array(out, dtype=np.float32), t
+
+
+@pytest.mark.gpu
+def test_geometry_set_matches_per_sample_meshes():
+    """Operators, mass terms and potentials of a geometry set (one launch per
+    kernel for all samples, the first sample's tiling) against a mesh handle
+    built for each sample: relative Frobenius <= 1e-6 (the P1 sums run in
+    another order)."""
+    from neuralat_amd.bem import BEM_Solver, GeometrySet, Mesh
+
+    dev = torch.device("cuda:0")
+    V, t = _placements(3)
+    ks = [7.0, -12.5, 21.0]
+    tt = torch.tensor(t, device=dev)
+    gs = GeometrySet(Mesh(torch.tensor(V[0], device=dev), tt), V)
+    rng = np.random.default_rng(1)
+    pts = torch.tensor((rng.normal(size=(3, 20, 3)) * 0.6).astype(np.float32), device=dev)
+    cases = [("P1", op, q) for op, q in (("single", False), ("double", False), ("hypersingular", True),
+                                         ("bm_lhs", True), ("bm_rhs", True), ("adjointdouble", "approx_1"))]
+    cases += [("DP0", "single", False), ("DP0", "double", True)]
+    per = [BEM_Solver(torch.tensor(V[z], device=dev), tt, basis="P1") for z in range(3)]
+    for basis, op, approx in cases:
+        A = gs.assemble(ks, op, approx=approx, basis=basis)
+        for z in range(3):
+            R = per[z].assemble_boundary_matrix(ks[z], op, approx=approx, basis=basis)
+            e = float(torch.linalg.matrix_norm(A[z] - R) / torch.linalg.matrix_norm(R))
+            assert e < 1e-6, (basis, op, approx, z, e)
+    for basis in ("P1", "DP0"):
+        n = gs.mesh.size(basis)
+        Z = torch.zeros((3, n, n), dtype=torch.complex64, device=dev)
+        gs.add_identity_(Z, [0.5, 1 - 2j, -0.25j], basis=basis)
+        for z, a in enumerate([0.5, 1 - 2j, -0.25j]):
+            M = per[z].identity_matrix(basis=basis) * a
+            assert float(torch.linalg.matrix_norm(Z[z] - M) / torch.linalg.matrix_norm(M)) < 1e-6
+        for op in ("single", "double"):
+            P = gs.potential(ks, pts, op, basis=basis)
+            for z in range(3):
+                R = per[z].potential_matrix(ks[z], pts[z], op, basis=basis)
+                assert float(torch.linalg.matrix_norm(P[z] - R) / torch.linalg.matrix_norm(R)) < 1e-6, (basis, op, z)
+
+
+@pytest.mark.gpu
+def test_generated_samples_vs_fp64_oracle():
+    """Two samples of the batched driver end to end against the FP64 oracle:
+    the Burton-Miller solve of each sample (bm_lhs, bm_rhs, approximate rules,
+    the mass terms) and |p| at its targets."""
+    import oracle
+
+    sc = SceneSampler(_objects(), trg_sample_num=16, freq_min=200.0, freq_max=2000.0, trg_pos_min=[-0.5] * 3,
+                      trg_pos_max=[0.5] * 3)
+    x, y, infos = sc.generate(2, seed=5, batch=2)
+    assert all(i["converged"] for i in infos), infos
+    rng = np.random.default_rng(5)
+    for q in range(2):
+        f = sc.sample_factors(rng)
+        v, t, neu, k, pts = sc.build(f)
+        v = v.astype(np.float32)
+        M = oracle.identity(v, t, "P1")
+        L = oracle.assemble(v, t, k, "bm_lhs", "P1", quad="approx") + 0.5 * M
+        R = oracle.assemble(v, t, k, "bm_rhs", "P1", quad="approx") - 0.5j / k * M
+        phi = np.linalg.solve(L, R @ neu.astype(np.complex128))
+        p = (oracle.potential(v, t, pts, k, "double", "P1") @ phi
+             - oracle.potential(v, t, pts, k, "single", "P1") @ neu.astype(np.complex128))
+        err = np.linalg.norm(y[q, :, 0].numpy() - np.abs(p)) / np.linalg.norm(np.abs(p))
+        print(f"generated sample {q} (k = {k:.2f}): |p| vs FP64 oracle {err:.2e}", flush=True)
+        assert err < 1e-4, (q, err)

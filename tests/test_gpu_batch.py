@@ -118,3 +118,35 @@ def test_batch_mixed_precision_rejects_bad_dtype(items):
     A = torch.eye(8, dtype=torch.complex64, device=_dev())
     with pytest.raises(ValueError):
         batch_solve_mixed([A], [torch.ones(8, dtype=torch.complex64, device=_dev())], dtype="int4")
+
+
+def test_batch_items_vs_fp64_oracle(items):
+    """Each item of the persistent batch against the FP64 oracle directly: the
+    batched system (M/2 - K, -V psi) against oracle.assemble of the item's
+    mesh and wavenumber (relative Frobenius <= 1e-5), and the lockstep
+    complex64 solve and the fp16 / fp8 mixed-precision solves against the
+    FP64 oracle's solution of that system (<= 1e-4)."""
+    import oracle
+    from neuralat_amd.bem.batch import batch_cbie, batch_gmres, batch_solve_mixed
+
+    outs = batch_cbie([s.mesh for s, _, _ in items], [k for _, k, _ in items], [p for _, _, p in items])
+    As, bs = [o[0] for o in outs], [o[1] for o in outs]
+    sols = {"c64": batch_gmres(As, bs, tol=1e-6, restart=30, max_cycles=60)[0]}
+    for dt in ("fp16", "fp8"):
+        sols[dt] = batch_solve_mixed(As, bs, dtype=dt, tol=1e-6, inner_tol=1e-2, outer_restart=10, inner_restart=50,
+                                     max_outer_cycles=4)[0]
+    for q, (s, k, psi) in enumerate(items):
+        v = s.vertices.cpu().numpy()
+        t = s.triangles.cpu().numpy()
+        S = oracle.assemble(v, t, k, "single", "DP0")
+        D = oracle.assemble(v, t, k, "double", "DP0")
+        A64 = 0.5 * oracle.identity(v, t, "DP0") - D
+        b64 = -S @ psi.cpu().numpy().astype(np.complex128)
+        eA = np.linalg.norm(As[q].cpu().numpy() - A64) / np.linalg.norm(A64)
+        eb = np.linalg.norm(bs[q].cpu().numpy() - b64) / np.linalg.norm(b64)
+        assert eA < 1e-5 and eb < 1e-5, (q, eA, eb)
+        x64 = np.linalg.solve(A64, b64)
+        for name, xs in sols.items():
+            e = np.linalg.norm(xs[q].cpu().numpy() - x64) / np.linalg.norm(x64)
+            print(f"batch item {q} ({t.shape[0]} triangles, k = {k:.1f}) {name}: vs FP64 oracle {e:.2e}", flush=True)
+            assert e < 1e-4, (q, name, e)
