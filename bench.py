@@ -88,9 +88,12 @@ def parse():
                         "solve of the first 8 batching meshes x 8 k, for kernel traces")
     p.add_argument("--p1-probe", action="store_true",
                    help="assemble the level-5 P1 single- and double-layer operators once each, for counter passes")
+    p.add_argument("--c64-probe", action="store_true",
+                   help="the 16-right-hand-side complex64 matrix-core product on the assembled level-6 system, "
+                        "timed, then a 4 GiB copy (kernel traces and counter passes)")
     p.add_argument("--only-form", type=str, default="",
                    help="run one form alone and print its JSON (p1, batching, batching_small_meshes, "
-                        "differentiation, datagen, modal, burton_miller_level6)")
+                        "differentiation, datagen, modal, burton_miller_level6, precision)")
     p.add_argument("--modal-sound-probe", action="store_true",
                    help="three batched modal-sound solves (32 modes of the modal form's block), for kernel traces")
     return p.parse_args()
@@ -1151,6 +1154,52 @@ def traffic_probe(level, k):
                       "copy_read_bytes": 4 << 30, "copy_write_bytes": 4 << 30, "y0": str(complex(y[0].item()))}))
 
 
+def c64_probe(level, k, nrhs=16, reps=6):
+    """Y = A X on the FP32 matrix cores for the assembled level-``level``
+    CBIE matrix A (complex64) and ``nrhs`` seeded right-hand sides, ``reps``
+    launches timed with events (the product the refinement's residual and
+    the potentials use), the first 512 rows checked against torch's product;
+    then a device copy of 4 GiB for the FETCH_SIZE / WRITE_SIZE calibration."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.precision import matmul
+
+    v, t, src, _ = make_workload(level)
+    s = Solver(v.astype(np.float32), t)
+    _, dn = mesh.sources_field(src, None, k)
+    psi = torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64), device=s.dev)
+    s.assemble(k, psi)
+    n = s.n
+    g = torch.Generator(device=s.dev).manual_seed(1)
+    X = torch.randn((nrhs, n), dtype=torch.complex64, device=s.dev, generator=g)
+    ref = torch.matmul(s.lhs[:512], X.T.contiguous()).T
+    algo = 8 * n * n + 16 * nrhs * n
+    out = {"rows": n, "nrhs": nrhs, "algorithmic_bytes": algo}
+    for split in (True, False):
+        Y = torch.empty((nrhs, n), dtype=torch.complex64, device=s.dev)
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            matmul(s.lhs, X, out=Y, split=split)
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        err = float((torch.linalg.matrix_norm(Y[:, :512] - ref) / torch.linalg.matrix_norm(ref)).item())
+        best = min(ms[1:])
+        out["column_split" if split else "one_pass"] = {
+            "ms": ms, "best_ms": best, "achieved_gb_s": algo / best / 1e6,
+            "frac_of_hbm_peak": algo / (best / 1e3) / HBM_PEAK, "rel_err_vs_torch_first_512_rows": err}
+        del Y
+    a = torch.ones(1 << 30, dtype=torch.float32, device=s.dev)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    out.update({"copy_read_bytes": 4 << 30, "copy_write_bytes": 4 << 30})
+    print(json.dumps(out))
+
+
 def modal_sound_probe(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=3):
     """The modal form's modal-sound step alone: the block's k lowest modes,
     then ``reps`` batched solves (every mode's Burton-Miller system and the
@@ -1217,6 +1266,9 @@ def main():
     if args.p1_probe:
         p1_probe()
         return
+    if args.c64_probe:
+        c64_probe(args.level, ks[-1])
+        return
     if args.batch_probe:
         batch_probe()
         return
@@ -1231,6 +1283,10 @@ def main():
         if args.only_form == "burton_miller_level6":
             v, t, src, pts = make_workload(args.level)
             res = burton_miller_large_form(v, t, src, pts, ks, restart=args.restart)
+            fp64_answers(CHECKS)
+        elif args.only_form == "precision":
+            v, t, src, pts = make_workload(args.level)
+            res = precision_form(Solver(v.astype(np.float32), t), v, t, ks[-1], pts, restart=args.restart)
             fp64_answers(CHECKS)
         else:
             res = forms[args.only_form]()

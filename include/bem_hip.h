@@ -273,9 +273,13 @@ int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int
                      void* Ai, int64_t ldlp, float* scale_used, void* stream);
 /* Y = alpha A X + beta Y for nrhs right-hand sides (X: nrhs systems of cols
  * entries at stride ldx; Y likewise with ldy), complex64 A on FP32 matrix
- * cores. */
+ * cores.  work (bem_matmul_work_size(rows) bytes, or NULL) holds the partial
+ * sums of a column split, which fills the GPU when the row tiles alone do
+ * not (summed in a fixed order: the result does not depend on the split);
+ * with NULL every product is one pass over the columns. */
+int64_t bem_matmul_work_size(int64_t rows);
 int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const void* X, int64_t ldx, int nrhs, void* Y,
-               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* stream);
+               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* work, void* stream);
 /* Same on the 16-bit planes (result multiplied by inv_scale = 1/scale). */
 int64_t bem_matmul_lowp_work_size(int64_t ldlp);
 int bem_matmul_lowp(const void* Ar, const void* Ai, int dtype, int64_t rows, int64_t cols, int64_t ldlp,

@@ -95,7 +95,8 @@ SIGNATURES = {
     "bem_lowp_ld": (_I64, [_I64]),
     "bem_lowp_plane_elems": (_I64, [_I64, _I64]),
     "bem_convert_lowp": (_I32, [_P, _I64, _I64, _I64, _I32, _F, _P, _P, _I64, ctypes.POINTER(_F), _P]),
-    "bem_matmul": (_I32, [_P, _I64, _I64, _I64, _P, _I64, _I32, _P, _I64, _F, _F, _F, _F, _P]),
+    "bem_matmul_work_size": (_I64, [_I64]),
+    "bem_matmul": (_I32, [_P, _I64, _I64, _I64, _P, _I64, _I32, _P, _I64, _F, _F, _F, _F, _P, _P]),
     "bem_matmul_lowp_work_size": (_I64, [_I64]),
     "bem_matmul_lowp": (_I32, [_P, _P, _I32, _I64, _I64, _I64, _F, _P, _I64, _I32, _P, _I64, _F, _F, _F, _F, _P,
                                _P]),

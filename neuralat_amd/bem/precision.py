@@ -69,8 +69,10 @@ def _as_block(X):
     return X
 
 
-def matmul(A: torch.Tensor, X: torch.Tensor, alpha=1.0, beta=0.0, out=None):
-    """Y[s] = alpha A X[s] + beta Y[s] on the FP32 matrix cores (X: [nrhs, cols])."""
+def matmul(A: torch.Tensor, X: torch.Tensor, alpha=1.0, beta=0.0, out=None, split=True):
+    """Y[s] = alpha A X[s] + beta Y[s] on the FP32 matrix cores (X: [nrhs, cols]).
+    ``split``: give the product the scratch of a column split (used when the
+    row tiles alone do not fill the GPU; the result is the same either way)."""
     check_tensor(A, torch.complex64)
     X = _as_block(X)
     nrhs = X.shape[0]
@@ -78,8 +80,11 @@ def matmul(A: torch.Tensor, X: torch.Tensor, alpha=1.0, beta=0.0, out=None):
     if out is None:
         out = torch.empty((nrhs, rows), dtype=torch.complex64, device=A.device)
     a, b = complex(alpha), complex(beta)
-    _lib.check(_lib.load().bem_matmul(_ptr(A), rows, cols, A.stride(0), _ptr(X), X.stride(0), nrhs, _ptr(out),
-                                      out.stride(0), a.real, a.imag, b.real, b.imag, _stream()), "bem_matmul")
+    lib = _lib.load()
+    work = torch.empty(lib.bem_matmul_work_size(rows), dtype=torch.uint8, device=A.device) if split else None
+    _lib.check(lib.bem_matmul(_ptr(A), rows, cols, A.stride(0), _ptr(X), X.stride(0), nrhs, _ptr(out),
+                              out.stride(0), a.real, a.imag, b.real, b.imag, _ptr(work) if split else None,
+                              _stream()), "bem_matmul")
     return out
 
 

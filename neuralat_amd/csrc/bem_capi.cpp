@@ -928,8 +928,13 @@ int bem_convert_lowp(const void* A, int64_t rows, int64_t cols, int64_t lda, int
     return BEM_OK;
 }
 
+int64_t bem_matmul_work_size(int64_t rows)
+{
+    return rows > 0 ? bem::splitk_part_elems(rows) * (int64_t)sizeof(float2) : 0;
+}
+
 int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const void* X, int64_t ldx, int nrhs, void* Y,
-               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* stream)
+               int64_t ldy, float alpha_re, float alpha_im, float beta_re, float beta_im, void* work, void* stream)
 {
     if (!A || !X || !Y) return fail(BEM_ERR_BAD_ARG, "null pointer");
     if (rows <= 0 || cols <= 0 || lda < cols || nrhs <= 0 || ldx < cols || ldy < rows)
@@ -938,7 +943,7 @@ int bem_matmul(const void* A, int64_t rows, int64_t cols, int64_t lda, const voi
         const int cnt = std::min(16, nrhs - s0);
         HIP_TRY(bem::launch_gemm_c64((const float2*)A, rows, cols, lda, (const float2*)X + (int64_t)s0 * ldx, ldx, cnt,
                                      (float2*)Y + (int64_t)s0 * ldy, ldy, make_float2(alpha_re, alpha_im),
-                                     make_float2(beta_re, beta_im), nullptr, S(stream)));
+                                     make_float2(beta_re, beta_im), (float2*)work, S(stream)));
     }
     return BEM_OK;
 }

@@ -19,9 +19,10 @@
 //    and a range of column tiles (split-K); four products per tile:
 //    Ar Xr, Ai Xi, Ar Xi, Ai Xr, accumulated in FP32; Y = (rr - ii) + i (ri + ir).
 //    One right-hand side uses the same kernel (the product is HBM-bound).
-//  * gemm_c64_mfma_kernel: the same 16-RHS product on the complex64 matrix with
-//    v_mfma_f32_16x16x4f32 (the FP32 residual of the refinement), right-hand
-//    sides staged in LDS per workgroup.
+//  * gemm_c64_mfma3_kernel: the same 16-RHS product on the complex64 matrix
+//    with v_mfma_f32_16x16x4f32, three real products per complex entry (the
+//    FP32 residual of the refinement), right-hand sides staged in LDS per
+//    workgroup, loads one step ahead of the products.
 // Right-hand-side blocks are stored per system: X[s * ldx + k].
 #include <hip/hip_runtime.h>
 
@@ -226,70 +227,150 @@ __global__ __launch_bounds__(256) void gemm_lowp_mfma_kernel(const uint16_t* __r
 
 // complex64 matrix, up to 16 right-hand sides, FP32 matrix cores.  A
 // workgroup of kC64Waves wavefronts owns kC64Waves row tiles of 16 rows and a
-// column range (split-K); per 32-column step the workgroup stages the 16 x 32
-// slice of the right-hand sides in LDS once (double-buffered, one barrier per
-// step) and every wavefront takes its B operands from there -- read per
-// wavefront from global memory, the right-hand sides cost as many bytes as the
-// matrix and the product ran at 1.8 TB/s.  Lane l owns row r0 + (l % 16) and
-// the columns kb + 8 (l / 16) + [0, 8) of each step; each of the 8 sub-steps
-// is one K = 4 slice over the four lane groups.
-constexpr int kC64Waves = 8;
-__global__ __launch_bounds__(64 * kC64Waves) void gemm_c64_mfma_kernel(
+// column range (split-K); lane l owns row r0 + (l % 16).  Three real
+// matrix-core products per complex entry
+// (P1 = Ar Xr, P2 = Ai Xi, P3 = (Ar + Ai)(Xr + Xi); Yr = P1 - P2,
+// Yi = P3 - P1 - P2): at 16 right-hand sides four FP32 products per entry
+// need 16 FLOP per matrix byte, three need 12 -- 644 GFLOP for an 81920 x
+// 81920 matrix, 4.1 ms of the FP32 matrix-core peak against 6.7 ms of HBM
+// time, so the loads must overlap the products.
+//  * The right-hand sides of kC64Steps 32-column steps are staged in LDS per
+//    barrier (double-buffered) as two planes (Xr, Xi; Xr + Xi is formed at
+//    the read) laid out [plane][lane group q][row i][8 kC64Steps + 1]: the
+//    lanes of each 32-lane half of a wavefront read 32 different banks.  The
+//    next stage's entries are read from global memory while a stage computes.
+//  * The matrix entries of step s + 1 are in flight while step s computes.
+template <bool VEC, int kC64Waves, int kC64Steps>
+__global__ __launch_bounds__(64 * kC64Waves) __attribute__((amdgpu_waves_per_eu(4))) void gemm_c64_mfma3_kernel(
     const float2* __restrict__ A, int64_t rows, int64_t cols, int64_t lda, const float2* __restrict__ X, int64_t ldx,
     int nrhs, float2* __restrict__ Y, int64_t ldy, float2 alpha, float2 beta, int beta_zero, int64_t kchunk,
     float2* __restrict__ part)
 {
-    __shared__ float2 xs[2][16][32];
+    constexpr int kC64Pad = 8 * kC64Steps + 1;
+    __shared__ float xs[2][2][4][16][kC64Pad];
+    constexpr int W = 32 * kC64Steps;                 // columns per stage
+    constexpr int kXPer = 16 * W / (64 * kC64Waves);  // right-hand-side entries staged per thread and stage
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int li = lane & 15, q = lane >> 4;
     const int64_t r0 = ((int64_t)blockIdx.x * kC64Waves + wv) * 16;
     const float2* pa = A + min(r0 + li, rows - 1) * lda;
-    const bool live = li < nrhs;
-    const bool vec = (lda & 1) == 0 && ((uintptr_t)A & 15) == 0;  // 16-byte aligned rows
-    f32x4 rr = {0, 0, 0, 0}, ii = rr, ri = rr, ir = rr;
+    f32x4 p1 = {0, 0, 0, 0}, p2 = p1, p3 = p1;
     const int64_t k0 = (int64_t)blockIdx.y * kchunk;  // columns [k0, k0 + kchunk) of this split
     const int64_t kend = min(cols, k0 + kchunk);
-    int buf = 0;
-    for (int64_t kb = k0; kb < kend; kb += 32) {
-        for (int t = threadIdx.x; t < 16 * 32; t += 64 * kC64Waves) {
-            const int sx = t >> 5, c = t & 31;
-            const int64_t cc = kb + c;
-            xs[buf][sx][c] = (sx < nrhs && cc < kend) ? X[(int64_t)sx * ldx + cc] : make_float2(0.f, 0.f);
-        }
-        __syncthreads();
-        float2 a[8];
-        if (vec && kb + 32 <= kend) {  // whole step in range: four 16-byte loads per lane
-            const f32x4* p4 = reinterpret_cast<const f32x4*>(pa + kb + 8 * q);
+
+    // Right-hand sides: the next stage's entries are read (clamped addresses)
+    // while this stage computes; entries outside the split are staged as zero.
+    float2 xv[kXPer];
+    auto load_x = [&](int64_t kb) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const f32x4 v = __builtin_nontemporal_load(p4 + e);
-                a[2 * e] = make_float2(v.x, v.y);
-                a[2 * e + 1] = make_float2(v.z, v.w);
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int64_t c = kb + 8 * q + e;
-                a[e] = c < kend ? pa[c] : make_float2(0.f, 0.f);
-            }
+        for (int i = 0; i < kXPer; ++i) {
+            const int t = threadIdx.x + i * 64 * kC64Waves;
+            const int sx = t / W, c = t - sx * W;  // right-hand side, column in the stage
+            xv[i] = X[(int64_t)min(sx, nrhs - 1) * ldx + min(kb + c, kend - 1)];
         }
+    };
+    auto store_x = [&](int b, int64_t kb) {
+#pragma unroll
+        for (int i = 0; i < kXPer; ++i) {
+            const int t = threadIdx.x + i * 64 * kC64Waves;
+            const int sx = t / W, c = t - sx * W;
+            const bool ok = sx < nrhs && kb + c < kend;
+            const float xr = ok ? xv[i].x : 0.f, xi = ok ? xv[i].y : 0.f;
+            // column 32 st + r is sub-step e of lane group cq (see the matrix loads)
+            const int st = c >> 5, r = c & 31, cq = (r >> 1) & 3, e = 2 * (r >> 3) + (r & 1);
+            xs[b][0][cq][sx][8 * st + e] = xr;
+            xs[b][1][cq][sx][8 * st + e] = xi;
+        }
+    };
+    // sub-step e (0..7) of a 32-column step at kc: 16 rows x 4 columns, lane
+    // group q holding column kc + 8 (e / 2) + 2 q + e % 2
+    auto mfma_step = [&](int b, int st, float2 (&a)[8]) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const float2 b = xs[buf][li][8 * q + e];
-            rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b.x, rr, 0, 0, 0);
-            ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b.y, ii, 0, 0, 0);
-            ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, b.y, ri, 0, 0, 0);
-            ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, b.x, ir, 0, 0, 0);
+            const float br = xs[b][0][q][li][8 * st + e];
+            const float bi = xs[b][1][q][li][8 * st + e];
+            const float bs = br + bi;
+            p1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x, br, p1, 0, 0, 0);
+            p2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].y, bi, p2, 0, 0, 0);
+            p3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a[e].x + a[e].y, bs, p3, 0, 0, 0);
         }
-        buf ^= 1;  // the next step writes the other buffer: every wave has left it (previous barrier)
+    };
+
+    // guarded loads, no look-ahead: every stage without VEC, the last
+    // (partial) stage with it
+    auto plain_stage = [&](int b, int64_t kb) {
+#pragma unroll
+        for (int st = 0; st < kC64Steps; ++st) {
+            const int64_t kc = kb + 32 * st;
+            float2 a[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int64_t c = kc + 8 * (e >> 1) + 2 * q + (e & 1);
+                a[e] = c < kend ? pa[c] : make_float2(0.f, 0.f);
+            }
+            mfma_step(b, st, a);
+        }
+    };
+
+    load_x(k0);
+    int buf = 0;
+    int64_t kb = k0;
+    if constexpr (VEC) {
+        // Whole stages (rows 16-byte aligned, kchunk a multiple of 32, cols
+        // even): each lane reads four 16-byte pieces per step (piece e / 2 of
+        // the four lanes of a row covers 64 contiguous bytes; the two halves
+        // of a 128-byte line follow each other).  The pieces of step s + 1 are
+        // in flight while step s computes: two register sets alternating with
+        // the step's parity, no branch or select between load and use (the
+        // look-ahead past the last whole stage reads a clamped address and is
+        // not used).
+        auto load_a = [&](int64_t kc, float2 (&d)[8]) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const f32x4 v = *reinterpret_cast<const f32x4*>(pa + min(kc + 8 * e + 2 * q, kend - 2));
+                d[2 * e] = make_float2(v.x, v.y);
+                d[2 * e + 1] = make_float2(v.z, v.w);
+            }
+        };
+        static_assert(kC64Steps % 2 == 0, "two register sets alternate over the steps of a stage");
+        const int64_t kfull = k0 + (kend - k0) / W * W;
+        float2 a0[8], a1[8];
+        load_a(k0, a0);
+        for (; kb < kfull; kb += W) {
+            store_x(buf, kb);
+            __syncthreads();
+            load_x(kb + W);
+#pragma unroll
+            for (int st = 0; st < kC64Steps; st += 2) {
+                // scheduling barriers keep each look-ahead load ahead of the
+                // step it overlaps (left alone, the compiler sinks the loads
+                // next to their use to save registers)
+                load_a(kb + 32 * (st + 1), a1);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma_step(buf, st, a0);
+                __builtin_amdgcn_sched_barrier(0);
+                load_a(kb + 32 * (st + 2), a0);
+                __builtin_amdgcn_sched_barrier(0);
+                mfma_step(buf, st + 1, a1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            buf ^= 1;  // the next stage writes the other buffer: every wave has left it (previous barrier)
+        }
     }
-    if (r0 >= rows || !live) return;  // after the loop: all waves take part in every barrier
+    for (; kb < kend; kb += W) {
+        store_x(buf, kb);
+        __syncthreads();
+        load_x(kb + W);
+        plain_stage(buf, kb);
+        buf ^= 1;
+    }
+    if (li >= nrhs) return;  // after the loop: all waves take part in every barrier
     if (part) {  // split-K partial, combined by splitk_reduce_kernel
         float2* P = part + ((int64_t)blockIdx.y * nrhs + li) * rows;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int64_t r = r0 + 4 * q + e;
-            if (r < rows) P[r] = make_float2(rr[e] - ii[e], ri[e] + ir[e]);
+            if (r < rows) P[r] = make_float2(p1[e] - p2[e], p3[e] - p1[e] - p2[e]);
         }
         return;
     }
@@ -297,7 +378,7 @@ __global__ __launch_bounds__(64 * kC64Waves) void gemm_c64_mfma_kernel(
     for (int e = 0; e < 4; ++e) {
         const int64_t r = r0 + 4 * q + e;
         if (r >= rows) continue;
-        const float yr = rr[e] - ii[e], yi = ri[e] + ir[e];
+        const float yr = p1[e] - p2[e], yi = p3[e] - p1[e] - p2[e];
         float2 o = make_float2(alpha.x * yr - alpha.y * yi, alpha.x * yi + alpha.y * yr);
         float2* dst = &Y[(int64_t)li * ldy + r];
         if (!beta_zero) {
@@ -417,15 +498,23 @@ hipError_t launch_gemm_lowp(int dt, const uint16_t* Ar, const uint16_t* Ai, int6
 hipError_t launch_gemm_c64(const float2* A, int64_t rows, int64_t cols, int64_t lda, const float2* X, int64_t ldx,
                            int nrhs, float2* Y, int64_t ldy, float2 alpha, float2 beta, float2* part, hipStream_t s)
 {
+    constexpr int kWaves = 8;
     const int bz = (beta.x == 0.f && beta.y == 0.f) ? 1 : 0;
     const int64_t tiles = (rows + 15) / 16;
-    const unsigned g = (unsigned)((tiles + kC64Waves - 1) / kC64Waves);
+    const unsigned g = (unsigned)((tiles + kWaves - 1) / kWaves);
     int ks = 1;
     int64_t kchunk = (cols + 31) / 32 * 32;
     if (part) choose_split(tiles, cols, ks, kchunk);
     float2* pp = ks > 1 ? part : nullptr;
-    hipLaunchKernelGGL(gemm_c64_mfma_kernel, dim3(g, (unsigned)ks), dim3(64 * kC64Waves), 0, s, A, rows, cols, lda, X,
-                       ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
+    // 16-byte pieces: rows 16-byte aligned and an even number of columns
+    // (splits start at multiples of 32 columns)
+    const bool vec = (lda % 2 == 0) && (cols % 2 == 0) && ((uintptr_t)A % 16 == 0);
+    if (vec)
+        hipLaunchKernelGGL((gemm_c64_mfma3_kernel<true, kWaves, 8>), dim3(g, (unsigned)ks), dim3(64 * kWaves), 0, s, A,
+                           rows, cols, lda, X, ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
+    else
+        hipLaunchKernelGGL((gemm_c64_mfma3_kernel<false, kWaves, 4>), dim3(g, (unsigned)ks), dim3(64 * kWaves), 0, s, A,
+                           rows, cols, lda, X, ldx, nrhs, Y, ldy, alpha, beta, bz, kchunk, pp);
     if (pp)
         hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((rows + 255) / 256), (unsigned)nrhs), dim3(256), 0, s,
                            pp, ks, nrhs, rows, 1.f, Y, ldy, alpha, beta, bz);

@@ -42,17 +42,24 @@ def _round_planes(A, dtype, scale):
     return (r + 1j * i) / scale
 
 
-@pytest.mark.parametrize("shape", [(100, 77, 3), (1000, 1000, 16), (257, 4097, 16), (64, 64, 20), (33, 50, 1)])
-def test_fp32_matrix_core_product(shape):
+# odd column counts take the guarded path, even ones the 16-byte-piece path
+# with look-ahead (whole 256-column stages, then a guarded partial stage):
+# 2048 columns are whole stages only, 1000 and 1538 end in a partial stage,
+# 64 is a partial stage alone; 7 and 20 right-hand sides are ragged blocks
+@pytest.mark.parametrize("split", [True, False])
+@pytest.mark.parametrize("shape", [(100, 77, 3), (1000, 1000, 16), (257, 4097, 16), (64, 64, 20), (33, 50, 1),
+                                   (512, 2048, 16), (300, 1538, 7)])
+def test_fp32_matrix_core_product(shape, split):
     from neuralat_amd.bem.precision import matmul
 
     rng = np.random.default_rng(0)
     r, c, k = shape
     A, X, Y0 = _rand(rng, r, c), _rand(rng, k, c), _rand(rng, k, r)
     out = torch.tensor(Y0, device=_dev())
-    matmul(torch.tensor(A, device=_dev()), torch.tensor(X, device=_dev()), alpha=0.5 - 1j, beta=2.0, out=out)
+    matmul(torch.tensor(A, device=_dev()), torch.tensor(X, device=_dev()), alpha=0.5 - 1j, beta=2.0, out=out,
+           split=split)
     ref = (0.5 - 1j) * (X.astype(np.complex128) @ A.T.astype(np.complex128)) + 2.0 * Y0
-    assert _rel(out.cpu().numpy(), ref) < 1e-5, shape
+    assert _rel(out.cpu().numpy(), ref) < 1e-5, (shape, split)
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "fp16"])
