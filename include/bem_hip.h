@@ -114,6 +114,13 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
  * flush to scattered addresses, slower).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream);
+/* The derivative with respect to k of the P1 operator op (SINGLE, DOUBLE,
+ * ADJOINT_DOUBLE, BM_LHS, BM_RHS; regular and neighbour pairs, no identity
+ * terms), [nv, nv] at out, same rules and clamp as bem_assemble.  No
+ * counterpart in the reference (it has no gradients): the wavenumber
+ * derivative the adjoint of the Burton-Miller pipeline (neumann2dirichlet,
+ * solver.py:118-126, operators bind.cu:180-215) needs. */
+int bem_assemble_dk(const bem_mesh* mesh, float k, int op, int quad, int clamp, void* out, int64_t ld, void* stream);
 /* bem_assemble at nk wavenumbers on one mesh (e.g. the modes of a vibrating
  * object, each solved by neumann2dirichlet: scene.py:384-397 loops over them
  * one call per operator): one launch per kernel for up to 64 wavenumbers.
@@ -201,6 +208,11 @@ int bem_triangle2vertex_raw(const int32_t* tris, int64_t nt, int64_t nv, const v
  * (single/double_boundary_potential, bind.cu:156-176,219-222). */
 int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
                   void* out, int64_t ld, void* stream);
+/* The derivative with respect to k of bem_potential's matrix (SINGLE or
+ * DOUBLE, either basis).  No counterpart in the reference: the wavenumber
+ * derivative of boundary2potential (solver.py:140-147) for its adjoint. */
+int bem_potential_dk(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
+                     void* out, int64_t ld, void* stream);
 /* bem_potential at nk wavenumbers (ks: host array [nk]) in one launch per
  * kernel for up to 64 of them: matrix z at out + z * stride complex elements,
  * [np, ld] each (stride >= np * ld). */

@@ -64,6 +64,7 @@ SIGNATURES = {
     "bem_mesh_normals_curls": (_I32, [_P, _P, _P, _P]),
     "bem_assemble": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_assemble_ex": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_assemble_dk": (_I32, [_P, _F, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_assemble_batch_k": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_cbie_work_size": (_I64, [_P]),
     "bem_cbie_dp0": (_I32, [_P, _F, _I32, _I32, _P, _P, _I64, _P, _P, _P]),
@@ -83,6 +84,7 @@ SIGNATURES = {
     "bem_triangle2vertex_work_size": (_I64, [_I64, _I64]),
     "bem_triangle2vertex_raw": (_I32, [_P, _I64, _I64, _P, _P, _P, _P]),
     "bem_potential": (_I32, [_P, _P, _I64, _F, _I32, _I32, _I32, _P, _I64, _P]),
+    "bem_potential_dk": (_I32, [_P, _P, _I64, _F, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_mesh_set_vertices": (_I32, [_P, _P, _I32, _P]),
     "bem_potential_batch_k": (_I32, [_P, _P, _I64, _P, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_matvec_work_size": (_I64, [_I64, _I64, _I32]),

@@ -1,4 +1,5 @@
-"""Differentiable exterior pressure of the DP0 CBIE solve.
+"""Differentiable exterior pressures: the DP0 CBIE solve and the P1
+Burton-Miller solve.
 
 ``exterior_pressure(vertices, triangles, k, psi, points)`` returns
 p = D_pot phi - S_pot psi with (M/2 - K) phi = -V psi (the reference's
@@ -9,6 +10,15 @@ to the Neumann data psi.  The backward pass solves the adjoint system
 the library's adjoint kernels, which recompute every derivative block on the
 fly.  The reference has no gradients; this is checked against central finite
 differences and the FP64 oracle in tests/test_gpu_grad.py.
+
+``burton_miller_pressure(vertices, triangles, k, neumann, points)`` is the
+same for the reference's production path, neumann2dirichlet +
+boundary2potential in the P1 basis with the approximate rules
+(src/bem/solver.py:118-126,140-147): (bm_lhs + M/2) phi = (bm_rhs -
+(i/2k) M) g, p = D_pot phi - S_pot g.  Its gradients reach k and the
+Neumann data g (adjoint solve with the stored matrix, the wavenumber
+derivatives of the operators from bem_assemble_dk / bem_potential_dk);
+vertex and listener gradients of this path are not built and are refused.
 
 A forward or adjoint GMRES solve that stops at ``maxiter`` before reaching
 ``tol`` raises ``NotConvergedError`` (``raise_on_fail=True``, the default);
@@ -31,11 +41,11 @@ from .solver import Mesh, _ptr, _stream, gmres, matvec
 last_solve_info = {"forward": None, "adjoint": None}
 
 
-def _report(which, info, raise_on_fail):
+def _report(which, info, raise_on_fail, what="exterior_pressure"):
     last_solve_info[which] = info
     if info["converged"]:
         return
-    msg = (f"exterior_pressure: {which} GMRES stopped after {info['iterations']} iterations at relative "
+    msg = (f"{what}: {which} GMRES stopped after {info['iterations']} iterations at relative "
            f"residual {info['rel_residual']:.2e}")
     if raise_on_fail:
         raise _lib.NotConvergedError(msg, info)
@@ -157,3 +167,97 @@ def exterior_pressure(vertices, triangles, k, psi, points, clamp=True, tol=1e-6,
         k = torch.tensor(float(k), dtype=torch.float32, device=vertices.device)
     return _ExteriorPressure.apply(vertices, k, psi, triangles, points, clamp, tol, restart, maxiter,
                                    bool(raise_on_fail), float(tol if adjoint_tol is None else adjoint_tol))
+
+
+def _adjoint_solve(A, c, tol, restart, maxiter):
+    """A^H x = c by GMRES on the stored matrix (bem_gmres_ex, trans = 2);
+    returns (x, info dict)."""
+    n = A.shape[0]
+    lib = _lib.load()
+    x = torch.zeros(n, dtype=torch.complex64, device=A.device)
+    restart = int(min(restart, n))
+    wk = torch.empty(lib.bem_gmres_work_size(n, restart), dtype=torch.uint8, device=A.device)
+    info = _lib.SolveInfo()
+    rc = lib.bem_gmres_ex(_ptr(A), n, A.stride(0), 2, _ptr(c), _ptr(x), float(tol), restart, int(maxiter), _ptr(wk),
+                          ctypes.byref(info), _stream())
+    if rc not in (_lib.BEM_OK, _lib.BEM_ERR_NOT_CONVERGED):
+        _lib.check(rc, "bem_gmres_ex")
+    return x, info.as_dict()
+
+
+class _BurtonMillerPressure(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, k, neumann, vertices, triangles, points, clamp, tol, restart, maxiter, raise_on_fail,
+                adjoint_tol):
+        from .solver import BEM_Solver, add_identity_
+
+        s = BEM_Solver(vertices.detach().contiguous().float(), triangles, basis="P1", clamp=clamp)
+        kf = float(k)
+        g = neumann.detach().to(torch.complex64).contiguous()
+        L = add_identity_(s, s.assemble_boundary_matrix(kf, "bm_lhs", approx=True), 0.5)
+        R = add_identity_(s, s.assemble_boundary_matrix(kf, "bm_rhs", approx=True), -0.5j / kf)
+        phi, info = gmres(L, matvec(R, g), tol=tol, restart=restart, maxiter=maxiter)
+        last_solve_info["adjoint"] = None
+        _report("forward", info, raise_on_fail, "burton_miller_pressure")
+        pts = points.detach().to(torch.float32).contiguous()
+        S = s.potential_matrix(kf, pts, "single")
+        D = s.potential_matrix(kf, pts, "double")
+        p = matvec(D, phi)
+        matvec(S, g, alpha=-1.0, beta=1.0, out=p)
+        ctx.s, ctx.L, ctx.R, ctx.S, ctx.D = s, L, R, S, D
+        ctx.kf, ctx.restart, ctx.maxiter = kf, restart, maxiter
+        ctx.raise_on_fail, ctx.adjoint_tol = raise_on_fail, adjoint_tol
+        ctx.save_for_backward(phi, g, pts)
+        return p
+
+    @staticmethod
+    def backward(ctx, gp):
+        from .solver import add_identity_
+
+        phi, g, pts = ctx.saved_tensors
+        s, L, R, S, D, kf = ctx.s, ctx.L, ctx.R, ctx.S, ctx.D, ctx.kf
+        pbar = (0.5 * gp).to(torch.complex64).contiguous()  # dl/dp* (torch's gradient is 2 dl/dp*)
+        # adjoint solve (bm_lhs + M/2)^H lambda = D^H pbar on the stored matrix
+        lam, info = _adjoint_solve(L, matvec(D, pbar, trans=2), ctx.adjoint_tol, ctx.restart, ctx.maxiter)
+        _report("adjoint", info, ctx.raise_on_fail, "burton_miller_pressure")
+        grad_k = grad_g = None
+        if ctx.needs_input_grad[1]:
+            # p = (D L^-1 R - S) g: dl/dg* = R^H lambda - S^H pbar
+            grad_g = matvec(R, lam, trans=2)
+            matvec(S, pbar, trans=2, alpha=-1.0, beta=1.0, out=grad_g)
+            grad_g = 2.0 * grad_g
+        if ctx.needs_input_grad[0]:
+            # dl/dk = 2 Re(pbar^H (D_k phi - S_k g) + lambda^H (R_k g - L_k phi)),
+            # L_k = d bm_lhs/dk (the mass term does not depend on k),
+            # R_k = d bm_rhs/dk + (i / 2k^2) M
+            Lk = s.assemble_boundary_matrix_dk(kf, "bm_lhs", approx=True)
+            Rk = add_identity_(s, s.assemble_boundary_matrix_dk(kf, "bm_rhs", approx=True), 0.5j / (kf * kf))
+            t2 = matvec(Rk, g)
+            matvec(Lk, phi, alpha=-1.0, beta=1.0, out=t2)
+            del Lk, Rk
+            Sk = s.potential_matrix_dk(kf, pts, "single")
+            Dk = s.potential_matrix_dk(kf, pts, "double")
+            t1 = matvec(Dk, phi)
+            matvec(Sk, g, alpha=-1.0, beta=1.0, out=t1)
+            gk = 2.0 * (torch.vdot(pbar, t1) + torch.vdot(lam, t2)).real
+            grad_k = gk.to(torch.float32).reshape(())
+        return grad_k, grad_g, None, None, None, None, None, None, None, None, None
+
+
+def burton_miller_pressure(vertices, triangles, k, neumann, points, clamp=False, tol=1e-6, restart=200,
+                           maxiter=2000, raise_on_fail=True, adjoint_tol=None):
+    """Differentiable p(points) of the P1 Burton-Miller solve (the reference's
+    neumann2dirichlet + boundary2potential, src/bem/solver.py:118-126,140-147):
+    gradients with respect to k and the Neumann data ``neumann`` (one complex
+    value per vertex).  Vertex and listener-position gradients of this path
+    are not built: tensors that ask for them are refused rather than given
+    silent zeros."""
+    if torch.is_tensor(vertices) and vertices.requires_grad:
+        raise NotImplementedError("burton_miller_pressure: gradients with respect to the vertices are not built")
+    if torch.is_tensor(points) and points.requires_grad:
+        raise NotImplementedError("burton_miller_pressure: gradients with respect to the listener points are not "
+                                  "built")
+    if not torch.is_tensor(k):
+        k = torch.tensor(float(k), dtype=torch.float32, device=vertices.device)
+    return _BurtonMillerPressure.apply(k, neumann, vertices, triangles, points, clamp, tol, restart, maxiter,
+                                       bool(raise_on_fail), float(tol if adjoint_tol is None else adjoint_tol))

@@ -425,6 +425,23 @@ class BEM_Solver:
                                     _lib.QUADS[quad], int(clamp), _ptr(out), n, _stream()), "bem_assemble")
         return out
 
+    def assemble_boundary_matrix_dk(self, wavenumber, layer_type, approx=False, clamp=None):
+        """d/dk of the P1 operator matrix (single, double, adjointdouble,
+        bm_lhs, bm_rhs), complex64 [nv, nv]; no identity terms.  The reference
+        has no derivatives: this serves the adjoint of neumann2dirichlet."""
+        if self.basis != "P1":
+            raise ValueError("wavenumber derivatives of the operators are built for the P1 basis")
+        clamp = self.clamp if clamp is None else clamp
+        if layer_type not in _lib.OPS:
+            raise ValueError(f"unknown layer type {layer_type!r}")
+        quad = "approx_1" if approx == "approx_1" else ("approx" if approx else "full")
+        n = self._n("P1")
+        out = torch.empty((n, n), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_assemble_dk(self.mesh.handle, float(wavenumber), _lib.OPS[layer_type],
+                                               _lib.QUADS[quad], int(clamp), _ptr(out), n, _stream()),
+                   "bem_assemble_dk")
+        return out
+
     def assemble_boundary_matrix_batch(self, wavenumbers, layer_type, approx=False, basis=None, clamp=None):
         """The operator at every wavenumber of ``wavenumbers`` (a sequence of
         floats), complex64 [len, n, n]: one launch per kernel for up to 64 of
@@ -571,6 +588,19 @@ class BEM_Solver:
         _lib.check(_lib.load().bem_potential(self.mesh.handle, _ptr(points), points.shape[0], float(k),
                                              _lib.OPS[layer_type], _lib.BASES[basis], int(clamp), _ptr(out), n,
                                              _stream()), "bem_potential")
+        return out
+
+    def potential_matrix_dk(self, k, points, layer_type, basis=None, clamp=None):
+        """d/dk of potential_matrix (single or double layer)."""
+        basis = basis or self.basis
+        clamp = self.clamp if clamp is None else clamp
+        points = points.to(torch.float32).contiguous()
+        check_tensor(points, torch.float32)
+        n = self._n(basis)
+        out = torch.empty((points.shape[0], n), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_potential_dk(self.mesh.handle, _ptr(points), points.shape[0], float(k),
+                                                _lib.OPS[layer_type], _lib.BASES[basis], int(clamp), _ptr(out), n,
+                                                _stream()), "bem_potential_dk")
         return out
 
     def potential_matrix_batch(self, wavenumbers, points, layer_type, basis=None, clamp=None):

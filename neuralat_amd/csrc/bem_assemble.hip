@@ -622,6 +622,10 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
                                      krev, vr, vi, gr, gi);
                 vr *= wy[b];
                 vi *= wy[b];
+                if constexpr (OP == (BM_LHS | kDk)) {  // w_b H for the curl term
+                    gr *= wy[b];
+                    gi *= wy[b];
+                }
             }
             const float p0 = 1.0f - TriRule<Q>::x1[b] - TriRule<Q>::x2[b];
             const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b];
@@ -633,8 +637,8 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
                 ur[1] = fmaf(wr, p1, ur[1]); ui[1] = fmaf(wi, p1, ui[1]);
                 ur[2] = fmaf(wr, p2, ur[2]); ui[2] = fmaf(wi, p2, ui[2]);
             }
-            if constexpr (OP == BM_LHS) {
-                tr9 += gr;  // gr, gi: w_b G
+            if constexpr (base_op(OP) == BM_LHS) {
+                tr9 += gr;  // gr, gi: w_b G (w_b H for the derivative)
                 ti9 += gi;
             }
         }
@@ -660,7 +664,7 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
             br[3 + be] = fmaf(q1, sr[be], br[3 + be]); bi[3 + be] = fmaf(q1, si[be], bi[3 + be]);
             br[6 + be] = fmaf(q2, sr[be], br[6 + be]); bi[6 + be] = fmaf(q2, si[be], bi[6 + be]);
         }
-        if constexpr (OP == BM_LHS) {
+        if constexpr (base_op(OP) == BM_LHS) {
             gr9 = fmaf(ua, tr9, gr9);
             gi9 = fmaf(ua, ti9, gi9);
         }
@@ -681,7 +685,7 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
             br[e] = sgr * cp[e] - br[e] * kk;
             bi[e] = sgi * cp[e] - bi[e] * kk;
         }
-    } else if constexpr (OP == BM_LHS) {
+    } else if constexpr (base_op(OP) == BM_LHS) {
         // (i/k) G summed, times curl.curl, plus the phi-weighted BM kernel (integrand.h:314-323)
         const float invk = 1.0f / k;
         const float s9r = -gi9 * invk, s9i = gr9 * invk;
@@ -765,7 +769,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         rtri[q] = tris[tl.perm[r0 + q]];
         rslot[q] = tl.row_slot[r0 + q];
     }
-    if constexpr (OP == HYP || OP == BM_LHS)
+    if constexpr (OP == HYP || base_op(OP) == BM_LHS)
         for (int e = threadIdx.x; e < (r1 - r0) * 9; e += CT) {
             const int q = e / 9;
             rcurl[q][e - q * 9] = curls[(int64_t)tl.perm[r0 + q] * 12 + (e - q * 9)];
@@ -784,7 +788,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const float ny0 = cr[4 * Q], ny1 = cr[4 * Q + 1], ny2 = cr[4 * Q + 2];
     const int4 tj = tris[j];
     float cj[9];
-    if constexpr (OP == HYP || OP == BM_LHS) {
+    if constexpr (OP == HYP || base_op(OP) == BM_LHS) {
 #pragma unroll
         for (int e = 0; e < 9; ++e) cj[e] = curls[(int64_t)j * 12 + e];
     }
@@ -842,7 +846,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         for (int e = 0; e < (PAIR ? 9 : 1); ++e) tr[e] = tii[e] = 0.f;
         if (active && !shared) {
             float ci[9];
-            if constexpr (OP == HYP || OP == BM_LHS) {
+            if constexpr (OP == HYP || base_op(OP) == BM_LHS) {
 #pragma unroll
                 for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
             }
@@ -1155,7 +1159,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
                         bi[3 * a + b] = fmaf(vi, pp, bi[3 * a + b]);
                     }
             }
-            if constexpr (OP == BM_LHS) {
+            if constexpr (base_op(OP) == BM_LHS) {
                 br[9] = fmaf(gr, ww, br[9]);
                 bi[9] = fmaf(gg, ww, bi[9]);
             } else {
@@ -1192,7 +1196,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
             for (int e = 0; e < 9; ++e)
                 if (e == ll) { vr = ar[e]; vi = ai[e]; }
             const int oa = pi[a], ob = pj[b];
-            if constexpr (OP == HYP || OP == BM_LHS) {
+            if constexpr (OP == HYP || base_op(OP) == BM_LHS) {
                 float ci[9], cj[9], cp[9];
 #pragma unroll
                 for (int e = 0; e < 9; ++e) {
@@ -1614,7 +1618,9 @@ static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, 
                               const KBatch& kb, int nz, float2* out, int64_t ld, hipStream_t s)
 {
     hipError_t e = hipSuccess;
-    if (basis == DP0) {
+    if constexpr (is_dk(OP)) {
+        if (basis == DP0) return hipErrorInvalidValue;  // derivatives: P1 only
+    } else if (basis == DP0) {
         if (parts & 1) {
             if (quad == 0) e = dp0_regular_t<OP, 6, CLAMP>(m, row0, row1, kb, nz, out, ld, s);
             else if (quad == 1) e = dp0_regular_t<OP, 3, CLAMP>(m, row0, row1, kb, nz, out, ld, s);
@@ -1682,6 +1688,11 @@ static hipError_t assemble_dispatch(const MeshDev& m, int op, int basis, int qua
         BEM_OP_CASE(HYP)
         BEM_OP_CASE(BM_LHS)
         BEM_OP_CASE(BM_RHS)
+        BEM_OP_CASE(SLP | kDk)
+        BEM_OP_CASE(DLP | kDk)
+        BEM_OP_CASE(ADLP | kDk)
+        BEM_OP_CASE(BM_LHS | kDk)
+        BEM_OP_CASE(BM_RHS | kDk)
     default:
         return hipErrorInvalidValue;
     }
@@ -1944,6 +1955,18 @@ static hipError_t potential_dispatch(const MeshDev& m, const float* pts, int64_t
                                        : potential_t<DLP, DP0, false>(m, pts, np, kb, nz, out, ld, s);
         return clamp ? potential_t<DLP, P1, true>(m, pts, np, kb, nz, out, ld, s)
                      : potential_t<DLP, P1, false>(m, pts, np, kb, nz, out, ld, s);
+    }
+    if (op == (SLP | kDk)) {
+        if (basis == DP0) return clamp ? potential_t<SLP | kDk, DP0, true>(m, pts, np, kb, nz, out, ld, s)
+                                       : potential_t<SLP | kDk, DP0, false>(m, pts, np, kb, nz, out, ld, s);
+        return clamp ? potential_t<SLP | kDk, P1, true>(m, pts, np, kb, nz, out, ld, s)
+                     : potential_t<SLP | kDk, P1, false>(m, pts, np, kb, nz, out, ld, s);
+    }
+    if (op == (DLP | kDk)) {
+        if (basis == DP0) return clamp ? potential_t<DLP | kDk, DP0, true>(m, pts, np, kb, nz, out, ld, s)
+                                       : potential_t<DLP | kDk, DP0, false>(m, pts, np, kb, nz, out, ld, s);
+        return clamp ? potential_t<DLP | kDk, P1, true>(m, pts, np, kb, nz, out, ld, s)
+                     : potential_t<DLP | kDk, P1, false>(m, pts, np, kb, nz, out, ld, s);
     }
     return hipErrorInvalidValue;
 }

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/bem_hip.h"
+#include "bem_device.hpp"
 #include "bem_internal.hpp"
 #include "bem_linalg.hpp"
 #include "bem_lowp.hpp"
@@ -232,6 +233,24 @@ int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, 
     if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
     hipError_t e = bem::launch_assemble(m, op, basis, quad, clamp, parts, 0, m.nt, k, (float2*)out, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_assemble launch");
+    return BEM_OK;
+}
+
+int bem_assemble_dk(const bem_mesh* mesh, float k, int op, int quad, int clamp, void* out, int64_t ld, void* stream)
+{
+    if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (op == BEM_OP_HYPERSINGULAR) return fail(BEM_ERR_BAD_ARG, "no wavenumber derivative of the hypersingular operator");
+    if (int rc = check_op(op, BEM_BASIS_P1, quad)) return rc;
+    const auto& m = mesh->dev;
+    if (ld < m.nv) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (!std::isfinite(k)) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+    if ((op == BEM_OP_BM_LHS || op == BEM_OP_BM_RHS) && k == 0.f)
+        return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs k != 0");
+    DeviceGuard g(m.device);
+    if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_assemble(m, op | bem::kDk, BEM_BASIS_P1, quad, clamp, 3, 0, m.nt, k, (float2*)out, ld,
+                                        S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_assemble_dk launch");
     return BEM_OK;
 }
 
@@ -564,6 +583,23 @@ int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_potential(m, points, np, k, op, basis, clamp, (float2*)out, ld, S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_potential launch");
+    return BEM_OK;
+}
+
+int bem_potential_dk(const bem_mesh* mesh, const float* points, int64_t np, float k, int op, int basis, int clamp,
+                     void* out, int64_t ld, void* stream)
+{
+    if (!mesh || !out || (np > 0 && !points)) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (op != BEM_OP_SINGLE && op != BEM_OP_DOUBLE)
+        return fail(BEM_ERR_BAD_ARG, "potential operators are SINGLE and DOUBLE");
+    if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
+    const auto& m = mesh->dev;
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
+    if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
+    DeviceGuard g(m.device);
+    hipError_t e = bem::launch_potential(m, points, np, k, op | bem::kDk, basis, clamp, (float2*)out, ld, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_potential_dk launch");
     return BEM_OK;
 }
 

@@ -13,6 +13,12 @@
 namespace bem {
 
 enum Op : int { SLP = 0, DLP = 1, ADLP = 2, HYP = 3, BM_LHS = 4, BM_RHS = 5 };
+// The derivative with respect to k of operator OP is evaluated as OP | kDk
+// (single, double, adjoint double, Burton-Miller LHS and RHS; P1 assembly
+// and the potentials).
+constexpr int kDk = 8;
+constexpr int base_op(int op) { return op & 7; }
+constexpr bool is_dk(int op) { return (op & kDk) != 0; }
 enum Basis : int { P1 = 0, DP0 = 1 };
 
 constexpr float kEps = 1e-3f;        // potential.h:16
@@ -85,7 +91,11 @@ __device__ __forceinline__ void sincos_rev(float t, float& s, float& c)
 // Kernel values for a displacement d = y - x.  Outputs are 4 pi times the
 // reference's potential<OP> (potential.h:18-52) and, for BM_LHS, 4 pi times
 // the second output of bm_lhs_potential (potential.h:55-65).  HYP evaluates
-// the single-layer kernel (its curl form is applied per pair).
+// the single-layer kernel (its curl form is applied per pair).  OP | kDk:
+// the derivatives of the same values with respect to k (with G = e^{ikr}/r,
+// dG/dk = i e^{ikr} and d/dk [G (1 - ikr)] = k r e^{ikr}); for BM_LHS | kDk
+// the second output is H with d/dk [(i/k) G] = (i/k) H, H = i e^{ikr} - G/k,
+// so that the caller's (i/k) scaling of the second output stays as it is.
 //   nx: row normal (x side), ny: column normal (y side)
 //   k: wavenumber, krev = k / (2 pi)
 template <int OP, bool CLAMP>
@@ -129,6 +139,35 @@ __device__ __forceinline__ void kernel4pi(float dx, float dy, float dz, float nx
         float tr = th * invk * q, ti = invk * q;
         vr = -gr + (gr * tr - gi * ti);
         vi = -gi + (gr * ti + gi * tr);
+    } else if constexpr (OP == (SLP | kDk)) {
+        vr = -s;  // i e^{ikr}
+        vi = c;
+    } else if constexpr (OP == (DLP | kDk) || OP == (ADLP | kDk)) {
+        // the double-layer kernel's e^{ikr} (1 - ikr) / r^3 becomes k e^{ikr} / r
+        float dn = (OP == (DLP | kDk)) ? -(dx * ny0 + dy * ny1 + dz * ny2) : (dx * nx0 + dy * nx1 + dz * nx2);
+        float f = dn * k * rinv;
+        vr = f * c;
+        vi = f * s;
+    } else if constexpr (OP == (BM_LHS | kDk)) {
+        // d/dk of G [(1 - ikr) a - ik nx.ny], a = (y-x).n_y / r^2:
+        // a k r e^{ikr} + k nx.ny e^{ikr} - i nx.ny G
+        float a = (dx * ny0 + dy * ny1 + dz * ny2) * rinv * rinv;
+        float nn = nx0 * ny0 + nx1 * ny1 + nx2 * ny2;
+        float t = k * (a * r + nn);
+        vr = fmaf(t, c, nn * s * rinv);
+        vi = fmaf(t, s, -nn * c * rinv);
+        float invk = 1.0f / k;
+        sr = -fmaf(c * rinv, invk, s);  // H = i e^{ikr} - G / k
+        si = fmaf(-s * rinv, invk, c);
+    } else if constexpr (OP == (BM_RHS | kDk)) {
+        // d/dk of -G + (i/k) G (1 - ikr) q, q = (x-y).n_x / r^2:
+        // -i e^{ikr} + q [-(i/k^2) G (1 - ikr) + i r e^{ikr}]
+        float q = -(dx * nx0 + dy * nx1 + dz * nx2) * rinv * rinv;
+        float th = k * r;
+        float invk = 1.0f / k;
+        float u = rinv * invk * invk;  // 1 / (k^2 r)
+        vr = s + q * (fmaf(-th, c, s) * u - r * s);
+        vi = -c + q * (r * c - fmaf(th, s, c) * u);
     }
 }
 
