@@ -93,7 +93,7 @@ def parse():
                         "timed, then a 4 GiB copy (kernel traces and counter passes)")
     p.add_argument("--only-form", type=str, default="",
                    help="run one form alone and print its JSON (p1, batching, batching_small_meshes, "
-                        "differentiation, datagen, modal, burton_miller_level6, precision)")
+                        "differentiation, differentiation_burton_miller, datagen, modal, burton_miller_level6, precision)")
     p.add_argument("--modal-sound-probe", action="store_true",
                    help="three batched modal-sound solves (32 modes of the modal form's block), for kernel traces")
     return p.parse_args()
@@ -463,6 +463,47 @@ def grad_form(ndir=64, h=1e-3, seed=0):
             "fd_vertex_rel_diff": float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)),
             "fd_vertex_rel_diff_richardson": float(np.linalg.norm(adr - fdr) / np.linalg.norm(fdr)),
             "dL_dk_adjoint": float(kt.grad), "dL_dk_fd": fdk}
+
+
+def grad_bm_form(level=4, k=20.0, reps=3, seed=0):
+    """The production path's gradient: the P1 Burton-Miller pressure
+    (neumann2dirichlet + boundary2potential, approximate rules) at 32
+    listeners on a level-``level`` sphere from one interior monopole; loss
+    sum |p|^2, gradients with respect to every vertex, k, the Neumann data and
+    the listener positions; forward and backward wall time (best of
+    ``reps``).  Parity against the FP64 oracle is in tests/test_gpu_grad_bm.py."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.grad import burton_miller_pressure
+
+    v, t = mesh.icosphere(level, 1.0)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=seed).astype(np.float32)
+    _, dn = mesh.sources_field(np.array([[0.1, -0.05, 0.02]]), None, k)
+    psi = mesh.project_p1(v, t, lambda x, n: dn(x, n)).astype(np.complex64)
+    pts = mesh.random_exterior_points(32, 1.5, 3.0, seed=seed + 3).astype(np.float32)
+    dev = torch.device("cuda:0")
+    tt = torch.tensor(t, device=dev)
+    tf, tb = [], []
+    for _ in range(reps):
+        vt = torch.tensor(v, device=dev, requires_grad=True)
+        kt = torch.tensor(k, device=dev, requires_grad=True)
+        gt = torch.tensor(psi, device=dev, requires_grad=True)
+        pt = torch.tensor(pts, device=dev, requires_grad=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p = burton_miller_pressure(vt, tt, kt, gt, pt)
+        L = (p.abs() ** 2).sum()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        L.backward()
+        torch.cuda.synchronize()
+        tf.append(t1 - t0)
+        tb.append(time.perf_counter() - t1)
+    return {"workload": f"P1 Burton-Miller, icosphere L{level} ({v.shape[0]} vertices) + SH noise, k={k}, "
+                        "32 listeners, loss sum |p|^2, gradients w.r.t. vertices, k, Neumann data, listeners",
+            "forward_ms": 1e3 * min(tf), "backward_ms": 1e3 * min(tb), "backward_over_forward": min(tb) / min(tf),
+            "dL_dk": float(kt.grad), "grad_vertices_norm": float(vt.grad.norm())}
 
 
 def p1_form(level=5, k=10.0, bm_level=4, bm_k=20.0, reps=3):
@@ -1279,6 +1320,7 @@ def main():
         forms = {"p1": lambda: p1_form(), "batching": lambda: batch_form(args.batch_meshes, args.batch_ks),
                  "batching_small_meshes": lambda: batch_form(16, args.batch_ks, levels=(2, 3), seed=9),
                  "differentiation": lambda: grad_form(), "datagen": lambda: datagen_form(),
+                 "differentiation_burton_miller": lambda: grad_bm_form(),
                  "modal": lambda: modal_form()}
         if args.only_form == "burton_miller_level6":
             v, t, src, pts = make_workload(args.level)
@@ -1378,6 +1420,7 @@ def main():
         forms["batching_small_meshes"] = batch_form(16, args.batch_ks, levels=(2, 3), seed=9)
         torch.cuda.empty_cache()
         forms["differentiation"] = grad_form()
+        forms["differentiation_burton_miller"] = grad_bm_form()
         torch.cuda.empty_cache()
         forms["p1"] = p1_form()
         torch.cuda.empty_cache()

@@ -334,6 +334,21 @@ int bem_adjoint_cbie_dp0(const bem_mesh* mesh, float k, int clamp, const void* l
 int bem_adjoint_potential_dp0(const bem_mesh* mesh, const float* points, int64_t np, float k, int clamp,
                               const void* pbar, const void* psi, const void* phi, double* grad_vertices,
                               double* grad_k, void* stream);
+/* The same for the P1 Burton-Miller pressure (neumann2dirichlet +
+ * boundary2potential with the approximate rules, solver.py:118-126,140-147):
+ * (bm_lhs + M/2) phi = (bm_rhs - (i/2k) M) g, p = D_pot phi - S_pot g, lambda
+ * solving (bm_lhs + M/2)^H lambda = D_pot^H pbar; all vectors P1 (one complex
+ * value per vertex).  bem_adjoint_bm_p1 adds 2 Re lambda^H (dR g - dL phi)
+ * (regular and neighbour pairs, curl terms, mass terms) to grad_vertices;
+ * bem_adjoint_potential_p1 adds 2 Re pbar^H (dD_pot phi - dS_pot g) to
+ * grad_vertices and, when grad_points [np,3] is not NULL, the listener
+ * gradients to it.  The k gradient comes from bem_assemble_dk /
+ * bem_potential_dk. */
+int bem_adjoint_bm_p1(const bem_mesh* mesh, float k, int clamp, const void* lam, const void* g, const void* phi,
+                      double* grad_vertices, void* stream);
+int bem_adjoint_potential_p1(const bem_mesh* mesh, const float* points, int64_t np, float k, int clamp,
+                             const void* pbar, const void* g, const void* phi, double* grad_vertices,
+                             double* grad_points, void* stream);
 
 /* ---- batched items (no reference counterpart; the reference solves one
  * (mesh, k) item at a time, src/scene.py:270-291) ---- */

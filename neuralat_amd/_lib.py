@@ -132,6 +132,8 @@ SIGNATURES = {
     "bem_adjoint_work_size": (_I64, [_P]),
     "bem_adjoint_cbie_dp0": (_I32, [_P, _F, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "bem_adjoint_potential_dp0": (_I32, [_P, _P, _I64, _F, _I32, _P, _P, _P, _P, _P, _P]),
+    "bem_adjoint_bm_p1": (_I32, [_P, _F, _I32, _P, _P, _P, _P, _P]),
+    "bem_adjoint_potential_p1": (_I32, [_P, _P, _I64, _F, _I32, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

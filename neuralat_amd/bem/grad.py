@@ -17,8 +17,9 @@ boundary2potential in the P1 basis with the approximate rules
 (src/bem/solver.py:118-126,140-147): (bm_lhs + M/2) phi = (bm_rhs -
 (i/2k) M) g, p = D_pot phi - S_pot g.  Its gradients reach k and the
 Neumann data g (adjoint solve with the stored matrix, the wavenumber
-derivatives of the operators from bem_assemble_dk / bem_potential_dk);
-vertex and listener gradients of this path are not built and are refused.
+derivatives of the operators from bem_assemble_dk / bem_potential_dk), to
+the vertices and to the listener positions (the library's P1 adjoint kernels,
+which recompute every derivative block on the fly).
 
 A forward or adjoint GMRES solve that stops at ``maxiter`` before reaching
 ``tol`` raises ``NotConvergedError`` (``raise_on_fail=True``, the default);
@@ -220,7 +221,25 @@ class _BurtonMillerPressure(torch.autograd.Function):
         # adjoint solve (bm_lhs + M/2)^H lambda = D^H pbar on the stored matrix
         lam, info = _adjoint_solve(L, matvec(D, pbar, trans=2), ctx.adjoint_tol, ctx.restart, ctx.maxiter)
         _report("adjoint", info, ctx.raise_on_fail, "burton_miller_pressure")
-        grad_k = grad_g = None
+        grad_k = grad_g = grad_v = grad_x = None
+        if ctx.needs_input_grad[2] or ctx.needs_input_grad[4]:
+            # vertex (and listener) gradients from the adjoint kernels, which
+            # recompute every derivative block on the fly
+            lib = _lib.load()
+            dev = phi.device
+            gvert = torch.zeros((s.mesh.nv, 3), dtype=torch.float64, device=dev)
+            gpts = torch.zeros((pts.shape[0], 3), dtype=torch.float64, device=dev) if ctx.needs_input_grad[4] else None
+            if ctx.needs_input_grad[2]:
+                _lib.check(lib.bem_adjoint_bm_p1(s.mesh.handle, kf, int(s.clamp), _ptr(lam), _ptr(g), _ptr(phi),
+                                                 _ptr(gvert), _stream()), "bem_adjoint_bm_p1")
+            _lib.check(lib.bem_adjoint_potential_p1(s.mesh.handle, _ptr(pts), pts.shape[0], kf, int(s.clamp),
+                                                    _ptr(pbar), _ptr(g), _ptr(phi), _ptr(gvert),
+                                                    _ptr(gpts) if gpts is not None else None, _stream()),
+                       "bem_adjoint_potential_p1")
+            if ctx.needs_input_grad[2]:
+                grad_v = gvert.to(torch.float32)
+            if gpts is not None:
+                grad_x = gpts.to(torch.float32)
         if ctx.needs_input_grad[1]:
             # p = (D L^-1 R - S) g: dl/dg* = R^H lambda - S^H pbar
             grad_g = matvec(R, lam, trans=2)
@@ -241,22 +260,16 @@ class _BurtonMillerPressure(torch.autograd.Function):
             matvec(Sk, g, alpha=-1.0, beta=1.0, out=t1)
             gk = 2.0 * (torch.vdot(pbar, t1) + torch.vdot(lam, t2)).real
             grad_k = gk.to(torch.float32).reshape(())
-        return grad_k, grad_g, None, None, None, None, None, None, None, None, None
+        return grad_k, grad_g, grad_v, None, grad_x, None, None, None, None, None, None
 
 
 def burton_miller_pressure(vertices, triangles, k, neumann, points, clamp=False, tol=1e-6, restart=200,
                            maxiter=2000, raise_on_fail=True, adjoint_tol=None):
     """Differentiable p(points) of the P1 Burton-Miller solve (the reference's
     neumann2dirichlet + boundary2potential, src/bem/solver.py:118-126,140-147):
-    gradients with respect to k and the Neumann data ``neumann`` (one complex
-    value per vertex).  Vertex and listener-position gradients of this path
-    are not built: tensors that ask for them are refused rather than given
-    silent zeros."""
-    if torch.is_tensor(vertices) and vertices.requires_grad:
-        raise NotImplementedError("burton_miller_pressure: gradients with respect to the vertices are not built")
-    if torch.is_tensor(points) and points.requires_grad:
-        raise NotImplementedError("burton_miller_pressure: gradients with respect to the listener points are not "
-                                  "built")
+    gradients with respect to the vertex positions, k, the Neumann data
+    ``neumann`` (one complex value per vertex, held fixed when the vertices
+    move) and the listener positions ``points``."""
     if not torch.is_tensor(k):
         k = torch.tensor(float(k), dtype=torch.float32, device=vertices.device)
     return _BurtonMillerPressure.apply(k, neumann, vertices, triangles, points, clamp, tol, restart, maxiter,

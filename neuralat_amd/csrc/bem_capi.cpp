@@ -536,6 +536,34 @@ int bem_adjoint_potential_dp0(const bem_mesh* mesh, const float* points, int64_t
     return BEM_OK;
 }
 
+int bem_adjoint_bm_p1(const bem_mesh* mesh, float k, int clamp, const void* lam, const void* g, const void* phi,
+                      double* grad_vertices, void* stream)
+{
+    if (!mesh || !lam || !g || !phi || !grad_vertices) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (!std::isfinite(k) || k == 0.f) return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs a finite k != 0");
+    DeviceGuard dg(mesh->dev.device);
+    hipError_t e = bem::launch_adjoint_bm_p1(mesh->dev, k, clamp, (const float2*)lam, (const float2*)g,
+                                             (const float2*)phi, grad_vertices, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_adjoint_bm_p1 launch");
+    return BEM_OK;
+}
+
+int bem_adjoint_potential_p1(const bem_mesh* mesh, const float* points, int64_t np, float k, int clamp,
+                             const void* pbar, const void* g, const void* phi, double* grad_vertices,
+                             double* grad_points, void* stream)
+{
+    if (!mesh || (np > 0 && (!points || !pbar)) || !g || !phi || !grad_vertices)
+        return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
+    if (!std::isfinite(k)) return fail(BEM_ERR_BAD_ARG, "non-finite wavenumber");
+    DeviceGuard dg(mesh->dev.device);
+    hipError_t e = bem::launch_adjoint_potential_p1(mesh->dev, points, np, k, clamp, (const float2*)pbar,
+                                                    (const float2*)g, (const float2*)phi, grad_vertices, grad_points,
+                                                    S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_adjoint_potential_p1 launch");
+    return BEM_OK;
+}
+
 int bem_triangle2vertex(const bem_mesh* mesh, const void* tri_values, void* vert_values, void* stream)
 {
     if (!mesh || !tri_values || !vert_values) return fail(BEM_ERR_BAD_ARG, "null pointer");

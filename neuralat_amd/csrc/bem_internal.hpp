@@ -144,6 +144,15 @@ hipError_t launch_adjoint_pairs(const MeshDev& m, float k, int clamp, const floa
 hipError_t launch_adjoint_potential(const MeshDev& m, const float* pts, int64_t np, float k, int clamp,
                                     const float2* pbar, const float2* psi, const float2* phi, double* gvert, double* gk,
                                     hipStream_t s);
+// P1 Burton-Miller adjoint: vertex gradients (FP64, accumulated into gvert
+// [nv, 3]) of 2 Re lambda^H (dR g - dL phi) (regular and neighbour pairs, the
+// mass terms) and of 2 Re pbar^H (dD_pot phi - dS_pot g) (plus the listener
+// gradients into gpts [np, 3] when it is not null).
+hipError_t launch_adjoint_bm_p1(const MeshDev& m, float k, int clamp, const float2* lam, const float2* g,
+                                const float2* phi, double* gvert, hipStream_t s);
+hipError_t launch_adjoint_potential_p1(const MeshDev& m, const float* pts, int64_t np, float k, int clamp,
+                                       const float2* pbar, const float2* g, const float2* phi, double* gvert,
+                                       double* gpts, hipStream_t s);
 hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 // records msg as the calling thread's bem_last_error() and returns code

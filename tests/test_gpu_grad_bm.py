@@ -8,7 +8,11 @@ oracle's own pipeline: in k a central difference with h = 1e-4 |k|
 data a central difference, exact up to rounding because the loss is
 quadratic in the data.  Loss: sum |p|^2 over 32 exterior listeners.
 Bar: 1e-4 relative (FP32 operators and GMRES to 1e-6 against FP64; measured
-1e-6 in k and 3e-7 along the Neumann directions).
+1e-6 in k, 3e-7 along the Neumann directions, 1e-6 over the vertex
+directions and 2.5e-7 along the listener directions, bar 1e-5 there).
+Vertex directions: central differences with h = 1e-6 (radius 0.1) of the
+FP64 pipeline re-assembled at the moved vertices, the Neumann data held
+fixed, four whole-mesh and four single-vertex directions.
 """
 import numpy as np
 import pytest
@@ -92,17 +96,66 @@ def test_neumann_gradient_vs_fp64_difference(k):
     assert rel < 1e-4, rel
 
 
-def test_vertex_and_listener_gradients_are_refused():
+def _gpu_geometry(v, t, k, psi, pts):
     from neuralat_amd.bem.grad import burton_miller_pressure
 
-    v, t, psi, pts = _case(7.0)
     dev = torch.device("cuda:0")
     vt = torch.tensor(v, device=dev, requires_grad=True)
-    with pytest.raises(NotImplementedError):
-        burton_miller_pressure(vt, torch.tensor(t, device=dev), 7.0, torch.tensor(psi.astype(np.complex64),
-                                                                                   device=dev),
-                               torch.tensor(pts.astype(np.float32), device=dev))
     pt = torch.tensor(pts.astype(np.float32), device=dev, requires_grad=True)
-    with pytest.raises(NotImplementedError):
-        burton_miller_pressure(torch.tensor(v, device=dev), torch.tensor(t, device=dev), 7.0,
-                               torch.tensor(psi.astype(np.complex64), device=dev), pt)
+    p = burton_miller_pressure(vt, torch.tensor(t, device=dev), k, torch.tensor(psi.astype(np.complex64), device=dev),
+                               pt, tol=1e-6)
+    (p.abs() ** 2).sum().backward()
+    return vt.grad.double().cpu().numpy(), pt.grad.double().cpu().numpy()
+
+
+@pytest.mark.parametrize("k", [18.3, -9.0])
+def test_vertex_and_listener_gradients_vs_fp64_difference(k):
+    # level 2 (162 vertices): whole-mesh and single-vertex directions; the
+    # Neumann data stay fixed while the vertices move
+    v, t = mesh.icosphere(2, 0.1)
+    v = mesh.sh_perturb(v, 4, 0.05, seed=6).astype(np.float32)
+    src = np.array([[0.01, -0.005, 0.002]])
+    _, dn = mesh.sources_field(src, None, k)
+    psi = mesh.project_p1(v, t, lambda x, n: dn(x, n))
+    pts = mesh.random_exterior_points(24, 0.2, 0.5, seed=2)
+    print(f"BM geometry gradients, k = {k}: GPU", flush=True)
+    gv, gx = _gpu_geometry(v, t, k, psi, pts)
+    v64 = v.astype(np.float64)
+    rng = np.random.default_rng(11)
+    h = 1e-6  # absolute (radius 0.1)
+    dirs = [rng.standard_normal(v.shape) for _ in range(4)]
+    for _ in range(4):
+        d = np.zeros(v.shape)
+        d[rng.integers(len(v))] = rng.standard_normal(3)
+        dirs.append(d)
+    ad, fd = [], []
+    print(f"BM geometry gradients, k = {k}: FP64 oracle differences over {len(dirs)} vertex directions", flush=True)
+    for d in dirs:
+        lp = _loss64(_fp64_ops(v64 + h * d, t, k, pts), psi)
+        lm = _loss64(_fp64_ops(v64 - h * d, t, k, pts), psi)
+        fd.append((lp - lm) / (2 * h))
+        ad.append(float(np.sum(gv * d)))
+    ad, fd = np.array(ad), np.array(fd)
+    rel = np.abs(ad - fd) / np.abs(fd)
+    print(f"  vertex directions: adjoint {ad}\n  FP64 {fd}\n  relative {rel}", flush=True)
+    assert float(np.linalg.norm(ad - fd) / np.linalg.norm(fd)) < 1e-4
+    assert float(rel.max()) < 1e-2, rel
+    # listener positions: only the potentials move
+    ops = _fp64_ops(v64, t, k, pts)
+    L, R = ops[0], ops[1]
+    phi64 = np.linalg.solve(L, R @ psi)
+
+    def lpts(x):
+        S = oracle.potential(v64, t, x, k, "single", "P1", clamp=False)
+        D = oracle.potential(v64, t, x, k, "double", "P1", clamp=False)
+        return float(np.sum(np.abs(D @ phi64 - S @ psi) ** 2))
+
+    ad, fd = [], []
+    for _ in range(4):
+        d = rng.standard_normal(pts.shape)
+        fd.append((lpts(pts + h * d) - lpts(pts - h * d)) / (2 * h))
+        ad.append(float(np.sum(gx * d)))
+    ad, fd = np.array(ad), np.array(fd)
+    relx = float(np.linalg.norm(ad - fd) / np.linalg.norm(fd))
+    print(f"  listener directions: adjoint {ad} FP64 {fd}: {relx:.2e}", flush=True)
+    assert relx < 1e-5, relx
