@@ -478,23 +478,39 @@ __global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
 // |w|^2 into *wn2, v_{j+1} = w / |w|, and the Givens update of column j
 // (as gmres_givens_kernel).  One launch instead of nine: for small systems
 // the steps are launch-bound (each kernel does microseconds of work).
-__global__ __launch_bounds__(kGivensChunk) void arnoldi_small_kernel(
+constexpr int kArnoldiThreads = 1024;  // 16 wavefronts share the dot products and the update
+__global__ __launch_bounds__(kArnoldiThreads) void arnoldi_small_kernel(
     float2* __restrict__ V, int64_t n, int j, float2* __restrict__ w, double2* __restrict__ h,
     double2* __restrict__ h2, double2* __restrict__ H, int m, double2* __restrict__ cs, double2* __restrict__ sn,
     double2* __restrict__ g, double* __restrict__ res, double* __restrict__ wn2)
 {
-    __shared__ double red[kGivensChunk / 64];
+    __shared__ double red[kArnoldiThreads / 64];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    constexpr int nw = kGivensChunk / 64;
+    constexpr int nw = kArnoldiThreads / 64;
     for (int pass = 0; pass < 2; ++pass) {
         double2* hh = pass ? h2 : h;
         for (int q = wave; q <= j; q += nw) {  // hh[q] = <v_q, w>
             const float2* vq = V + (int64_t)q * n;
             double sr = 0.0, si = 0.0;
-            for (int64_t e = lane; e < n; e += 64) {
-                const float2 a = vq[e], b = w[e];
-                sr += (double)a.x * b.x + (double)a.y * b.y;
-                si += (double)a.x * b.y - (double)a.y * b.x;
+            // eight iterations' loads issued before their sums (clamped
+            // addresses, masked values): issued one at a time, each waited
+            // for before the next, the step took ~95 us at n = 2562
+            for (int64_t e0 = lane; e0 < n; e0 += 64 * 8) {
+                float2 a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t e = min(e0 + 64 * u, n - 1);
+                    a[u] = vq[e];
+                    b[u] = w[e];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (e0 + 64 * u < n) {
+                        sr += (double)a[u].x * b[u].x + (double)a[u].y * b[u].y;
+                        si += (double)a[u].x * b[u].y - (double)a[u].y * b[u].x;
+                    }
+                }
             }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
@@ -504,13 +520,21 @@ __global__ __launch_bounds__(kGivensChunk) void arnoldi_small_kernel(
             if (lane == 0) hh[q] = make_double2(sr, si);
         }
         __syncthreads();
-        for (int64_t e = t; e < n; e += kGivensChunk) {  // w -= sum_q hh[q] v_q
+        for (int64_t e = t; e < n; e += kArnoldiThreads) {  // w -= sum_q hh[q] v_q
             double ar = 0.0, ai = 0.0;
-            for (int q = 0; q <= j; ++q) {
-                const double2 c = hh[q];
-                const float2 v = V[(int64_t)q * n + e];
-                ar += c.x * v.x - c.y * v.y;
-                ai += c.x * v.y + c.y * v.x;
+            for (int q0 = 0; q0 <= j; q0 += 8) {  // eight basis vectors' loads at once
+                float2 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = V[(int64_t)min(q0 + u, j) * n + e];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (q0 + u <= j) {
+                        const double2 c = hh[q0 + u];
+                        ar += c.x * v[u].x - c.y * v[u].y;
+                        ai += c.x * v[u].y + c.y * v[u].x;
+                    }
+                }
             }
             const float2 wo = w[e];
             w[e] = make_float2((float)((double)wo.x - ar), (float)((double)wo.y - ai));
@@ -518,7 +542,7 @@ __global__ __launch_bounds__(kGivensChunk) void arnoldi_small_kernel(
         __syncthreads();
     }
     double a2 = 0.0;
-    for (int64_t e = t; e < n; e += kGivensChunk) {
+    for (int64_t e = t; e < n; e += kArnoldiThreads) {
         const float2 v = w[e];
         a2 += (double)v.x * v.x + (double)v.y * v.y;
     }
@@ -530,7 +554,7 @@ __global__ __launch_bounds__(kGivensChunk) void arnoldi_small_kernel(
     for (int q = 0; q < nw; ++q) nrm2 += red[q];
     const float inv = nrm2 > 0.0 ? (float)(1.0 / sqrt(nrm2)) : 0.f;
     float2* vn = V + (int64_t)(j + 1) * n;
-    for (int64_t e = t; e < n; e += kGivensChunk) {
+    for (int64_t e = t; e < n; e += kArnoldiThreads) {
         const float2 v = w[e];
         vn[e] = make_float2(v.x * inv, v.y * inv);
     }
@@ -977,7 +1001,7 @@ hipError_t launch_givens(double2* H, int m, int j, const double2* h, const doubl
 hipError_t launch_arnoldi_small(float2* V, int64_t n, int j, float2* w, double2* h, double2* h2, double2* H, int m,
                                 double2* cs, double2* sn, double2* g, double* res, double* wn2, hipStream_t s)
 {
-    hipLaunchKernelGGL(arnoldi_small_kernel, dim3(1), dim3(kGivensChunk), 0, s, V, n, j, w, h, h2, H, m, cs, sn, g,
+    hipLaunchKernelGGL(arnoldi_small_kernel, dim3(1), dim3(kArnoldiThreads), 0, s, V, n, j, w, h, h2, H, m, cs, sn, g,
                        res, wn2);
     return hipGetLastError();
 }
