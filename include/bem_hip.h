@@ -114,6 +114,16 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
  * flush to scattered addresses, slower).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
                     int64_t ld, void* stream);
+/* The system neumann2dirichlet solves (solver.py:118-126) in the P1 basis:
+ * lhs = bm_lhs + M/2 ([nv, ld]) and rhs = (bm_rhs - (i/2k) M) g ([nv], g the
+ * Neumann data per vertex), quad 0 / 1 / 2 as bem_assemble (the reference's
+ * neumann2dirichlet uses the approximate rules, quad 1).  The reference
+ * assembles bm_rhs as a matrix and multiplies; here its entries are
+ * contracted with g inside the bm_lhs kernels (never stored), so the system
+ * costs about one operator assembly instead of two plus a product.  Results
+ * agree with that route to rounding. */
+int bem_burton_miller_system(const bem_mesh* mesh, float k, int quad, int clamp, const void* g, void* lhs, int64_t ld,
+                             void* rhs, void* stream);
 /* The derivative with respect to k of the P1 operator op (SINGLE, DOUBLE,
  * ADJOINT_DOUBLE, BM_LHS, BM_RHS; regular and neighbour pairs, no identity
  * terms), [nv, nv] at out, same rules and clamp as bem_assemble.  No

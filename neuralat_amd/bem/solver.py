@@ -520,15 +520,23 @@ class BEM_Solver:
         RHS = matvec(R, neumann.to(torch.complex64).contiguous())
         return self._solve(LHS, RHS, solver, tol, restart, maxiter)
 
-    def burton_miller_system(self, k, neumann):
+    def burton_miller_system(self, k, neumann, approx=True):
         """The system neumann2dirichlet solves (solver.py:120-125), with the
-        approximate rules: (bm_lhs + M/2, (bm_rhs - (i/k) M/2) psi).  The mass
-        terms are added in place (no dense identity) and the right-hand side
-        is one library GEMV."""
-        beta = 1j / k
-        LHS = add_identity_(self, self.assemble_boundary_matrix(k, "bm_lhs", approx=True), 0.5)
-        R = add_identity_(self, self.assemble_boundary_matrix(k, "bm_rhs", approx=True), -0.5 * beta)
-        RHS = matvec(R, neumann.to(torch.complex64).contiguous())
+        approximate rules: (bm_lhs + M/2, (bm_rhs - (i/k) M/2) psi).  One
+        library call (bem_burton_miller_system): the right-hand side's operator
+        is contracted with psi inside the LHS kernels, never stored, and the
+        mass terms are added in place (no dense identity)."""
+        if self.basis != "P1":
+            raise ValueError("the Burton-Miller system is built for the P1 basis")
+        quad = "approx_1" if approx == "approx_1" else ("approx" if approx else "full")
+        n = self._n("P1")
+        g = neumann.to(torch.complex64).contiguous()
+        check_tensor(g, torch.complex64)
+        LHS = torch.empty((n, n), dtype=torch.complex64, device=self.device)
+        RHS = torch.empty(n, dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_burton_miller_system(self.mesh.handle, float(k), _lib.QUADS[quad],
+                                                        int(self.clamp), _ptr(g), _ptr(LHS), n, _ptr(RHS),
+                                                        _stream()), "bem_burton_miller_system")
         return LHS, RHS
 
     def neumann2dirichlet(self, k, neumann, solver="gmres", tol=1e-6, restart=100, maxiter=1000):

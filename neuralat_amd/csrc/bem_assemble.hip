@@ -504,7 +504,10 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // (row vertices, column vertices), as B(i, j) is; p1_add_transpose_kernel
 // then adds outT^T to out.  The adjoint double layer is the transpose of the
 // double layer, so it evaluates the double layer and swaps the two blocks.
-enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2 };
+// kSweepAllRg: every ordered pair of the Burton-Miller LHS, with the
+// Burton-Miller RHS contracted with a vertex vector g in the same pass
+// (the RHS operator itself is never stored)
+enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2, kSweepAllRg = 3 };
 
 // The 3 x 3 block B(i, j) (row-basis a, column-basis b) of one regular pair
 // of the P1 operator OP with the Q-point rule: rr = the row triangle's
@@ -513,12 +516,28 @@ enum { kSweepAll = 0, kSweepSym = 1, kSweepPair = 2 };
 // (zero them first); with PAIR also tr, tii = B(j, i)^T of the paired sweep.
 // EOP is the evaluated operator (the paired adjoint double layer evaluates
 // the double layer).
-template <int OP, int Q, bool CLAMP, bool PAIR, int EOP>
+template <int OP, int Q, bool CLAMP, bool PAIR, int EOP, bool RG = false>
 __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, const float* __restrict__ rci,
                                               const float* py, const float* wy, float ny0, float ny1, float ny2,
                                               const float* cj, float k, float krev, float invk, float* br, float* bi,
-                                              float* tr, float* tii)
+                                              float* tr, float* tii, const float* ghat = nullptr,
+                                              float* rgs = nullptr)
 {
+    static_assert(!RG || OP == BM_LHS, "the right-hand-side contraction rides on the Burton-Miller LHS");
+    // RG: the Burton-Miller RHS kernel -G + (i/k) G (1 - ikr) (x - y).n_x / r^2
+    // at the same Gauss-point pairs, contracted with ghat (g at the column
+    // points, 2Q floats) into rgs (the three row corners, complex)
+    float qR[RG ? Q : 1], qRk[RG ? Q : 1];
+    if constexpr (RG) {
+        const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            const float yn = (py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 +
+                             (py[3 * b + 2] - rr[2]) * nx2;
+            qR[b] = -yn;  // (x_0 - y_b).n_x = (x_a - y_b).n_x on the row plane
+            qRk[b] = -yn * invk;
+        }
+    }
     float gr9 = 0.f, gi9 = 0.f;  // plain-G sum (HYP/BM_LHS secondary)
     // ADLP: the normal factor (y_b - x_0).n_x is constant along the row
     // triangle's plane, so it is formed once per column point
@@ -556,6 +575,7 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
         float sr[3] = {0.f, 0.f, 0.f}, si[3] = {0.f, 0.f, 0.f};
         float ur[3] = {0.f, 0.f, 0.f}, ui[3] = {0.f, 0.f, 0.f};  // PAIR: transposed sums
         float tr9 = 0.f, ti9 = 0.f;
+        float sRr = 0.f, sRi = 0.f;  // RG: sum_b (RHS kernel) w_b ghat_b
         float mca = 0.f, kca = 0.f;
         if constexpr (OP == BM_LHS) {
             const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
@@ -610,6 +630,13 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
                     ei = fmaf(kca, rinv, mknn);
                     gr = g_r;  // w_b G for the curl term
                     gi = g_i;
+                    if constexpr (RG) {
+                        // RHS: w_b G [(x - y).n_x / r - 1 + i (x - y).n_x / (k r^2)], times g(y_b)
+                        const float e2r = fmaf(qR[b], rinv, -1.0f), e2i = qRk[b] * rinv * rinv;
+                        const float v2r = g_r * e2r - g_i * e2i, v2i = g_r * e2i + g_i * e2r;
+                        sRr = fmaf(v2r, ghat[2 * b], fmaf(-v2i, ghat[2 * b + 1], sRr));
+                        sRi = fmaf(v2r, ghat[2 * b + 1], fmaf(v2i, ghat[2 * b], sRi));
+                    }
                 } else {
                     // -1 + (i/k)(1 - i k r)(x - y).n_x / r^2
                     er = fmaf(qb[b], rinv, -1.0f);
@@ -668,6 +695,11 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
             gr9 = fmaf(ua, tr9, gr9);
             gi9 = fmaf(ua, ti9, gi9);
         }
+        if constexpr (RG) {
+            rgs[0] = fmaf(q0, sRr, rgs[0]); rgs[1] = fmaf(q0, sRi, rgs[1]);
+            rgs[2] = fmaf(q1, sRr, rgs[2]); rgs[3] = fmaf(q1, sRi, rgs[3]);
+            rgs[4] = fmaf(q2, sRr, rgs[4]); rgs[5] = fmaf(q2, sRi, rgs[5]);
+        }
     }
     if constexpr (OP == HYP) {
         // curl.curl * (sum of G) - k^2 (n_x.n_y) * (phi-weighted G) (integrand.h:304-313)
@@ -717,7 +749,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                                                          const float* __restrict__ curls, P1Tiles tl,
                                                          const KBatch kb, float2* __restrict__ out, int64_t ld,
                                                          float2* __restrict__ outT, int64_t ldT,
-                                                         const int32_t* __restrict__ vmap)
+                                                         const int32_t* __restrict__ vmap,
+                                                         const float2* __restrict__ rg_g,
+                                                         float2* __restrict__ rg_out)
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
@@ -728,8 +762,10 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     static_assert(CT == 256 && NCV <= CT, "one lane per column triangle and per column vertex");
     static_assert(SWEEP != kSweepSym || OP == SLP || OP == HYP, "symmetric sweep only for symmetric pair blocks");
     static_assert(SWEEP != kSweepPair || OP == DLP || OP == ADLP, "paired sweep operators");
-    constexpr bool HALF = SWEEP != kSweepAll;  // one evaluation per unordered pair
+    static_assert(SWEEP != kSweepAllRg || OP == BM_LHS, "the right-hand-side contraction rides on the LHS");
+    constexpr bool HALF = SWEEP == kSweepSym || SWEEP == kSweepPair;  // one evaluation per unordered pair
     constexpr bool PAIR = SWEEP == kSweepPair;
+    constexpr bool RG = SWEEP == kSweepAllRg;
     // the evaluated operator: the paired adjoint double layer evaluates the
     // double layer and swaps the blocks
     constexpr int EOP = (PAIR && OP == ADLP) ? DLP : OP;
@@ -742,6 +778,9 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     __shared__ int4 rtri[kP1RowTris];
     __shared__ int4 rslot[kP1RowTris];  // the row triangles' vertex slots in the row tile
     __shared__ float rcurl[kP1RowTris][9];
+    // RG: the right-hand-side contraction per row-tile vertex slot, summed
+    // over the workgroup's columns, then stored as this column tile's partial
+    __shared__ float2 rgslot[RG ? kP1RowVerts : 1];
     int ct = blockIdx.x, rt = blockIdx.y;
     bool own = false;  // half sweeps: the column tile holds the row tile
     if constexpr (HALF) {
@@ -761,6 +800,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     const int rv0 = tl.row_vptr[rt], nrv = tl.row_vptr[rt + 1] - rv0;
     const int cv0 = tl.col_vptr[ct], ncv = tl.col_vptr[ct + 1] - cv0;
     for (int e = threadIdx.x; e < nrv * ncv; e += CT) acc[e / ncv][e % ncv] = make_float2(0.f, 0.f);
+    if constexpr (RG)
+        if (threadIdx.x < kP1RowVerts) rgslot[threadIdx.x] = make_float2(0.f, 0.f);
     for (int e = threadIdx.x; e < (r1 - r0) * S; e += CT) {  // r1 - r0 <= kP1RowTris
         const int q = e / S;
         rrec[q][e - q * S] = rec[(int64_t)tl.perm[r0 + q] * S + (e - q * S)];
@@ -794,6 +835,18 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     }
     const float krev = k * kInv2Pi;
     const float invk = 1.0f / k;
+    // RG: g at the column triangle's quadrature points (caller's vertex ids)
+    float ghat[RG ? 2 * Q : 1];
+    float2 rgacc = make_float2(0.f, 0.f);  // RG: lane 3 r + c keeps row triangle r's corner c
+    if constexpr (RG) {
+        const float2 g0 = rg_g[tj.x], g1 = rg_g[tj.y], g2 = rg_g[tj.z];
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            const float p1 = TriRule<Q>::x1[b], p2 = TriRule<Q>::x2[b], p0 = 1.0f - p1 - p2;
+            ghat[2 * b] = p0 * g0.x + p1 * g1.x + p2 * g2.x;
+            ghat[2 * b + 1] = p0 * g0.y + p1 * g1.y + p2 * g2.y;
+        }
+    }
     // phase-2 role: column vertex slot w = threadIdx.x
     const int w = (int)threadIdx.x;
     const bool owner = w < ncv;
@@ -844,14 +897,25 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
 #pragma unroll
         for (int e = 0; e < (PAIR ? 9 : 1); ++e) tr[e] = tii[e] = 0.f;
+        float rgs[RG ? 6 : 1];
+#pragma unroll
+        for (int e = 0; e < (RG ? 6 : 1); ++e) rgs[e] = 0.f;
         if (active && !shared) {
             float ci[9];
             if constexpr (OP == HYP || base_op(OP) == BM_LHS) {
 #pragma unroll
                 for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
             }
-            p1_pair_block<OP, Q, CLAMP, PAIR, EOP>(rrec[rp - r0], ci, py, wy, ny0, ny1, ny2, cj, k, krev, invk, br,
-                                                    bi, tr, tii);
+            p1_pair_block<OP, Q, CLAMP, PAIR, EOP, RG>(rrec[rp - r0], ci, py, wy, ny0, ny1, ny2, cj, k, krev, invk,
+                                                        br, bi, tr, tii, ghat, rgs);
+        }
+        if constexpr (RG) {  // the row triangle's three corner sums over this wavefront's columns
+            const int lane = threadIdx.x & 63;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float sr = wave_sum_f32(rgs[2 * c]), si = wave_sum_f32(rgs[2 * c + 1]);
+                if (lane == 3 * (rp - r0) + c) rgacc = make_float2(sr, si);
+            }
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
@@ -894,6 +958,24 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         const float2 v = acc[rvi][cvi];
         unsafeAtomicAdd(dst, v.x);
         unsafeAtomicAdd(dst + 1, v.y);
+    }
+    if constexpr (RG) {
+        // the wavefronts' corner sums into the row tile's vertex slots (LDS),
+        // then one plain store per slot into this column tile's partial row
+        // (rg_out [ncol_tiles, row slots]; p1_rg_reduce_kernel sums the
+        // column tiles in a fixed order): no global atomics, which at level 6
+        // (every row vertex hit from 320 column tiles) serialised the kernel
+        const int lane = threadIdx.x & 63;
+        if (lane < 3 * (r1 - r0)) {
+            const int4 rs = rslot[lane / 3];
+            const int c = lane % 3;
+            const int slot = c == 0 ? rs.x : (c == 1 ? rs.y : rs.z);
+            atomicAdd(&rgslot[slot].x, rgacc.x);
+            atomicAdd(&rgslot[slot].y, rgacc.y);
+        }
+        __syncthreads();
+        const int64_t nslots = tl.row_vptr[tl.nrow_tiles];
+        if ((int)threadIdx.x < nrv) rg_out[(int64_t)ct * nslots + rv0 + threadIdx.x] = rgslot[threadIdx.x];
     }
     if constexpr (PAIR) {
         if (owner) {
@@ -1007,7 +1089,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     // lanes per pair: a wavefront per pair, except the 2-point line rule of the
     // P1 operators (16 points): 16 lanes per pair, 4 pairs per wavefront, so
     // the FP64 reductions run over 16 lanes and serve four pairs
-    constexpr int LPP = (NL == 2 && BASIS == P1 && MODE == 0) ? 16 : 64;
+    constexpr int LPP = (NL == 2 && BASIS == P1 && MODE != 1) ? 16 : 64;
     constexpr int PPW = 64 / LPP;
     const int ll = lane % LPP;
     const int64_t pw = p0 + ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * PPW;  // the wavefront's first pair
@@ -1216,10 +1298,65 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
                     vi += ar[9] * invk * c;
                 }
             }
-            float* dst = reinterpret_cast<float*>(&out[(int64_t)ia[oa] * ld + ja[ob]]);
-            unsafeAtomicAdd(dst, (float)vr);
-            unsafeAtomicAdd(dst + 1, (float)vi);
+            if constexpr (MODE == 2) {
+                // P1 contraction: the entry times psi at its column vertex, into
+                // vpsi at its row vertex (the operator is not stored)
+                const float2 gv = psi[ja[ob]];
+                float* dst = reinterpret_cast<float*>(&vpsi[ia[oa]]);
+                unsafeAtomicAdd(dst, (float)(vr * gv.x - vi * gv.y));
+                unsafeAtomicAdd(dst + 1, (float)(vr * gv.y + vi * gv.x));
+            } else {
+                float* dst = reinterpret_cast<float*>(&out[(int64_t)ia[oa] * ld + ja[ob]]);
+                unsafeAtomicAdd(dst, (float)vr);
+                unsafeAtomicAdd(dst + 1, (float)vi);
+            }
         }
+    }
+}
+
+// rhs[caller id of slot s] += sum over column tiles (fixed order) of
+// part[ct][s]; one thread per row-tile vertex slot (a vertex owns one slot in
+// each row tile that touches it, so the atomics meet at most a few times)
+__global__ __launch_bounds__(256) void p1_rg_reduce_kernel(const float2* __restrict__ part, int ncol_tiles,
+                                                           int64_t nslots, const int32_t* __restrict__ row_verts,
+                                                           const int32_t* __restrict__ vold,
+                                                           float2* __restrict__ rhs)
+{
+    const int64_t sl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (sl >= nslots) return;
+    float sx = 0.f, sy = 0.f;
+    for (int c = 0; c < ncol_tiles; ++c) {
+        const float2 v = part[(int64_t)c * nslots + sl];
+        sx += v.x;
+        sy += v.y;
+    }
+    float* dst = reinterpret_cast<float*>(&rhs[vold[row_verts[sl]]]);
+    unsafeAtomicAdd(dst, sx);
+    unsafeAtomicAdd(dst + 1, sy);
+}
+
+// out[v] += alpha (M g)[v] for the P1 mass matrix M_ab = |c| (1 + delta_ab) / 24
+// per triangle (the closed form of identity_p1_kernel's entries); one thread
+// per triangle, atomics into the three corners.
+__global__ __launch_bounds__(256) void p1_mass_apply_kernel(const float* __restrict__ geo,
+                                                            const int4* __restrict__ tris, int64_t nt,
+                                                            const float2* __restrict__ g, float2 alpha,
+                                                            float2* __restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= nt) return;
+    const int4 tr = tris[t];
+    const float c = geo[t * 16 + 12] * (1.0f / 24.0f);
+    const float2 g0 = g[tr.x], g1 = g[tr.y], g2 = g[tr.z];
+    const float sx = g0.x + g1.x + g2.x, sy = g0.y + g1.y + g2.y;
+    const int v3[3] = {tr.x, tr.y, tr.z};
+    const float2 gv[3] = {g0, g1, g2};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float mx = c * (sx + gv[a].x), my = c * (sy + gv[a].y);  // (M g)_a from this triangle
+        float* dst = reinterpret_cast<float*>(&out[v3[a]]);
+        unsafeAtomicAdd(dst, alpha.x * mx - alpha.y * my);
+        unsafeAtomicAdd(dst + 1, alpha.x * my + alpha.y * mx);
     }
 }
 
@@ -1507,7 +1644,8 @@ __global__ __launch_bounds__(256) void p1_permute_kernel(const float2* __restric
 constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
 static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float2* out, int64_t ld, bool one_sided,
-                               const int32_t* vmap, hipStream_t s)
+                               const int32_t* vmap, hipStream_t s, const float2* rg_g = nullptr,
+                               float2* rg_out = nullptr)
 {
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
     const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
@@ -1516,7 +1654,8 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
     if (!one_sided && m.tiles.nsym <= 0x7fffffffLL && nb * nb <= 0x7fffffffLL) {
         if constexpr (sym) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym, 1, nz),
-                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap);
+                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap,
+                               nullptr, nullptr);
             hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2), 1, nz), dim3(256), 0, s, out,
                                n, ld, kb.zstride);
             return hipGetLastError();
@@ -1526,7 +1665,8 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
                 hipError_t e = launch_fill(t, n, n, n, make_float2(0.f, 0.f), s);
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
-                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n, vmap);
+                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n, vmap,
+                                       nullptr, nullptr);
                     hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
                                        (const float2*)t, n, nb);
                     e = hipGetLastError();
@@ -1538,8 +1678,15 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
         }
     }
     dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles, (unsigned)nz);
+    if constexpr (OP == BM_LHS) {
+        if (rg_out) {
+            hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAllRg>), grid, dim3(256), 0, s, rec, m.tris,
+                               m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, rg_g, rg_out);
+            return hipGetLastError();
+        }
+    }
     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap);
+                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, nullptr, nullptr);
     return hipGetLastError();
 }
 
@@ -1799,6 +1946,93 @@ hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2
     AlphaBatch ab{};
     ab.a[0] = alpha;
     return add_identity_z(m, 0, basis, ab, 1, out, ld, 0, s);
+}
+
+// The Burton-Miller system of neumann2dirichlet in the P1 basis:
+// lhs = bm_lhs + M/2, rhs = (bm_rhs - (i/2k) M) g.  The RHS operator is never
+// stored: its regular pairs are contracted with g inside the LHS tile kernel
+// (the same Gauss-point pairs, sharing e^{ikr}/r), its neighbour pairs by the
+// contraction mode of the neighbour-pair kernel, its mass term by one pass
+// over the triangles.
+template <bool CLAMP>
+static hipError_t bm_system_p1_t(const MeshDev& m, float k, int quad, const float2* g, float2* lhs, int64_t ld,
+                                 float2* rhs, hipStream_t s)
+{
+    const KBatch kb = kbatch1(k);
+    const int64_t n = m.nv;
+    hipError_t e = launch_fill(rhs, 1, n, n, make_float2(0.f, 0.f), s);
+    if (e != hipSuccess) return e;
+    // the right-hand side's regular pairs: one partial row per column tile
+    const int64_t nslots = m.h_row_slots;
+    float2* part = nullptr;
+    e = hipMallocAsync((void**)&part, sizeof(float2) * (size_t)m.tiles.ncol_tiles * (size_t)nslots, s);
+    if (e != hipSuccess) return e;
+    auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap) -> hipError_t {
+        hipError_t er = launch_fill(dst, n, n, ldd, make_float2(0.f, 0.f), s);
+        if (er != hipSuccess) return er;
+        if (quad == 0) return p1_regular_t<BM_LHS, 6, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
+        if (quad == 1) return p1_regular_t<BM_LHS, 3, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
+        return p1_regular_t<BM_LHS, 1, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
+    };
+    float2* S = nullptr;
+    if (hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n, s) == hipSuccess) {
+        KBatch kbs = kb;
+        kbs.zstride = n * n;
+        e = regular(S, n, kbs, nullptr);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, 1), dim3(256), 0, s, S, n, m.p1_vnew, lhs, ld,
+                               (int64_t)0);
+            e = hipGetLastError();
+        }
+        const hipError_t ef = hipFreeAsync(S, s);
+        if (e == hipSuccess) e = ef;
+    } else {
+        (void)hipGetLastError();  // no room for the scratch: flush through the vertex map
+        e = regular(lhs, ld, kb, m.p1_vold);
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(p1_rg_reduce_kernel, dim3(grid1d(nslots)), dim3(256), 0, s, (const float2*)part,
+                           m.tiles.ncol_tiles, nslots, m.tiles.row_verts, m.p1_vold, rhs);
+        e = hipGetLastError();
+    }
+    {
+        const hipError_t ef = hipFreeAsync(part, s);
+        if (e == hipSuccess) e = ef;
+    }
+    if (e != hipSuccess) return e;
+    // neighbour pairs: the LHS into lhs, the RHS contracted with g into rhs
+    if (quad == 0) e = singular_t<BM_LHS, 4, P1, CLAMP>(m, 0, m.npairs, kb, 1, lhs, ld, 0, s);
+    else e = singular_t<BM_LHS, 2, P1, CLAMP>(m, 0, m.npairs, kb, 1, lhs, ld, 0, s);
+    if (e != hipSuccess) return e;
+    if (m.npairs > 0) {
+        const int64_t per = 16;  // 2-point rule: 4 pairs per wavefront; 4-point: one, 4 wavefronts
+        if (quad == 0) {
+            const unsigned gs = (unsigned)((m.npairs + 3) / 4);
+            hipLaunchKernelGGL((singular_kernel<BM_RHS, 4, P1, CLAMP, 2>), dim3(gs), dim3(256), 0, s, m.geo, m.tris,
+                               m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr, (int64_t)0,
+                               (int64_t)0, 1.0f, g, rhs);
+        } else {
+            const unsigned gs = (unsigned)((m.npairs + per - 1) / per);
+            hipLaunchKernelGGL((singular_kernel<BM_RHS, 2, P1, CLAMP, 2>), dim3(gs), dim3(256), 0, s, m.geo, m.tris,
+                               m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr, (int64_t)0,
+                               (int64_t)0, 1.0f, g, rhs);
+        }
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    // mass terms: rhs -= (i/2k) M g, lhs += M/2
+    hipLaunchKernelGGL(p1_mass_apply_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, m.geo, m.tris, m.nt, g,
+                       make_float2(0.f, -0.5f / k), rhs);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_add_identity(m, P1, make_float2(0.5f, 0.f), lhs, ld, s);
+}
+
+hipError_t launch_bm_system_p1(const MeshDev& m, float k, int quad, int clamp, const float2* g, float2* lhs,
+                               int64_t ld, float2* rhs, hipStream_t s)
+{
+    return clamp ? bm_system_p1_t<true>(m, k, quad, g, lhs, ld, rhs, s)
+                 : bm_system_p1_t<false>(m, k, quad, g, lhs, ld, rhs, s);
 }
 
 // The position-dependent pointers of m moved by off floats (the geometry of

@@ -236,6 +236,22 @@ int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, 
     return BEM_OK;
 }
 
+int bem_burton_miller_system(const bem_mesh* mesh, float k, int quad, int clamp, const void* g, void* lhs, int64_t ld,
+                             void* rhs, void* stream)
+{
+    if (!mesh || !g || !lhs || !rhs) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (quad < 0 || quad > 2) return fail(BEM_ERR_BAD_ARG, "unknown quadrature");
+    const auto& m = mesh->dev;
+    if (ld < m.nv) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (!std::isfinite(k) || k == 0.f) return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs a finite k != 0");
+    DeviceGuard dg(m.device);
+    if (!dg.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_bm_system_p1(m, k, quad, clamp, (const float2*)g, (float2*)lhs, ld, (float2*)rhs,
+                                            S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_burton_miller_system launch");
+    return BEM_OK;
+}
+
 int bem_assemble_dk(const bem_mesh* mesh, float k, int op, int quad, int clamp, void* out, int64_t ld, void* stream)
 {
     if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");

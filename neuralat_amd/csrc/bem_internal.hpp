@@ -72,6 +72,7 @@ struct MeshDev {
     int32_t* p1_vnew = nullptr;
     int32_t* p1_vold = nullptr;
     std::vector<int64_t> h_pair_ptr;  // [nt+1] host copy: pair range of each row triangle
+    int64_t h_row_slots = 0;          // total vertex slots of the row tiles (tiles.row_vptr[nrow_tiles])
     int32_t max_neighbors = 0;
     double diameter = 0.0;  // bounding-box diagonal
     std::vector<int32_t> h_tris;  // [nt*3] host copy (for mesh_set_vertices)
@@ -154,6 +155,10 @@ hipError_t launch_adjoint_potential_p1(const MeshDev& m, const float* pts, int64
                                        const float2* pbar, const float2* g, const float2* phi, double* gvert,
                                        double* gpts, hipStream_t s);
 hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2* out, int64_t ld, hipStream_t s);
+// neumann2dirichlet's P1 system: lhs = bm_lhs + M/2 [nv, ld], rhs = (bm_rhs -
+// (i/2k) M) g [nv] without storing bm_rhs (contracted inside the LHS kernels)
+hipError_t launch_bm_system_p1(const MeshDev& m, float k, int quad, int clamp, const float2* g, float2* lhs,
+                               int64_t ld, float2* rhs, hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 // records msg as the calling thread's bem_last_error() and returns code
 int report_failure(int code, const std::string& msg);

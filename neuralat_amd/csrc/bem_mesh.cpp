@@ -468,6 +468,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     m.tiles.nsym = sym_ptr.back();
     m.tiles.nrow_tiles = nrowt;
     m.tiles.ncol_tiles = ncolt;
+    m.h_row_slots = rowT.vptr.empty() ? 0 : rowT.vptr.back();
     return 0;
 }
 

@@ -27,8 +27,7 @@ import numpy as np
 import torch
 
 from .bem.batch import batch_gmres
-from .bem.solver import (BEM_Solver, GeometrySet, Mesh, add_identity_, boundary2potential_samples,
-                         burton_miller_samples, matvec)
+from .bem.solver import BEM_Solver, GeometrySet, Mesh, boundary2potential_samples, burton_miller_samples
 
 SPEED_OF_SOUND = 343.2  # src/scene.py:275
 
@@ -172,11 +171,7 @@ class SceneSampler:
 def burton_miller_system(solver, k, neumann):
     """(bm_lhs + M/2, (bm_rhs - beta M/2) psi) with the approximate rules, the
     system BEM_Solver.neumann2dirichlet solves (src/bem/solver.py:118-126)."""
-    beta = 1j / k
-    lhs = add_identity_(solver, solver.assemble_boundary_matrix(k, "bm_lhs", approx=True), 0.5)
-    R = add_identity_(solver, solver.assemble_boundary_matrix(k, "bm_rhs", approx=True), -0.5 * beta)
-    rhs = matvec(R, neumann.to(torch.complex64).contiguous())
-    return lhs, rhs
+    return solver.burton_miller_system(k, neumann)
 
 
 def burton_miller_systems(solver, ks, neumanns):
