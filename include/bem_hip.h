@@ -124,6 +124,12 @@ int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, 
  * agree with that route to rounding. */
 int bem_burton_miller_system(const bem_mesh* mesh, float k, int quad, int clamp, const void* g, void* lhs, int64_t ld,
                              void* rhs, void* stream);
+/* The same at nk wavenumbers on one mesh (the modes of scene.py:384-397):
+ * ks host [nk], g [nk, nv] contiguous, system z at lhs + z * stride complex
+ * elements ([nv, ld] each) and rhs + z * nv; one launch per kernel for up to
+ * 64 systems. */
+int bem_burton_miller_systems(const bem_mesh* mesh, const float* ks, int nk, int quad, int clamp, const void* g,
+                              void* lhs, int64_t ld, int64_t stride, void* rhs, void* stream);
 /* The derivative with respect to k of the P1 operator op (SINGLE, DOUBLE,
  * ADJOINT_DOUBLE, BM_LHS, BM_RHS; regular and neighbour pairs, no identity
  * terms), [nv, nv] at out, same rules and clamp as bem_assemble.  No
@@ -164,6 +170,11 @@ int bem_add_identity_samples(const bem_geomset* set, int basis, const float* alp
                              int64_t stride, void* stream);
 /* bem_potential for every sample z at its own points (device [ns, np, 3])
  * and wavenumber ks[z]: matrix z at out + z * stride ([np, ld] each). */
+/* The fused Burton-Miller systems (bem_burton_miller_system) of every sample
+ * of a geometry set, each at its wavenumber (ks host [ns]): g [ns, nv]
+ * contiguous, system z at lhs + z * stride and rhs + z * nv. */
+int bem_burton_miller_samples(const bem_geomset* set, const float* ks, int quad, int clamp, const void* g, void* lhs,
+                              int64_t ld, int64_t stride, void* rhs, void* stream);
 int bem_potential_samples(const bem_geomset* set, const float* points, int64_t np, const float* ks, int op, int basis,
                           int clamp, void* out, int64_t ld, int64_t stride, void* stream);
 /* y_z = alpha A_z x_z + beta y_z for nz complex64 matrices [rows, cols] (A_z at

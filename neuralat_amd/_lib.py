@@ -66,6 +66,8 @@ SIGNATURES = {
     "bem_assemble_ex": (_I32, [_P, _F, _I32, _I32, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_assemble_dk": (_I32, [_P, _F, _I32, _I32, _I32, _P, _I64, _P]),
     "bem_burton_miller_system": (_I32, [_P, _F, _I32, _I32, _P, _P, _I64, _P, _P]),
+    "bem_burton_miller_systems": (_I32, [_P, _P, _I32, _I32, _I32, _P, _P, _I64, _I64, _P, _P]),
+    "bem_burton_miller_samples": (_I32, [_P, _P, _I32, _I32, _P, _P, _I64, _I64, _P, _P]),
     "bem_assemble_batch_k": (_I32, [_P, _P, _I32, _I32, _I32, _I32, _I32, _P, _I64, _I64, _P]),
     "bem_cbie_work_size": (_I64, [_P]),
     "bem_cbie_dp0": (_I32, [_P, _F, _I32, _I32, _P, _P, _I64, _P, _P, _P]),

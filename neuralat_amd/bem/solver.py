@@ -189,14 +189,17 @@ def burton_miller_samples(gs: GeometrySet, ks, neumanns, clamp=True):
     """The neumann2dirichlet systems (solver.py:120-125, approximate rules) of
     every sample of a geometry set at its wavenumber: lhs [ns, V, V] =
     bm_lhs + M/2, rhs [ns, V] = (bm_rhs - (i/k) M/2) psi; one launch per
-    kernel for all samples."""
-    ks = [float(k) for k in ks]
+    kernel for all samples (bem_burton_miller_samples: the right-hand side's
+    operator contracted with psi inside the LHS kernels, never stored)."""
+    kh = gs._ks(ks)
     psi = torch.as_tensor(neumanns, device=gs.device).to(torch.complex64).reshape(gs.ns, -1).contiguous()
-    lhs = gs.assemble(ks, "bm_lhs", approx=True, clamp=clamp)
-    gs.add_identity_(lhs, [0.5] * gs.ns)
-    R = gs.assemble(ks, "bm_rhs", approx=True, clamp=clamp)
-    gs.add_identity_(R, [-0.5j / k for k in ks])
-    return lhs, matvec_strided(R, psi)
+    n = psi.shape[1]
+    lhs = torch.empty((gs.ns, n, n), dtype=torch.complex64, device=gs.device)
+    rhs = torch.empty((gs.ns, n), dtype=torch.complex64, device=gs.device)
+    _lib.check(_lib.load().bem_burton_miller_samples(gs.handle, kh.ctypes.data, _lib.QUADS["approx"], int(clamp),
+                                                     _ptr(psi), _ptr(lhs), n, n * n, _ptr(rhs), _stream()),
+               "bem_burton_miller_samples")
+    return lhs, rhs
 
 
 def boundary2potential_samples(gs: GeometrySet, ks, neumanns, dirichlets, points, clamp=True):
@@ -548,18 +551,22 @@ class BEM_Solver:
     def burton_miller_systems(self, ks, neumanns):
         """The systems neumann2dirichlet solves, at every wavenumber of ``ks``
         (``neumanns``: [K, n]): lhs [K, n, n] = bm_lhs + M/2, rhs [K, n] =
-        (bm_rhs - (i/k) M/2) psi.  Each operator kind is assembled for all
-        wavenumbers in one launch per kernel; the mass terms are added in
-        place and the right-hand sides come from one strided GEMV launch."""
-        ks = [float(k) for k in ks]
-        neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(ks), -1)
+        (bm_rhs - (i/k) M/2) psi, all wavenumbers in one launch per kernel
+        (bem_burton_miller_systems: the right-hand side's operator contracted
+        with psi inside the LHS kernels, never stored)."""
+        if self.basis != "P1":
+            raise ValueError("the Burton-Miller system is built for the P1 basis")
+        kh = np.ascontiguousarray(np.asarray([float(k) for k in ks], dtype=np.float32))
+        neumanns = torch.as_tensor(neumanns, device=self.device).to(torch.complex64).reshape(len(kh), -1)
         neumanns = neumanns.contiguous()
-        lhs = self.assemble_boundary_matrix_batch(ks, "bm_lhs", approx=True)
-        R = self.assemble_boundary_matrix_batch(ks, "bm_rhs", approx=True)
-        for z, k in enumerate(ks):
-            add_identity_(self, lhs[z], 0.5)
-            add_identity_(self, R[z], -0.5j / k)
-        return lhs, matvec_strided(R, neumanns)
+        n = self._n("P1")
+        lhs = torch.empty((len(kh), n, n), dtype=torch.complex64, device=self.device)
+        rhs = torch.empty((len(kh), n), dtype=torch.complex64, device=self.device)
+        _lib.check(_lib.load().bem_burton_miller_systems(self.mesh.handle, kh.ctypes.data, len(kh),
+                                                         _lib.QUADS["approx"], int(self.clamp), _ptr(neumanns),
+                                                         _ptr(lhs), n, n * n, _ptr(rhs), _stream()),
+                   "bem_burton_miller_systems")
+        return lhs, rhs
 
     def neumann2dirichlet_batch(self, ks, neumanns, tol=1e-6, restart=100, max_cycles=10):
         """neumann2dirichlet at every wavenumber of ``ks`` (the per-mode loop of

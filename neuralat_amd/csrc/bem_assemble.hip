@@ -835,6 +835,10 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     }
     const float krev = k * kInv2Pi;
     const float invk = 1.0f / k;
+    if constexpr (RG) {  // this launch's vertex vector and partial rows
+        rg_g += (int64_t)blockIdx.z * kb.vstride;
+        rg_out += (int64_t)blockIdx.z * tl.ncol_tiles * (int64_t)tl.row_vptr[tl.nrow_tiles];
+    }
     // RG: g at the column triangle's quadrature points (caller's vertex ids)
     float ghat[RG ? 2 * Q : 1];
     float2 rgacc = make_float2(0.f, 0.f);  // RG: lane 3 r + c keeps row triangle r's corner c
@@ -1085,6 +1089,10 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     out += (int64_t)blockIdx.z * kb.zstride;
     geo += (int64_t)blockIdx.z * kb.gstride;
     curls += (int64_t)blockIdx.z * kb.gstride;
+    if constexpr (MODE == 2) {
+        psi += (int64_t)blockIdx.z * kb.vstride;
+        vpsi += (int64_t)blockIdx.z * kb.vstride;
+    }
     const int lane = threadIdx.x & 63;
     // lanes per pair: a wavefront per pair, except the 2-point line rule of the
     // P1 operators (16 points): 16 lanes per pair, 4 pairs per wavefront, so
@@ -1320,10 +1328,12 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
 __global__ __launch_bounds__(256) void p1_rg_reduce_kernel(const float2* __restrict__ part, int ncol_tiles,
                                                            int64_t nslots, const int32_t* __restrict__ row_verts,
                                                            const int32_t* __restrict__ vold,
-                                                           float2* __restrict__ rhs)
+                                                           float2* __restrict__ rhs, int64_t vstride)
 {
     const int64_t sl = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (sl >= nslots) return;
+    part += (int64_t)blockIdx.y * ncol_tiles * nslots;  // system blockIdx.y
+    rhs += (int64_t)blockIdx.y * vstride;
     float sx = 0.f, sy = 0.f;
     for (int c = 0; c < ncol_tiles; ++c) {
         const float2 v = part[(int64_t)c * nslots + sl];
@@ -1338,13 +1348,24 @@ __global__ __launch_bounds__(256) void p1_rg_reduce_kernel(const float2* __restr
 // out[v] += alpha (M g)[v] for the P1 mass matrix M_ab = |c| (1 + delta_ab) / 24
 // per triangle (the closed form of identity_p1_kernel's entries); one thread
 // per triangle, atomics into the three corners.
-__global__ __launch_bounds__(256) void p1_mass_apply_kernel(const float* __restrict__ geo,
+struct AlphaBatch {
+    float2 a[kMaxBatchK];
+};
+
+// system z = blockIdx.y: geometry z (gstride floats on), g and out z * vstride
+// elements on, factor ab.a[z]
+__global__ __launch_bounds__(256) void p1_mass_apply_kernel(const float* __restrict__ geo, int64_t gstride,
                                                             const int4* __restrict__ tris, int64_t nt,
-                                                            const float2* __restrict__ g, float2 alpha,
-                                                            float2* __restrict__ out)
+                                                            const float2* __restrict__ g, const AlphaBatch ab,
+                                                            float2* __restrict__ out, int64_t vstride)
 {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= nt) return;
+    const int z = blockIdx.y;
+    geo += (int64_t)z * gstride;
+    g += (int64_t)z * vstride;
+    out += (int64_t)z * vstride;
+    const float2 alpha = ab.a[z];
     const int4 tr = tris[t];
     const float c = geo[t * 16 + 12] * (1.0f / 24.0f);
     const float2 g0 = g[tr.x], g1 = g[tr.y], g2 = g[tr.z];
@@ -1883,9 +1904,6 @@ hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld,
 
 // out_z += alpha_z * M_z (mass matrix of geometry z) in place, z = blockIdx.y
 // (one geometry: gstride 0): DP0 diagonal, P1 one thread per row
-struct AlphaBatch {
-    float2 a[kMaxBatchK];
-};
 
 __global__ __launch_bounds__(256) void add_identity_dp0_kernel(const float* __restrict__ geo, int64_t gstride,
                                                                int64_t nt, const AlphaBatch ab,
@@ -1955,33 +1973,37 @@ hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2
 // contraction mode of the neighbour-pair kernel, its mass term by one pass
 // over the triangles.
 template <bool CLAMP>
-static hipError_t bm_system_p1_t(const MeshDev& m, float k, int quad, const float2* g, float2* lhs, int64_t ld,
-                                 float2* rhs, hipStream_t s)
+static hipError_t bm_system_p1_t(const MeshDev& m, int64_t gstride, const KBatch& kbin, int nz, int quad,
+                                 const float2* g, float2* lhs, int64_t ld, int64_t lstride, float2* rhs,
+                                 hipStream_t s)
 {
-    const KBatch kb = kbatch1(k);
     const int64_t n = m.nv;
-    hipError_t e = launch_fill(rhs, 1, n, n, make_float2(0.f, 0.f), s);
+    KBatch kb = kbin;
+    kb.zstride = lstride;
+    kb.gstride = gstride;
+    kb.vstride = n;
+    hipError_t e = launch_fill(rhs, nz, n, n, make_float2(0.f, 0.f), s);
     if (e != hipSuccess) return e;
-    // the right-hand side's regular pairs: one partial row per column tile
+    // the right-hand side's regular pairs: one partial row per column tile and system
     const int64_t nslots = m.h_row_slots;
     float2* part = nullptr;
-    e = hipMallocAsync((void**)&part, sizeof(float2) * (size_t)m.tiles.ncol_tiles * (size_t)nslots, s);
+    e = hipMallocAsync((void**)&part, sizeof(float2) * (size_t)m.tiles.ncol_tiles * (size_t)nslots * (size_t)nz, s);
     if (e != hipSuccess) return e;
     auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap) -> hipError_t {
-        hipError_t er = launch_fill(dst, n, n, ldd, make_float2(0.f, 0.f), s);
+        hipError_t er = launch_fill_z(dst, n, n, ldd, kbd.zstride, nz, make_float2(0.f, 0.f), s);
         if (er != hipSuccess) return er;
-        if (quad == 0) return p1_regular_t<BM_LHS, 6, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
-        if (quad == 1) return p1_regular_t<BM_LHS, 3, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
-        return p1_regular_t<BM_LHS, 1, CLAMP>(m, kbd, 1, dst, ldd, true, vmap, s, g, part);
+        if (quad == 0) return p1_regular_t<BM_LHS, 6, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
+        if (quad == 1) return p1_regular_t<BM_LHS, 3, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
+        return p1_regular_t<BM_LHS, 1, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
     };
     float2* S = nullptr;
-    if (hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n, s) == hipSuccess) {
+    if (hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n * (size_t)nz, s) == hipSuccess) {
         KBatch kbs = kb;
         kbs.zstride = n * n;
         e = regular(S, n, kbs, nullptr);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, 1), dim3(256), 0, s, S, n, m.p1_vnew, lhs, ld,
-                               (int64_t)0);
+            hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n, m.p1_vnew,
+                               lhs, ld, lstride);
             e = hipGetLastError();
         }
         const hipError_t ef = hipFreeAsync(S, s);
@@ -1991,8 +2013,8 @@ static hipError_t bm_system_p1_t(const MeshDev& m, float k, int quad, const floa
         e = regular(lhs, ld, kb, m.p1_vold);
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(p1_rg_reduce_kernel, dim3(grid1d(nslots)), dim3(256), 0, s, (const float2*)part,
-                           m.tiles.ncol_tiles, nslots, m.tiles.row_verts, m.p1_vold, rhs);
+        hipLaunchKernelGGL(p1_rg_reduce_kernel, dim3(grid1d(nslots), (unsigned)nz), dim3(256), 0, s,
+                           (const float2*)part, m.tiles.ncol_tiles, nslots, m.tiles.row_verts, m.p1_vold, rhs, n);
         e = hipGetLastError();
     }
     {
@@ -2001,38 +2023,43 @@ static hipError_t bm_system_p1_t(const MeshDev& m, float k, int quad, const floa
     }
     if (e != hipSuccess) return e;
     // neighbour pairs: the LHS into lhs, the RHS contracted with g into rhs
-    if (quad == 0) e = singular_t<BM_LHS, 4, P1, CLAMP>(m, 0, m.npairs, kb, 1, lhs, ld, 0, s);
-    else e = singular_t<BM_LHS, 2, P1, CLAMP>(m, 0, m.npairs, kb, 1, lhs, ld, 0, s);
+    if (quad == 0) e = singular_t<BM_LHS, 4, P1, CLAMP>(m, 0, m.npairs, kb, nz, lhs, ld, 0, s);
+    else e = singular_t<BM_LHS, 2, P1, CLAMP>(m, 0, m.npairs, kb, nz, lhs, ld, 0, s);
     if (e != hipSuccess) return e;
     if (m.npairs > 0) {
-        const int64_t per = 16;  // 2-point rule: 4 pairs per wavefront; 4-point: one, 4 wavefronts
         if (quad == 0) {
-            const unsigned gs = (unsigned)((m.npairs + 3) / 4);
-            hipLaunchKernelGGL((singular_kernel<BM_RHS, 4, P1, CLAMP, 2>), dim3(gs), dim3(256), 0, s, m.geo, m.tris,
-                               m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr, (int64_t)0,
-                               (int64_t)0, 1.0f, g, rhs);
+            const unsigned gs = (unsigned)((m.npairs + 3) / 4);  // one pair per wavefront
+            hipLaunchKernelGGL((singular_kernel<BM_RHS, 4, P1, CLAMP, 2>), dim3(gs, 1, (unsigned)nz), dim3(256), 0, s,
+                               m.geo, m.tris, m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr,
+                               (int64_t)0, (int64_t)0, 1.0f, g, rhs);
         } else {
-            const unsigned gs = (unsigned)((m.npairs + per - 1) / per);
-            hipLaunchKernelGGL((singular_kernel<BM_RHS, 2, P1, CLAMP, 2>), dim3(gs), dim3(256), 0, s, m.geo, m.tris,
-                               m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr, (int64_t)0,
-                               (int64_t)0, 1.0f, g, rhs);
+            const unsigned gs = (unsigned)((m.npairs + 15) / 16);  // four pairs per wavefront
+            hipLaunchKernelGGL((singular_kernel<BM_RHS, 2, P1, CLAMP, 2>), dim3(gs, 1, (unsigned)nz), dim3(256), 0, s,
+                               m.geo, m.tris, m.curls, m.pairs_by_kind, (int64_t)0, m.npairs, kb, (float2*)nullptr,
+                               (int64_t)0, (int64_t)0, 1.0f, g, rhs);
         }
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    // mass terms: rhs -= (i/2k) M g, lhs += M/2
-    hipLaunchKernelGGL(p1_mass_apply_kernel, dim3(grid1d(m.nt)), dim3(256), 0, s, m.geo, m.tris, m.nt, g,
-                       make_float2(0.f, -0.5f / k), rhs);
+    // mass terms: rhs_z -= (i/2k_z) M_z g_z, lhs_z += M_z/2
+    AlphaBatch ar{}, al{};
+    for (int z = 0; z < nz; ++z) {
+        ar.a[z] = make_float2(0.f, -0.5f / kb.k[z]);
+        al.a[z] = make_float2(0.5f, 0.f);
+    }
+    hipLaunchKernelGGL(p1_mass_apply_kernel, dim3(grid1d(m.nt), (unsigned)nz), dim3(256), 0, s, m.geo, gstride, m.tris,
+                       m.nt, g, ar, rhs, n);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_add_identity(m, P1, make_float2(0.5f, 0.f), lhs, ld, s);
+    return add_identity_z(m, gstride, P1, al, nz, lhs, ld, lstride, s);
 }
 
 hipError_t launch_bm_system_p1(const MeshDev& m, float k, int quad, int clamp, const float2* g, float2* lhs,
                                int64_t ld, float2* rhs, hipStream_t s)
 {
-    return clamp ? bm_system_p1_t<true>(m, k, quad, g, lhs, ld, rhs, s)
-                 : bm_system_p1_t<false>(m, k, quad, g, lhs, ld, rhs, s);
+    const KBatch kb = kbatch1(k);
+    return clamp ? bm_system_p1_t<true>(m, 0, kb, 1, quad, g, lhs, ld, 0, rhs, s)
+                 : bm_system_p1_t<false>(m, 0, kb, 1, quad, g, lhs, ld, 0, rhs, s);
 }
 
 // The position-dependent pointers of m moved by off floats (the geometry of
@@ -2047,6 +2074,25 @@ static MeshDev geometry_shift(const MeshDev& m, int64_t off)
     g.curls += off;
     g.geo += off;
     return g;
+}
+
+hipError_t launch_bm_systems_p1(const MeshDev& mg, int64_t gstride, const float* ks, int nk, int quad, int clamp,
+                                const float2* g, float2* lhs, int64_t ld, int64_t stride, float2* rhs, hipStream_t s)
+{
+    const int64_t n = mg.nv;
+    for (int z0 = 0; z0 < nk; z0 += kMaxBatchK) {
+        const int nz = nk - z0 < kMaxBatchK ? nk - z0 : kMaxBatchK;
+        KBatch kb{};
+        for (int z = 0; z < nz; ++z) kb.k[z] = ks[z0 + z];
+        const MeshDev m = geometry_shift(mg, (int64_t)z0 * gstride);
+        const float2* gz = g + (int64_t)z0 * n;
+        float2* lz = lhs + (int64_t)z0 * stride;
+        float2* rz = rhs + (int64_t)z0 * n;
+        const hipError_t e = clamp ? bm_system_p1_t<true>(m, gstride, kb, nz, quad, gz, lz, ld, stride, rz, s)
+                                   : bm_system_p1_t<false>(m, gstride, kb, nz, quad, gz, lz, ld, stride, rz, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_assemble_samples(const MeshDev& mg, int64_t gstride, const float* ks, int ns, int op, int basis,

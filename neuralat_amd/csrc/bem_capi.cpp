@@ -252,6 +252,32 @@ int bem_burton_miller_system(const bem_mesh* mesh, float k, int quad, int clamp,
     return BEM_OK;
 }
 
+static int check_bm_ks(const float* ks, int nk)
+{
+    if (!ks || nk < 1) return fail(BEM_ERR_BAD_ARG, "need at least one wavenumber");
+    for (int z = 0; z < nk; ++z)
+        if (!std::isfinite(ks[z]) || ks[z] == 0.f)
+            return fail(BEM_ERR_BAD_ARG, "Burton-Miller coupling i/k needs finite wavenumbers != 0");
+    return BEM_OK;
+}
+
+int bem_burton_miller_systems(const bem_mesh* mesh, const float* ks, int nk, int quad, int clamp, const void* g,
+                              void* lhs, int64_t ld, int64_t stride, void* rhs, void* stream)
+{
+    if (!mesh || !g || !lhs || !rhs) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (quad < 0 || quad > 2) return fail(BEM_ERR_BAD_ARG, "unknown quadrature");
+    if (int rc = check_bm_ks(ks, nk)) return rc;
+    const auto& m = mesh->dev;
+    if (ld < m.nv) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (nk > 1 && stride < m.nv * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (nv * ld)");
+    DeviceGuard dg(m.device);
+    if (!dg.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_bm_systems_p1(m, 0, ks, nk, quad, clamp, (const float2*)g, (float2*)lhs, ld, stride,
+                                             (float2*)rhs, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_burton_miller_systems launch");
+    return BEM_OK;
+}
+
 int bem_assemble_dk(const bem_mesh* mesh, float k, int op, int quad, int clamp, void* out, int64_t ld, void* stream)
 {
     if (!mesh || !out) return fail(BEM_ERR_BAD_ARG, "null pointer");
@@ -394,6 +420,23 @@ int bem_add_identity_samples(const bem_geomset* gs, int basis, const float* alph
     hipError_t e = bem::launch_add_identity_samples(m, gs->gstride, basis, alphas, gs->ns, (float2*)out, ld, stride,
                                                     S(stream));
     if (e != hipSuccess) return hip_fail(e, "bem_add_identity_samples launch");
+    return BEM_OK;
+}
+
+int bem_burton_miller_samples(const bem_geomset* gs, const float* ks, int quad, int clamp, const void* g, void* lhs,
+                              int64_t ld, int64_t stride, void* rhs, void* stream)
+{
+    if (!gs || !g || !lhs || !rhs) return fail(BEM_ERR_BAD_ARG, "null pointer");
+    if (quad < 0 || quad > 2) return fail(BEM_ERR_BAD_ARG, "unknown quadrature");
+    if (int rc = check_bm_ks(ks, gs->ns)) return rc;
+    const auto& m = gs->view;
+    if (ld < m.nv) return fail(BEM_ERR_BAD_ARG, "ld smaller than the matrix width");
+    if (gs->ns > 1 && stride < m.nv * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (nv * ld)");
+    DeviceGuard dg(m.device);
+    if (!dg.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
+    hipError_t e = bem::launch_bm_systems_p1(m, gs->gstride, ks, gs->ns, quad, clamp, (const float2*)g, (float2*)lhs,
+                                             ld, stride, (float2*)rhs, S(stream));
+    if (e != hipSuccess) return hip_fail(e, "bem_burton_miller_samples launch");
     return BEM_OK;
 }
 

@@ -93,6 +93,7 @@ struct KBatch {
     int64_t zstride;
     int64_t gstride = 0;
     int64_t pstride = 0;
+    int64_t vstride = 0;  // per-launch vertex vectors (the fused Burton-Miller right-hand sides)
 };
 inline KBatch kbatch1(float k)
 {
@@ -159,6 +160,12 @@ hipError_t launch_add_identity(const MeshDev& m, int basis, float2 alpha, float2
 // (i/2k) M) g [nv] without storing bm_rhs (contracted inside the LHS kernels)
 hipError_t launch_bm_system_p1(const MeshDev& m, float k, int quad, int clamp, const float2* g, float2* lhs,
                                int64_t ld, float2* rhs, hipStream_t s);
+// the same for nk wavenumbers on one mesh (gstride 0) or for the ns samples of
+// a geometry set (mg its region view): system z at lhs + z * stride, g and rhs
+// [nk, nv] contiguous
+hipError_t launch_bm_systems_p1(const MeshDev& mg, int64_t gstride, const float* ks, int nk, int quad, int clamp,
+                                const float2* g, float2* lhs, int64_t ld, int64_t stride, float2* rhs,
+                                hipStream_t s);
 hipError_t launch_identity(const MeshDev& m, int basis, float2* out, int64_t ld, hipStream_t s);
 // records msg as the calling thread's bem_last_error() and returns code
 int report_failure(int code, const std::string& msg);
