@@ -195,9 +195,8 @@ class _BurtonMillerPressure(torch.autograd.Function):
         s = BEM_Solver(vertices.detach().contiguous().float(), triangles, basis="P1", clamp=clamp)
         kf = float(k)
         g = neumann.detach().to(torch.complex64).contiguous()
-        L = add_identity_(s, s.assemble_boundary_matrix(kf, "bm_lhs", approx=True), 0.5)
-        R = add_identity_(s, s.assemble_boundary_matrix(kf, "bm_rhs", approx=True), -0.5j / kf)
-        phi, info = gmres(L, matvec(R, g), tol=tol, restart=restart, maxiter=maxiter)
+        L, b = s.burton_miller_system(kf, g)  # the RHS operator contracted with g, not stored
+        phi, info = gmres(L, b, tol=tol, restart=restart, maxiter=maxiter)
         last_solve_info["adjoint"] = None
         _report("forward", info, raise_on_fail, "burton_miller_pressure")
         pts = points.detach().to(torch.float32).contiguous()
@@ -205,7 +204,7 @@ class _BurtonMillerPressure(torch.autograd.Function):
         D = s.potential_matrix(kf, pts, "double")
         p = matvec(D, phi)
         matvec(S, g, alpha=-1.0, beta=1.0, out=p)
-        ctx.s, ctx.L, ctx.R, ctx.S, ctx.D = s, L, R, S, D
+        ctx.s, ctx.L, ctx.S, ctx.D = s, L, S, D
         ctx.kf, ctx.restart, ctx.maxiter = kf, restart, maxiter
         ctx.raise_on_fail, ctx.adjoint_tol = raise_on_fail, adjoint_tol
         ctx.save_for_backward(phi, g, pts)
@@ -216,7 +215,7 @@ class _BurtonMillerPressure(torch.autograd.Function):
         from .solver import add_identity_
 
         phi, g, pts = ctx.saved_tensors
-        s, L, R, S, D, kf = ctx.s, ctx.L, ctx.R, ctx.S, ctx.D, ctx.kf
+        s, L, S, D, kf = ctx.s, ctx.L, ctx.S, ctx.D, ctx.kf
         pbar = (0.5 * gp).to(torch.complex64).contiguous()  # dl/dp* (torch's gradient is 2 dl/dp*)
         # adjoint solve (bm_lhs + M/2)^H lambda = D^H pbar on the stored matrix
         lam, info = _adjoint_solve(L, matvec(D, pbar, trans=2), ctx.adjoint_tol, ctx.restart, ctx.maxiter)
@@ -241,8 +240,11 @@ class _BurtonMillerPressure(torch.autograd.Function):
             if gpts is not None:
                 grad_x = gpts.to(torch.float32)
         if ctx.needs_input_grad[1]:
-            # p = (D L^-1 R - S) g: dl/dg* = R^H lambda - S^H pbar
+            # p = (D L^-1 R - S) g: dl/dg* = R^H lambda - S^H pbar (R assembled here:
+            # the forward pass contracted it with g without storing it)
+            R = add_identity_(s, s.assemble_boundary_matrix(kf, "bm_rhs", approx=True), -0.5j / kf)
             grad_g = matvec(R, lam, trans=2)
+            del R
             matvec(S, pbar, trans=2, alpha=-1.0, beta=1.0, out=grad_g)
             grad_g = 2.0 * grad_g
         if ctx.needs_input_grad[0]:
