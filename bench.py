@@ -1291,7 +1291,8 @@ def modal_sound_probe(cells=(20, 10, 10), size=(0.2, 0.1, 0.1), k=32, reps=3):
 def p1_probe(level=5, k=10.0):
     """One assembly of each P1 layer operator and of the two Burton-Miller
     operators with the approximate rules (the data-generation path) on the
-    level-``level`` mesh of the P1 measurement (for hardware-counter passes)."""
+    level-``level`` mesh of the P1 measurement, then neumann2dirichlet's
+    system in one pass (for kernel traces and hardware-counter passes)."""
     import torch
 
     from neuralat_amd import mesh
@@ -1306,6 +1307,12 @@ def p1_probe(level=5, k=10.0):
         print(json.dumps({"op": op, "approx": approx, "vertices": int(v.shape[0]),
                           "entry_0_5": str(complex(A[0, 5].item()))}))
         del A
+    # neumann2dirichlet's system in one pass (bm_rhs contracted with g in the bm_lhs kernels)
+    g = torch.ones(v.shape[0], dtype=torch.complex64, device="cuda")
+    L, b = s.burton_miller_system(k, g)
+    torch.cuda.synchronize()
+    print(json.dumps({"op": "burton_miller_system", "vertices": int(v.shape[0]), "rhs_0": str(complex(b[0].item()))}))
+    del L, b
 
 
 def main():
