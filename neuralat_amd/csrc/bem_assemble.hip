@@ -743,6 +743,148 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
     }
 }
 
+// The same block for the two Burton-Miller operators with the 3-point rule
+// (the reference's approximate rules), built from sums.  The rule's points
+// give every corner the basis value (1 + 3 delta)/6 (2/3 at its own point,
+// 1/6 at the others), and its three weights are equal, so with u_ab the
+// kernel value at row point a and column point b (column weight w folded
+// in) and ua the row weight,
+//   B[c][d] = sum_ab ua (1 + 3 d_ca)(1 + 3 d_db) u_ab / 36
+//           = (ua / 36) (9 u_cd + 3 R_c + 3 C_d + S)
+// with R, C the row and column sums of u and S its total: 28 additions and 36
+// fused multiply-adds per pair instead of 117 basis-weighted accumulations.
+// w = the column weight / 4 pi (all three equal), kw = k w.  RG as above.
+template <int OP, bool CLAMP, bool RG>
+__device__ __forceinline__ void p1_pair_block_bm3(const float* __restrict__ rr, const float* __restrict__ rci,
+                                                  const float* py, float w, float kw, float ny0, float ny1, float ny2,
+                                                  const float* cj, float k, float krev, float invk, float* br,
+                                                  float* bi, const float* ghat, float* rgs)
+{
+    static_assert(OP == BM_LHS || OP == BM_RHS, "Burton-Miller operators");
+    static_assert(!RG || OP == BM_LHS, "the right-hand-side contraction rides on the Burton-Miller LHS");
+    constexpr int Q = 3;
+    const float nx0 = rr[4 * Q], nx1 = rr[4 * Q + 1], nx2 = rr[4 * Q + 2];
+    // (x_0 - y_b).n_x per column point (constant along the row plane)
+    float qn[Q];
+    if constexpr (OP == BM_RHS || RG) {
+#pragma unroll
+        for (int b = 0; b < Q; ++b)
+            qn[b] = -((py[3 * b] - rr[0]) * nx0 + (py[3 * b + 1] - rr[1]) * nx1 + (py[3 * b + 2] - rr[2]) * nx2);
+    }
+    float qw[Q], qkw[Q];  // BM_RHS: w qn and w qn / k
+    if constexpr (OP == BM_RHS) {
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            qw[b] = w * qn[b];
+            qkw[b] = qw[b] * invk;
+        }
+    }
+    // BM_LHS: -i k w n_x.n_y once per pair
+    float mknn = 0.f;
+    if constexpr (OP == BM_LHS) mknn = -kw * (nx0 * ny0 + nx1 * ny1 + nx2 * ny2);
+    float uR[Q][Q], uI[Q][Q];
+    float gsr = 0.f, gsi = 0.f;  // BM_LHS: sum of exp(ikr)/r (the curl term)
+    float tr_[RG ? Q : 1], ti_[RG ? Q : 1];  // RG: per row point, sum_b exp(ikr)/r e2_ab ghat_b
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+        float mca = 0.f, kca = 0.f;
+        if constexpr (OP == BM_LHS) {
+            const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
+            mca = -w * ca;
+            kca = kw * ca;
+        }
+        if constexpr (RG) tr_[a] = ti_[a] = 0.f;
+#pragma unroll
+        for (int b = 0; b < Q; ++b) {
+            const float dx = py[3 * b] - xa0, dy = py[3 * b + 1] - xa1, dz = py[3 * b + 2] - xa2;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            if (CLAMP) r2 = fmaxf(r2, kEps2);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            float sn, cs;
+            sincos_rev(r * krev, sn, cs);
+            const float gr = cs * rinv, gi = sn * rinv;  // exp(ikr)/r
+            float er, ei;
+            if constexpr (OP == BM_LHS) {
+                // w [(1 - i k r)(y - x).n_y / r^2 - i k n_x.n_y]
+                er = mca * rinv * rinv;
+                ei = fmaf(kca, rinv, mknn);
+                gsr += gr;
+                gsi += gi;
+                if constexpr (RG) {
+                    // RHS: (x - y).n_x / r - 1 + i (x - y).n_x / (k r^2), times g(y_b)
+                    const float e2r = fmaf(qn[b], rinv, -1.0f), e2i = qn[b] * invk * rinv * rinv;
+                    const float v2r = gr * e2r - gi * e2i, v2i = gr * e2i + gi * e2r;
+                    tr_[a] = fmaf(v2r, ghat[2 * b], fmaf(-v2i, ghat[2 * b + 1], tr_[a]));
+                    ti_[a] = fmaf(v2r, ghat[2 * b + 1], fmaf(v2i, ghat[2 * b], ti_[a]));
+                }
+            } else {
+                // w [-1 + (i/k)(1 - i k r)(x - y).n_x / r^2]
+                er = fmaf(qw[b], rinv, -w);
+                ei = qkw[b] * rinv * rinv;
+            }
+            uR[a][b] = gr * er - gi * ei;
+            uI[a][b] = gr * ei + gi * er;
+        }
+    }
+    const float ua = rr[3 * Q];  // the row weight (all three equal)
+    const float f = ua * (1.0f / 36.0f), f3 = 3.0f * f, f9 = 9.0f * f;
+    float Rr[Q], Ri[Q], Cr[Q], Ci[Q];
+#pragma unroll
+    for (int a = 0; a < Q; ++a) {
+        Rr[a] = uR[a][0] + uR[a][1] + uR[a][2];
+        Ri[a] = uI[a][0] + uI[a][1] + uI[a][2];
+        Cr[a] = uR[0][a] + uR[1][a] + uR[2][a];
+        Ci[a] = uI[0][a] + uI[1][a] + uI[2][a];
+    }
+    const float Sr = Rr[0] + Rr[1] + Rr[2], Si = Ri[0] + Ri[1] + Ri[2];
+#pragma unroll
+    for (int c = 0; c < Q; ++c) {
+        const float pr = fmaf(f3, Rr[c], f * Sr), pi = fmaf(f3, Ri[c], f * Si);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) {
+            br[3 * c + d] = fmaf(f9, uR[c][d], fmaf(f3, Cr[d], pr));
+            bi[3 * c + d] = fmaf(f9, uI[c][d], fmaf(f3, Ci[d], pi));
+        }
+    }
+    if constexpr (OP == BM_LHS) {
+        // (i/k) w ua (sum of G) times curl.curl
+        const float sc = invk * w * ua;
+        const float s9r = -gsi * sc, s9i = gsr * sc;
+        // a triangle's three curls sum to zero, so the 2 x 2 leading dot
+        // products fix the other five
+        float ci[9], cp[9];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) ci[e] = rci[e];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+                cp[3 * c + d] = ci[3 * c] * cj[3 * d] + ci[3 * c + 1] * cj[3 * d + 1] + ci[3 * c + 2] * cj[3 * d + 2];
+        cp[2] = -(cp[0] + cp[1]);
+        cp[5] = -(cp[3] + cp[4]);
+        cp[6] = -(cp[0] + cp[3]);
+        cp[7] = -(cp[1] + cp[4]);
+        cp[8] = -(cp[6] + cp[7]);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            br[e] = fmaf(s9r, cp[e], br[e]);
+            bi[e] = fmaf(s9i, cp[e], bi[e]);
+        }
+    }
+    if constexpr (RG) {
+        // corner c: sum_a ua w (1 + 3 d_ca) / 6 T_a
+        const float h = ua * w * (1.0f / 6.0f), h3 = 3.0f * h;
+        const float Tr = tr_[0] + tr_[1] + tr_[2], Ti = ti_[0] + ti_[1] + ti_[2];
+#pragma unroll
+        for (int c = 0; c < Q; ++c) {
+            rgs[2 * c] = fmaf(h3, tr_[c], h * Tr);
+            rgs[2 * c + 1] = fmaf(h3, ti_[c], h * Ti);
+        }
+    }
+}
+
 
 template <int OP, int Q, bool CLAMP, int SWEEP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
@@ -751,7 +893,7 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                                                          float2* __restrict__ outT, int64_t ldT,
                                                          const int32_t* __restrict__ vmap,
                                                          const float2* __restrict__ rg_g,
-                                                         float2* __restrict__ rg_out)
+                                                         float2* __restrict__ rg_out, int64_t plane)
 {
     const float k = kb.k[blockIdx.z];
     out += (int64_t)blockIdx.z * kb.zstride;
@@ -769,6 +911,8 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     // the evaluated operator: the paired adjoint double layer evaluates the
     // double layer and swaps the blocks
     constexpr int EOP = (PAIR && OP == ADLP) ? DLP : OP;
+    // the Burton-Miller operators with the 3-point rule: blocks built from sums
+    constexpr bool kBm3 = Q == 3 && !PAIR && (OP == BM_LHS || OP == BM_RHS);
     constexpr int S = Rec<Q>::kFloats;
     __shared__ float2 acc[kP1RowVerts][NCV];
     __shared__ float2 stage[9][CT];
@@ -863,11 +1007,18 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
     int cinc[kIncRegs];
 #pragma unroll
     for (int u = 0; u < kIncRegs; ++u) cinc[u] = ib + u < ie ? tl.col_inc[ib + u] : 0;
-    // s_c += the stage entries of row corner c over the owner's incident
-    // column triangles, in incidence order
+    // s_c = the sum of the stage entries of row corner c over the owner's
+    // incident column triangles, in incidence order (an owner has at least
+    // one: its vertex belongs to a triangle of the tile)
     auto gather = [&](float2& s0, float2& s1, float2& s2) {
+        {
+            const int p = cinc[0] >> 2, b = cinc[0] & 3;
+            s0 = stage[b][p];
+            s1 = stage[3 + b][p];
+            s2 = stage[6 + b][p];
+        }
 #pragma unroll
-        for (int u = 0; u < kIncRegs; ++u) {
+        for (int u = 1; u < kIncRegs; ++u) {
             if (ib + u < ie) {
                 const int code = cinc[u];
                 const int p = code >> 2, b = code & 3;
@@ -897,21 +1048,31 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                             ti.y == tj.z || ti.z == tj.x || ti.z == tj.y || ti.z == tj.z || (HALF && own && pos <= rp);
         float br[9], bi[9];
         float tr[PAIR ? 9 : 1], tii[PAIR ? 9 : 1];  // PAIR: B(j, i)^T
+        // the sum-built blocks are assigned whole: skipped lanes stage zeros
+        // from a branch of their own instead of clearing 18 registers at
+        // every row triangle (the kernels are bound by vector issue)
+        if constexpr (!kBm3) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
+            for (int e = 0; e < 9; ++e) br[e] = bi[e] = 0.f;
+        }
 #pragma unroll
         for (int e = 0; e < (PAIR ? 9 : 1); ++e) tr[e] = tii[e] = 0.f;
         float rgs[RG ? 6 : 1];
 #pragma unroll
         for (int e = 0; e < (RG ? 6 : 1); ++e) rgs[e] = 0.f;
-        if (active && !shared) {
+        const bool eval = active && !shared;
+        if (eval) {
             float ci[9];
             if constexpr (OP == HYP || base_op(OP) == BM_LHS) {
 #pragma unroll
                 for (int e = 0; e < 9; ++e) ci[e] = rcurl[rp - r0][e];
             }
-            p1_pair_block<OP, Q, CLAMP, PAIR, EOP, RG>(rrec[rp - r0], ci, py, wy, ny0, ny1, ny2, cj, k, krev, invk,
-                                                        br, bi, tr, tii, ghat, rgs);
+            if constexpr (kBm3)
+                p1_pair_block_bm3<OP, CLAMP, RG>(rrec[rp - r0], ci, py, wy[0], k * wy[0], ny0, ny1, ny2, cj, k, krev,
+                                                 invk, br, bi, ghat, rgs);
+            else
+                p1_pair_block<OP, Q, CLAMP, PAIR, EOP, RG>(rrec[rp - r0], ci, py, wy, ny0, ny1, ny2, cj, k, krev,
+                                                            invk, br, bi, tr, tii, ghat, rgs);
         }
         if constexpr (RG) {  // the row triangle's three corner sums over this wavefront's columns
             const int lane = threadIdx.x & 63;
@@ -921,12 +1082,17 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
                 if (lane == 3 * (rp - r0) + c) rgacc = make_float2(sr, si);
             }
         }
+        if (!kBm3 || eval) {
 #pragma unroll
-        for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
+            for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(br[e], bi[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) stage[e][threadIdx.x] = make_float2(0.f, 0.f);
+        }
         __syncthreads();
         if (owner) {
             const int4 rs = rslot[rp - r0];
-            float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
+            float2 s0, s1, s2;
             gather(s0, s1, s2);
             acc[rs.x][w].x += s0.x; acc[rs.x][w].y += s0.y;
             acc[rs.y][w].x += s1.x; acc[rs.y][w].y += s1.y;
@@ -958,10 +1124,16 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             gr = vmap[gr];
             gc = vmap[gc];
         }
-        float* dst = reinterpret_cast<float*>(&out[gr * ld + gc]);
         const float2 v = acc[rvi][cvi];
-        unsafeAtomicAdd(dst, v.x);
-        unsafeAtomicAdd(dst + 1, v.y);
+        if (plane) {  // real and imaginary planes: a wavefront's 64 atomics span half the cache lines
+            float* dst = reinterpret_cast<float*>(out) + (gr * ld + gc);
+            unsafeAtomicAdd(dst, v.x);
+            unsafeAtomicAdd(dst + plane, v.y);
+        } else {
+            float* dst = reinterpret_cast<float*>(&out[gr * ld + gc]);
+            unsafeAtomicAdd(dst, v.x);
+            unsafeAtomicAdd(dst + 1, v.y);
+        }
     }
     if constexpr (RG) {
         // the wavefronts' corner sums into the row tile's vertex slots (LDS),
@@ -1653,6 +1825,22 @@ __global__ __launch_bounds__(256) void p1_permute_kernel(const float2* __restric
     for (int64_t d = threadIdx.x; d < n; d += 256) dst[d] = src[vnew[d]];
 }
 
+// The same move from a planar scratch: per z, the real parts [n, n], then
+// the imaginary parts [n, n].
+__global__ __launch_bounds__(256) void p1_permute_planar_kernel(const float* __restrict__ S, int64_t n,
+                                                                const int32_t* __restrict__ vnew,
+                                                                float2* __restrict__ out, int64_t ld, int64_t zstride)
+{
+    const int64_t c = blockIdx.x;
+    const float* re = S + (int64_t)blockIdx.y * 2 * n * n + (int64_t)vnew[c] * n;
+    const float* im = re + n * n;
+    float2* dst = out + (int64_t)blockIdx.y * zstride + c * ld;
+    for (int64_t d = threadIdx.x; d < n; d += 256) {
+        const int64_t q = vnew[d];
+        dst[d] = make_float2(re[q], im[q]);
+    }
+}
+
 // Half sweeps where they pay (single layer and hypersingular: symmetric;
 // double layer and its adjoint with the 6-point rule on meshes of at least
 // kP1PairMinVerts vertices: paired, with an [nv, nv] scratch matrix from the
@@ -1666,8 +1854,11 @@ constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
 static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float2* out, int64_t ld, bool one_sided,
                                const int32_t* vmap, hipStream_t s, const float2* rg_g = nullptr,
-                               float2* rg_out = nullptr)
+                               float2* rg_out = nullptr, bool* planar = nullptr)
 {
+    // planar (an [n, n] scratch per z, ld == n): the every-ordered-pair
+    // sweeps flush real and imaginary planes; *planar says whether they did
+    if (planar) *planar = false;
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
     const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
     constexpr bool sym = OP == SLP || OP == HYP;
@@ -1676,7 +1867,7 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
         if constexpr (sym) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym, 1, nz),
                                dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap,
-                               nullptr, nullptr);
+                               nullptr, nullptr, (int64_t)0);
             hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2), 1, nz), dim3(256), 0, s, out,
                                n, ld, kb.zstride);
             return hipGetLastError();
@@ -1687,7 +1878,7 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
                 if (e == hipSuccess) {
                     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
                                        dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n, vmap,
-                                       nullptr, nullptr);
+                                       nullptr, nullptr, (int64_t)0);
                     hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
                                        (const float2*)t, n, nb);
                     e = hipGetLastError();
@@ -1699,15 +1890,17 @@ static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float
         }
     }
     dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles, (unsigned)nz);
+    const int64_t plane = planar ? n * n : 0;
+    if (planar) *planar = true;
     if constexpr (OP == BM_LHS) {
         if (rg_out) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAllRg>), grid, dim3(256), 0, s, rec, m.tris,
-                               m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, rg_g, rg_out);
+                               m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, rg_g, rg_out, plane);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, nullptr, nullptr);
+                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, nullptr, nullptr, plane);
     return hipGetLastError();
 }
 
@@ -1811,29 +2004,37 @@ static hipError_t assemble_op(const MeshDev& m, int basis, int quad, int parts, 
         // are in P1 order, so the mapped addresses are scattered).
         const bool one_sided = (parts & 4) != 0;
         const int64_t n = m.nv;
-        auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap) -> hipError_t {
+        auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap,
+                           bool* planar) -> hipError_t {
             hipError_t er = launch_fill_z(dst, n, n, ldd, kbd.zstride, nz, make_float2(0.f, 0.f), s);
             if (er != hipSuccess) return er;
-            if (quad == 0) return p1_regular_t<OP, 6, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
-            if (quad == 1) return p1_regular_t<OP, 3, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
-            return p1_regular_t<OP, 1, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s);
+            if (quad == 0) return p1_regular_t<OP, 6, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s, nullptr,
+                                                             nullptr, planar);
+            if (quad == 1) return p1_regular_t<OP, 3, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s, nullptr,
+                                                             nullptr, planar);
+            return p1_regular_t<OP, 1, CLAMP>(m, kbd, nz, dst, ldd, one_sided, vmap, s, nullptr, nullptr, planar);
         };
         float2* S = nullptr;
         if (!(parts & 8) &&
             hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n * (size_t)nz, s) == hipSuccess) {
             KBatch kbs = kb;
             kbs.zstride = n * n;
-            e = regular(S, n, kbs, nullptr);
+            bool planar = false;
+            e = regular(S, n, kbs, nullptr, &planar);
             if (e == hipSuccess) {
-                hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n,
-                                   m.p1_vnew, out, ld, kb.zstride);
+                if (planar)
+                    hipLaunchKernelGGL(p1_permute_planar_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s,
+                                       (const float*)S, n, m.p1_vnew, out, ld, kb.zstride);
+                else
+                    hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n,
+                                       m.p1_vnew, out, ld, kb.zstride);
                 e = hipGetLastError();
             }
             const hipError_t ef = hipFreeAsync(S, s);
             if (e == hipSuccess) e = ef;
         } else {
             (void)hipGetLastError();  // no room for the scratch, or none asked for (parts & 8)
-            e = regular(out, ld, kb, m.p1_vold);
+            e = regular(out, ld, kb, m.p1_vold, nullptr);
         }
         if (e != hipSuccess) return e;
     }
@@ -1989,28 +2190,34 @@ static hipError_t bm_system_p1_t(const MeshDev& m, int64_t gstride, const KBatch
     float2* part = nullptr;
     e = hipMallocAsync((void**)&part, sizeof(float2) * (size_t)m.tiles.ncol_tiles * (size_t)nslots * (size_t)nz, s);
     if (e != hipSuccess) return e;
-    auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap) -> hipError_t {
+    auto regular = [&](float2* dst, int64_t ldd, const KBatch& kbd, const int32_t* vmap,
+                       bool* planar) -> hipError_t {
         hipError_t er = launch_fill_z(dst, n, n, ldd, kbd.zstride, nz, make_float2(0.f, 0.f), s);
         if (er != hipSuccess) return er;
-        if (quad == 0) return p1_regular_t<BM_LHS, 6, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
-        if (quad == 1) return p1_regular_t<BM_LHS, 3, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
-        return p1_regular_t<BM_LHS, 1, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part);
+        if (quad == 0) return p1_regular_t<BM_LHS, 6, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part, planar);
+        if (quad == 1) return p1_regular_t<BM_LHS, 3, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part, planar);
+        return p1_regular_t<BM_LHS, 1, CLAMP>(m, kbd, nz, dst, ldd, true, vmap, s, g, part, planar);
     };
     float2* S = nullptr;
     if (hipMallocAsync((void**)&S, sizeof(float2) * (size_t)n * (size_t)n * (size_t)nz, s) == hipSuccess) {
         KBatch kbs = kb;
         kbs.zstride = n * n;
-        e = regular(S, n, kbs, nullptr);
+        bool planar = false;
+        e = regular(S, n, kbs, nullptr, &planar);
         if (e == hipSuccess) {
-            hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n, m.p1_vnew,
-                               lhs, ld, lstride);
+            if (planar)
+                hipLaunchKernelGGL(p1_permute_planar_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s,
+                                   (const float*)S, n, m.p1_vnew, lhs, ld, lstride);
+            else
+                hipLaunchKernelGGL(p1_permute_kernel, dim3((unsigned)n, (unsigned)nz), dim3(256), 0, s, S, n,
+                                   m.p1_vnew, lhs, ld, lstride);
             e = hipGetLastError();
         }
         const hipError_t ef = hipFreeAsync(S, s);
         if (e == hipSuccess) e = ef;
     } else {
         (void)hipGetLastError();  // no room for the scratch: flush through the vertex map
-        e = regular(lhs, ld, kb, m.p1_vold);
+        e = regular(lhs, ld, kb, m.p1_vold, nullptr);
     }
     if (e == hipSuccess) {
         hipLaunchKernelGGL(p1_rg_reduce_kernel, dim3(grid1d(nslots), (unsigned)nz), dim3(256), 0, s,
