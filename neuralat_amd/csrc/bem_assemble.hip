@@ -496,13 +496,13 @@ __global__ __launch_bounds__(256) void cbie_finalize_kernel(const float2* __rest
 // Gauss-point pair's exp(ikr)(1 - ikr)/r^3 serves both blocks of the
 // unordered pair: B(i, j) with the normal factor (x - y).n_y and B(j, i)^T
 // with (y - x).n_x.  B(i, j) goes through the LDS accumulator to out;
-// B(j, i)^T through the same stage and owner gather into 20 registers per
-// lane (one per row-tile vertex slot: the LDS has no room for a second
-// accumulator at 3 workgroups per CU; each slot is updated by a select on the
-// wave-uniform slot numbers, since a register array indexed by them would be
-// placed in scratch memory) and is flushed to the scratch matrix outT at
-// (row vertices, column vertices), as B(i, j) is; p1_add_transpose_kernel
-// then adds outT^T to out.  The adjoint double layer is the transpose of the
+// B(j, i)^T through the same stage and owner gather into 2 x 20 registers
+// per lane (one per row-tile vertex slot, real and imaginary: the LDS has no
+// room for a second accumulator at 3 workgroups per CU; the slot numbers are
+// the same on every lane, so the registers are addressed by them through
+// indexed register moves); after B(i, j) is flushed, the LDS accumulator takes
+// B(j, i)^T and flushes it at the transposed positions (column vertices, row
+// vertices) of out.  The adjoint double layer is the transpose of the
 // double layer, so it evaluates the double layer and swaps the two blocks.
 // kSweepAllRg: every ordered pair of the Burton-Miller LHS, with the
 // Burton-Miller RHS contracted with a vertex vector g in the same pass
@@ -754,6 +754,31 @@ __device__ __forceinline__ void p1_pair_block(const float* __restrict__ rr, cons
 // with R, C the row and column sums of u and S its total: 28 additions and 36
 // fused multiply-adds per pair instead of 117 basis-weighted accumulations.
 // w = the column weight / 4 pi (all three equal), kw = k w.  RG as above.
+template <int Q3>
+__device__ __forceinline__ void bm3_block(const float (&uR)[Q3][Q3], const float (&uI)[Q3][Q3], float ua, float* br,
+                                          float* bi)
+{
+    const float f = ua * (1.0f / 36.0f), f3 = 3.0f * f, f9 = 9.0f * f;
+    float Rr[Q3], Ri[Q3], Cr[Q3], Ci[Q3];
+#pragma unroll
+    for (int a = 0; a < Q3; ++a) {
+        Rr[a] = uR[a][0] + uR[a][1] + uR[a][2];
+        Ri[a] = uI[a][0] + uI[a][1] + uI[a][2];
+        Cr[a] = uR[0][a] + uR[1][a] + uR[2][a];
+        Ci[a] = uI[0][a] + uI[1][a] + uI[2][a];
+    }
+    const float Sr = Rr[0] + Rr[1] + Rr[2], Si = Ri[0] + Ri[1] + Ri[2];
+#pragma unroll
+    for (int c = 0; c < Q3; ++c) {
+        const float pr = fmaf(f3, Rr[c], f * Sr), pi = fmaf(f3, Ri[c], f * Si);
+#pragma unroll
+        for (int d = 0; d < Q3; ++d) {
+            br[3 * c + d] = fmaf(f9, uR[c][d], fmaf(f3, Cr[d], pr));
+            bi[3 * c + d] = fmaf(f9, uI[c][d], fmaf(f3, Ci[d], pi));
+        }
+    }
+}
+
 template <int OP, bool CLAMP, bool RG>
 __device__ __forceinline__ void p1_pair_block_bm3(const float* __restrict__ rr, const float* __restrict__ rci,
                                                   const float* py, float w, float kw, float ny0, float ny1, float ny2,
@@ -788,6 +813,7 @@ __device__ __forceinline__ void p1_pair_block_bm3(const float* __restrict__ rr, 
 #pragma unroll
     for (int a = 0; a < Q; ++a) {
         const float xa0 = rr[3 * a], xa1 = rr[3 * a + 1], xa2 = rr[3 * a + 2];
+        // -w (x_a - y_0).n_y and k w (x_a - y_0).n_y
         float mca = 0.f, kca = 0.f;
         if constexpr (OP == BM_LHS) {
             const float ca = (xa0 - py[0]) * ny0 + (xa1 - py[1]) * ny1 + (xa2 - py[2]) * ny2;
@@ -805,16 +831,17 @@ __device__ __forceinline__ void p1_pair_block_bm3(const float* __restrict__ rr, 
             float sn, cs;
             sincos_rev(r * krev, sn, cs);
             const float gr = cs * rinv, gi = sn * rinv;  // exp(ikr)/r
+            const float rinv2 = rinv * rinv;
             float er, ei;
             if constexpr (OP == BM_LHS) {
                 // w [(1 - i k r)(y - x).n_y / r^2 - i k n_x.n_y]
-                er = mca * rinv * rinv;
+                er = mca * rinv2;
                 ei = fmaf(kca, rinv, mknn);
                 gsr += gr;
                 gsi += gi;
                 if constexpr (RG) {
                     // RHS: (x - y).n_x / r - 1 + i (x - y).n_x / (k r^2), times g(y_b)
-                    const float e2r = fmaf(qn[b], rinv, -1.0f), e2i = qn[b] * invk * rinv * rinv;
+                    const float e2r = fmaf(qn[b], rinv, -1.0f), e2i = qn[b] * invk * rinv2;
                     const float v2r = gr * e2r - gi * e2i, v2i = gr * e2i + gi * e2r;
                     tr_[a] = fmaf(v2r, ghat[2 * b], fmaf(-v2i, ghat[2 * b + 1], tr_[a]));
                     ti_[a] = fmaf(v2r, ghat[2 * b + 1], fmaf(v2i, ghat[2 * b], ti_[a]));
@@ -822,32 +849,14 @@ __device__ __forceinline__ void p1_pair_block_bm3(const float* __restrict__ rr, 
             } else {
                 // w [-1 + (i/k)(1 - i k r)(x - y).n_x / r^2]
                 er = fmaf(qw[b], rinv, -w);
-                ei = qkw[b] * rinv * rinv;
+                ei = qkw[b] * rinv2;
             }
             uR[a][b] = gr * er - gi * ei;
             uI[a][b] = gr * ei + gi * er;
         }
     }
     const float ua = rr[3 * Q];  // the row weight (all three equal)
-    const float f = ua * (1.0f / 36.0f), f3 = 3.0f * f, f9 = 9.0f * f;
-    float Rr[Q], Ri[Q], Cr[Q], Ci[Q];
-#pragma unroll
-    for (int a = 0; a < Q; ++a) {
-        Rr[a] = uR[a][0] + uR[a][1] + uR[a][2];
-        Ri[a] = uI[a][0] + uI[a][1] + uI[a][2];
-        Cr[a] = uR[0][a] + uR[1][a] + uR[2][a];
-        Ci[a] = uI[0][a] + uI[1][a] + uI[2][a];
-    }
-    const float Sr = Rr[0] + Rr[1] + Rr[2], Si = Ri[0] + Ri[1] + Ri[2];
-#pragma unroll
-    for (int c = 0; c < Q; ++c) {
-        const float pr = fmaf(f3, Rr[c], f * Sr), pi = fmaf(f3, Ri[c], f * Si);
-#pragma unroll
-        for (int d = 0; d < Q; ++d) {
-            br[3 * c + d] = fmaf(f9, uR[c][d], fmaf(f3, Cr[d], pr));
-            bi[3 * c + d] = fmaf(f9, uI[c][d], fmaf(f3, Ci[d], pi));
-        }
-    }
+    bm3_block<Q>(uR, uI, ua, br, bi);
     if constexpr (OP == BM_LHS) {
         // (i/k) w ua (sum of G) times curl.curl
         const float sc = invk * w * ua;
@@ -890,7 +899,6 @@ template <int OP, int Q, bool CLAMP, int SWEEP>
 __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict__ rec, const int4* __restrict__ tris,
                                                          const float* __restrict__ curls, P1Tiles tl,
                                                          const KBatch kb, float2* __restrict__ out, int64_t ld,
-                                                         float2* __restrict__ outT, int64_t ldT,
                                                          const int32_t* __restrict__ vmap,
                                                          const float2* __restrict__ rg_g,
                                                          float2* __restrict__ rg_out, int64_t plane)
@@ -1037,9 +1045,10 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             s2.x += e2.x; s2.y += e2.y;
         }
     };
-    float2 accT[PAIR ? kP1RowVerts : 1];  // PAIR: this lane's column of B(j, i)^T, by row slot
+    // PAIR: this lane's column of B(j, i)^T by row slot, real and imaginary
+    float accTr[PAIR ? kP1RowVerts : 1], accTi[PAIR ? kP1RowVerts : 1];
 #pragma unroll
-    for (int q = 0; q < (PAIR ? kP1RowVerts : 1); ++q) accT[q] = make_float2(0.f, 0.f);
+    for (int q = 0; q < (PAIR ? kP1RowVerts : 1); ++q) accTr[q] = accTi[q] = 0.f;
     __syncthreads();
 
     for (int rp = r0; rp < r1; ++rp) {
@@ -1106,13 +1115,13 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
             const int4 rs = rslot[rp - r0];
             float2 s0 = make_float2(0.f, 0.f), s1 = s0, s2 = s0;
             if (owner) gather(s0, s1, s2);
-#pragma unroll
-            for (int q = 0; q < kP1RowVerts; ++q) {
-                const float2 z = make_float2(0.f, 0.f);
-                const float2 add = q == rs.x ? s0 : q == rs.y ? s1 : q == rs.z ? s2 : z;
-                accT[q].x += add.x;
-                accT[q].y += add.y;
-            }
+            // the slots are the same on every lane: indexed register moves
+            // (a select per slot costs 8 vector instructions x 20 slots)
+            const int sx = __builtin_amdgcn_readfirstlane(rs.x), sy = __builtin_amdgcn_readfirstlane(rs.y),
+                      sz = __builtin_amdgcn_readfirstlane(rs.z);
+            accTr[sx] += s0.x; accTi[sx] += s0.y;
+            accTr[sy] += s1.x; accTi[sy] += s1.y;
+            accTr[sz] += s2.x; accTi[sz] += s2.y;
             __syncthreads();
         }
     }
@@ -1154,17 +1163,33 @@ __global__ __launch_bounds__(256) void p1_regular_kernel(const float* __restrict
         if ((int)threadIdx.x < nrv) rg_out[(int64_t)ct * nslots + rv0 + threadIdx.x] = rgslot[threadIdx.x];
     }
     if constexpr (PAIR) {
+        // B(j, i)^T lands at the transposed positions of out: the owners'
+        // registers through the LDS accumulator (free once flushed), then a
+        // flush with consecutive lanes along the row-tile slots of one column
+        // vertex (a few cache lines per wavefront in the tiles' numbering)
+        __syncthreads();
         if (owner) {
-            const int64_t gc = vmap ? vmap[tl.col_verts[cv0 + w]] : tl.col_verts[cv0 + w];
 #pragma unroll
-            for (int q = 0; q < kP1RowVerts; ++q) {
-                if (q < nrv) {
-                    const int64_t gr = vmap ? vmap[tl.row_verts[rv0 + q]] : tl.row_verts[rv0 + q];
-                    float* dst = reinterpret_cast<float*>(&outT[gr * ldT + gc]);
-                    const float2 v = accT[q];
-                    unsafeAtomicAdd(dst, v.x);
-                    unsafeAtomicAdd(dst + 1, v.y);
-                }
+            for (int q = 0; q < kP1RowVerts; ++q)
+                if (q < nrv) acc[q][w] = make_float2(accTr[q], accTi[q]);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nrv * ncv; e += CT) {
+            const int cvi = e / nrv, rvi = e - cvi * nrv;
+            int64_t gr = tl.row_verts[rv0 + rvi], gc = tl.col_verts[cv0 + cvi];
+            if (vmap) {
+                gr = vmap[gr];
+                gc = vmap[gc];
+            }
+            const float2 v = acc[rvi][cvi];
+            if (plane) {
+                float* dst = reinterpret_cast<float*>(out) + (gc * ld + gr);
+                unsafeAtomicAdd(dst, v.x);
+                unsafeAtomicAdd(dst + plane, v.y);
+            } else {
+                float* dst = reinterpret_cast<float*>(&out[gc * ld + gr]);
+                unsafeAtomicAdd(dst, v.x);
+                unsafeAtomicAdd(dst + 1, v.y);
             }
         }
     }
@@ -1198,32 +1223,6 @@ __global__ __launch_bounds__(256) void p1_symmetrize_kernel(float2* __restrict__
         if (I != J) {
             const float2 wb = b[rr][tx], wa = a[tx][rr];  // B(rr, tx) + A(tx, rr)
             if (rb < n && cb < n) out[rb * ld + cb] = make_float2(wb.x + wa.x, wb.y + wa.y);
-        }
-    }
-}
-
-// out += t^T on an n x n matrix (t: [n, n] with leading dimension n).  One
-// workgroup per 32 x 32 tile of out; the matching tile of t through LDS.
-__global__ __launch_bounds__(256) void p1_add_transpose_kernel(float2* __restrict__ out, int64_t ld,
-                                                               const float2* __restrict__ t, int64_t n, int64_t nb)
-{
-    __shared__ float2 b[kSymT][kSymT + 1];
-    const int64_t I = (int64_t)blockIdx.x / nb, J = (int64_t)blockIdx.x - I * nb;
-    const int64_t r0 = I * kSymT, c0 = J * kSymT;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 8 rows per pass
-    for (int rr = ty; rr < kSymT; rr += 8) {  // b(rr, tx) = t(c0 + rr, r0 + tx)
-        const int64_t tr = c0 + rr, tc = r0 + tx;
-        b[rr][tx] = (tr < n && tc < n) ? t[tr * n + tc] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    for (int rr = ty; rr < kSymT; rr += 8) {  // out(r0 + rr, c0 + tx) += t(c0 + tx, r0 + rr)
-        const int64_t ra = r0 + rr, ca = c0 + tx;
-        if (ra < n && ca < n) {
-            const float2 v = b[tx][rr];
-            float2 o = out[ra * ld + ca];
-            o.x += v.x;
-            o.y += v.y;
-            out[ra * ld + ca] = o;
         }
     }
 }
@@ -1843,64 +1842,54 @@ __global__ __launch_bounds__(256) void p1_permute_planar_kernel(const float* __r
 
 // Half sweeps where they pay (single layer and hypersingular: symmetric;
 // double layer and its adjoint with the 6-point rule on meshes of at least
-// kP1PairMinVerts vertices: paired, with an [nv, nv] scratch matrix from the
-// stream-ordered pool -- with 9 Gauss-point pairs per triangle pair the
-// second stage pass costs more than the halved evaluations save, and on
-// smaller meshes the halved grid under-fills the GPU while the scratch fill
-// and transpose pass stay); one_sided, or no room for the scratch matrix,
-// evaluates every ordered pair.  A batch of wavenumbers (nz > 1) is not
-// paired (it would need one scratch matrix per wavenumber).
+// kP1PairMinVerts vertices: paired -- on smaller meshes the halved grid
+// under-fills the GPU; with the 3-point rule the per-pair exchange and the
+// transposed flush, whose destination rows take 20 values each, cost more
+// than the halved evaluations save); one_sided evaluates every ordered pair.
 constexpr int64_t kP1PairMinVerts = 4096;
 template <int OP, int Q, bool CLAMP>
 static hipError_t p1_regular_t(const MeshDev& m, const KBatch& kb, int nz, float2* out, int64_t ld, bool one_sided,
                                const int32_t* vmap, hipStream_t s, const float2* rg_g = nullptr,
                                float2* rg_out = nullptr, bool* planar = nullptr)
 {
-    // planar (an [n, n] scratch per z, ld == n): the every-ordered-pair
-    // sweeps flush real and imaginary planes; *planar says whether they did
+    // planar (an [n, n] scratch per z, ld == n): the every-ordered-pair and
+    // paired sweeps flush real and imaginary planes; *planar says whether the
+    // sweep taken did
     if (planar) *planar = false;
     const float* rec = (Q == 6) ? m.rec6 : (Q == 3) ? m.rec3 : m.rec1;
     const int64_t n = m.nv, nb = (n + kSymT - 1) / kSymT;
     constexpr bool sym = OP == SLP || OP == HYP;
     constexpr bool pair = (OP == DLP || OP == ADLP) && Q == 6;
+    const int64_t plane = planar ? n * n : 0;
     if (!one_sided && m.tiles.nsym <= 0x7fffffffLL && nb * nb <= 0x7fffffffLL) {
         if constexpr (sym) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepSym>), dim3((unsigned)m.tiles.nsym, 1, nz),
-                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap,
-                               nullptr, nullptr, (int64_t)0);
+                               dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, vmap, nullptr, nullptr,
+                               (int64_t)0);
             hipLaunchKernelGGL(p1_symmetrize_kernel, dim3((unsigned)(nb * (nb + 1) / 2), 1, nz), dim3(256), 0, s, out,
                                n, ld, kb.zstride);
             return hipGetLastError();
         } else if constexpr (pair) {
-            float2* t = nullptr;
-            if (nz == 1 && n >= kP1PairMinVerts && hipMallocAsync((void**)&t, sizeof(float2) * n * n, s) == hipSuccess) {
-                hipError_t e = launch_fill(t, n, n, n, make_float2(0.f, 0.f), s);
-                if (e == hipSuccess) {
-                    hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym),
-                                       dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, t, n, vmap,
-                                       nullptr, nullptr, (int64_t)0);
-                    hipLaunchKernelGGL(p1_add_transpose_kernel, dim3((unsigned)(nb * nb)), dim3(256), 0, s, out, ld,
-                                       (const float2*)t, n, nb);
-                    e = hipGetLastError();
-                }
-                const hipError_t ef = hipFreeAsync(t, s);
-                return e != hipSuccess ? e : ef;
+            if (n >= kP1PairMinVerts) {
+                if (planar) *planar = true;
+                hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepPair>), dim3((unsigned)m.tiles.nsym, 1, nz),
+                                   dim3(256), 0, s, rec, m.tris, m.curls, m.tiles, kb, out, ld, vmap, nullptr, nullptr,
+                                   plane);
+                return hipGetLastError();
             }
-            (void)hipGetLastError();  // no room for the scratch, or none asked for (parts & 8) matrix
         }
     }
     dim3 grid((unsigned)m.tiles.ncol_tiles, (unsigned)m.tiles.nrow_tiles, (unsigned)nz);
-    const int64_t plane = planar ? n * n : 0;
     if (planar) *planar = true;
     if constexpr (OP == BM_LHS) {
         if (rg_out) {
             hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAllRg>), grid, dim3(256), 0, s, rec, m.tris,
-                               m.curls, m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, rg_g, rg_out, plane);
+                               m.curls, m.tiles, kb, out, ld, vmap, rg_g, rg_out, plane);
             return hipGetLastError();
         }
     }
     hipLaunchKernelGGL((p1_regular_kernel<OP, Q, CLAMP, kSweepAll>), grid, dim3(256), 0, s, rec, m.tris, m.curls,
-                       m.tiles, kb, out, ld, nullptr, (int64_t)0, vmap, nullptr, nullptr, plane);
+                       m.tiles, kb, out, ld, vmap, nullptr, nullptr, plane);
     return hipGetLastError();
 }
 
