@@ -340,89 +340,43 @@ __global__ __launch_bounds__(256) void adj_singular_kernel(const float* __restri
 #pragma unroll
     for (int e = 0; e < 25; ++e) acc[e] = 0.0;
     constexpr int NP = NL * NL * NL * NL;
-    for (int q = lane; q < NP; q += 64) {
-        const int q0 = q / (NL * NL * NL), q1 = (q / (NL * NL)) % NL, q2 = (q / NL) % NL, q3 = q % NL;
-        const float xs = LineRule01<NL>::t[q0], e_1 = LineRule01<NL>::t[q1];
-        const float e_2 = LineRule01<NL>::t[q2], e_3 = LineRule01<NL>::t[q3];
-        const float wq = LineRule01<NL>::w[q0] * LineRule01<NL>::w[q1] * LineRule01<NL>::w[q2] * LineRule01<NL>::w[q3];
-        const float e12 = e_1 * e_2, e123 = e12 * e_3;
-        float reg[6][7];
-        int nreg;
-        float jac;
-        if (shared == 3) {
-            jac = xs * xs * xs * e_1 * e_1 * e_2;
-            nreg = 6;
-            const float e12c = e12 * (1.f - e_3);
-            const float t[6][7] = {
-                {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1), 1.f, -xs * e123, -xs * e12},
-                {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12), 1.f, xs * e123, xs * e12},
-                {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12), 1.f, -xs * e12, -xs * e123},
-                {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123), 1.f, xs * e12, xs * e123},
-                {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12), 1.f, xs * e123, -xs * e12c},
-                {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123), 1.f, -xs * e123, xs * e12c},
-            };
-            for (int r = 0; r < 6; ++r)
-                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-        } else if (shared == 2) {
-            jac = xs * xs * xs * e_1 * e_1;
-            nreg = 5;
-            const float t[5][7] = {
-                {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f, -xs * e12, 0.f},
-                {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2, -xs * e123, 0.f},
-                {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2, xs * e12, 0.f},
-                {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2, xs * e123, 0.f},
-                {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2, xs * e123, 0.f},
-            };
-            for (int r = 0; r < 5; ++r)
-                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-        } else {
-            jac = xs * xs * xs * e_2;
-            nreg = 2;
-            const float t[2][7] = {
-                {xs, xs * e_1, xs * e_2, xs * e_2 * e_3, 1.f, 0.f, 0.f},
-                {xs * e_2, xs * e_2 * e_3, xs, xs * e_1, 1.f, 0.f, 0.f},
-            };
-            for (int r = 0; r < 2; ++r)
-                for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-        }
-        const float W = jac * wq * jij;
-        for (int r = 0; r < nreg; ++r) {
-            const float x1 = reg[r][0], x2 = reg[r][1], y1 = reg[r][2], y2 = reg[r][3];
-            const float d1 = reg[r][5], d2 = reg[r][6];
-            float dx, dy, dz;
-            if (shared == 3) {
-                dx = d1 * e1[0] + d2 * e2[0];
-                dy = d1 * e1[1] + d2 * e2[1];
-                dz = d1 * e1[2] + d2 * e2[2];
-            } else if (shared == 2) {
-                dx = d1 * e1[0] + y2 * f2[0] - x2 * e2[0];
-                dy = d1 * e1[1] + y2 * f2[1] - x2 * e2[1];
-                dz = d1 * e1[2] + y2 * f2[2] - x2 * e2[2];
-            } else {
-                dx = y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
-                dy = y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
-                dz = y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
+    ss_dispatch(shared, [&](auto kind) {
+        constexpr int S = decltype(kind)::value;
+        constexpr int NR = kSSRegions<S>;
+        for (int q = lane; q < NP; q += 64) {
+            const int q0 = q / (NL * NL * NL), q1 = (q / (NL * NL)) % NL, q2 = (q / NL) % NL, q3 = q % NL;
+            const float wq =
+                LineRule01<NL>::w[q0] * LineRule01<NL>::w[q1] * LineRule01<NL>::w[q2] * LineRule01<NL>::w[q3];
+            float reg[NR][7];
+            const float jac = ss_regions<S>(LineRule01<NL>::t[q0], LineRule01<NL>::t[q1], LineRule01<NL>::t[q2],
+                                            LineRule01<NL>::t[q3], reg);
+            const float W = jac * wq * jij;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                const float x1 = reg[r][0], x2 = reg[r][1], y1 = reg[r][2], y2 = reg[r][3];
+                float dx, dy, dz;
+                ss_disp<S>(reg[r], e1, e2, f1, f2, dx, dy, dz);
+                AdjEval o;
+                adj_eval<CLAMP>(dx, dy, dz, ny0, ny1, ny2, k, krev, ar, ai, br, bi, o);
+                const float ww = W * reg[r][4];
+                const float px[3] = {1.f - x1, x1 - x2, x2};
+                const float py[3] = {1.f - y1, y1 - y2, y2};
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        acc[3 * a + c] -= (double)(ww * px[a] * o.gd[c]);
+                        acc[9 + 3 * a + c] += (double)(ww * py[a] * o.gd[c]);
+                    }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[18 + c] += (double)(ww * o.gn[c]);
+                acc[21] += (double)(ww * o.e);
+                acc[22] += (double)(ww * o.gk);
+                acc[23] += (double)(ww * o.gr);
+                acc[24] += (double)(ww * o.gi);
             }
-            AdjEval o;
-            adj_eval<CLAMP>(dx, dy, dz, ny0, ny1, ny2, k, krev, ar, ai, br, bi, o);
-            const float ww = W * reg[r][4];
-            const float px[3] = {1.f - x1, x1 - x2, x2};
-            const float py[3] = {1.f - y1, y1 - y2, y2};
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    acc[3 * a + c] -= (double)(ww * px[a] * o.gd[c]);
-                    acc[9 + 3 * a + c] += (double)(ww * py[a] * o.gd[c]);
-                }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[18 + c] += (double)(ww * o.gn[c]);
-            acc[21] += (double)(ww * o.e);
-            acc[22] += (double)(ww * o.gk);
-            acc[23] += (double)(ww * o.gr);
-            acc[24] += (double)(ww * o.gi);
         }
-    }
+    });
 #pragma unroll
     for (int e = 0; e < 25; ++e) acc[e] = wsum_d(acc[e]);
     if (lane != 0) return;

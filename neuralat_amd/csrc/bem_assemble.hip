@@ -13,8 +13,6 @@
 // Neighbour pairs (sharing >= 1 vertex; bind.cu:81-152, integrand.h:63-185,
 // 331-394) go to a separate Sauter-Schwab kernel: one wavefront per pair,
 // the n^4 tensor Gauss points spread over the lanes, FP64 accumulation.
-#include <type_traits>
-
 #include "bem_device.hpp"
 #include "bem_internal.hpp"
 
@@ -1333,88 +1331,25 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     for (int e = 0; e < 10; ++e) ar[e] = ai[e] = 0.0;
 
     constexpr int NP = NL * NL * NL * NL;
-    // The pair kind (shared vertices: 3 coincident, 2 edge, 1 vertex) picks a
-    // fully unrolled region loop, so the region table stays in registers (a
-    // table indexed by a run-time region count was placed in scratch memory).
-    auto accumulate = [&](auto kind) {
+    // The pair kind (shared vertices) picks a fully unrolled region loop.
+    ss_dispatch(shared, [&](auto kind) {
         constexpr int S = decltype(kind)::value;
-        constexpr int NR = S == 3 ? 6 : (S == 2 ? 5 : 2);
+        constexpr int NR = kSSRegions<S>;
         for (int q = ll; q < NP; q += LPP) {
             const int q0 = q / (NL * NL * NL), q1 = (q / (NL * NL)) % NL, q2 = (q / NL) % NL, q3 = q % NL;
-            const float xs = LineRule01<NL>::t[q0], e_1 = LineRule01<NL>::t[q1];
-            const float e_2 = LineRule01<NL>::t[q2], e_3 = LineRule01<NL>::t[q3];
             const float wq =
                 LineRule01<NL>::w[q0] * LineRule01<NL>::w[q1] * LineRule01<NL>::w[q2] * LineRule01<NL>::w[q3];
-            const float e12 = e_1 * e_2, e123 = e12 * e_3;
-            // Region table rows: (x1, x2, y1, y2, extra weight, d1, d2) with
-            // (d1, d2) = (y1 - x1, y2 - x2) formed from the eta products
-            // directly (coincident, edge: d1 only), which keeps y - x accurate
-            // where the two points nearly coincide.
-            float jac;
             float reg[NR][7];
-            if constexpr (S == 3) {
-                jac = xs * xs * xs * e_1 * e_1 * e_2;
-                const float e12c = e12 * (1.f - e_3);  // eta1 eta2 - eta1 eta2 eta3
-                const float t[6][7] = {
-                    {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1), 1.f, -xs * e123, -xs * e12},
-                    {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12), 1.f, xs * e123, xs * e12},
-                    {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12), 1.f, -xs * e12, -xs * e123},
-                    {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123), 1.f, xs * e12, xs * e123},
-                    {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12), 1.f, xs * e123, -xs * e12c},
-                    {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123), 1.f, -xs * e123, xs * e12c},
-                };
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-#pragma unroll
-                    for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-            } else if constexpr (S == 2) {
-                jac = xs * xs * xs * e_1 * e_1;
-                const float t[5][7] = {
-                    {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f, -xs * e12, 0.f},
-                    {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2, -xs * e123, 0.f},
-                    {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2, xs * e12, 0.f},
-                    {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2, xs * e123, 0.f},
-                    {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2, xs * e123, 0.f},
-                };
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-#pragma unroll
-                    for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-            } else {
-                jac = xs * xs * xs * e_2;
-                const float t[2][7] = {
-                    {xs, xs * e_1, xs * e_2, xs * e_2 * e_3, 1.f, 0.f, 0.f},
-                    {xs * e_2, xs * e_2 * e_3, xs, xs * e_1, 1.f, 0.f, 0.f},
-                };
-#pragma unroll
-                for (int r = 0; r < NR; ++r)
-#pragma unroll
-                    for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
-            }
+            const float jac = ss_regions<S>(LineRule01<NL>::t[q0], LineRule01<NL>::t[q1], LineRule01<NL>::t[q2],
+                                            LineRule01<NL>::t[q3], reg);
             float br[10], bi[10];
 #pragma unroll
             for (int e = 0; e < 10; ++e) br[e] = bi[e] = 0.f;
 #pragma unroll
             for (int r = 0; r < NR; ++r) {
                 const float x1 = reg[r][0], x2 = reg[r][1], y1 = reg[r][2], y2 = reg[r][3];
-                const float d1 = reg[r][5], d2 = reg[r][6];
                 float dx, dy, dz;
-                if constexpr (S == 3) {
-                    // same triangle: y - x = d1 e1 + d2 e2
-                    dx = d1 * e1[0] + d2 * e2[0];
-                    dy = d1 * e1[1] + d2 * e2[1];
-                    dz = d1 * e1[2] + d2 * e2[2];
-                } else if constexpr (S == 2) {
-                    // common edge v0 v1 = w0 w1 (so f1 = e1): y - x = d1 e1 + y2 f2 - x2 e2
-                    dx = d1 * e1[0] + y2 * f2[0] - x2 * e2[0];
-                    dy = d1 * e1[1] + y2 * f2[1] - x2 * e2[1];
-                    dz = d1 * e1[2] + y2 * f2[2] - x2 * e2[2];
-                } else {
-                    // common vertex v0 = w0
-                    dx = y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
-                    dy = y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
-                    dz = y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
-                }
+                ss_disp<S>(reg[r], e1, e2, f1, f2, dx, dy, dz);
                 float vr, vi, gr = 0.f, gg = 0.f;
                 kernel4pi<OP, CLAMP>(dx, dy, dz, nx0, nx1, nx2, ny0, ny1, ny2, k, krev, vr, vi, gr, gg);
                 const float ww = reg[r][4];
@@ -1447,10 +1382,7 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
                 ai[e] += sc * (double)bi[e];
             }
         }
-    };
-    if (shared == 3) accumulate(std::integral_constant<int, 3>{});
-    else if (shared == 2) accumulate(std::integral_constant<int, 2>{});
-    else accumulate(std::integral_constant<int, 1>{});
+    });
 #pragma unroll
     for (int e = 0; e < 10; ++e) {
         ar[e] = group_sum<LPP>(ar[e]) * (double)kInv4Pi;

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace bem {
 
 enum Op : int { SLP = 0, DLP = 1, ADLP = 2, HYP = 3, BM_LHS = 4, BM_RHS = 5 };
@@ -180,6 +182,96 @@ __device__ __forceinline__ void curl_product(const float* ci, const float* cj, f
 #pragma unroll
         for (int b = 0; b < 3; ++b)
             cp[3 * a + b] = ci[3 * a] * cj[3 * b] + ci[3 * a + 1] * cj[3 * b + 1] + ci[3 * a + 2] * cj[3 * b + 2];
+}
+
+// ---- Sauter-Schwab regions of a neighbour pair ------------------------------
+// A pair with S shared vertices (3 coincident, 2 edge, 1 vertex; the shared
+// ones at local 0 (and 1), integrand.h:117-153) on the reference triangle
+// (0,0),(1,0),(1,1): at the tensor point (xs, e_1, e_2, e_3) of [0,1]^4 each
+// of the kSSRegions<S> regions gives a row (x1, x2, y1, y2, extra weight, d1,
+// d2) with (d1, d2) = (y1 - x1, y2 - x2) formed from the eta products directly
+// (coincident; edge: d1 only), which keeps y - x accurate where the points
+// nearly coincide; the return value is the Duffy Jacobian.  S is a template
+// argument so that callers unroll the region loop and keep the table in
+// registers (indexed by a run-time region count it lands in scratch memory).
+template <int S> constexpr int kSSRegions = S == 3 ? 6 : (S == 2 ? 5 : 2);
+
+template <int S>
+__device__ __forceinline__ float ss_regions(float xs, float e_1, float e_2, float e_3,
+                                            float (&reg)[kSSRegions<S>][7])
+{
+    const float e12 = e_1 * e_2, e123 = e12 * e_3;
+    if constexpr (S == 3) {
+        const float e12c = e12 * (1.f - e_3);  // eta1 eta2 - eta1 eta2 eta3
+        const float t[6][7] = {
+            {xs, xs * (1.f - e_1 + e12), xs * (1.f - e123), xs * (1.f - e_1), 1.f, -xs * e123, -xs * e12},
+            {xs * (1.f - e123), xs * (1.f - e_1), xs, xs * (1.f - e_1 + e12), 1.f, xs * e123, xs * e12},
+            {xs, xs * (e_1 - e12 + e123), xs * (1.f - e12), xs * (e_1 - e12), 1.f, -xs * e12, -xs * e123},
+            {xs * (1.f - e12), xs * (e_1 - e12), xs, xs * (e_1 - e12 + e123), 1.f, xs * e12, xs * e123},
+            {xs * (1.f - e123), xs * (e_1 - e123), xs, xs * (e_1 - e12), 1.f, xs * e123, -xs * e12c},
+            {xs, xs * (e_1 - e12), xs * (1.f - e123), xs * (e_1 - e123), 1.f, -xs * e123, xs * e12c},
+        };
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
+        return xs * xs * xs * e_1 * e_1 * e_2;
+    } else if constexpr (S == 2) {
+        const float t[5][7] = {
+            {xs, xs * e_1 * e_3, xs * (1.f - e12), xs * e_1 * (1.f - e_2), 1.f, -xs * e12, 0.f},
+            {xs, xs * e_1, xs * (1.f - e123), xs * e12 * (1.f - e_3), e_2, -xs * e123, 0.f},
+            {xs * (1.f - e12), xs * e_1 * (1.f - e_2), xs, xs * e123, e_2, xs * e12, 0.f},
+            {xs * (1.f - e123), xs * e12 * (1.f - e_3), xs, xs * e_1, e_2, xs * e123, 0.f},
+            {xs * (1.f - e123), xs * e_1 * (1.f - e_2 * e_3), xs, xs * e12, e_2, xs * e123, 0.f},
+        };
+#pragma unroll
+        for (int r = 0; r < 5; ++r)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
+        return xs * xs * xs * e_1 * e_1;
+    } else {
+        const float t[2][7] = {
+            {xs, xs * e_1, xs * e_2, xs * e_2 * e_3, 1.f, 0.f, 0.f},
+            {xs * e_2, xs * e_2 * e_3, xs, xs * e_1, 1.f, 0.f, 0.f},
+        };
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 7; ++c) reg[r][c] = t[r][c];
+        return xs * xs * xs * e_2;
+    }
+}
+
+// y - x for a region row, with x = v0 + x1 e1 + x2 e2 and y = w0 + y1 f1 + y2 f2
+// (e1 = v1 - v0, e2 = v2 - v1, f likewise on the column triangle); the shared
+// local vertex 0 (v0 = w0) cancels, and for an edge pair f1 = e1.
+template <int S>
+__device__ __forceinline__ void ss_disp(const float (&row)[7], const float* e1, const float* e2, const float* f1,
+                                        const float* f2, float& dx, float& dy, float& dz)
+{
+    const float x1 = row[0], x2 = row[1], y1 = row[2], y2 = row[3], d1 = row[5], d2 = row[6];
+    if constexpr (S == 3) {  // same triangle: y - x = d1 e1 + d2 e2
+        dx = d1 * e1[0] + d2 * e2[0];
+        dy = d1 * e1[1] + d2 * e2[1];
+        dz = d1 * e1[2] + d2 * e2[2];
+    } else if constexpr (S == 2) {  // common edge: y - x = d1 e1 + y2 f2 - x2 e2
+        dx = d1 * e1[0] + y2 * f2[0] - x2 * e2[0];
+        dy = d1 * e1[1] + y2 * f2[1] - x2 * e2[1];
+        dz = d1 * e1[2] + y2 * f2[2] - x2 * e2[2];
+    } else {  // common vertex
+        dx = y1 * f1[0] + y2 * f2[0] - x1 * e1[0] - x2 * e2[0];
+        dy = y1 * f1[1] + y2 * f2[1] - x1 * e1[1] - x2 * e2[1];
+        dz = y1 * f1[2] + y2 * f2[2] - x1 * e1[2] - x2 * e2[2];
+    }
+}
+
+// f(std::integral_constant<int, S>) for the pair's shared-vertex count
+template <class F>
+__device__ __forceinline__ void ss_dispatch(int shared, F&& f)
+{
+    if (shared == 3) f(std::integral_constant<int, 3>{});
+    else if (shared == 2) f(std::integral_constant<int, 2>{});
+    else f(std::integral_constant<int, 1>{});
 }
 
 }  // namespace bem
