@@ -106,10 +106,8 @@ int bem_assemble(const bem_mesh* mesh, float k, int op, int basis, int quad, int
  * pairs one evaluation per ordered triangle pair; without it each unordered
  * pair is evaluated once where that is faster (single layer and hypersingular
  * by symmetry; double layer and adjoint double layer with quad 0, on meshes of
- * at least 4096 vertices, for both orders at once, with an [nv, nv] complex
- * scratch matrix taken from the
- * stream-ordered pool and released before return; if none can be had, the
- * ordered-pair sweep runs).  Adding 8 makes the P1 regular pairs accumulate
+ * at least 4096 vertices, for both orders at once).  Adding 8 makes the P1
+ * regular pairs accumulate
  * directly in out (no [nv, nv] scratch matrix per wavenumber: the kernels then
  * flush to scattered addresses, slower).  The results agree to rounding. */
 int bem_assemble_ex(const bem_mesh* mesh, float k, int op, int basis, int quad, int clamp, int parts, void* out,
