@@ -91,6 +91,8 @@ def parse():
     p.add_argument("--c64-probe", action="store_true",
                    help="the 16-right-hand-side complex64 matrix-core product on the assembled level-6 system, "
                         "timed, then a 4 GiB copy (kernel traces and counter passes)")
+    p.add_argument("--read-probe", action="store_true",
+                   help="the GEMV beside torch.mv and a torch sum over the assembled level-6 matrix, timed")
     p.add_argument("--only-form", type=str, default="",
                    help="run one form alone and print its JSON (p1, batching, batching_small_meshes, "
                         "differentiation, differentiation_burton_miller, datagen, modal, burton_miller_level6, precision)")
@@ -1214,6 +1216,54 @@ def traffic_probe(level, k):
                       "copy_read_bytes": 4 << 30, "copy_write_bytes": 4 << 30, "y0": str(complex(y[0].item()))}))
 
 
+def read_probe(level, k, reps=6):
+    """The GEMV's bandwidth beside two library reads of the same assembled
+    level-``level`` CBIE matrix (complex64, 8 N^2 bytes), each timed with
+    events, best of ``reps`` after one warm-up: this build's GEMV
+    (``matvec``), torch's complex GEMV (``torch.mv``, the hipBLAS path that the
+    reference's ``@`` takes) and torch's full sum over the matrix as real
+    numbers (a plain read of every byte)."""
+    import torch
+
+    from neuralat_amd import mesh
+    from neuralat_amd.bem.solver import matvec
+
+    v, t, src, _ = make_workload(level)
+    s = Solver(v.astype(np.float32), t)
+    _, dn = mesh.sources_field(src, None, k)
+    psi = torch.tensor(mesh.project_dp0(v, t, lambda x, nn: dn(x, nn)).astype(np.complex64), device=s.dev)
+    s.assemble(k, psi)
+    torch.cuda.synchronize()
+    n = s.n
+    nbytes = 8 * n * n
+    A = s.lhs  # [n, n], contiguous
+
+    def best(f):
+        f()
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            f()
+            e1.record()
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        return min(ms)
+
+    out = {"rows": n, "matrix_bytes": nbytes}
+    y_ours = matvec(A, psi)
+    y_torch = torch.mv(A, psi)
+    out["torch_mv_rel_diff"] = float((torch.linalg.vector_norm(y_torch - y_ours) /
+                                      torch.linalg.vector_norm(y_torch)).item())
+    del y_ours, y_torch
+    for name, f in (("gemv", lambda: matvec(A, psi)), ("torch_mv", lambda: torch.mv(A, psi)),
+                    ("torch_sum", lambda: torch.view_as_real(A).sum())):
+        ms = best(f)
+        out[name] = {"ms": ms, "achieved_gb_s": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / (ms / 1e3) / HBM_PEAK}
+    print(json.dumps(out))
+
+
 def c64_probe(level, k, nrhs=16, reps=6):
     """Y = A X on the FP32 matrix cores for the assembled level-``level``
     CBIE matrix A (complex64) and ``nrhs`` seeded right-hand sides, ``reps``
@@ -1335,6 +1385,9 @@ def main():
         return
     if args.c64_probe:
         c64_probe(args.level, ks[-1])
+        return
+    if args.read_probe:
+        read_probe(args.level, ks[-1])
         return
     if args.batch_probe:
         batch_probe()
