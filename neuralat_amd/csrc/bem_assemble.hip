@@ -2323,6 +2323,7 @@ __global__ __launch_bounds__(256) void p1_point_gather_kernel(const float2* __re
 // Scratch for the P1 point gather, per launch: at most this many bytes
 // (points are taken in chunks that fit).
 constexpr int64_t kPotScratchBytes = (int64_t)512 << 20;
+constexpr int64_t kPotMaxPoints = (int64_t)65535 * kPotRowTile;  // points per launch (grid y <= 65535)
 
 template <int OP, int BASIS, bool CLAMP>
 static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, const KBatch& kb, int nz, float2* out,
@@ -2335,6 +2336,7 @@ static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, co
         int64_t pc = kPotScratchBytes / per_point;
         pc = pc < np ? pc : np;
         pc = pc < kPotRowTile ? kPotRowTile : pc;
+        pc = pc < kPotMaxPoints ? pc : kPotMaxPoints;
         float2* t = nullptr;
         if (np > 0 && hipMallocAsync((void**)&t, (size_t)(per_point * pc), s) == hipSuccess) {
             hipError_t e = hipSuccess;
@@ -2358,11 +2360,17 @@ static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, co
         hipError_t e = launch_fill_z(out, np, m.nv, ld, kb.zstride, nz, make_float2(0.f, 0.f), s);
         if (e != hipSuccess) return e;
     }
-    if (np <= 0) return hipSuccess;
-    dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((np + kPotRowTile - 1) / kPotRowTile), (unsigned)nz);
-    hipLaunchKernelGGL((potential_kernel<OP, BASIS, CLAMP>), grid, dim3(256), 0, s, m.rec6, m.tris, m.nt, pts, np, kb,
-                       out, ld);
-    return hipGetLastError();
+    // grid y covers kPotRowTile points per workgroup: chunks of at most
+    // kPotMaxPoints points (the points of sample z stay at pts + z pstride)
+    for (int64_t c0 = 0; c0 < np; c0 += kPotMaxPoints) {
+        const int64_t cn = np - c0 < kPotMaxPoints ? np - c0 : kPotMaxPoints;
+        dim3 grid((unsigned)((m.nt + 255) / 256), (unsigned)((cn + kPotRowTile - 1) / kPotRowTile), (unsigned)nz);
+        hipLaunchKernelGGL((potential_kernel<OP, BASIS, CLAMP>), grid, dim3(256), 0, s, m.rec6, m.tris, m.nt,
+                           pts + 3 * c0, cn, kb, out + c0 * ld, ld);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 static hipError_t potential_dispatch(const MeshDev& m, const float* pts, int64_t np, const KBatch& kb, int nz, int op,

@@ -450,7 +450,6 @@ int bem_potential_samples(const bem_geomset* gs, const float* points, int64_t np
     if (int rc = check_ks(ks, gs->ns, op)) return rc;
     const auto& m = gs->view;
     if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
-    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
     const int64_t n = basis == BEM_BASIS_DP0 ? m.nt : m.nv;
     if (ld < n) return fail(BEM_ERR_BAD_ARG, "ld too small");
     if (gs->ns > 1 && stride < np * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (np * ld)");
@@ -665,7 +664,6 @@ int bem_potential(const bem_mesh* mesh, const float* points, int64_t np, float k
     if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
     const auto& m = mesh->dev;
     if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
-    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
     if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_potential(m, points, np, k, op, basis, clamp, (float2*)out, ld, S(stream));
@@ -682,7 +680,6 @@ int bem_potential_dk(const bem_mesh* mesh, const float* points, int64_t np, floa
     if (basis != BEM_BASIS_P1 && basis != BEM_BASIS_DP0) return fail(BEM_ERR_BAD_ARG, "unknown basis");
     const auto& m = mesh->dev;
     if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
-    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
     if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
     DeviceGuard g(m.device);
     hipError_t e = bem::launch_potential(m, points, np, k, op | bem::kDk, basis, clamp, (float2*)out, ld, S(stream));
@@ -705,7 +702,6 @@ int bem_potential_batch_k(const bem_mesh* mesh, const float* points, int64_t np,
     if (int rc = check_ks(ks, nk, op)) return rc;
     const auto& m = mesh->dev;
     if (np < 0) return fail(BEM_ERR_BAD_ARG, "negative point count");
-    if (np > (int64_t)65535 * 16) return fail(BEM_ERR_BAD_ARG, "too many points in one call (max 1048560)");
     if (ld < (basis == BEM_BASIS_DP0 ? m.nt : m.nv)) return fail(BEM_ERR_BAD_ARG, "ld too small");
     if (nk > 1 && stride < np * ld) return fail(BEM_ERR_BAD_ARG, "stride smaller than one matrix (np * ld)");
     DeviceGuard g(m.device);

@@ -622,6 +622,29 @@ def test_bad_meshes_raise():
         s.assemble_boundary_matrix(1.0, "nonsense")
 
 
+@pytest.mark.parametrize("basis", ["P1", "DP0"])
+def test_potential_more_points_than_one_launch(basis):
+    """65535 x 16 = 1048560 points fill one launch's grid; more are split
+    into several launches.  Entries of points on either side of the split
+    (and the last point) equal the same points evaluated in a small call,
+    bit for bit: each (point, triangle) term is formed the same way."""
+    v, t = mesh.icosphere(0, 1.0)
+    s = _solver(v.astype(np.float32), t, basis)
+    npts = 1048560 + 1000
+    rng = np.random.default_rng(17)
+    d = rng.standard_normal((npts, 3))
+    pts = (d / np.linalg.norm(d, axis=1, keepdims=True) * rng.uniform(1.5, 3.0, (npts, 1))).astype(np.float32)
+    pt = torch.tensor(pts, device=_dev())
+    _say("potential beyond one launch", basis, npts)
+    idx = torch.tensor([0, 1, 1048559, 1048560, 1048561, npts - 1], device=_dev())
+    for op in ("single", "double"):
+        P = s.potential_matrix(5.0, pt, op)
+        Q = s.potential_matrix(5.0, pt[idx], op)
+        assert P.shape[0] == npts
+        assert torch.equal(P[idx], Q), (basis, op)
+        del P
+
+
 def test_single_triangle_mesh():
     v = np.array([[0, 0, 0], [0.1, 0, 0], [0.02, 0.09, 0]], dtype=np.float32)
     t = np.array([[0, 1, 2]], dtype=np.int32)
