@@ -1396,6 +1396,34 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
     } else if constexpr (BASIS == DP0) {
         if (lane == 0)
             out[((int64_t)i - row0) * ld + j] = make_float2((float)(alpha * ar[9]), (float)(alpha * ai[9]));
+    } else if constexpr (MODE == 2) {
+        // P1 contraction (the operator is not stored): entry (a, b) times psi
+        // at its column vertex; the three lanes 3a, 3a + 1, 3a + 2 of the
+        // pair's group are summed, so each pair adds three values (one per
+        // row corner) into vpsi instead of nine (on small meshes every row
+        // vertex takes hundreds of these atomics)
+        static_assert(OP == BM_RHS, "the contraction is the Burton-Miller right-hand side's");
+        float cr = 0.f, cim = 0.f;
+        int oa = 0;
+        if (valid && ll < 9) {
+            const int a = ll / 3, b = ll % 3;
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int e = 0; e < 9; ++e)
+                if (e == ll) { vr = ar[e]; vi = ai[e]; }
+            oa = pi[a];
+            const float2 gv = psi[ja[pj[b]]];
+            cr = (float)(vr * gv.x - vi * gv.y);
+            cim = (float)(vr * gv.y + vi * gv.x);
+        }
+        // every lane takes part in the shuffles; lanes 3a + 1, 3a + 2 stay in the group (ll <= 8 < 16)
+        const float r1 = __shfl_down(cr, 1, 64), i1 = __shfl_down(cim, 1, 64);
+        const float r2 = __shfl_down(cr, 2, 64), i2 = __shfl_down(cim, 2, 64);
+        if (valid && ll < 9 && ll % 3 == 0) {
+            float* dst = reinterpret_cast<float*>(&vpsi[ia[oa]]);
+            unsafeAtomicAdd(dst, cr + r1 + r2);
+            unsafeAtomicAdd(dst + 1, cim + i1 + i2);
+        }
     } else {
         if (valid && ll < 9) {
             // ll = reordered (a, b) pair; its original local indices are (pi[a], pj[b])
@@ -1425,18 +1453,9 @@ __global__ __launch_bounds__(256) void singular_kernel(const float* __restrict__
                     vi += ar[9] * invk * c;
                 }
             }
-            if constexpr (MODE == 2) {
-                // P1 contraction: the entry times psi at its column vertex, into
-                // vpsi at its row vertex (the operator is not stored)
-                const float2 gv = psi[ja[ob]];
-                float* dst = reinterpret_cast<float*>(&vpsi[ia[oa]]);
-                unsafeAtomicAdd(dst, (float)(vr * gv.x - vi * gv.y));
-                unsafeAtomicAdd(dst + 1, (float)(vr * gv.y + vi * gv.x));
-            } else {
-                float* dst = reinterpret_cast<float*>(&out[(int64_t)ia[oa] * ld + ja[ob]]);
-                unsafeAtomicAdd(dst, (float)vr);
-                unsafeAtomicAdd(dst + 1, (float)vi);
-            }
+            float* dst = reinterpret_cast<float*>(&out[(int64_t)ia[oa] * ld + ja[ob]]);
+            unsafeAtomicAdd(dst, (float)vr);
+            unsafeAtomicAdd(dst + 1, (float)vi);
         }
     }
 }
