@@ -678,10 +678,12 @@ def burton_miller_large_form(v, t, src, pts, ks, restart=1500, tol=1e-6, nrows=1
     return out
 
 
-def datagen_form(n_samples=48, batch=16, level=2, seed=11):
+def datagen_form(n_samples=48, batch=None, level=2, seed=11):
     """The data-generation path (P1 Burton-Miller per sampled scene, |p| at
     64 targets): one neumann2dirichlet + boundary2potential per sample against
-    the batched driver (per-sample assembly, one lockstep GMRES per batch).
+    the batched driver (the samples of a batch as one geometry set, their
+    systems in one pass per kernel, one lockstep GMRES per batch; ``batch``
+    None: the driver's default, here all 48 samples in one batch).
     Scene: two level-``level`` spheres, one rotating and resized, one moving."""
     import torch
 
@@ -697,7 +699,7 @@ def datagen_form(n_samples=48, batch=16, level=2, seed=11):
     sc = SceneSampler(objs, trg_sample_num=64, freq_min=200.0, freq_max=4000.0, trg_pos_min=[-0.5] * 3,
                       trg_pos_max=[0.5] * 3)
     dev = torch.device("cuda:0")
-    sc.generate(2, seed=seed, batch=2)  # warm-up
+    sc.generate(n_samples, seed=seed + 1, batch=batch)  # warm-up of the same shape (pool growth, first launches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     x, y, infos = sc.generate(n_samples, seed=seed, batch=batch)
@@ -719,6 +721,7 @@ def datagen_form(n_samples=48, batch=16, level=2, seed=11):
     diff = float((torch.linalg.vector_norm(y[:, :, 0] - yr) / torch.linalg.vector_norm(yr)).item())
     log(f"bench: datagen {n_samples} samples: loop {t_loop:.2f} s, batched {t_batch:.2f} s")
     return {"workload": f"{n_samples} scenes of two level-{level} spheres (P1 Burton-Miller, 64 targets)",
+            "samples_per_batch": int(min(n_samples, sc.batch_size() if batch is None else batch)),
             "vertices": int(2 * v.shape[0]), "loop_seconds": t_loop, "batched_seconds": t_batch,
             "speedup": t_loop / t_batch, "samples_per_s_batched": n_samples / t_batch,
             "converged": int(sum(i["converged"] for i in infos)), "rel_diff_batched_vs_loop": diff}

@@ -138,12 +138,25 @@ class SceneSampler:
         x[:, -1] = f["freq"]
         return x
 
-    def generate(self, n_samples, seed=0, batch=16, device="cuda", tol=1e-6, restart=100, max_cycles=20):
+    def batch_size(self, max_batch=64, max_batch_bytes=64 << 30):
+        """Samples per geometry set: up to ``max_batch``, as many as keep the
+        batch's dense systems (n^2 complex64 entries each, n the scene's vertex
+        count) within ``max_batch_bytes``.  One lockstep GMRES call per batch:
+        on small scenes the launches and host round trips of a call, not the
+        arithmetic, set its time, so larger batches are faster."""
+        n = sum(len(o.vertices) for o in self.objects)
+        return int(max(1, min(max_batch, max_batch_bytes // (8 * n * n))))
+
+    def generate(self, n_samples, seed=0, batch=None, device="cuda", tol=1e-6, restart=100, max_cycles=20):
         """x [S, T, D], y [S, T, 1] (generate.py:13-39), ``batch`` samples per
-        geometry set and lockstep GMRES call.  Returns (x, y, infos)."""
+        geometry set and lockstep GMRES call (default ``batch_size()``).
+        Returns (x, y, infos)."""
         rng = np.random.default_rng(seed)
         dev = torch.device(device)
         S, T = int(n_samples), self.trg_sample_num
+        batch = self.batch_size() if batch is None else int(batch)
+        if batch < 1:
+            raise ValueError("batch must be at least 1")
         x = torch.zeros((S, T, self.feature_dim), dtype=torch.float32)
         y = torch.zeros((S, T, 1), dtype=torch.float32)
         infos = []

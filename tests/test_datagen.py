@@ -58,6 +58,21 @@ def test_sampler_layout_and_targets():
     assert np.isclose(x[0, 4], f["move"][0]) and np.isclose(x[0, -2], f["resize"]) and np.isclose(x[0, -1], f["freq"])
 
 
+def test_default_batch_size():
+    """Up to 64 samples per batch, fewer when the batch's dense systems
+    (8 n^2 bytes each, n the scene's vertex count) would pass the budget."""
+    a, b = _objects()
+    sc = SceneSampler([a, b], trg_sample_num=8, freq_min=100.0, freq_max=1000.0, trg_pos_min=[-1] * 3,
+                      trg_pos_max=[1] * 3)
+    n = len(a.vertices) + len(b.vertices)
+    assert sc.batch_size() == 64
+    assert sc.batch_size(max_batch=16) == 16
+    assert sc.batch_size(max_batch_bytes=8 * n * n * 5 + 7) == 5
+    assert sc.batch_size(max_batch_bytes=1) == 1
+    with pytest.raises(ValueError):
+        sc.generate(1, batch=0)
+
+
 @pytest.mark.gpu
 def test_batched_generation_matches_per_sample_solves():
     from neuralat_amd.bem import BEM_Solver
