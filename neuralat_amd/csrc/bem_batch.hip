@@ -9,8 +9,9 @@
 //    all items (grid y = item).  Per-item Givens rotations and convergence
 //    flags stay on the device; a converged item skips its work.  A restart
 //    cycle (residual and set-up, Arnoldi steps in segments of kSeg, update) is
-//    captured once as hipGraphs and replayed; the host reads the items' flags
-//    after each segment and ends the cycle once every item is done.
+//    replayed from hipGraphs, each part captured on its second use; the host
+//    reads the items' flags after each segment and ends the cycle once every
+//    item is done.
 //  * bem_batch_fgmres: the mixed-precision variant -- flexible GMRES in FP32
 //    on the complex64 matrices whose directions come from the same lockstep
 //    GMRES run on 16-bit copies of the matrices (diagonal kept in FP32).
@@ -814,61 +815,69 @@ hipError_t xcycle_fin(const XEng& E, hipStream_t st)
 
 constexpr int kSeg = 20;  // Arnoldi steps per graph segment; the host reads the flags between segments
 
-// A restart cycle as graphs: [0] start, [1 .. nseg] step segments, [nseg + 1] end
+// A restart cycle as graphs: slot 0 the start, 1 .. nseg the step segments,
+// nseg + 1 the end.  A slot runs as direct launches on its first use, is
+// captured on its second and replayed from then on: a capture pays only when
+// the graph is replayed (later restart cycles, the inner solves of later
+// outer steps), and solves done in one cycle capture nothing.
 struct XGraphs {
     std::vector<hipGraph_t> g;
     std::vector<hipGraphExec_t> x;
+    std::vector<int> uses;
     hipStream_t cap = nullptr;
     XGraphs() = default;
     XGraphs(const XGraphs&) = delete;
     XGraphs& operator=(const XGraphs&) = delete;
     ~XGraphs()
     {
-        for (auto e : x) (void)hipGraphExecDestroy(e);
-        for (auto q : g) (void)hipGraphDestroy(q);
+        for (auto e : x)
+            if (e) (void)hipGraphExecDestroy(e);
+        for (auto q : g)
+            if (q) (void)hipGraphDestroy(q);
         if (cap) (void)hipStreamDestroy(cap);
     }
 };
 
+// slot of G through f (a function of the stream that launches the slot's work)
 template <class F>
-hipError_t xcapture(XGraphs& G, F&& f)
+hipError_t xrun_slot(XGraphs* G, int nslots, int slot, hipStream_t s, F&& f)
 {
-    hipError_t e = hipStreamBeginCapture(G.cap, hipStreamCaptureModeThreadLocal);
+    if (!G) return f(s);
+    if (G->x.empty()) {
+        G->g.assign(nslots, nullptr);
+        G->x.assign(nslots, nullptr);
+        G->uses.assign(nslots, 0);
+    }
+    if (G->x[slot]) return hipGraphLaunch(G->x[slot], s);
+    if (++G->uses[slot] < 2) return f(s);
+    hipError_t e = G->cap ? hipSuccess : hipStreamCreateWithFlags(&G->cap, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
-    const hipError_t e1 = f(G.cap);
+    e = hipStreamBeginCapture(G->cap, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t e1 = f(G->cap);
     hipGraph_t graph = nullptr;
-    const hipError_t e2 = hipStreamEndCapture(G.cap, &graph);
-    if (graph) G.g.push_back(graph);
+    const hipError_t e2 = hipStreamEndCapture(G->cap, &graph);
+    G->g[slot] = graph;
     if (e1 != hipSuccess) return e1;
     if (e2 != hipSuccess) return e2;
     hipGraphExec_t ex = nullptr;
     e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return e;
-    G.x.push_back(ex);
-    return hipSuccess;
+    G->x[slot] = ex;
+    return hipGraphLaunch(ex, s);
 }
 
-hipError_t xbuild_graphs(const XEng& E, XGraphs& G)
+// One restart cycle on stream s (through the graphs of G when given).  The
+// host reads the items' flags after each segment of Arnoldi steps and ends
+// the cycle early once every item is done.
+hipError_t xrun_cycle(const XEng& E, XGraphs* G, hipStream_t s, std::vector<int>& hd)
 {
-    hipError_t e = hipStreamCreateWithFlags(&G.cap, hipStreamNonBlocking);
-    if (e != hipSuccess) return e;
-    e = xcapture(G, [&](hipStream_t st) { return xcycle_init(E, st); });
-    for (int j0 = 0; e == hipSuccess && j0 < E.m; j0 += kSeg)
-        e = xcapture(G, [&](hipStream_t st) { return xcycle_steps(E, j0, std::min(E.m, j0 + kSeg), st); });
-    if (e == hipSuccess) e = xcapture(G, [&](hipStream_t st) { return xcycle_fin(E, st); });
-    return e;
-}
-
-// One restart cycle on stream s (graph replays when G is given).  The host
-// reads the items' flags after each segment of Arnoldi steps and ends the
-// cycle early once every item is done.
-hipError_t xrun_cycle(const XEng& E, const XGraphs* G, hipStream_t s, std::vector<int>& hd)
-{
-    const int nseg = (E.m + kSeg - 1) / kSeg;
-    hipError_t e = G ? hipGraphLaunch(G->x[0], s) : xcycle_init(E, s);
+    const int nseg = (E.m + kSeg - 1) / kSeg, nslots = nseg + 2;
+    hipError_t e = xrun_slot(G, nslots, 0, s, [&](hipStream_t st) { return xcycle_init(E, st); });
     if (e != hipSuccess) return e;
     for (int k = 0; k < nseg; ++k) {
-        e = G ? hipGraphLaunch(G->x[1 + k], s) : xcycle_steps(E, k * kSeg, std::min(E.m, (k + 1) * kSeg), s);
+        e = xrun_slot(G, nslots, 1 + k, s,
+                      [&](hipStream_t st) { return xcycle_steps(E, k * kSeg, std::min(E.m, (k + 1) * kSeg), st); });
         if (e != hipSuccess) return e;
         e = hipMemcpyAsync(hd.data(), E.done, sizeof(int) * E.ns, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return e;
@@ -878,7 +887,7 @@ hipError_t xrun_cycle(const XEng& E, const XGraphs* G, hipStream_t s, std::vecto
         for (int q = 0; q < E.ns && all; ++q) all = hd[q] != 0;
         if (all) break;
     }
-    return G ? hipGraphLaunch(G->x[nseg + 1], s) : xcycle_fin(E, s);
+    return xrun_slot(G, nslots, nseg + 1, s, [&](hipStream_t st) { return xcycle_fin(E, st); });
 }
 
 // Workspace of the mixed-precision batch solve
@@ -1136,11 +1145,7 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
     const XEng E = xeng_at(W, L, (const BSys*)(W + L.sys), ns, ntot, nmax, restart);
     std::vector<double> hb, hr(ns);
     if (int rc = gather_systems(sys, hs, E, tol, hb, s)) return rc;
-    XGraphs G;
-    if (use_graph && xbuild_graphs(E, G) != hipSuccess) {
-        g_batch_err = "graph capture failed";
-        return BEM_ERR_HIP;
-    }
+    XGraphs G;  // captured by the first cycle that runs past one segment
     std::vector<int> hc(ns, 0), hk(ns, 0), hd(ns, 0), its(ns, 0);
     bool all = false;
     for (int cycles = 0; cycles < max_cycles && !all; ++cycles) {
@@ -1239,11 +1244,7 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
         hipLaunchKernelGGL(xto_lowp_kernel<kDtFP16>, grow, dim3(256), 0, s, dsys, dlow, scale);
     XTRY(hipGetLastError());
 
-    XGraphs G;
-    if (use_graph && xbuild_graphs(Ei, G) != hipSuccess) {
-        g_batch_err = "graph capture failed";
-        return BEM_ERR_HIP;
-    }
+    XGraphs G;  // the inner solves' graphs, captured by the first inner cycle that runs past one segment
     const dim3 gx((unsigned)((nmax + 255) / 256), (unsigned)ns);
     const dim3 gc((unsigned)((nmax + 63) / 64), (unsigned)ns);
     std::vector<int> hc(ns, 0), hd(ns, 0), hdi(ns, 0), hci(ns, 0), hko(ns, 0), hki(ns, 0), its_o(ns, 0),
