@@ -39,13 +39,6 @@ struct bem_mesh {
 
 namespace bem {
 
-struct BSys {
-    const float2* A;
-    int64_t lda;
-    int64_t n;
-    int64_t off;
-};
-
 __device__ __forceinline__ double bsum_d(double v, double* sh)
 {
 #pragma unroll
@@ -767,6 +760,12 @@ hipError_t xcycle_init(const XEng& E, hipStream_t st)
     return hipGetLastError();
 }
 
+// Steps whose Gram-Schmidt reads at most this many basis entries per system
+// ((j + 1) n) run after the product as one launch, a workgroup per system
+// (launch_barnoldi_small); larger ones spread each system's dot products and
+// combination over many workgroups in separate launches.
+constexpr int64_t kXFusedStep = (int64_t)1 << 16;
+
 // Arnoldi steps [j0, j1): product, classical Gram-Schmidt with one
 // re-orthogonalisation, normalisation, Givens rotation and convergence flag
 hipError_t xcycle_steps(const XEng& E, int j0, int j1, hipStream_t st)
@@ -777,6 +776,12 @@ hipError_t xcycle_steps(const XEng& E, int j0, int j1, hipStream_t st)
     for (int j = j0; j < j1; ++j) {
         hipError_t e = xmatvec(E, E.V + (int64_t)j * E.ntot, E.w, kOne, kZero, 1, E.done, st);
         if (e != hipSuccess) return e;
+        if ((int64_t)(j + 1) * E.nmax <= kXFusedStep) {  // small Gram-Schmidt: the rest of the step in one launch
+            e = launch_barnoldi_small(E.dsys, E.ns, E.V, E.ntot, j, E.w, E.h, E.h2, m + 1, E.H, m, E.cs, E.sn, E.g,
+                                      E.res, E.tgt, E.done, E.nk, E.wn2, E.one_pass ? 1 : 2, st);
+            if (e != hipSuccess) return e;
+            continue;
+        }
         hipLaunchKernelGGL(xmultidot_kernel, dim3(j + 1, E.ns), dim3(256), 0, st, E.dsys, E.V, E.ntot, E.w, E.h,
                            m + 1, (const int*)E.done);
         hipLaunchKernelGGL(xcombine_kernel, gc, dim3(64 * kXWaves), 0, st, E.dsys, E.V, E.ntot, j + 1,

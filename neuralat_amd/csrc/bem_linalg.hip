@@ -402,46 +402,6 @@ __global__ __launch_bounds__(256) void normalize_kernel(const float2* __restrict
 //   column j of H gets h (length j+1, from CGS2) and ||w||; previous Givens
 //   rotations are applied, a new one zeroes H[j+1][j], g is rotated and
 //   |g[j+1]| is stored in res[j].  H: (m+1) x m column-major (double2).
-// New Hessenberg column j = h + h2 (the two Gram-Schmidt passes), rotated by
-// the j previous Givens rotations, then the new rotation and the residual
-// estimate.  The rotations are a serial chain: the workgroup stages them (and
-// the column) in LDS 256 at a time so that thread 0 never waits on a global
-// load inside the chain; the value carried from one rotation to the next
-// stays in registers.  Returns, in thread 0, rotated entry j (the value
-// entering the new rotation); called by every thread of the workgroup.
-constexpr int kGivensChunk = 256;
-__device__ double2 rotate_column_wg(double2* __restrict__ col, int j, const double2* __restrict__ h,
-                                    const double2* __restrict__ h2, const double2* __restrict__ cs,
-                                    const double2* __restrict__ sn)
-{
-    __shared__ double2 scs[kGivensChunk], ssn[kGivensChunk], sb[kGivensChunk];
-    const int t = threadIdx.x;
-    double2 carry = make_double2(h[0].x + h2[0].x, h[0].y + h2[0].y);  // col[q] entering rotation q
-    for (int q0 = 0; q0 < j; q0 += kGivensChunk) {
-        const int cnt = min(kGivensChunk, j - q0);
-        if (t < cnt) {
-            scs[t] = cs[q0 + t];
-            ssn[t] = sn[q0 + t];
-            const int e = q0 + t + 1;
-            sb[t] = make_double2(h[e].x + h2[e].x, h[e].y + h2[e].y);
-        }
-        __syncthreads();
-        if (t == 0) {
-            for (int u = 0; u < cnt; ++u) {
-                // [c s; -conj(s) conj(c)] applied to (col[q], col[q+1])
-                const double2 c = scs[u], sv = ssn[u], a = carry, b = sb[u];
-                const double2 top = make_double2(c.x * a.x - c.y * a.y + sv.x * b.x - sv.y * b.y,
-                                                 c.x * a.y + c.y * a.x + sv.x * b.y + sv.y * b.x);
-                carry = make_double2(-(sv.x * a.x + sv.y * a.y) + (c.x * b.x + c.y * b.y),
-                                     -(sv.x * a.y - sv.y * a.x) + (c.x * b.y - c.y * b.x));
-                col[q0 + u] = top;
-            }
-        }
-        __syncthreads();
-    }
-    return carry;
-}
-
 __global__ __launch_bounds__(kGivensChunk) void gmres_givens_kernel(
     double2* __restrict__ H, int m, int j, const double2* __restrict__ h, const double2* __restrict__ h2,
     const double* __restrict__ wn2, double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g,
@@ -766,6 +726,124 @@ __global__ __launch_bounds__(kGivensChunk) void bgivens_kernel(
     if (rj <= target[sy]) done[sy] = 1;
 }
 
+// The lockstep form of arnoldi_small_kernel: one workgroup per system (grid
+// = systems), the Givens bookkeeping of bgivens_kernel.  One launch instead
+// of seven per step: with small systems a step's kernels each do microseconds
+// of work, so the steps are launch-bound.
+__global__ __launch_bounds__(kArnoldiThreads) void barnoldi_small_kernel(
+    const BSys* __restrict__ sys, float2* __restrict__ V, int64_t ntot, int j, float2* __restrict__ w,
+    double2* __restrict__ h, double2* __restrict__ h2, int hs, double2* __restrict__ H, int m,
+    double2* __restrict__ cs, double2* __restrict__ sn, double2* __restrict__ g, double* __restrict__ res,
+    const double* __restrict__ target, int* __restrict__ done, int* __restrict__ nk, double* __restrict__ wn2,
+    int passes)
+{
+    __shared__ double red[kArnoldiThreads / 64];
+    const int sy = blockIdx.x;
+    if (done[sy]) return;  // uniform over the workgroup
+    const BSys S = sys[sy];
+    const int64_t n = S.n;
+    float2* Vs = V + S.off;  // basis vector q at Vs + q ntot
+    float2* ws = w + S.off;
+    double2* hsy = h + (int64_t)sy * hs;
+    double2* h2sy = h2 + (int64_t)sy * hs;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    constexpr int nw = kArnoldiThreads / 64;
+    for (int pass = 0; pass < passes; ++pass) {
+        double2* hh = pass ? h2sy : hsy;
+        for (int q = wave; q <= j; q += nw) {  // hh[q] = <v_q, w>
+            const float2* vq = Vs + (int64_t)q * ntot;
+            double sr = 0.0, si = 0.0;
+            for (int64_t e0 = lane; e0 < n; e0 += 64 * 8) {  // eight loads in flight (clamped, masked)
+                float2 a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t e = min(e0 + 64 * u, n - 1);
+                    a[u] = vq[e];
+                    b[u] = ws[e];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (e0 + 64 * u < n) {
+                        sr += (double)a[u].x * b[u].x + (double)a[u].y * b[u].y;
+                        si += (double)a[u].x * b[u].y - (double)a[u].y * b[u].x;
+                    }
+                }
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                sr += __shfl_xor(sr, o, 64);
+                si += __shfl_xor(si, o, 64);
+            }
+            if (lane == 0) hh[q] = make_double2(sr, si);
+        }
+        __syncthreads();
+        for (int64_t e = t; e < n; e += kArnoldiThreads) {  // w -= sum_q hh[q] v_q
+            double ar = 0.0, ai = 0.0;
+            for (int q0 = 0; q0 <= j; q0 += 8) {  // eight basis vectors' loads at once
+                float2 v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = Vs[(int64_t)min(q0 + u, j) * ntot + e];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    if (q0 + u <= j) {
+                        const double2 c = hh[q0 + u];
+                        ar += c.x * v[u].x - c.y * v[u].y;
+                        ai += c.x * v[u].y + c.y * v[u].x;
+                    }
+                }
+            }
+            const float2 wo = ws[e];
+            ws[e] = make_float2((float)((double)wo.x - ar), (float)((double)wo.y - ai));
+        }
+        __syncthreads();
+    }
+    double a2 = 0.0;
+    for (int64_t e = t; e < n; e += kArnoldiThreads) {
+        const float2 v = ws[e];
+        a2 += (double)v.x * v.x + (double)v.y * v.y;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+    if (lane == 0) red[wave] = a2;
+    __syncthreads();
+    double nrm2 = 0.0;
+    for (int q = 0; q < nw; ++q) nrm2 += red[q];
+    const float inv = nrm2 > 0.0 ? (float)(1.0 / sqrt(nrm2)) : 0.f;
+    float2* vn = Vs + (int64_t)(j + 1) * ntot;
+    for (int64_t e = t; e < n; e += kArnoldiThreads) {
+        const float2 v = ws[e];
+        vn[e] = make_float2(v.x * inv, v.y * inv);
+    }
+    double2* col = H + (int64_t)sy * (m + 1) * m + (int64_t)j * (m + 1);
+    double2* c_ = cs + (int64_t)sy * m;
+    double2* s_ = sn + (int64_t)sy * m;
+    const double2 carry = rotate_column_wg(col, j, hsy, h2sy, c_, s_);
+    if (t != 0) return;
+    wn2[sy] = nrm2;
+    double2* gg = g + (int64_t)sy * (m + 1);
+    const double2 aa = carry, bb = make_double2(sqrt(nrm2), 0.0);
+    const double den = sqrt(aa.x * aa.x + aa.y * aa.y + bb.x * bb.x + bb.y * bb.y);
+    double2 c = make_double2(1.0, 0.0), sv = make_double2(0.0, 0.0);
+    if (den > 0.0) {
+        c = make_double2(aa.x / den, -aa.y / den);
+        sv = make_double2(bb.x / den, -bb.y / den);
+    }
+    c_[j] = c;
+    s_[j] = sv;
+    col[j] = make_double2(den, 0.0);
+    col[j + 1] = make_double2(0.0, 0.0);
+    const double2 gj = gg[j];
+    const double2 gn = make_double2(-(sv.x * gj.x + sv.y * gj.y), -(sv.x * gj.y - sv.y * gj.x));
+    gg[j + 1] = gn;
+    gg[j] = make_double2(c.x * gj.x - c.y * gj.y, c.x * gj.y + c.y * gj.x);
+    const double rj = sqrt(gn.x * gn.x + gn.y * gn.y);
+    res[sy] = rj;
+    nk[sy] = j + 1;
+    if (rj <= target[sy]) done[sy] = 1;
+}
+
 // y = R^{-1} g for one system per workgroup (R: leading nk x nk of the
 // rotated Hessenberg, column c at H + c (m + 1)).  Column-oriented: once y_q
 // is known every thread removes R[t][q] y_q from the right-hand side entries
@@ -1003,6 +1081,15 @@ hipError_t launch_arnoldi_small(float2* V, int64_t n, int j, float2* w, double2*
 {
     hipLaunchKernelGGL(arnoldi_small_kernel, dim3(1), dim3(kArnoldiThreads), 0, s, V, n, j, w, h, h2, H, m, cs, sn, g,
                        res, wn2);
+    return hipGetLastError();
+}
+hipError_t launch_barnoldi_small(const BSys* sys, int ns, float2* V, int64_t ntot, int j, float2* w, double2* h,
+                                 double2* h2, int hs, double2* H, int m, double2* cs, double2* sn, double2* g,
+                                 double* res, const double* target, int* done, int* nk, double* wn2, int passes,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(barnoldi_small_kernel, dim3((unsigned)ns), dim3(kArnoldiThreads), 0, s, sys, V, ntot, j, w, h,
+                       h2, hs, H, m, cs, sn, g, res, target, done, nk, wn2, passes);
     return hipGetLastError();
 }
 hipError_t launch_backsolve(const double2* H, int m, int nk, const double2* g, double2* y, hipStream_t s)

@@ -7,6 +7,56 @@ namespace bem {
 
 constexpr int kGemvRows = 4;  // rows per wavefront in the streaming GEMV
 
+// One system of a lockstep batch: its matrix A [n, lda] and its vectors at
+// offset off of the batch's concatenated vector space.
+struct BSys {
+    const float2* A;
+    int64_t lda;
+    int64_t n;
+    int64_t off;
+};
+
+// New Hessenberg column j = h + h2 (the two Gram-Schmidt passes), rotated by
+// the j previous Givens rotations, then the new rotation and the residual
+// estimate.  The rotations are a serial chain: the workgroup stages them (and
+// the column) in LDS 256 at a time so that thread 0 never waits on a global
+// load inside the chain; the value carried from one rotation to the next
+// stays in registers.  Returns, in thread 0, rotated entry j (the value
+// entering the new rotation); called by every thread of the workgroup
+// (at least kGivensChunk threads).
+constexpr int kGivensChunk = 256;
+__device__ inline double2 rotate_column_wg(double2* __restrict__ col, int j, const double2* __restrict__ h,
+                                           const double2* __restrict__ h2, const double2* __restrict__ cs,
+                                           const double2* __restrict__ sn)
+{
+    __shared__ double2 scs[kGivensChunk], ssn[kGivensChunk], sb[kGivensChunk];
+    const int t = threadIdx.x;
+    double2 carry = make_double2(h[0].x + h2[0].x, h[0].y + h2[0].y);  // col[q] entering rotation q
+    for (int q0 = 0; q0 < j; q0 += kGivensChunk) {
+        const int cnt = min(kGivensChunk, j - q0);
+        if (t < cnt) {
+            scs[t] = cs[q0 + t];
+            ssn[t] = sn[q0 + t];
+            const int e = q0 + t + 1;
+            sb[t] = make_double2(h[e].x + h2[e].x, h[e].y + h2[e].y);
+        }
+        __syncthreads();
+        if (t == 0) {
+            for (int u = 0; u < cnt; ++u) {
+                // [c s; -conj(s) conj(c)] applied to (col[q], col[q+1])
+                const double2 c = scs[u], sv = ssn[u], a = carry, b = sb[u];
+                const double2 top = make_double2(c.x * a.x - c.y * a.y + sv.x * b.x - sv.y * b.y,
+                                                 c.x * a.y + c.y * a.x + sv.x * b.y + sv.y * b.x);
+                carry = make_double2(-(sv.x * a.x + sv.y * a.y) + (c.x * b.x + c.y * b.y),
+                                     -(sv.x * a.y - sv.y * a.x) + (c.x * b.y - c.y * b.x));
+                col[q0 + u] = top;
+            }
+        }
+        __syncthreads();
+    }
+    return carry;
+}
+
 // y_z = alpha A_z x_z + beta y_z for nz matrices [rows, cols] (A_z at A + z sa,
 // x_z at x + z sx, y_z at y + z sy; elements)
 hipError_t launch_gemv_strided(const float2* A, int64_t rows, int64_t cols, int64_t lda, int64_t sa, const float2* x,
@@ -57,6 +107,15 @@ hipError_t launch_bgivens(double2* H, int m, int j, const double2* h, const doub
                           int ns, hipStream_t s);
 hipError_t launch_bbacksolve(const double2* H, int m, const int* nk, const double2* g, double2* y, int ns,
                              hipStream_t s);
+// One lockstep Arnoldi step after w = A v_j, one workgroup per system (meant
+// for small systems, whose steps are launch-bound): Gram-Schmidt (passes = 1
+// or 2; with 1, h2 is left as it is), |w|^2 into wn2, v_{j+1}, and the
+// Givens bookkeeping of launch_bgivens.  Vector q of system s at
+// V + q ntot + sys[s].off; h, h2 [ns, hs]; H [ns, (m + 1) m].
+hipError_t launch_barnoldi_small(const BSys* sys, int ns, float2* V, int64_t ntot, int j, float2* w, double2* h,
+                                 double2* h2, int hs, double2* H, int m, double2* cs, double2* sn, double2* g,
+                                 double* res, const double* target, int* done, int* nk, double* wn2, int passes,
+                                 hipStream_t s);
 hipError_t launch_bupdate_x(const float2* V, int64_t ldn, int ns, int64_t n, const int* nk, const double2* y, int m,
                             float2* x, int64_t ldx, hipStream_t s);
 hipError_t launch_bcopy(const float2* src, int64_t lds, float2* dst, int64_t ldd, int64_t n, int ns, hipStream_t s);
