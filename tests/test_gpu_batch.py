@@ -102,6 +102,36 @@ def test_batch_gmres_graph_capture_mid_cycle_equals_direct_launches():
         assert np.linalg.norm(x1.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-5
 
 
+def test_batch_gmres_across_fused_step_threshold():
+    """Arnoldi steps whose Gram-Schmidt reads at most 65536 basis entries per
+    item ((j + 1) n) run as one launch per step, deeper ones as separate
+    kernels.  With n = 3000 the first 21 steps of every cycle take the fused
+    launch and the rest the separate kernels; n = 40 (nmax 40) stays fused.
+    Both batches reach the FP64 direct solution and the step counts of one
+    GMRES per item."""
+    from neuralat_amd.bem.batch import batch_gmres
+    from neuralat_amd.bem.solver import gmres
+
+    rng = np.random.default_rng(11)
+    for sizes in ((3000, 2200, 1700), (40, 33)):
+        As, bs = [], []
+        for n in sizes:
+            A = (np.eye(n) * (1.2 + 0.4j) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+                 * (0.6 / np.sqrt(n))).astype(np.complex64)
+            As.append(torch.tensor(A, device=_dev()))
+            bs.append(torch.tensor((rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64),
+                                   device=_dev()))
+        xs, infos = batch_gmres(As, bs, tol=1e-6, restart=60, max_cycles=20)
+        for a, b, x, inf in zip(As, bs, xs, infos):
+            assert inf["converged"], inf
+            if a.shape[0] == 3000:
+                assert inf["iterations"] > 22, inf  # crossed the threshold
+            _, ir = gmres(a, b, tol=1e-6, restart=60)
+            assert abs(inf["iterations"] - ir["iterations"]) <= 1, (inf, ir)
+            ref = np.linalg.solve(a.cpu().numpy().astype(np.complex128), b.cpu().numpy().astype(np.complex128))
+            assert np.linalg.norm(x.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-5
+
+
 def test_batch_gmres_iterations_match_per_item(items):
     """Per-item Arnoldi step counts of the lockstep solve (cycles ended early
     once every item is done) agree with one GMRES per item at the same restart."""
