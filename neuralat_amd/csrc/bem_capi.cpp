@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/bem_hip.h"
@@ -345,11 +346,28 @@ int bem_geomset_create(const bem_mesh* mesh, const float* verts, int ns, void* s
     DeviceGuard g(m.device);
     if (!g.ok) return fail(BEM_ERR_HIP, "cannot select the mesh's device");
     std::vector<char> stage(gb * (size_t)ns);
-    std::string err;
-    for (int z = 0; z < ns; ++z) {
-        const int rc = bem::mesh_geometry_region(m, verts + (size_t)z * 3 * (size_t)m.nv, stage.data() + z * gb, err);
-        if (rc) return fail(rc == 2 ? BEM_ERR_BAD_MESH : BEM_ERR_BAD_ARG, "sample " + std::to_string(z) + ": " + err);
+    // Each sample's geometry is independent host work (quadrature records,
+    // curls): up to 16 threads, each on an interleaved share of the samples.
+    std::vector<int> rcs((size_t)ns, 0);
+    std::vector<std::string> errs((size_t)ns);
+    auto work = [&](int t0, int step) {
+        for (int z = t0; z < ns; z += step)
+            rcs[z] = bem::mesh_geometry_region(m, verts + (size_t)z * 3 * (size_t)m.nv, stage.data() + z * gb, errs[z]);
+    };
+    const int nth = (int)std::min<int64_t>({(int64_t)ns, 16, (int64_t)std::max(1u, std::thread::hardware_concurrency()),
+                                            std::max<int64_t>(1, (int64_t)ns * m.nt / 4096)});
+    if (nth <= 1) {
+        work(0, 1);
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve(nth - 1);
+        for (int t = 1; t < nth; ++t) pool.emplace_back(work, t, nth);
+        work(0, nth);
+        for (auto& th : pool) th.join();
     }
+    for (int z = 0; z < ns; ++z)
+        if (rcs[z])
+            return fail(rcs[z] == 2 ? BEM_ERR_BAD_MESH : BEM_ERR_BAD_ARG, "sample " + std::to_string(z) + ": " + errs[z]);
     auto* gs = new bem_geomset();
     if (hipMalloc(&gs->block, stage.size()) != hipSuccess) {
         (void)hipGetLastError();

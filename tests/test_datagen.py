@@ -181,6 +181,30 @@ def test_geometry_set_matches_per_sample_meshes():
 
 
 @pytest.mark.gpu
+def test_geometry_set_of_many_samples():
+    """A geometry set of 64 level-2 placements (its host geometry built by
+    several threads, each on an interleaved share of the samples): every
+    sample's single layer equals that of a mesh handle of its own, and a
+    non-finite vertex in one sample is reported with that sample's index."""
+    from neuralat_amd.bem import BEM_Solver, GeometrySet, Mesh
+
+    dev = torch.device("cuda:0")
+    V, t = _placements(64)
+    tt = torch.tensor(t, device=dev)
+    m = Mesh(torch.tensor(V[0], device=dev), tt)
+    gs = GeometrySet(m, V)
+    ks = np.linspace(5.0, 25.0, 64)
+    A = gs.assemble(ks, "single", basis="P1")
+    for z in (0, 1, 17, 38, 63):
+        R = BEM_Solver(torch.tensor(V[z], device=dev), tt, basis="P1").assemble_boundary_matrix(ks[z], "single")
+        assert float(torch.linalg.matrix_norm(A[z] - R) / torch.linalg.matrix_norm(R)) < 1e-6, z
+    W = V.copy()
+    W[37, 5, 1] = np.nan
+    with pytest.raises(ValueError, match="sample 37"):
+        GeometrySet(m, W)
+
+
+@pytest.mark.gpu
 def test_generated_samples_vs_fp64_oracle():
     """Two samples of the batched driver end to end against the FP64 oracle:
     the Burton-Miller solve of each sample (bm_lhs, bm_rhs, approximate rules,
