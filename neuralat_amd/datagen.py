@@ -80,6 +80,28 @@ class SceneObject:
             v = v + np.asarray(self.position, dtype=np.float64)
         return v
 
+    def place_batch(self, resize_factors, rot_factors, move_factors, moving_scene=True):
+        """``place`` for S samples at once (factor arrays of length S):
+        vertices [S, V, 3]."""
+        from scipy.spatial.transform import Rotation
+
+        base = np.zeros(3) if self.resize is None else np.asarray(self.resize, dtype=np.float64)
+        scale = base[None, :] * np.asarray(resize_factors, dtype=np.float64)[:, None]
+        scale[scale == 0] = 1.0
+        v = self.vertices[None] * scale[:, None, :]
+        if self.rot_axis is not None:
+            pivot = (np.asarray(self.rot_pos, dtype=np.float64)[None, :] * scale)[:, None, :]
+            ang = np.asarray(rot_factors, dtype=np.float64) * self.rot_max_deg
+            R = Rotation.from_euler(self.rot_axis, ang, degrees=True).as_matrix()  # [S, 3, 3]
+            d = v - pivot
+            v = sum(d[..., j, None] * R[:, None, :, j] for j in range(3)) + pivot  # d R^T per sample
+        if moving_scene:
+            if self.move is not None:
+                v = v + np.asarray(self.move, dtype=np.float64) * np.asarray(move_factors, dtype=np.float64)[:, None,
+                                                                                                           None]
+            v = v + np.asarray(self.position, dtype=np.float64)
+        return v
+
 
 class SceneSampler:
     """Samples scenes and solves them in batches (src/scene.py:193-291)."""
@@ -129,6 +151,36 @@ class SceneSampler:
         return (np.concatenate(verts), np.concatenate(tris), np.concatenate(neu), float(k),
                 pts + self.bbox_center)
 
+    def build_batch(self, facs):
+        """``build`` for a list of samples at once: (vertices [S, V, 3],
+        triangles, neumann [S, V], k [S], target points [S, T, 3]); the
+        samples share the triangles."""
+        S = len(facs)
+        resize = np.array([f["resize"] for f in facs]) * (self.max_resize - 1) + 1
+        verts, tris, neu = [], [], []
+        ri = mi = 0
+        nv = 0
+        for o in self.objects:
+            rf = np.array([f["rot"][ri] for f in facs]) if o.rot_axis is not None else np.zeros(S)
+            mf = np.array([f["move"][mi] for f in facs]) if o.move is not None else np.zeros(S)
+            v = o.place_batch(resize, rf, mf, self.move_num > 0)
+            ri += o.rot_axis is not None and ri < self.rot_num - 1
+            mi += o.move is not None and mi < self.move_num - 1
+            verts.append(v)
+            tris.append(o.triangles + nv)
+            neu.append(o.neumann)
+            nv += v.shape[1]
+        fr = np.array([f["freq"] for f in facs])
+        freq = 10 ** (fr * (self.freq_max_log - self.freq_min_log) + self.freq_min_log)
+        k = 2 * np.pi * freq / SPEED_OF_SOUND
+        u = np.stack([f["trg"] for f in facs])
+        rs = (u[..., 0] + 1) * self.bbox_size
+        theta = u[..., 1] * 2 * np.pi - np.pi
+        phi = u[..., 2] * np.pi
+        pts = np.stack([rs * np.sin(phi) * np.cos(theta), rs * np.sin(phi) * np.sin(theta), rs * np.cos(phi)], -1)
+        return (np.concatenate(verts, axis=1), np.concatenate(tris), np.broadcast_to(np.concatenate(neu), (S, nv)),
+                k, pts + self.bbox_center)
+
     def features(self, f):
         x = np.zeros((self.trg_sample_num, self.feature_dim), dtype=np.float32)
         x[:, :3] = f["trg"]
@@ -163,14 +215,14 @@ class SceneSampler:
         mesh = None  # the scene's triangles (every sample has the same ones)
         for s0 in range(0, S, batch):
             facs = [self.sample_factors(rng) for _ in range(min(batch, S - s0))]
-            built = [self.build(f) for f in facs]
-            verts = np.stack([b[0] for b in built]).astype(np.float32)
+            vb, tb, nb, kb, pb = self.build_batch(facs)
+            verts = vb.astype(np.float32)
             if mesh is None:
-                mesh = Mesh(torch.tensor(verts[0], device=dev), torch.tensor(built[0][1], device=dev))
+                mesh = Mesh(torch.tensor(verts[0], device=dev), torch.tensor(tb, device=dev))
             gs = GeometrySet(mesh, verts)
-            ks = [b[3] for b in built]
-            neu = torch.tensor(np.stack([b[2] for b in built]), device=dev)
-            pts = torch.tensor(np.stack([b[4] for b in built]).astype(np.float32), device=dev)
+            ks = [float(k) for k in kb]
+            neu = torch.tensor(np.ascontiguousarray(nb), device=dev)
+            pts = torch.tensor(pb.astype(np.float32), device=dev)
             lhs, rhs = burton_miller_samples(gs, ks, neu)
             xs, inf = batch_gmres(list(lhs), list(rhs), tol=tol, restart=restart, max_cycles=max_cycles)
             infos.extend(inf)

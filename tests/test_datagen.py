@@ -58,6 +58,24 @@ def test_sampler_layout_and_targets():
     assert np.isclose(x[0, 4], f["move"][0]) and np.isclose(x[0, -2], f["resize"]) and np.isclose(x[0, -1], f["freq"])
 
 
+def test_build_batch_equals_per_sample_build():
+    """The batched driver's vectorised build of a batch (one rotation per
+    sample from one scipy call, broadcast placement) gives each sample's
+    vertices, Neumann data, wavenumber and targets of the per-sample build."""
+    c = SceneObject(*mesh.icosphere(1, 0.05), resize=(0.0, 1.0, 1.0), rot_axis="x", rot_pos=(0.0, 0.02, 0.0),
+                    rot_max_deg=45.0, move=(0.1, 0.0, -0.1), vibration="y")
+    sc = SceneSampler(list(_objects()) + [c], trg_sample_num=5, freq_min=100.0, freq_max=4000.0,
+                      trg_pos_min=[-0.5] * 3, trg_pos_max=[0.7] * 3)
+    rng = np.random.default_rng(9)
+    facs = [sc.sample_factors(rng) for _ in range(6)]
+    vb, tb, nb, kb, pb = sc.build_batch(facs)
+    assert vb.shape == (6, 3 * 42, 3) and nb.shape == (6, 3 * 42) and pb.shape == (6, 5, 3)
+    for q, f in enumerate(facs):
+        v, t, neu, k, pts = sc.build(f)
+        assert np.allclose(vb[q], v, rtol=0, atol=1e-14) and np.array_equal(tb, t)
+        assert np.array_equal(nb[q], neu) and np.isclose(kb[q], k, rtol=1e-15) and np.allclose(pb[q], pts, atol=1e-14)
+
+
 def test_default_batch_size():
     """Up to 64 samples per batch, fewer when the batch's dense systems
     (8 n^2 bytes each, n the scene's vertex count) would pass the budget."""
