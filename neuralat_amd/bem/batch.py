@@ -76,11 +76,18 @@ def batch_autotune(meshes, ks, psis, outs=None, clamp=True):
     return int(best.value), dict(zip((16, 32, 64), list(ms)))
 
 
+def _zero_vectors(bs):
+    """Zero initial guesses shaped like ``bs``: views of one allocation (one
+    fill for the whole batch)."""
+    flat = torch.zeros(sum(b.numel() for b in bs), dtype=bs[0].dtype, device=bs[0].device)
+    return [v.view(b.shape) for v, b in zip(flat.split([b.numel() for b in bs]), bs)]
+
+
 def batch_gmres(As, bs, xs=None, tol=1e-6, restart=50, max_cycles=40, use_graph=True, raise_on_fail=False):
     """Lockstep GMRES on systems As[q] x = bs[q] (different sizes).  Returns
     (xs, infos)."""
     n = len(As)
-    xs = xs or [torch.zeros_like(b) for b in bs]
+    xs = xs or _zero_vectors(bs)
     arr = (_Sys * n)()
     for q in range(n):
         check_tensor(As[q], torch.complex64)
@@ -110,7 +117,7 @@ def batch_solve_mixed(As, bs, xs=None, dtype="bf16", tol=1e-6, inner_tol=1e-6, o
     codes = {"bf16": 1, "fp16": 2, "fp8": 3}
     if dtype not in codes:
         raise ValueError("dtype must be 'bf16', 'fp16' or 'fp8'")
-    xs = xs or [torch.zeros_like(b) for b in bs]
+    xs = xs or _zero_vectors(bs)
     arr = (_Sys * n)()
     for q in range(n):
         check_tensor(As[q], torch.complex64)

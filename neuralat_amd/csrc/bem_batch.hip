@@ -223,7 +223,31 @@ __global__ __launch_bounds__(256) void xnormalize_kernel(const BSys* __restrict_
     dst[S.off + e] = make_float2(v.x * sc, v.y * sc);
 }
 
-// x_s += sum_{q < nk_s} y_s[q] V_q ; grid (element blocks, items)
+// Caller-side vectors of one item: its right-hand side and its solution
+// (initial guess on entry)
+struct XVec {
+    const float2* b;
+    float2* x;
+};
+
+// B_s = b_s and X_s = x_s into the concatenated space (scatter = false), or
+// x_s = X_s back (scatter = true); grid (element blocks, items)
+__global__ __launch_bounds__(256) void xgather_kernel(const BSys* __restrict__ sys, const XVec* __restrict__ vec,
+                                                      float2* __restrict__ B, float2* __restrict__ X, bool scatter)
+{
+    const int s = blockIdx.y;
+    const BSys S = sys[s];
+    const XVec v = vec[s];
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < S.n; e += (int64_t)gridDim.x * 256) {
+        if (scatter) {
+            v.x[e] = X[S.off + e];
+        } else {
+            B[S.off + e] = v.b[e];
+            X[S.off + e] = v.x[e];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void xcopy_kernel(const float2* __restrict__ src, float2* __restrict__ dst, int64_t n)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -623,7 +647,7 @@ namespace {
 size_t al(size_t b) { return (b + 255) / 256 * 256; }
 
 struct XLayout {
-    size_t sys, V, w, r, b, x, H, h, h2, cs, sn, g, y, res, rn2, wn2, tgt, done, nk, conv, total;
+    size_t sys, vec, V, w, r, b, x, H, h, h2, cs, sn, g, y, res, rn2, wn2, tgt, done, nk, conv, total;
 };
 
 XLayout xlayout(int ns, int64_t ntot, int m)
@@ -636,6 +660,7 @@ XLayout xlayout(int ns, int64_t ntot, int m)
         return at;
     };
     L.sys = take(sizeof(BSys) * ns);
+    L.vec = take(sizeof(XVec) * ns);
     L.V = take(sizeof(float2) * (size_t)ntot * (m + 1));
     L.w = take(sizeof(float2) * ntot);
     L.r = take(sizeof(float2) * ntot);
@@ -1116,16 +1141,24 @@ static int check_systems(const bem_batch_system* sys, int ns, std::vector<BSys>&
     return BEM_OK;
 }
 
-// b and the initial x into the concatenated space; targets tol |b_s|
+static dim3 xgather_grid(const XEng& E)
+{
+    return dim3((unsigned)std::min<int64_t>((E.nmax + 255) / 256, 1024), (unsigned)E.ns);
+}
+
+// b and the initial x into the concatenated space (one launch for all items;
+// dvec: the items' vector table, hv its host copy, alive until the stream
+// has run the copy); targets tol |b_s|
 static int gather_systems(const bem_batch_system* sys, const std::vector<BSys>& hs, const XEng& E, float tol,
-                          std::vector<double>& hb, hipStream_t s)
+                          std::vector<double>& hb, XVec* dvec, std::vector<XVec>& hv, hipStream_t s)
 {
     const int ns = E.ns;
+    hv.resize(ns);
+    for (int q = 0; q < ns; ++q) hv[q] = XVec{(const float2*)sys[q].b, (float2*)sys[q].x};
     XTRY(hipMemcpyAsync((void*)E.dsys, hs.data(), sizeof(BSys) * ns, hipMemcpyHostToDevice, s));
-    for (int q = 0; q < ns; ++q) {
-        XTRY(hipMemcpyAsync(E.B + hs[q].off, sys[q].b, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
-        XTRY(hipMemcpyAsync(E.X + hs[q].off, sys[q].x, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
-    }
+    XTRY(hipMemcpyAsync(dvec, hv.data(), sizeof(XVec) * ns, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(xgather_kernel, xgather_grid(E), dim3(256), 0, s, E.dsys, (const XVec*)dvec, E.B, E.X, false);
+    XTRY(hipGetLastError());
     hb.assign(ns, 0.0);
     std::vector<double> ht(ns);
     hipLaunchKernelGGL(xnorm2_kernel, dim3(ns), dim3(256), 0, s, E.dsys, E.B, E.rn2);
@@ -1149,7 +1182,9 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
     char* W = (char*)work;
     const XEng E = xeng_at(W, L, (const BSys*)(W + L.sys), ns, ntot, nmax, restart);
     std::vector<double> hb, hr(ns);
-    if (int rc = gather_systems(sys, hs, E, tol, hb, s)) return rc;
+    std::vector<XVec> hv;
+    XVec* dvec = (XVec*)(W + L.vec);
+    if (int rc = gather_systems(sys, hs, E, tol, hb, dvec, hv, s)) return rc;
     XGraphs G;  // captured by the first cycle that runs past one segment
     std::vector<int> hc(ns, 0), hk(ns, 0), hd(ns, 0), its(ns, 0);
     bool all = false;
@@ -1165,8 +1200,8 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
             its[q] += hk[q];  // Arnoldi steps of this item in the cycle (0 once converged)
         }
     }
-    for (int q = 0; q < ns; ++q)
-        XTRY(hipMemcpyAsync(sys[q].x, E.X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(xgather_kernel, xgather_grid(E), dim3(256), 0, s, E.dsys, (const XVec*)dvec, E.B, E.X, true);
+    XTRY(hipGetLastError());
     if (info)
         for (int q = 0; q < ns; ++q) {
             info[q].iterations = its[q];
@@ -1225,7 +1260,9 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
                        (unsigned int*)Ei.h2, (int64_t)4 * ns * (mi + 1), 0u);
 
     std::vector<double> hb, hr(ns);
-    if (int rc = gather_systems(sys, hs, Eo, tol, hb, s)) return rc;
+    std::vector<XVec> hv;
+    XVec* dvec = (XVec*)(W + F.o_at + F.o.vec);
+    if (int rc = gather_systems(sys, hs, Eo, tol, hb, dvec, hv, s)) return rc;
     // 16-bit copies of the matrices
     std::vector<BLow> hl(ns);
     float2* diag = (float2*)(W + F.diag_at);
@@ -1323,8 +1360,9 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
             its_o[q] += hko[q];
         }
     }
-    for (int q = 0; q < ns; ++q)
-        XTRY(hipMemcpyAsync(sys[q].x, Eo.X + hs[q].off, sizeof(float2) * hs[q].n, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(xgather_kernel, xgather_grid(Eo), dim3(256), 0, s, Eo.dsys, (const XVec*)dvec, Eo.B, Eo.X,
+                       true);
+    XTRY(hipGetLastError());
     if (info)
         for (int q = 0; q < ns; ++q) {
             info[q].iterations = its_o[q];
