@@ -400,10 +400,10 @@ typedef struct {
     void* x;       /* [n] initial guess in, solution out */
 } bem_batch_system;
 /* GMRES(restart) on all systems in lockstep (different matrices and sizes);
- * with use_graph != 0 the parts of a restart cycle (start, segments of 20
- * Arnoldi steps, end) are replayed from hipGraphs, each captured on its
- * second use (a solve done in one cycle captures nothing), and the cycle
- * ends early once every
+ * with use_graph != 0 the parts of a restart cycle (start, segments of 4,
+ * then 20 Arnoldi steps, end) are replayed from hipGraphs, each captured on
+ * its third use (a solve done in one or two cycles captures nothing), and the
+ * cycle ends early once every
  * system meets its target; stops when every system has
  * ||b - A x|| <= tol ||b|| or after max_cycles cycles.  info[q].iterations =
  * Arnoldi steps taken by system q. */

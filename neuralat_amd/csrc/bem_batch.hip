@@ -8,8 +8,8 @@
 //    space (item s at offset off_s), and every kernel of an Arnoldi step covers
 //    all items (grid y = item).  Per-item Givens rotations and convergence
 //    flags stay on the device; a converged item skips its work.  A restart
-//    cycle (residual and set-up, Arnoldi steps in segments of kSeg, update) is
-//    replayed from hipGraphs, each part captured on its second use; the host
+//    cycle (residual and set-up, Arnoldi steps in segments, update) is
+//    replayed from hipGraphs, each part captured on its third use; the host
 //    reads the items' flags after each segment and ends the cycle once every
 //    item is done.
 //  * bem_batch_fgmres: the mixed-precision variant -- flexible GMRES in FP32
@@ -843,13 +843,19 @@ hipError_t xcycle_fin(const XEng& E, hipStream_t st)
     return hipGetLastError();
 }
 
-constexpr int kSeg = 20;  // Arnoldi steps per graph segment; the host reads the flags between segments
+// Arnoldi steps per graph segment; the host reads the flags between segments.
+// The first segment is short: a restart cycle that follows a nearly converged
+// one (the implicit residual reached the target, the true one not quite)
+// usually needs a step or two.
+constexpr int kSeg0 = 4, kSeg = 20;
+inline int seg_count(int m) { return m <= kSeg0 ? 1 : 1 + (m - kSeg0 + kSeg - 1) / kSeg; }
+inline int seg_begin(int k) { return k == 0 ? 0 : kSeg0 + (k - 1) * kSeg; }
 
 // A restart cycle as graphs: slot 0 the start, 1 .. nseg the step segments,
-// nseg + 1 the end.  A slot runs as direct launches on its first use, is
-// captured on its second and replayed from then on: a capture pays only when
+// nseg + 1 the end.  A slot runs as direct launches on its first two uses, is
+// captured on its third and replayed from then on: a capture pays only when
 // the graph is replayed (later restart cycles, the inner solves of later
-// outer steps), and solves done in one cycle capture nothing.
+// outer steps), and solves done in one or two cycles capture nothing.
 struct XGraphs {
     std::vector<hipGraph_t> g;
     std::vector<hipGraphExec_t> x;
@@ -879,7 +885,7 @@ hipError_t xrun_slot(XGraphs* G, int nslots, int slot, hipStream_t s, F&& f)
         G->uses.assign(nslots, 0);
     }
     if (G->x[slot]) return hipGraphLaunch(G->x[slot], s);
-    if (++G->uses[slot] < 2) return f(s);
+    if (++G->uses[slot] < 3) return f(s);
     hipError_t e = G->cap ? hipSuccess : hipStreamCreateWithFlags(&G->cap, hipStreamNonBlocking);
     if (e != hipSuccess) return e;
     e = hipStreamBeginCapture(G->cap, hipStreamCaptureModeThreadLocal);
@@ -902,12 +908,12 @@ hipError_t xrun_slot(XGraphs* G, int nslots, int slot, hipStream_t s, F&& f)
 // the cycle early once every item is done.
 hipError_t xrun_cycle(const XEng& E, XGraphs* G, hipStream_t s, std::vector<int>& hd)
 {
-    const int nseg = (E.m + kSeg - 1) / kSeg, nslots = nseg + 2;
+    const int nseg = seg_count(E.m), nslots = nseg + 2;
     hipError_t e = xrun_slot(G, nslots, 0, s, [&](hipStream_t st) { return xcycle_init(E, st); });
     if (e != hipSuccess) return e;
     for (int k = 0; k < nseg; ++k) {
         e = xrun_slot(G, nslots, 1 + k, s,
-                      [&](hipStream_t st) { return xcycle_steps(E, k * kSeg, std::min(E.m, (k + 1) * kSeg), st); });
+                      [&](hipStream_t st) { return xcycle_steps(E, seg_begin(k), std::min(E.m, seg_begin(k + 1)), st); });
         if (e != hipSuccess) return e;
         e = hipMemcpyAsync(hd.data(), E.done, sizeof(int) * E.ns, hipMemcpyDeviceToHost, s);
         if (e != hipSuccess) return e;
