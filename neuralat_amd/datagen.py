@@ -93,8 +93,7 @@ class SceneObject:
             pivot = (np.asarray(self.rot_pos, dtype=np.float64)[None, :] * scale)[:, None, :]
             ang = np.asarray(rot_factors, dtype=np.float64) * self.rot_max_deg
             R = Rotation.from_euler(self.rot_axis, ang, degrees=True).as_matrix()  # [S, 3, 3]
-            d = v - pivot
-            v = sum(d[..., j, None] * R[:, None, :, j] for j in range(3)) + pivot  # d R^T per sample
+            v = np.matmul(v - pivot, np.ascontiguousarray(R.transpose(0, 2, 1))) + pivot
         if moving_scene:
             if self.move is not None:
                 v = v + np.asarray(self.move, dtype=np.float64) * np.asarray(move_factors, dtype=np.float64)[:, None,
@@ -181,6 +180,18 @@ class SceneSampler:
         return (np.concatenate(verts, axis=1), np.concatenate(tris), np.broadcast_to(np.concatenate(neu), (S, nv)),
                 k, pts + self.bbox_center)
 
+    def features_batch(self, facs):
+        """``features`` of a list of samples: [S, T, D]."""
+        x = np.zeros((len(facs), self.trg_sample_num, self.feature_dim), dtype=np.float32)
+        x[:, :, :3] = np.stack([f["trg"] for f in facs])
+        if self.rot_num:
+            x[:, :, 3:3 + self.rot_num] = np.stack([f["rot"] for f in facs])[:, None, :]
+        if self.move_num:
+            x[:, :, 3 + self.rot_num:3 + self.rot_num + self.move_num] = np.stack([f["move"] for f in facs])[:, None, :]
+        x[:, :, -2] = np.array([f["resize"] for f in facs])[:, None]
+        x[:, :, -1] = np.array([f["freq"] for f in facs])[:, None]
+        return x
+
     def features(self, f):
         x = np.zeros((self.trg_sample_num, self.feature_dim), dtype=np.float32)
         x[:, :3] = f["trg"]
@@ -228,8 +239,7 @@ class SceneSampler:
             infos.extend(inf)
             p = boundary2potential_samples(gs, ks, neu, torch.stack(xs), pts)
             y[s0:s0 + len(facs), :, 0] = p.abs().float().cpu()  # one copy per batch
-            for q, f in enumerate(facs):
-                x[s0 + q] = torch.from_numpy(self.features(f))
+            x[s0:s0 + len(facs)] = torch.from_numpy(self.features_batch(facs))
         return x, y, infos
 
 

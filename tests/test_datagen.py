@@ -61,7 +61,8 @@ def test_sampler_layout_and_targets():
 def test_build_batch_equals_per_sample_build():
     """The batched driver's vectorised build of a batch (one rotation per
     sample from one scipy call, broadcast placement) gives each sample's
-    vertices, Neumann data, wavenumber and targets of the per-sample build."""
+    vertices, Neumann data, wavenumber, targets and features of the
+    per-sample build."""
     c = SceneObject(*mesh.icosphere(1, 0.05), resize=(0.0, 1.0, 1.0), rot_axis="x", rot_pos=(0.0, 0.02, 0.0),
                     rot_max_deg=45.0, move=(0.1, 0.0, -0.1), vibration="y")
     sc = SceneSampler(list(_objects()) + [c], trg_sample_num=5, freq_min=100.0, freq_max=4000.0,
@@ -74,6 +75,8 @@ def test_build_batch_equals_per_sample_build():
         v, t, neu, k, pts = sc.build(f)
         assert np.allclose(vb[q], v, rtol=0, atol=1e-14) and np.array_equal(tb, t)
         assert np.array_equal(nb[q], neu) and np.isclose(kb[q], k, rtol=1e-15) and np.allclose(pb[q], pts, atol=1e-14)
+    xb = sc.features_batch(facs)
+    assert all(np.array_equal(xb[q], sc.features(f)) for q, f in enumerate(facs))
 
 
 def test_default_batch_size():
