@@ -307,6 +307,7 @@ class EditableModalSound:
         ys = torch.linspace(0, 1, ffat_res[1], device="cuda", dtype=torch.float32)
         self.gridx, self.gridy = torch.meshgrid(xs, ys, indexing="ij")
         self.uniform = uniform
+        self._solver = None  # the surface's mesh handle, built by the first solve
         if uniform:
             self.point_num_per_sample = ffat_res[0] * ffat_res[1]
 
@@ -343,7 +344,14 @@ class EditableModalSound:
         """Pressure of every mode at the target points [modes, P] complex64:
         the modes' Burton-Miller systems (one per wavenumber on the same
         surface) solved together by the lockstep batched GMRES."""
-        solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
+        # one mesh handle across samples: a sample only rescales the surface,
+        # so the handle's connectivity (neighbour pairs, tiling) is kept and
+        # only its positions are replaced
+        if self._solver is None:
+            self._solver = BEM_Solver(self.vertices.contiguous(), self.triangles)
+        else:
+            self._solver.set_vertices(self.vertices.contiguous())
+        solver = self._solver
         ks = [float(k) for k in self.ks]
         phi, self.solve_infos = solver.neumann2dirichlet_batch(ks, self.neumann_vtx, tol=tol, restart=restart,
                                                                max_cycles=max_cycles)

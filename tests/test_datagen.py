@@ -145,6 +145,13 @@ def test_modal_sound_pipeline_batched_equals_loop():
     assert rel < 2e-5, rel
     xs, ys = generate_modal_dataset(ms, 2, seed=1)
     assert xs.shape == (2, 32, 5) and ys.shape == (2, 32, 6) and bool(torch.isfinite(ys).all())
+    # later samples move the first solve's mesh handle to the new size: the
+    # same pressure as a handle built for that size (the loop builds its own)
+    ms.sample(freqK_base=torch.tensor([0.8], device="cuda"), sizeK_base=torch.tensor([0.1], device="cuda"))
+    pb = ms.solve()
+    pl = ms.solve_loop()
+    rel = (torch.linalg.vector_norm(pb - pl) / torch.linalg.vector_norm(pl)).item()
+    assert rel < 2e-5, rel
 
 
 def _placements(ns, level=2, seed=4):
