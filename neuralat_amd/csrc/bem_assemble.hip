@@ -2314,10 +2314,12 @@ hipError_t launch_tri2vert_raw(const int32_t* tris, int64_t nt, int64_t nv, cons
 // out[p, v] = sum of the corner values of v's triangles for point p (the
 // P1 potential from the per-(point, triangle) scratch of potential_kernel):
 // one thread per (point, vertex), every entry written once (no fill, no
-// atomics, a fixed summation order).
-__global__ __launch_bounds__(256) void p1_point_gather_kernel(const float2* __restrict__ t, const int4* __restrict__ tris,
+// atomics, a fixed summation order: v's triangles ascending).  vt_corner
+// holds each incidence's scratch column (3 t + corner), so a value costs one
+// index load and one scratch load.
+__global__ __launch_bounds__(256) void p1_point_gather_kernel(const float2* __restrict__ t,
                                                               const int32_t* __restrict__ vt_ptr,
-                                                              const int32_t* __restrict__ vt_idx, int64_t nv,
+                                                              const int32_t* __restrict__ vt_corner, int64_t nv,
                                                               int64_t nt, int64_t np, int64_t tzstride,
                                                               float2* __restrict__ out, int64_t ld, int64_t zstride)
 {
@@ -2328,10 +2330,7 @@ __global__ __launch_bounds__(256) void p1_point_gather_kernel(const float2* __re
         const float2* row = t + p * nt * 3;
         float sr = 0.f, si = 0.f;
         for (int q = vt_ptr[v]; q < vt_ptr[v + 1]; ++q) {
-            const int tri = vt_idx[q];
-            const int4 tv = tris[tri];
-            const int c = tv.x == v ? 0 : (tv.y == v ? 1 : 2);
-            const float2 x = row[(int64_t)tri * 3 + c];
+            const float2 x = row[vt_corner[q]];
             sr += x.x;
             si += x.y;
         }
@@ -2368,7 +2367,7 @@ static hipError_t potential_t(const MeshDev& m, const float* pts, int64_t np, co
                 hipLaunchKernelGGL((potential_kernel<OP, P1, CLAMP, true>), grid, dim3(256), 0, s, m.rec6, m.tris,
                                    m.nt, pts + 3 * c0, cn, kt, t, (int64_t)0);
                 hipLaunchKernelGGL(p1_point_gather_kernel, dim3(grid1d(cn * m.nv, 256, 65536), 1, (unsigned)nz),
-                                   dim3(256), 0, s, (const float2*)t, m.tris, m.vt_ptr, m.vt_idx, m.nv, m.nt, cn,
+                                   dim3(256), 0, s, (const float2*)t, m.vt_ptr, m.vt_corner, m.nv, m.nt, cn,
                                    kt.zstride, out + c0 * ld, ld, kb.zstride);
                 e = hipGetLastError();
             }

@@ -65,6 +65,7 @@ struct MeshDev {
     int64_t npairs = 0;
     int32_t* vt_ptr = nullptr;  // [nv+1] vertex -> triangle CSR (ascending)
     int32_t* vt_idx = nullptr;
+    int32_t* vt_corner = nullptr;  // [3nt] in vt_idx order: 3 t + the vertex's corner in triangle t
     P1Tiles tiles{};
     // the P1 numbering of the tiles (vertices in order of first appearance
     // along the tiles' triangle order): p1_vnew [nv] caller id -> P1 id,

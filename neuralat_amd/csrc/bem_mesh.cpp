@@ -256,13 +256,17 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     m.h_tris.assign(h_tris, h_tris + 3 * nt);
 
     // ---- vertex -> triangle CSR ----
-    std::vector<int32_t> vt_ptr((size_t)nv + 1, 0), vt_idx((size_t)nt * 3);
+    std::vector<int32_t> vt_ptr((size_t)nv + 1, 0), vt_idx((size_t)nt * 3), vt_corner((size_t)nt * 3);
     for (int64_t t = 0; t < 3 * nt; ++t) vt_ptr[h_tris[t] + 1]++;
     for (int64_t v = 0; v < nv; ++v) vt_ptr[v + 1] += vt_ptr[v];
     {
         std::vector<int32_t> fillp(vt_ptr.begin(), vt_ptr.end() - 1);
         for (int64_t t = 0; t < nt; ++t)
-            for (int a = 0; a < 3; ++a) vt_idx[fillp[h_tris[3 * t + a]]++] = (int32_t)t;
+            for (int a = 0; a < 3; ++a) {
+                const int32_t q = fillp[h_tris[3 * t + a]]++;
+                vt_idx[q] = (int32_t)t;
+                vt_corner[q] = (int32_t)(3 * t + a);
+            }
     }
     for (int64_t v = 0; v < nv; ++v)
         if (vt_ptr[v + 1] == vt_ptr[v]) {
@@ -404,6 +408,7 @@ int mesh_build(const float* h_verts, int64_t nv, const int32_t* h_tris, int64_t 
     parts.push_back({m.h_pair_ptr.data(), m.h_pair_ptr.size() * 8, (void**)&m.pair_ptr});
     parts.push_back({vt_ptr.data(), vt_ptr.size() * 4, (void**)&m.vt_ptr});
     parts.push_back({vt_idx.data(), vt_idx.size() * 4, (void**)&m.vt_idx});
+    parts.push_back({vt_corner.data(), vt_corner.size() * 4, (void**)&m.vt_corner});
     int32_t *d_perm, *d_rp, *d_rvp, *d_rv, *d_cp, *d_cvp, *d_cv, *d_cip, *d_ci, *d_rct;
     int64_t* d_sym;
     int4 *d_rs, *d_cs;
