@@ -1237,12 +1237,34 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// sum over aligned groups of N lanes (N a power of two <= 64), in every lane
+// v from another lane of the same 16-lane row (DPP control CTRL), both
+// 32-bit halves moved by the VALU (no LDS permute)
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// sum over aligned groups of N lanes (N a power of two <= 64), in every lane.
+// N = 16 (one DPP row): quad exchanges (lanes 1 0 3 2, then 2 3 0 1), then
+// the half-row mirror (i <-> 7 - i within 8 lanes) and the row mirror
+// (i <-> 15 - i); each step adds the same two values in every lane, so all
+// 16 lanes end with the same sum.  Other N: cross-lane permutes.
 template <int N>
 __device__ __forceinline__ double group_sum(double v)
 {
+    if constexpr (N == 16) {
+        v += dpp_double<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+        v += dpp_double<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+        v += dpp_double<0x141>(v);  // row_half_mirror
+        v += dpp_double<0x140>(v);  // row_mirror
+    } else {
 #pragma unroll
-    for (int off = N / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        for (int off = N / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    }
     return v;
 }
 
