@@ -1249,18 +1249,24 @@ __device__ __forceinline__ double dpp_double(double v)
 }
 
 // sum over aligned groups of N lanes (N a power of two <= 64), in every lane.
-// N = 16 (one DPP row): quad exchanges (lanes 1 0 3 2, then 2 3 0 1), then
-// the half-row mirror (i <-> 7 - i within 8 lanes) and the row mirror
-// (i <-> 15 - i); each step adds the same two values in every lane, so all
-// 16 lanes end with the same sum.  Other N: cross-lane permutes.
+// N = 16 or 64: the sum of each 16-lane DPP row first -- quad exchanges
+// (lanes 1 0 3 2, then 2 3 0 1), the half-row mirror (i <-> 7 - i within 8
+// lanes) and the row mirror (i <-> 15 - i) -- then, for 64, the rows by
+// cross-lane permutes (xor 16, xor 32).  Each step adds the same two values
+// in every lane, so all lanes of a group end with the same sum.  Other N:
+// cross-lane permutes throughout.
 template <int N>
 __device__ __forceinline__ double group_sum(double v)
 {
-    if constexpr (N == 16) {
+    if constexpr (N == 16 || N == 64) {
         v += dpp_double<0xB1>(v);   // quad_perm [1, 0, 3, 2]
         v += dpp_double<0x4E>(v);   // quad_perm [2, 3, 0, 1]
         v += dpp_double<0x141>(v);  // row_half_mirror
         v += dpp_double<0x140>(v);  // row_mirror
+        if constexpr (N == 64) {
+            v += __shfl_xor(v, 16, 64);
+            v += __shfl_xor(v, 32, 64);
+        }
     } else {
 #pragma unroll
         for (int off = N / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
