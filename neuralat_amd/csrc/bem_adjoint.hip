@@ -86,12 +86,7 @@ __constant__ float c_rule6_x2[6] = {0.445948490915965f, 0.091576213509771f, 0.44
                                     1.0f - 2.0f * 0.445948490915965f, 0.091576213509771f,
                                     1.0f - 2.0f * 0.091576213509771f};
 
-__device__ __forceinline__ float wsum(float v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ float wsum(float v) { return wave_sum64(v); }
 
 // d|J|/dv for the vertex triangle (v0, v1, v2) contracted with g_c (dL/dc,
 // c = (v1 - v0) x (v2 - v0)); adds into gv[3][3].
@@ -268,12 +263,7 @@ __global__ __launch_bounds__(256) void adj_regular_kernel(const float* __restric
 // ---------------------------------------------------------------------------
 // neighbour pairs: Sauter-Schwab, one wavefront per pair
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wsum_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wsum_d(double v) { return wave_sum64(v); }
 
 template <bool CLAMP>
 __global__ __launch_bounds__(256) void adj_singular_kernel(const float* __restrict__ geo, const int4* __restrict__ tris,

@@ -97,19 +97,9 @@ __device__ __forceinline__ void bm_adj_eval(float dx, float dy, float dz, const 
     o.Gi = G.i;
 }
 
-__device__ __forceinline__ float wsum(float v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ float wsum(float v) { return wave_sum64(v); }
 
-__device__ __forceinline__ double wsum_d(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ __forceinline__ double wsum_d(double v) { return wave_sum64(v); }
 
 // dL/dc (c = (v1 - v0) x (v2 - v0)) to the three vertices, added into gv
 __device__ __forceinline__ void chain_c(const float* v, const float* gc, float gv[3][3])

@@ -1237,41 +1237,20 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// v from another lane of the same 16-lane row (DPP control CTRL), both
-// 32-bit halves moved by the VALU (no LDS permute)
-template <int CTRL>
-__device__ __forceinline__ double dpp_double(double v)
-{
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
-// sum over aligned groups of N lanes (N a power of two <= 64), in every lane.
-// N = 16 or 64: the sum of each 16-lane DPP row first -- quad exchanges
-// (lanes 1 0 3 2, then 2 3 0 1), the half-row mirror (i <-> 7 - i within 8
-// lanes) and the row mirror (i <-> 15 - i) -- then, for 64, the rows by
-// cross-lane permutes (xor 16, xor 32).  Each step adds the same two values
-// in every lane, so all lanes of a group end with the same sum.  Other N:
-// cross-lane permutes throughout.
+// sum over aligned groups of N lanes (N a power of two <= 64), in every lane
+// (16 and 64: DPP row moves, bem_device.hpp; others: cross-lane permutes)
 template <int N>
 __device__ __forceinline__ double group_sum(double v)
 {
-    if constexpr (N == 16 || N == 64) {
-        v += dpp_double<0xB1>(v);   // quad_perm [1, 0, 3, 2]
-        v += dpp_double<0x4E>(v);   // quad_perm [2, 3, 0, 1]
-        v += dpp_double<0x141>(v);  // row_half_mirror
-        v += dpp_double<0x140>(v);  // row_mirror
-        if constexpr (N == 64) {
-            v += __shfl_xor(v, 16, 64);
-            v += __shfl_xor(v, 32, 64);
-        }
+    if constexpr (N == 16) {
+        return row_sum16(v);
+    } else if constexpr (N == 64) {
+        return wave_sum64(v);
     } else {
 #pragma unroll
         for (int off = N / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        return v;
     }
-    return v;
 }
 
 // MODE 0: write (DP0: out = alpha * entry) or add (P1) the entries;

@@ -83,6 +83,51 @@ struct cf {
 
 // sin(2 pi t), cos(2 pi t) for t in revolutions, on the hardware
 // transcendental unit after exact range reduction to [0, 1).
+// Cross-lane sums in every lane of a wavefront.  Within each 16-lane row the
+// values move by DPP (quad exchanges [1 0 3 2] and [2 3 0 1], the half-row
+// mirror i <-> 7 - i, the row mirror i <-> 15 - i), across rows by two
+// permutes (xor 16, xor 32).  Each step adds the same two values in every
+// lane, so every lane ends with the same sum.
+template <int CTRL>
+__device__ __forceinline__ float dpp_row_f(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_row_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float wave_sum64(float v)
+{
+    v += dpp_row_f<0xB1>(v);
+    v += dpp_row_f<0x4E>(v);
+    v += dpp_row_f<0x141>(v);
+    v += dpp_row_f<0x140>(v);
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
+}
+__device__ __forceinline__ double wave_sum64(double v)
+{
+    v += dpp_row_d<0xB1>(v);
+    v += dpp_row_d<0x4E>(v);
+    v += dpp_row_d<0x141>(v);
+    v += dpp_row_d<0x140>(v);
+    v += __shfl_xor(v, 16, 64);
+    return v + __shfl_xor(v, 32, 64);
+}
+// the same over each 16-lane row alone
+__device__ __forceinline__ double row_sum16(double v)
+{
+    v += dpp_row_d<0xB1>(v);
+    v += dpp_row_d<0x4E>(v);
+    v += dpp_row_d<0x141>(v);
+    return v + dpp_row_d<0x140>(v);
+}
+
 __device__ __forceinline__ void sincos_rev(float t, float& s, float& c)
 {
     float f = __builtin_amdgcn_fractf(t);
