@@ -11,6 +11,7 @@
 //    reference-compatible BiCGSTAB (src/bem/BiCGSTAB.py:5-107).
 #include <hip/hip_runtime.h>
 
+#include "bem_device.hpp"
 #include "bem_linalg.hpp"
 
 #include <algorithm>
@@ -472,11 +473,8 @@ __global__ __launch_bounds__(kArnoldiThreads) void arnoldi_small_kernel(
                     }
                 }
             }
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                sr += __shfl_xor(sr, o, 64);
-                si += __shfl_xor(si, o, 64);
-            }
+            sr = wave_sum64(sr);
+            si = wave_sum64(si);
             if (lane == 0) hh[q] = make_double2(sr, si);
         }
         __syncthreads();
@@ -506,8 +504,7 @@ __global__ __launch_bounds__(kArnoldiThreads) void arnoldi_small_kernel(
         const float2 v = w[e];
         a2 += (double)v.x * v.x + (double)v.y * v.y;
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+    a2 = wave_sum64(a2);
     if (lane == 0) red[wave] = a2;
     __syncthreads();
     double nrm2 = 0.0;
@@ -770,11 +767,8 @@ __global__ __launch_bounds__(kArnoldiThreads) void barnoldi_small_kernel(
                     }
                 }
             }
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                sr += __shfl_xor(sr, o, 64);
-                si += __shfl_xor(si, o, 64);
-            }
+            sr = wave_sum64(sr);
+            si = wave_sum64(si);
             if (lane == 0) hh[q] = make_double2(sr, si);
         }
         __syncthreads();
@@ -804,8 +798,7 @@ __global__ __launch_bounds__(kArnoldiThreads) void barnoldi_small_kernel(
         const float2 v = ws[e];
         a2 += (double)v.x * v.x + (double)v.y * v.y;
     }
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) a2 += __shfl_xor(a2, o, 64);
+    a2 = wave_sum64(a2);
     if (lane == 0) red[wave] = a2;
     __syncthreads();
     double nrm2 = 0.0;
