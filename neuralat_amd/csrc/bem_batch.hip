@@ -1191,7 +1191,7 @@ int bem_batch_gmres(const bem_batch_system* sys, int ns, float tol, int restart,
     std::vector<XVec> hv;
     XVec* dvec = (XVec*)(W + L.vec);
     if (int rc = gather_systems(sys, hs, E, tol, hb, dvec, hv, s)) return rc;
-    XGraphs G;  // captured by the first cycle that runs past one segment
+    XGraphs G;  // each part of a cycle captured on its third use (xrun_slot)
     std::vector<int> hc(ns, 0), hk(ns, 0), hd(ns, 0), its(ns, 0);
     bool all = false;
     for (int cycles = 0; cycles < max_cycles && !all; ++cycles) {
@@ -1292,7 +1292,7 @@ int bem_batch_fgmres(const bem_batch_system* sys, int ns, int dtype, float tol, 
         hipLaunchKernelGGL(xto_lowp_kernel<kDtFP16>, grow, dim3(256), 0, s, dsys, dlow, scale);
     XTRY(hipGetLastError());
 
-    XGraphs G;  // the inner solves' graphs, captured by the first inner cycle that runs past one segment
+    XGraphs G;  // the inner solves' graphs, shared by every outer step: each part captured on its third use
     const dim3 gx((unsigned)((nmax + 255) / 256), (unsigned)ns);
     const dim3 gc((unsigned)((nmax + 63) / 64), (unsigned)ns);
     std::vector<int> hc(ns, 0), hd(ns, 0), hdi(ns, 0), hci(ns, 0), hko(ns, 0), hki(ns, 0), its_o(ns, 0),

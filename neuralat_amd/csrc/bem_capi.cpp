@@ -395,10 +395,17 @@ int bem_geomset_create(const bem_mesh* mesh, const float* verts, int ns, void* s
     if (nth <= 1) {
         work(0, 1);
     } else {
+        // a thread that cannot be started (std::system_error) leaves its
+        // share to this one: no exception crosses the C ABI
         std::vector<std::thread> pool;
-        pool.reserve(nth - 1);
-        for (int t = 1; t < nth; ++t) pool.emplace_back(work, t, nth);
+        int started = 1;
+        try {
+            pool.reserve(nth - 1);
+            for (; started < nth; ++started) pool.emplace_back(work, started, nth);
+        } catch (...) {
+        }
         work(0, nth);
+        for (int t = started; t < nth; ++t) work(t, nth);
         for (auto& th : pool) th.join();
     }
     for (int z = 0; z < ns; ++z)
