@@ -77,6 +77,17 @@ def test_build_batch_equals_per_sample_build():
         assert np.array_equal(nb[q], neu) and np.isclose(kb[q], k, rtol=1e-15) and np.allclose(pb[q], pts, atol=1e-14)
     xb = sc.features_batch(facs)
     assert all(np.array_equal(xb[q], sc.features(f)) for q, f in enumerate(facs))
+    # a scene with neither a rotating nor a moving object (no move/position step)
+    st = SceneSampler([SceneObject(*mesh.icosphere(1, 0.05), resize=(1.0, 1.0, 0.0), position=(0.3, 0.0, 0.0))],
+                      trg_sample_num=4, freq_min=50.0, freq_max=500.0, trg_pos_min=[-1] * 3, trg_pos_max=[1] * 3)
+    facs = [st.sample_factors(rng) for _ in range(3)]
+    vb, tb, nb, kb, pb = st.build_batch(facs)
+    xb = st.features_batch(facs)
+    assert xb.shape == (3, 4, 5)
+    for q, f in enumerate(facs):
+        v, t, neu, k, pts = st.build(f)
+        assert np.allclose(vb[q], v, rtol=0, atol=1e-14) and np.isclose(kb[q], k, rtol=1e-15)
+        assert np.allclose(pb[q], pts, atol=1e-14) and np.array_equal(xb[q], st.features(f))
 
 
 def test_default_batch_size():
